@@ -1,0 +1,291 @@
+/*
+ * libhmec_hip — C ABI of the MI355X (gfx950) implementation of the H-MEC 2D
+ * per-timestep hot path: marker->node scatter, visco-elasto-plastic update,
+ * rate-and-state friction (RSF) fault solve, fixed-pattern CSR assembly of the
+ * 6-unknown-per-cell Stokes+Darcy system, its sparse LU factorization/solve,
+ * and node->marker RK4 advection.
+ *
+ * Reference: lucadalzilio/H-MECs-2D, cpp/h_mec_rsf_v9.cpp and cpp/constants.hpp.
+ * The reference has no library interface for this path: it is inline code in
+ * main() (h_mec_rsf_v9.cpp:376-1726) plus Eigen's sparse-solver concept
+ * (PardisoLU: analyzePattern / factorize / solve, h_mec_rsf_v9.cpp:96,1050-1052).
+ * Each entry point below names the reference lines it replaces.
+ *
+ * Conventions
+ *  - Every int-returning call returns HMEC_OK (0) or a negative HMEC_E_* code;
+ *    hmec_strerror() names it.  No exceptions cross the ABI.
+ *  - A context is not reentrant; distinct contexts may be used from distinct
+ *    host threads.  Every context owns its device buffers and one HIP stream.
+ *  - Nodal arrays use the reference's Eigen layout: column-major, element
+ *    (i, j) at [i + j * rows], rows = Ny (basic grid) or Ny1 = Ny + 1
+ *    (pressure / vx / vy grids).  hmec_get_field can also return row-major
+ *    (the HDF5 layout of cpp/hdf5.cpp:46-53).
+ *  - The global unknown index is the reference's: k = (j * Ny1 + i) * 6 + v,
+ *    v = 0:Pt 1:vxs 2:vys 3:Pf 4:vxD 5:vyD (h_mec_rsf_v9.cpp:690-695).
+ */
+#ifndef HMEC_H
+#define HMEC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ errors */
+#define HMEC_OK 0
+#define HMEC_E_ARG (-1)      /* invalid argument or call order               */
+#define HMEC_E_HIP (-2)      /* HIP runtime error                            */
+#define HMEC_E_OOM (-3)      /* device allocation failed / over footprint    */
+#define HMEC_E_PIVOT (-4)    /* zero pivot that perturbation could not fix   */
+#define HMEC_E_NOCONV (-5)   /* iterative refinement did not reach tolerance */
+#define HMEC_E_NODEV (-6)    /* no HIP device visible                        */
+#define HMEC_E_SOLVER (-7)   /* external solver error                        */
+const char *hmec_strerror(int code);
+
+/* --------------------------------------------------------------- parameters
+ * Runtime mirror of cpp/constants.hpp:14-151 (same names, same defaults from
+ * hmec_params_default).  Derived quantities (dx, dy, faultwidth, line_fault,
+ * upper_block, lower_block) are filled by hmec_params_finalize, which follows
+ * constants.hpp:36-76 unless a field was overridden (block bounds).          */
+typedef struct hmec_params {
+    /* grid (constants.hpp:23-26) */
+    double xsize, ysize;
+    int Nx, Ny;
+    /* markers per basic cell in x and y (constants.hpp:81-82 use 4 and 4) */
+    int mx_per_cell, my_per_cell;
+    /* RSF profile tapers along x (constants.hpp:29-33) */
+    double TS_1, TS_2, TS_3, TS_4;
+    /* RSF parameters, index 0 = block, 1 = fault (constants.hpp:93-96) */
+    double arsfm[2], brsfm[2], lrsfm[2], omm[2];
+    double V0;          /* reference slip velocity (constants.hpp:97)       */
+    double alpha;       /* porous-matrix viscosity exponent (:98)           */
+    double POR0;        /* standard porosity (:101)                         */
+    double cohes, friction, dilatation, tensile, shearmod;   /* :105-109   */
+    double BETAFLUID, BETASOLID;                              /* :110-111   */
+    double gx, gy, PCONF, PTFDIFF;                            /* :115-118   */
+    double etamin, etamax, kkkmin, kkkmax, stpmax, stpmax1;   /* :122-127   */
+    double bcupper, bclower, bcvyflower;                      /* :130-132   */
+    int niterglobal, ynlastmax, dtstep;                       /* :135-137   */
+    double vratiomax, dtkoef, dtkoefup, dtkoefv, errmin, etawt, syieldmin, tyield; /* :138-145 */
+    double dtelastic0, dtmin, inertia;                        /* :71-74     */
+    /* derived (hmec_params_finalize) */
+    double dx, dy, faultwidth;       /* constants.hpp:47-48,66            */
+    double upper_block, lower_block; /* constants.hpp:76-77 (fault band)  */
+    int line_fault;                  /* constants.hpp:43                  */
+    /* switches */
+    int advect_markers;   /* 1: run the node->marker RK4 advection loop, which
+                             the reference disables (h_mec_rsf_v9.cpp:1563)  */
+    int bisect_cap;       /* iteration cap of the RSF stress bisection (the
+                             reference has none, h_mec_rsf_v9.cpp:1224)      */
+    int block_override;   /* 1: keep caller's upper_block / lower_block       */
+} hmec_params;
+
+/* sizeof(hmec_params) / sizeof(hmec_step_info), for binding checks */
+int hmec_params_size(void);
+int hmec_step_info_size(void);
+
+/* constants.hpp defaults (401x51 grid, 40 km x 10 km) */
+void hmec_params_default(hmec_params *p);
+/* compute derived fields; returns HMEC_E_ARG on an invalid grid */
+int hmec_params_finalize(hmec_params *p);
+/* Workload presets.  cfg 1: 101x51, 50x10 km, 4x4 markers/cell, central
+ * 20 km velocity-weakening patch.  cfg 2: 401x101 (+-5% lognormal marker
+ * properties).  cfg 3: 801x201 (fault viscosity contrast up to 1e6, log-uniform
+ * permeability).  cfg 4: 601x151 coseismic.  cfg 5 grid sizes come from the
+ * serving mix.  Returns HMEC_E_ARG for an unknown cfg. */
+int hmec_params_config(int cfg, hmec_params *p);
+
+/* ------------------------------------------------------------------ markers
+ * Marker structure-of-arrays, constants.hpp:153-173 (the 19 marker vectors). */
+enum hmec_marker_field {
+    HMEC_M_XM = 0, HMEC_M_YM, HMEC_M_PORM, HMEC_M_KKKM, HMEC_M_T_MARKER,
+    HMEC_M_COHESCM, HMEC_M_COHESTM, HMEC_M_FRICTCM, HMEC_M_DILATCM,
+    HMEC_M_FRICTTM, HMEC_M_ETASM, HMEC_M_ETAM, HMEC_M_RHOM, HMEC_M_RHOFM,
+    HMEC_M_ETAFM, HMEC_M_GM, HMEC_M_SXXM, HMEC_M_SYYM, HMEC_M_SXYM,
+    HMEC_NMARKER_FIELDS
+};
+/* number of markers of a grid: (Nx-1)*mx * (Ny-1)*my (constants.hpp:81-85) */
+int64_t hmec_marker_count(const hmec_params *p);
+/* Seeded synthetic marker generator (host only, no GPU).  Lattice order
+ * m = ix * Ny_markers + iy as in h_mec_rsf_v9.cpp:188-191; each marker is
+ * jittered uniformly inside its marker sub-cell when jitter != 0 (jitter == 0
+ * reproduces the reference's regular lattice).  Materials follow
+ * h_mec_rsf_v9.cpp:192-208 with the per-config perturbations of the presets.
+ * soa[f] must hold hmec_marker_count(p) doubles for each of the 19 fields. */
+int hmec_generate_markers(const hmec_params *p, int cfg, uint64_t seed,
+                          int jitter, double *const soa[HMEC_NMARKER_FIELDS]);
+
+/* ------------------------------------------------------------------- fields
+ * Nodal fields; B = basic grid (Ny x Nx), S = staggered (Ny1 x Nx1).
+ * Names are the reference's globals (h_mec_rsf_v9.cpp:55-109).            */
+enum hmec_field {
+    /* basic grid */
+    HMEC_F_OM0 = 0, HMEC_F_OM, HMEC_F_OM5, HMEC_F_ARSF, HMEC_F_BRSF, HMEC_F_LRSF,
+    HMEC_F_RHO, HMEC_F_ETA, HMEC_F_ETA0, HMEC_F_ETA1, HMEC_F_ETA5, HMEC_F_ETA00,
+    HMEC_F_ETA50, HMEC_F_IETAPLB, HMEC_F_SXY, HMEC_F_SXY0, HMEC_F_YNY0,
+    HMEC_F_YNY, HMEC_F_KKK, HMEC_F_GGG, HMEC_F_COHC, HMEC_F_COHT, HMEC_F_FRIC,
+    HMEC_F_FRIT, HMEC_F_DILC, HMEC_F_TTT, HMEC_F_EIIB, HMEC_F_VSLIPB,
+    HMEC_F_ESP, HMEC_F_EXY, HMEC_F_DSY, HMEC_F_SIGMAY, HMEC_F_SII_FAULT,
+    HMEC_F_SIIB,
+    HMEC_NBASIC_FIELDS,
+    /* staggered grids */
+    HMEC_F_PT = HMEC_NBASIC_FIELDS, HMEC_F_VXS, HMEC_F_VYS, HMEC_F_PF,
+    HMEC_F_VXD, HMEC_F_VYD, HMEC_F_ETAB, HMEC_F_ETAB0, HMEC_F_ETAP,
+    HMEC_F_ETAP0, HMEC_F_POR, HMEC_F_GGGP, HMEC_F_GGGB, HMEC_F_PTF0,
+    HMEC_F_PT0, HMEC_F_PF0, HMEC_F_SXX, HMEC_F_SXX0, HMEC_F_SYY, HMEC_F_SYY0,
+    HMEC_F_DILP, HMEC_F_RHOX, HMEC_F_RHOFX, HMEC_F_ETADX, HMEC_F_PORX,
+    HMEC_F_VX0, HMEC_F_VXF0, HMEC_F_RHOY, HMEC_F_RHOFY, HMEC_F_ETADY,
+    HMEC_F_PORY, HMEC_F_VY0, HMEC_F_VYF0, HMEC_F_EXX, HMEC_F_EYY,
+    HMEC_NFIELDS
+};
+#define HMEC_COLMAJOR 0   /* Eigen MatrixXd layout                        */
+#define HMEC_ROWMAJOR 1   /* HDF5 dataset layout (cpp/hdf5.cpp:46-53)      */
+/* rows/cols of a field: Ny x Nx (basic) or Ny1 x Nx1 (staggered) */
+int hmec_field_shape(const hmec_params *p, int field_id, int *rows, int *cols);
+
+/* ------------------------------------------------------------------ scalars
+ * Host-visible per-step state: the reference's scalar globals
+ * (h_mec_rsf_v9.cpp:99-105) and the step's outcome.                        */
+typedef struct hmec_step_info {
+    int timestep;          /* 1-based step just completed                   */
+    int iterations;        /* Picard iterations (iterstep + 1)              */
+    int ynlast;            /* iterations since last dt decrease             */
+    int yndtdecrease;
+    double dt, dt00, dtx, dty, dtlapusta, Vmax, maxvxy, timesum;
+    double ptscale;        /* last Picard iteration's P scaling             */
+    int bisect_capped;     /* RSF nodes whose bisection hit bisect_cap      */
+    /* linear-solver statistics accumulated over the step */
+    int factorizations;    /* numeric factorizations performed              */
+    int solves;            /* linear solves (one per Picard iteration)      */
+    int ir_iterations;     /* iterative-refinement sweeps (all solves)      */
+    int fallbacks;         /* FP32 factor -> FP64 factor fallbacks          */
+    int perturbed_pivots;  /* static-pivot perturbations                    */
+    double max_rel_residual; /* max over solves of ||b-Ax||inf/||b||inf     */
+    /* wall-clock split of the step (ms) */
+    double ms_scatter, ms_assemble, ms_factor, ms_solve, ms_update, ms_total;
+} hmec_step_info;
+
+/* ------------------------------------------------------------------ context */
+typedef struct hmec_ctx hmec_ctx;
+
+/* Linear-solver precision (the factorization; residuals, refinement and all
+ * physics stay FP64). */
+#define HMEC_PREC_FP64 0  /* FP64 factor, refactored every Picard iteration
+                             like Eigen PardisoLU (h_mec_rsf_v9.cpp:1050-1052) */
+#define HMEC_PREC_FP32 1  /* equilibrated FP32 factor + FP64 iterative
+                             refinement, factor reused across Picard iterations
+                             while refinement converges; FP64 fallback         */
+
+/* Create a context on the current HIP device; copies params.  The fresh
+ * nodal state follows h_mec_rsf_v9.cpp:283-338 (replaces main()'s init). */
+int hmec_create(const hmec_params *p, hmec_ctx **out);
+void hmec_destroy(hmec_ctx *ctx);
+/* Device footprint estimate in bytes of a context (markers, nodal state,
+ * CSR, factor fill) before it is created; used for admission. */
+int64_t hmec_footprint_bytes(const hmec_params *p);
+int hmec_set_precision(hmec_ctx *ctx, int precision);
+
+/* Upload the 19 marker arrays (host, M doubles each); marker order kept.
+ * Replaces the marker init of h_mec_rsf_v9.cpp:188-209. */
+int hmec_set_markers(hmec_ctx *ctx, const double *const soa[HMEC_NMARKER_FIELDS], int64_t M);
+int hmec_get_markers(hmec_ctx *ctx, double *const soa[HMEC_NMARKER_FIELDS]);
+/* Per-marker anchor cells, 4 grids x M ints: basic, P, vx, vy; each value is
+ * i + j * Ny1 as computed by fix()/check_bounds() (h_mec_rsf_v9.cpp:432-433,
+ * 468-469, 493-494, 515-516).  Diagnostic for the cell-assignment parity. */
+int hmec_get_marker_cells(hmec_ctx *ctx, int32_t *cells4xM);
+
+int hmec_set_field(hmec_ctx *ctx, int field_id, const double *host, int layout);
+int hmec_get_field(hmec_ctx *ctx, int field_id, double *host, int layout);
+/* Scalar state (dt, timesum, ...) in/out; timestep is the next step to run. */
+int hmec_get_state(hmec_ctx *ctx, hmec_step_info *info);
+int hmec_set_state(hmec_ctx *ctx, const hmec_step_info *info);
+
+/* -------------------------------------------------------------- step kernels
+ * Stage-level entry points, each one HIP launch sequence on the context's
+ * stream.  Together they make up hmec_step().                              */
+
+/* Marker->node scatter and normalization, h_mec_rsf_v9.cpp:384-583. */
+int hmec_scatter(hmec_ctx *ctx);
+/* Start of a timestep: ETA/ETA1/ETA50 from ETA0 or ETA00, OM = OM0, dt
+ * growth, h_mec_rsf_v9.cpp:586-608. */
+int hmec_step_begin(hmec_ctx *ctx);
+/* Picard-iteration prologue: copy_bounds(pt, pf), viscosity floor, plastic
+ * strain rate, viscoplastic ETAP/ETAB, GGGB, DILP; h_mec_rsf_v9.cpp:612-669. */
+int hmec_vep_pre(hmec_ctx *ctx, double dt);
+/* ptscale = pfscale (h_mec_rsf_v9.cpp:671-677) and the fixed-pattern CSR
+ * values + RHS of the coupled system (h_mec_rsf_v9.cpp:682-1047). */
+int hmec_assemble(hmec_ctx *ctx, double dt, double *ptscale_out);
+/* CSR of the last assembly (row_ptr N+1, col_idx nnz, values nnz, rhs N);
+ * any pointer may be NULL.  hmec_csr_dims gives N and nnz. */
+int hmec_csr_dims(hmec_ctx *ctx, int64_t *n, int64_t *nnz);
+int hmec_get_csr(hmec_ctx *ctx, int32_t *row_ptr, int32_t *col_idx, double *values, double *rhs);
+/* Replace the CSR values / RHS of the fixed pattern (for solver tests). */
+int hmec_set_csr_values(hmec_ctx *ctx, const double *values, const double *rhs);
+
+/* Sparse direct solver, mirroring Eigen's split (h_mec_rsf_v9.cpp:1050-1052):
+ * analyze = nested-dissection ordering + symbolic multifrontal analysis,
+ * done once per grid (the reference redoes it every Picard iteration);
+ * factorize = numeric multifrontal LU in the given precision;
+ * solve = triangular solves + FP64 iterative refinement of the last CSR. */
+typedef struct hmec_solve_stats {
+    int ir_iterations;
+    int fallback;            /* 1: FP32 factor stalled, FP64 factor used    */
+    int perturbed_pivots;
+    double rel_residual;     /* ||b - A x||inf / ||b||inf after refinement   */
+    double factor_flops;     /* flops of one factorization (symbolic count)  */
+    int64_t factor_nnz;      /* entries of L+U                               */
+    int nfronts, nlevels, max_front;
+} hmec_solve_stats;
+int hmec_analyze(hmec_ctx *ctx);
+int hmec_factorize(hmec_ctx *ctx, int precision);
+int hmec_solve(hmec_ctx *ctx, hmec_solve_stats *stats);
+/* solution vector of the last solve (N doubles, reference unknown order) */
+int hmec_get_solution(hmec_ctx *ctx, double *x);
+
+/* Reload (h_mec_rsf_v9.cpp:1060-1075), stress update (:1102-1127), RSF fault
+ * solve (:1152-1308) and the reductions feeding the dt controller
+ * (:1310-1415).  Returns the scalars the host-side controller needs. */
+typedef struct hmec_iter_scalars {
+    int ynpl;
+    double ddd, dtlapusta, Vmax;       /* Vmax: VSLIPB max before the RSF   */
+    double vxs_max, vxs_min, vys_max, vys_min;
+    double maxvxs_band, maxvys_band;   /* |v| max inside the fault band     */
+    double dtslip;                     /* min dx*stpmax/V over V > 0        */
+    int bisect_capped;
+} hmec_iter_scalars;
+int hmec_update(hmec_ctx *ctx, double dt, double ptscale, hmec_iter_scalars *out);
+/* Picard-iteration epilogue when not converged: ETA = clamp(ETA5), ETA50,
+ * YNY0 = YNY, OM = OM5 (h_mec_rsf_v9.cpp:1440-1448); ETA = ETA0 if ynpl==0
+ * (:1315-1317). */
+int hmec_iter_commit(hmec_ctx *ctx, int ynpl, int converged);
+/* After the Picard loop: ETA00 = ETA50, OM0 = OM, final stresses with the
+ * final dt (h_mec_rsf_v9.cpp:1469-1521), RK4 advection when enabled
+ * (:1561-1656) and the history update (:1658-1726). */
+int hmec_step_end(hmec_ctx *ctx, double dt);
+int hmec_advect(hmec_ctx *ctx, double dt);
+
+/* One complete timestep (h_mec_rsf_v9.cpp:376-1726): the C++ host driver
+ * with the reference's Picard / dt controller around the kernels above. */
+int hmec_step(hmec_ctx *ctx, hmec_step_info *info);
+
+/* ------------------------------------------------------------------ serving
+ * Factor several same-size contexts' current CSR matrices in one batched
+ * launch sequence (all contexts must share Nx, Ny and the current device). */
+int hmec_batch_factorize(hmec_ctx *const *ctxs, int n, int precision);
+/* Advance n contexts by one timestep each, overlapping their work on the
+ * device; same-size members share batched factorizations. */
+int hmec_batch_step(hmec_ctx *const *ctxs, int n, hmec_step_info *infos);
+
+/* ------------------------------------------------------------------ timing
+ * Average device duration (ms, HIP events on the context's stream) of the
+ * dominant kernels since the last reset; name = "scatter", "assemble",
+ * "factor", "solve".  Returns the number of samples. */
+int hmec_kernel_time(hmec_ctx *ctx, const char *name, double *avg_ms, double *bytes);
+int hmec_timing_enable(hmec_ctx *ctx, int on);
+int hmec_timing_reset(hmec_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HMEC_H */
