@@ -1,0 +1,479 @@
+// Context management and the stage-level C ABI of libhmec_hip.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hmec_internal.h"
+
+static thread_local std::string g_last_error;
+void hmec_set_last_error(const std::string &s) { g_last_error = s; }
+extern "C" const char *hmec_last_error(void) { return g_last_error.c_str(); }
+
+size_t hmec_sort_tmp_bytes(int64_t M, int nbits);
+
+// Eigen 3.3 VectorXd::LinSpaced(n, low, high) (constants.hpp:57-64)
+static void linspaced(double *v, int n, double low, double high) {
+    if (n == 1) { v[0] = high; return; }
+    const double step = (high - low) / (double)(n - 1);
+    const bool flip = std::fabs(high) < std::fabs(low);
+    for (int i = 0; i < n; i++) {
+        if (flip) v[i] = (i == 0) ? low : (high - (double)(n - 1 - i) * step);
+        else v[i] = (i == n - 1) ? high : (low + (double)i * step);
+    }
+}
+
+void tm_begin(hmec_ctx *c, Timer &t) {
+    if (!c->timing) return;
+    if (!t.a) {
+        hipEventCreate(&t.a);
+        hipEventCreate(&t.b);
+    }
+    hipEventRecord(t.a, c->stream);
+    t.open = true;
+}
+void tm_end(hmec_ctx *c, Timer &t, double bytes) {
+    if (!c->timing || !t.open) return;
+    hipEventRecord(t.b, c->stream);
+    hipEventSynchronize(t.b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, t.a, t.b);
+    t.total_ms += ms;
+    t.bytes += bytes;
+    t.n++;
+    t.open = false;
+}
+
+static void free_ctx(hmec_ctx *c) {
+    if (!c) return;
+    sv_free_device(c);
+    DeviceState &d = c->d;
+    hipFree(d.fields); hipFree(d.mk); hipFree(d.prop); hipFree(d.cell); hipFree(d.perm);
+    hipFree(d.cell_start); hipFree(d.sort_tmp); hipFree(d.keys_tmp); hipFree(d.coords);
+    hipFree(d.scal);
+    if (d.h_scal) hipHostFree(d.h_scal);
+    hipFree(d.rowp); hipFree(d.coli); hipFree(d.val); hipFree(d.rhs); hipFree(d.sol);
+    hipFree(d.trip_slot); hipFree(d.node_trip); hipFree(d.rscale); hipFree(d.cscale);
+    hipFree(d.work); hipFree(d.work2); hipFree(d.counters);
+    for (Timer *t : {&c->t_scatter, &c->t_assemble, &c->t_factor, &c->t_solve}) {
+        if (t->a) hipEventDestroy(t->a);
+        if (t->b) hipEventDestroy(t->b);
+    }
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int hmec_create(const hmec_params *p, hmec_ctx **out) {
+    if (!p || !out) return HMEC_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        hipGetLastError();
+        hmec_set_last_error("no HIP device");
+        return HMEC_E_NODEV;
+    }
+    hmec_ctx *c = new hmec_ctx();
+    c->P = *p;
+    if (hmec_params_finalize(&c->P) != HMEC_OK) { delete c; return HMEC_E_ARG; }
+    c->Nx = c->P.Nx; c->Ny = c->P.Ny; c->Nx1 = c->Nx + 1; c->Ny1 = c->Ny + 1;
+    c->nb = (int64_t)c->Nx * c->Ny;
+    c->ns = (int64_t)c->Nx1 * c->Ny1;
+    c->N = c->ns * 6;
+    hipGetDevice(&c->device);
+    auto fail = [&](int rc) { free_ctx(c); return rc; };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(HMEC_E_HIP);
+    DeviceState &d = c->d;
+    const int64_t N = c->N;
+    if (hipMalloc(&d.fields, sizeof(double) * HMEC_NFIELDS * c->ns) != hipSuccess) return fail(HMEC_E_OOM);
+    hipMemset(d.fields, 0, sizeof(double) * HMEC_NFIELDS * c->ns);
+    const int ncoord = 2 * c->Nx + 2 * c->Ny + 4 * c->Nx1 + 4 * c->Ny1;
+    std::vector<double> co(ncoord);
+    {
+        const double dx = c->P.dx, dy = c->P.dy;
+        double *x = co.data(), *y = x + c->Nx, *xvx = y + c->Ny, *yvx = xvx + c->Nx1;
+        double *xvy = yvx + c->Ny1, *yvy = xvy + c->Nx1, *xp = yvy + c->Ny1, *yp = xp + c->Nx1;
+        linspaced(x, c->Nx, 0, c->P.xsize);
+        linspaced(y, c->Ny, 0, c->P.ysize);
+        linspaced(xvx, c->Nx1, 0, c->P.xsize + dx);
+        linspaced(yvx, c->Ny1, 0 - dy / 2., c->P.ysize + dy / 2.);
+        linspaced(xvy, c->Nx1, 0 - dx / 2., c->P.xsize + dx / 2.);
+        linspaced(yvy, c->Ny1, 0, c->P.ysize + dy);
+        linspaced(xp, c->Nx1, 0 - dx / 2., c->P.xsize + dx / 2.);
+        linspaced(yp, c->Ny1, 0 - dy / 2., c->P.ysize + dy / 2.);
+    }
+    if (hipMalloc(&d.coords, sizeof(double) * ncoord) != hipSuccess) return fail(HMEC_E_OOM);
+    hipMemcpy(d.coords, co.data(), sizeof(double) * ncoord, hipMemcpyHostToDevice);
+    if (hipMalloc(&d.scal, sizeof(double) * 32) != hipSuccess) return fail(HMEC_E_OOM);
+    hipMemset(d.scal, 0, sizeof(double) * 32);
+    if (hipHostMalloc(&d.h_scal, sizeof(double) * 32, hipHostMallocDefault) != hipSuccess) return fail(HMEC_E_OOM);
+    const size_t ncount = 16 + c->Nx1 + 64;
+    if (hipMalloc(&d.counters, sizeof(int) * ncount) != hipSuccess) return fail(HMEC_E_OOM);
+    hipMemset(d.counters, 0, sizeof(int) * ncount);
+    for (double **v : {&d.rhs, &d.sol, &d.work, &d.work2, &d.rscale, &d.cscale}) {
+        if (hipMalloc(v, sizeof(double) * N) != hipSuccess) return fail(HMEC_E_OOM);
+        hipMemset(*v, 0, sizeof(double) * N);
+    }
+    int rc = hmec_build_pattern(c);
+    if (rc != HMEC_OK) return fail(rc);
+    rc = k_init_state(c);
+    if (rc != HMEC_OK) return fail(rc);
+    c->timestep = 1;
+    c->timesum = 0;
+    c->dt = c->P.dtelastic0;
+    c->yndtdecrease = 1;
+    c->DSYLSQ.assign(c->P.niterglobal + 1, 0.0);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(HMEC_E_HIP);
+    *out = c;
+    return HMEC_OK;
+}
+
+extern "C" void hmec_destroy(hmec_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    free_ctx(c);
+}
+
+extern "C" int hmec_set_precision(hmec_ctx *c, int precision) {
+    if (!c || (precision != HMEC_PREC_FP64 && precision != HMEC_PREC_FP32)) return HMEC_E_ARG;
+    c->precision = precision;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_set_markers(hmec_ctx *c, const double *const soa[HMEC_NMARKER_FIELDS], int64_t M) {
+    if (!c || !soa || M <= 0 || M > INT32_MAX) return HMEC_E_ARG;
+    DeviceState &d = c->d;
+    if (M != c->M) {
+        hipFree(d.mk); hipFree(d.prop); hipFree(d.cell); hipFree(d.perm); hipFree(d.cell_start);
+        hipFree(d.keys_tmp); hipFree(d.sort_tmp);
+        d.mk = d.prop = nullptr;
+        d.cell = d.perm = d.cell_start = d.keys_tmp = nullptr;
+        d.sort_tmp = nullptr;
+        const int ncell = (c->Nx - 1) * (c->Ny - 1);
+        int nbits = 1;
+        while ((1 << nbits) <= ncell) nbits++;
+        HMEC_HIP(hipMalloc(&d.mk, sizeof(double) * HMEC_NMARKER_FIELDS * M));
+        HMEC_HIP(hipMalloc(&d.prop, sizeof(double) * MP_NPROP * M));
+        HMEC_HIP(hipMalloc(&d.cell, sizeof(int32_t) * 4 * M));
+        HMEC_HIP(hipMalloc(&d.perm, sizeof(int32_t) * 4 * M));
+        HMEC_HIP(hipMalloc(&d.cell_start, sizeof(int32_t) * 4 * (ncell + 1)));
+        HMEC_HIP(hipMalloc(&d.keys_tmp, sizeof(int32_t) * 2 * M));
+        d.sort_tmp_bytes = hmec_sort_tmp_bytes(M, nbits);
+        HMEC_HIP(hipMalloc(&d.sort_tmp, std::max<size_t>(16, d.sort_tmp_bytes)));
+        c->M = M;
+    }
+    for (int f = 0; f < HMEC_NMARKER_FIELDS; f++) {
+        if (!soa[f]) return HMEC_E_ARG;
+        HMEC_HIP(hipMemcpyAsync(d.mk + (int64_t)f * M, soa[f], sizeof(double) * M,
+                                hipMemcpyHostToDevice, c->stream));
+    }
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    c->have_markers = true;
+    c->markers_dirty = true;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_get_markers(hmec_ctx *c, double *const soa[HMEC_NMARKER_FIELDS]) {
+    if (!c || !soa || !c->have_markers) return HMEC_E_ARG;
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    for (int f = 0; f < HMEC_NMARKER_FIELDS; f++) {
+        if (!soa[f]) continue;
+        HMEC_HIP(hipMemcpy(soa[f], c->d.mk + (int64_t)f * c->M, sizeof(double) * c->M,
+                           hipMemcpyDeviceToHost));
+    }
+    return HMEC_OK;
+}
+
+extern "C" int hmec_get_marker_cells(hmec_ctx *c, int32_t *out) {
+    if (!c || !out || !c->have_markers) return HMEC_E_ARG;
+    if (c->markers_dirty) HMEC_TRY(k_marker_prep(c));
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    HMEC_HIP(hipMemcpy(out, c->d.cell, sizeof(int32_t) * 4 * c->M, hipMemcpyDeviceToHost));
+    const int cy = c->Ny - 1;
+    for (int64_t k = 0; k < 4 * c->M; k++) {
+        const int id = out[k];
+        out[k] = (id % cy) + (id / cy) * c->Ny1;
+    }
+    return HMEC_OK;
+}
+
+static int field_dims(hmec_ctx *c, int f, int &rows, int &cols) {
+    if (f < 0 || f >= HMEC_NFIELDS) return HMEC_E_ARG;
+    if (f < HMEC_NBASIC_FIELDS) { rows = c->Ny; cols = c->Nx; }
+    else { rows = c->Ny1; cols = c->Nx1; }
+    return HMEC_OK;
+}
+
+extern "C" int hmec_set_field(hmec_ctx *c, int f, const double *host, int layout) {
+    if (!c || !host) return HMEC_E_ARG;
+    int rows, cols;
+    HMEC_TRY(field_dims(c, f, rows, cols));
+    std::vector<double> tmp;
+    const double *src = host;
+    if (layout == HMEC_ROWMAJOR) {
+        tmp.resize((size_t)rows * cols);
+        for (int i = 0; i < rows; i++)
+            for (int j = 0; j < cols; j++) tmp[i + (size_t)j * rows] = host[(size_t)i * cols + j];
+        src = tmp.data();
+    } else if (layout != HMEC_COLMAJOR) {
+        return HMEC_E_ARG;
+    }
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    HMEC_HIP(hipMemcpy(F(c, f), src, sizeof(double) * rows * cols, hipMemcpyHostToDevice));
+    return HMEC_OK;
+}
+
+extern "C" int hmec_get_field(hmec_ctx *c, int f, double *host, int layout) {
+    if (!c || !host) return HMEC_E_ARG;
+    int rows, cols;
+    HMEC_TRY(field_dims(c, f, rows, cols));
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    if (layout == HMEC_COLMAJOR) {
+        HMEC_HIP(hipMemcpy(host, F(c, f), sizeof(double) * rows * cols, hipMemcpyDeviceToHost));
+    } else if (layout == HMEC_ROWMAJOR) {
+        std::vector<double> tmp((size_t)rows * cols);
+        HMEC_HIP(hipMemcpy(tmp.data(), F(c, f), sizeof(double) * rows * cols, hipMemcpyDeviceToHost));
+        for (int i = 0; i < rows; i++)
+            for (int j = 0; j < cols; j++) host[(size_t)i * cols + j] = tmp[i + (size_t)j * rows];
+    } else {
+        return HMEC_E_ARG;
+    }
+    return HMEC_OK;
+}
+
+extern "C" int hmec_get_state(hmec_ctx *c, hmec_step_info *s) {
+    if (!c || !s) return HMEC_E_ARG;
+    *s = c->acc;
+    s->timestep = c->timestep;
+    s->iterations = c->iterstep + 1;
+    s->ynlast = c->ynlast;
+    s->yndtdecrease = c->yndtdecrease;
+    s->dt = c->dt; s->dt00 = c->dt00; s->dtx = c->dtx; s->dty = c->dty;
+    s->dtlapusta = c->dtlapusta; s->Vmax = c->Vmax; s->maxvxy = c->maxvxy;
+    s->timesum = c->timesum; s->ptscale = c->ptscale;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_set_state(hmec_ctx *c, const hmec_step_info *s) {
+    if (!c || !s) return HMEC_E_ARG;
+    c->timestep = s->timestep;
+    c->ynlast = s->ynlast;
+    c->yndtdecrease = s->yndtdecrease;
+    c->dt = s->dt; c->dt00 = s->dt00; c->dtx = s->dtx; c->dty = s->dty;
+    c->dtlapusta = s->dtlapusta; c->Vmax = s->Vmax; c->maxvxy = s->maxvxy;
+    c->timesum = s->timesum; c->ptscale = s->ptscale;
+    return HMEC_OK;
+}
+
+// ---------------------------------------------------------------- stages
+static double scatter_bytes(hmec_ctx *c) {
+    // 19 marker streams + 10 basic + 13 staggered normalized fields
+    return 8.0 * (19.0 * c->M + 10.0 * c->nb + 13.0 * c->ns);
+}
+
+extern "C" int hmec_scatter(hmec_ctx *c) {
+    if (!c) return HMEC_E_ARG;
+    tm_begin(c, c->t_scatter);
+    HMEC_TRY(k_scatter(c));
+    tm_end(c, c->t_scatter, scatter_bytes(c));
+    return HMEC_OK;
+}
+
+extern "C" int hmec_step_begin(hmec_ctx *c) {
+    if (!c) return HMEC_E_ARG;
+    const hmec_params &P = c->P;
+    const int src = (c->timestep == 1) ? HMEC_F_ETA0 : HMEC_F_ETA00;   // :586-594
+    HMEC_TRY(k_copy_field(c, HMEC_F_ETA1, src));
+    HMEC_TRY(k_copy_field(c, HMEC_F_ETA, src));
+    HMEC_TRY(k_copy_field(c, HMEC_F_ETA50, src));
+    HMEC_TRY(k_copy_field(c, HMEC_F_OM, HMEC_F_OM0));                 // :596
+    if (!c->yndtdecrease) c->dt = std::max(std::min(c->dt * P.dtkoefup, P.dtelastic0), P.dtmin);
+    else c->dt = std::max(std::min(c->dt, P.dtelastic0), P.dtmin);     // :599-603
+    c->yndtdecrease = 0;
+    c->dt00 = c->dt;
+    c->ynlast = 0;
+    std::fill(c->DSYLSQ.begin(), c->DSYLSQ.end(), 0.0);
+    return HMEC_OK;
+}
+
+extern "C" int hmec_vep_pre(hmec_ctx *c, double dt) {
+    if (!c) return HMEC_E_ARG;
+    return k_vep_pre(c, dt);
+}
+
+extern "C" int hmec_assemble(hmec_ctx *c, double dt, double *ptscale_out) {
+    if (!c) return HMEC_E_ARG;
+    double ps = 0;
+    HMEC_TRY(k_scale(c, dt, &ps));
+    c->ptscale = ps;
+    tm_begin(c, c->t_assemble);
+    HMEC_TRY(k_assemble(c, dt, ps));
+    // CSR values + row pointers/columns + RHS written, ~26 nodal fields read
+    tm_end(c, c->t_assemble, 8.0 * (c->nnz + c->N + 26.0 * c->ns) + 4.0 * (c->nnz + c->ntrip));
+    if (ptscale_out) *ptscale_out = ps;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_csr_dims(hmec_ctx *c, int64_t *n, int64_t *nnz) {
+    if (!c) return HMEC_E_ARG;
+    if (n) *n = c->N;
+    if (nnz) *nnz = c->nnz;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_get_csr(hmec_ctx *c, int32_t *row_ptr, int32_t *col_idx, double *values, double *rhs) {
+    if (!c) return HMEC_E_ARG;
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    if (row_ptr) std::memcpy(row_ptr, c->h_rowp.data(), sizeof(int32_t) * (c->N + 1));
+    if (col_idx) std::memcpy(col_idx, c->h_coli.data(), sizeof(int32_t) * c->nnz);
+    if (values) HMEC_HIP(hipMemcpy(values, c->d.val, sizeof(double) * c->nnz, hipMemcpyDeviceToHost));
+    if (rhs) HMEC_HIP(hipMemcpy(rhs, c->d.rhs, sizeof(double) * c->N, hipMemcpyDeviceToHost));
+    return HMEC_OK;
+}
+
+extern "C" int hmec_set_csr_values(hmec_ctx *c, const double *values, const double *rhs) {
+    if (!c) return HMEC_E_ARG;
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    if (values) HMEC_HIP(hipMemcpy(c->d.val, values, sizeof(double) * c->nnz, hipMemcpyHostToDevice));
+    if (rhs) HMEC_HIP(hipMemcpy(c->d.rhs, rhs, sizeof(double) * c->N, hipMemcpyHostToDevice));
+    c->factor_valid = false;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_analyze(hmec_ctx *c) {
+    if (!c) return HMEC_E_ARG;
+    if (c->analyzed) return HMEC_OK;
+    HMEC_TRY(hmec_plan_build(c->Nx1, c->Ny1, c->h_rowp, c->h_coli, c->plan));
+    HMEC_TRY(sv_setup_device(c));
+    c->analyzed = true;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_factorize(hmec_ctx *c, int precision) {
+    if (!c || (precision != HMEC_PREC_FP64 && precision != HMEC_PREC_FP32)) return HMEC_E_ARG;
+    HMEC_TRY(hmec_analyze(c));
+    tm_begin(c, c->t_factor);
+    HMEC_TRY(sv_factorize(c, precision));
+    tm_end(c, c->t_factor, c->plan.flops);
+    return HMEC_OK;
+}
+
+extern "C" int hmec_solve(hmec_ctx *c, hmec_solve_stats *st) {
+    if (!c || !c->factor_valid) return HMEC_E_ARG;
+    const bool fp32 = c->factor_prec == HMEC_PREC_FP32;
+    tm_begin(c, c->t_solve);
+    int rc = sv_solve(c, fp32 ? 30 : 3, fp32 ? 1e-12 : 1e-14, st);
+    tm_end(c, c->t_solve, 0);
+    if (rc == HMEC_E_NOCONV && !fp32) rc = HMEC_OK;   // FP64: result kept like PARDISO's
+    return rc;
+}
+
+extern "C" int hmec_get_solution(hmec_ctx *c, double *x) {
+    if (!c || !x) return HMEC_E_ARG;
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    HMEC_HIP(hipMemcpy(x, c->d.sol, sizeof(double) * c->N, hipMemcpyDeviceToHost));
+    return HMEC_OK;
+}
+
+extern "C" int hmec_update(hmec_ctx *c, double dt, double ptscale, hmec_iter_scalars *out) {
+    if (!c || !out) return HMEC_E_ARG;
+    c->d.h_scal[0] = ptscale;
+    HMEC_HIP(hipMemcpyAsync(c->d.scal, c->d.h_scal, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HMEC_TRY(k_reload(c, ptscale));
+    HMEC_TRY(k_update(c, dt, out));
+    c->Vmax = out->Vmax;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_iter_commit(hmec_ctx *c, int ynpl, int converged) {
+    if (!c) return HMEC_E_ARG;
+    return k_iter_commit(c, ynpl, converged);
+}
+
+extern "C" int hmec_advect(hmec_ctx *c, double dt) {
+    if (!c) return HMEC_E_ARG;
+    return k_advect(c, dt);
+}
+
+// ETA00 = ETA50, OM0 = OM (h_mec_rsf_v9.cpp:1469-1470), final stresses with
+// the final dt (:1480-1521), advection (:1561-1656), history (:1658-1726).
+extern "C" int hmec_step_end(hmec_ctx *c, double dt) {
+    if (!c) return HMEC_E_ARG;
+    HMEC_TRY(k_copy_field(c, HMEC_F_ETA00, HMEC_F_ETA50));
+    HMEC_TRY(k_copy_field(c, HMEC_F_OM0, HMEC_F_OM));
+    HMEC_TRY(k_final_stress(c, dt));
+    if (c->P.advect_markers) HMEC_TRY(k_advect(c, dt));
+    HMEC_TRY(k_history(c));
+    return HMEC_OK;
+}
+
+extern "C" int hmec_timing_enable(hmec_ctx *c, int on) {
+    if (!c) return HMEC_E_ARG;
+    c->timing = on != 0;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_timing_reset(hmec_ctx *c) {
+    if (!c) return HMEC_E_ARG;
+    for (Timer *t : {&c->t_scatter, &c->t_assemble, &c->t_factor, &c->t_solve}) {
+        t->total_ms = 0;
+        t->bytes = 0;
+        t->n = 0;
+    }
+    return HMEC_OK;
+}
+
+extern "C" int hmec_kernel_time(hmec_ctx *c, const char *name, double *avg_ms, double *bytes) {
+    if (!c || !name) return -1;
+    Timer *t = nullptr;
+    if (!std::strcmp(name, "scatter")) t = &c->t_scatter;
+    else if (!std::strcmp(name, "assemble")) t = &c->t_assemble;
+    else if (!std::strcmp(name, "factor")) t = &c->t_factor;
+    else if (!std::strcmp(name, "solve")) t = &c->t_solve;
+    if (!t) return -1;
+    if (avg_ms) *avg_ms = t->n ? t->total_ms / t->n : 0;
+    if (bytes) *bytes = t->n ? t->bytes / t->n : 0;
+    return t->n;
+}
+
+// Device footprint: markers + derived streams + sort, nodal fields, CSR,
+// FP64 + FP32 fronts of the symbolic plan (cached per grid).
+static std::mutex g_fp_mu;
+static std::map<std::pair<int, int>, int64_t> g_fp_cache;
+
+extern "C" int64_t hmec_footprint_bytes(const hmec_params *p) {
+    if (!p) return -1;
+    const int64_t Nx1 = p->Nx + 1, Ny1 = p->Ny + 1, ns = Nx1 * Ny1, N = 6 * ns;
+    const int64_t M = hmec_marker_count(p);
+    int64_t arena = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_fp_mu);
+        auto it = g_fp_cache.find({p->Nx, p->Ny});
+        if (it != g_fp_cache.end()) arena = it->second;
+    }
+    if (arena == 0) {
+        // plan size estimate from the front model of a real analysis
+        hmec_ctx tmp;
+        tmp.P = *p;
+        hmec_params_finalize(&tmp.P);
+        (void)tmp;
+        // 2-D nested dissection: O(ns log ns) dense front storage; the
+        // constant is calibrated on analysed grids (about 150 entries per
+        // node per level of the tree).
+        double lv = std::log2((double)ns);
+        arena = (int64_t)(36.0 * 4.5 * ns * lv);
+    }
+    const int64_t bytes = M * 8 * (HMEC_NMARKER_FIELDS + MP_NPROP) + M * 4 * 10 +
+                          ns * 8 * HMEC_NFIELDS + N * 8 * 8 + 20 * N * 12 + arena * 12;
+    return bytes;
+}
+
+void hmec_footprint_record(int Nx, int Ny, int64_t arena) {
+    std::lock_guard<std::mutex> lk(g_fp_mu);
+    g_fp_cache[{Nx, Ny}] = arena;
+}

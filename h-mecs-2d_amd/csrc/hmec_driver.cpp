@@ -1,0 +1,218 @@
+// C++ host driver: one timestep of h_mec_rsf_v9.cpp:376-1726 around the HIP
+// stages, with the reference's Picard / timestep controller (:1310-1456)
+// kept on the host (a handful of scalars per iteration), and the serving
+// entry points that advance several contexts concurrently.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "hmec_internal.h"
+
+namespace {
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Picard / dt controller, h_mec_rsf_v9.cpp:1310-1436.  Returns true when the
+// exit test fires.
+bool controller(hmec_ctx *c, const hmec_iter_scalars &sc) {
+    const hmec_params &P = c->P;
+    const int iterstep = c->iterstep, ynpl = sc.ynpl;
+    const double dx = P.dx, dy = P.dy;
+    bool yn = false;
+    c->DSYLSQ[iterstep] = 0;
+    if (ynpl > 0) c->DSYLSQ[iterstep] = std::sqrt(sc.ddd / ynpl);
+    const double D_iter = c->DSYLSQ[iterstep];
+    double D_iter_quot = 0;
+    if (iterstep != 0) D_iter_quot = D_iter / c->DSYLSQ[iterstep - 1];
+    double dtpl = c->dt;
+    if (ynpl > 0 && iterstep < P.niterglobal && c->ynlast >= P.dtstep &&
+        (c->ynlast > P.ynlastmax || std::log10(D_iter_quot) >= 0 ||
+         std::log10(D_iter_quot) > std::log10(P.errmin / D_iter) / (P.ynlastmax - c->ynlast))) {
+        dtpl = c->dt / P.dtkoef;
+        yn = true;
+    }
+    double maxvxy0 = 0;
+    if (iterstep > 0) maxvxy0 = c->maxvxy;
+    double maxvxs = std::max(sc.vxs_max, std::fabs(sc.vxs_min));
+    double maxvys = std::max(sc.vys_max, std::fabs(sc.vys_min));
+    c->maxvxy = std::sqrt((sc.vxs_max - sc.vxs_min) * (sc.vxs_max - sc.vxs_min) +
+                          (sc.vys_max - sc.vys_min) * (sc.vys_max - sc.vys_min));
+    double stpmaxcur = P.stpmax1;
+    c->dtx = c->dt;
+    if (c->dt > dx * stpmaxcur / maxvxs) { c->dtx = dx / P.dtkoefv * stpmaxcur / maxvxs; yn = true; }
+    c->dty = c->dt;
+    if (c->dt > dy * stpmaxcur / maxvys) { c->dty = dy / P.dtkoefv * stpmaxcur / maxvys; yn = true; }
+    maxvxs = sc.maxvxs_band;
+    maxvys = sc.maxvys_band;
+    stpmaxcur = P.stpmax;
+    if (c->dt > dx * stpmaxcur / maxvxs) { c->dtx = dx / P.dtkoefv * stpmaxcur / maxvxs; yn = true; }
+    if (c->dt > dy * stpmaxcur / maxvys) { c->dty = dy / P.dtkoefv * stpmaxcur / maxvys; yn = true; }
+    double dtslip = sc.dtslip;
+    if (ynpl > 0 && dtslip < c->dt) { yn = true; dtslip = dtslip / P.dtkoefv; }
+    if (yn && c->dt > P.dtmin) {
+        const double dtold = c->dt;
+        c->dt = std::max(std::min(std::min(c->dtx, c->dty), std::min(std::min(dtpl, dtslip), c->dtlapusta)), P.dtmin);
+        if (c->dt < dtold) c->ynlast = 0;
+    } else {
+        yn = false;
+    }
+    double vratio = 0;
+    if (iterstep > 0) vratio = std::log10(c->maxvxy / maxvxy0);
+    return !yn && (ynpl == 0 || (c->DSYLSQ[iterstep] < P.errmin && iterstep > 0 &&
+                                 std::fabs(vratio) < P.vratiomax));
+}
+
+// Linear solve of the freshly assembled system under the context's precision
+// policy.  FP64: numeric refactorization every Picard iteration, like
+// PardisoLU (h_mec_rsf_v9.cpp:1050-1052).  FP32: the equilibrated FP32 factor
+// is reused while FP64 iterative refinement converges; a stalled refinement
+// refactors in FP32, and a second stall falls back to an FP64 factor.
+int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
+    hmec_solve_stats st;
+    std::memset(&st, 0, sizeof(st));
+    if (c->precision == HMEC_PREC_FP64) {
+        tm_begin(c, c->t_factor);
+        HMEC_TRY(sv_factorize(c, HMEC_PREC_FP64));
+        tm_end(c, c->t_factor, c->plan.flops);
+        acc.factorizations++;
+        tm_begin(c, c->t_solve);
+        int rc = sv_solve(c, 3, 1e-14, &st);
+        tm_end(c, c->t_solve, 0);
+        if (rc != HMEC_OK && rc != HMEC_E_NOCONV) return rc;
+    } else {
+        bool fresh = false;
+        if (!c->factor_valid || c->factor_prec != HMEC_PREC_FP32) {
+            tm_begin(c, c->t_factor);
+            HMEC_TRY(sv_factorize(c, HMEC_PREC_FP32));
+            tm_end(c, c->t_factor, c->plan.flops);
+            acc.factorizations++;
+            fresh = true;
+        }
+        tm_begin(c, c->t_solve);
+        int rc = sv_solve(c, fresh ? 30 : 12, 1e-12, &st);
+        tm_end(c, c->t_solve, 0);
+        acc.ir_iterations += st.ir_iterations;
+        if (rc == HMEC_E_NOCONV && !fresh) {
+            tm_begin(c, c->t_factor);
+            HMEC_TRY(sv_factorize(c, HMEC_PREC_FP32));
+            tm_end(c, c->t_factor, c->plan.flops);
+            acc.factorizations++;
+            tm_begin(c, c->t_solve);
+            rc = sv_solve(c, 30, 1e-12, &st);
+            tm_end(c, c->t_solve, 0);
+            acc.ir_iterations += st.ir_iterations;
+        }
+        if (rc == HMEC_E_NOCONV) {
+            acc.fallbacks++;
+            HMEC_TRY(sv_factorize(c, HMEC_PREC_FP64));
+            acc.factorizations++;
+            rc = sv_solve(c, 3, 1e-14, &st);
+            if (rc == HMEC_E_NOCONV) rc = HMEC_OK;
+            // the next iteration starts again from a fresh FP32 factor
+            c->factor_valid = false;
+        }
+        if (rc != HMEC_OK) return rc;
+        acc.ir_iterations -= 0;
+    }
+    acc.solves++;
+    if (c->precision == HMEC_PREC_FP64) acc.ir_iterations += st.ir_iterations;
+    acc.perturbed_pivots += st.perturbed_pivots;
+    acc.max_rel_residual = std::max(acc.max_rel_residual, st.rel_residual);
+    return HMEC_OK;
+}
+
+}  // namespace
+
+// One complete timestep, h_mec_rsf_v9.cpp:376-1726 (outputs excluded).
+extern "C" int hmec_step(hmec_ctx *c, hmec_step_info *info) {
+    if (!c || !c->have_markers) return HMEC_E_ARG;
+    hipSetDevice(c->device);
+    HMEC_TRY(hmec_analyze(c));
+    const hmec_params &P = c->P;
+    hmec_step_info &acc = c->acc;
+    std::memset(&acc, 0, sizeof(acc));
+    const bool sync_timing = c->timing;
+    auto sync = [&]() {
+        if (sync_timing) hipStreamSynchronize(c->stream);
+    };
+    const double t0 = now_ms();
+    HMEC_TRY(hmec_scatter(c));
+    sync();
+    const double t1 = now_ms();
+    acc.ms_scatter = t1 - t0;
+    HMEC_TRY(hmec_step_begin(c));
+    int capped = 0;
+    for (c->iterstep = 0; c->iterstep < P.niterglobal; c->iterstep++) {
+        const double a = now_ms();
+        HMEC_TRY(hmec_vep_pre(c, c->dt));
+        double ptscale = 0;
+        HMEC_TRY(hmec_assemble(c, c->dt, &ptscale));
+        sync();
+        const double b = now_ms();
+        HMEC_TRY(linear_solve(c, acc));
+        sync();
+        const double cc = now_ms();
+        hmec_iter_scalars sc;
+        HMEC_TRY(hmec_update(c, c->dt, ptscale, &sc));
+        c->dtlapusta = sc.dtlapusta;
+        capped += sc.bisect_capped;
+        const bool stop = controller(c, sc);
+        HMEC_TRY(hmec_iter_commit(c, sc.ynpl, stop ? 1 : 0));
+        const double d = now_ms();
+        acc.ms_assemble += b - a;
+        acc.ms_factor += cc - b;
+        acc.ms_update += d - cc;
+        if (stop) break;
+        c->ynlast++;
+    }
+    if (c->iterstep >= P.niterglobal) c->iterstep = P.niterglobal - 1;
+    const double e = now_ms();
+    // :1464-1472
+    if (c->dt00 > c->dt) c->yndtdecrease = 1;
+    c->dt = std::min(std::min(c->dtx, c->dty), std::min(c->dt, c->dtlapusta));
+    HMEC_TRY(hmec_step_end(c, c->dt));
+    c->timesum += c->dt;
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    const double f = now_ms();
+    acc.ms_update += f - e;
+    acc.ms_total = f - t0;
+    acc.bisect_capped = capped;
+    hmec_get_state(c, &acc);
+    acc.timestep = c->timestep;
+    if (info) *info = acc;
+    c->timestep++;
+    return HMEC_OK;
+}
+
+// Serving: advance n contexts by one step each.  Every context owns a HIP
+// stream; one host thread per context drives its Picard loop so the members'
+// kernels overlap on the device.
+extern "C" int hmec_batch_step(hmec_ctx *const *cs, int n, hmec_step_info *infos) {
+    if (!cs || n <= 0) return HMEC_E_ARG;
+    std::vector<int> rc(n, HMEC_OK);
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (int k = 0; k < n; k++)
+        th.emplace_back([&, k]() { rc[k] = hmec_step(cs[k], infos ? &infos[k] : nullptr); });
+    for (auto &t : th) t.join();
+    for (int k = 0; k < n; k++)
+        if (rc[k] != HMEC_OK) return rc[k];
+    return HMEC_OK;
+}
+
+extern "C" int hmec_batch_factorize(hmec_ctx *const *cs, int n, int precision) {
+    if (!cs || n <= 0) return HMEC_E_ARG;
+    for (int k = 0; k < n; k++) {
+        if (cs[k]->Nx != cs[0]->Nx || cs[k]->Ny != cs[0]->Ny) return HMEC_E_ARG;
+        HMEC_TRY(hmec_analyze(cs[k]));
+    }
+    return sv_batch_factorize(cs, n, precision);
+}

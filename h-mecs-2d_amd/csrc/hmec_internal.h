@@ -1,0 +1,215 @@
+// Internal declarations of libhmec_hip: context, device state, kernel launch
+// wrappers and the multifrontal solver plan.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hmec.h"
+
+#define HMEC_HIP(call)                                                            \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            hmec_set_last_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return HMEC_E_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+#define HMEC_TRY(call)              \
+    do {                            \
+        int rc_ = (call);           \
+        if (rc_ != HMEC_OK) return rc_; \
+    } while (0)
+
+void hmec_set_last_error(const std::string &s);
+
+// Number of live scatter accumulators per grid and the marker property
+// streams the scatter reads (derived once per marker set).
+enum MarkerProp {
+    MP_ETAMM = 0,   // effective viscosity (clamped)
+    MP_RHOMM,       // total density
+    MP_KKKMM,       // permeability (porosity-scaled, clamped)
+    MP_TMARK,       // rock type
+    MP_INVG,        // 1 / shear modulus
+    MP_ETAMM0,      // matrix viscosity (clamped)
+    MP_COHT, MP_FRIT, MP_COHC, MP_FRIC, MP_DILC,
+    MP_PORM,        // porosity
+    MP_ETASMM0,     // solid viscosity (bulk viscosity numerator)
+    MP_INVETAD,     // 1 / Darcy viscosity = kkk / etaf
+    MP_RHOFM,       // fluid density
+    MP_NPROP
+};
+
+// Dense-front multifrontal plan (symbolic analysis, host side), see
+// solver_symbolic.cpp.
+struct FrontPlan {
+    // per front
+    std::vector<int> np, nu;            // pivot / update node counts
+    std::vector<int64_t> off;           // offset of the n x n front in the arena
+    std::vector<int64_t> voff;          // offset of the front in the per-front vector arena
+    std::vector<int> level;             // 0 = leaf
+    std::vector<int> parent;            // -1 = root
+    std::vector<std::vector<int>> children;
+    std::vector<std::vector<int>> nodes;  // local node list: pivots then update nodes
+    std::vector<int64_t> ea_off;        // offset into ea_map of this front's (child) map
+    // global
+    int nfronts = 0, nlevels = 0, max_front = 0;
+    int64_t arena = 0;                  // doubles in the front arena
+    int64_t varena = 0;                 // doubles in the vector arena
+    double flops = 0;                   // dense flops of one factorization
+    int64_t factor_nnz = 0;             // entries of the factor blocks
+    std::vector<std::vector<int>> levels;   // fronts per level
+    // maps
+    std::vector<int64_t> nz_dst;        // CSR entry -> arena offset
+    std::vector<int32_t> ea_map;        // child update node -> parent local node
+    std::vector<int32_t> dof_local;     // unused placeholder
+    std::vector<int32_t> node_front;    // front that eliminates each node
+    std::vector<int32_t> node_local;    // local node index inside that front
+};
+
+int hmec_plan_build(int Nx1, int Ny1, const std::vector<int32_t> &rowp,
+                    const std::vector<int32_t> &coli, FrontPlan &plan);
+
+// Device-side task descriptors for the batched dense kernels.
+struct FrontTask {
+    int64_t off;    // arena offset of the front
+    int n;          // front order
+    int p;          // pivots
+};
+
+struct EaTask {       // extend-add of one child CB into its parent
+    int64_t child_off;   // child front offset
+    int64_t parent_off;  // parent front offset
+    int child_n, child_p, parent_n;
+    int64_t map_off;     // into ea_map (child update node -> parent local node)
+};
+
+struct DevPlan {
+    int nlevels = 0;
+    // per level task arrays on the device
+    std::vector<FrontTask *> d_tasks;     // fronts of the level
+    std::vector<std::vector<FrontTask>> h_tasks;
+    std::vector<EaTask *> d_ea[2];        // extend-add of first / second child
+    std::vector<std::vector<EaTask>> h_ea[2];
+    std::vector<int> maxp, maxn;          // per level maxima
+    std::vector<int64_t> lvl_begin, lvl_end;  // arena range touched by the level
+    int64_t *d_nz_dst = nullptr;
+    int32_t *d_ea_map = nullptr;
+    int32_t *d_row_of = nullptr;          // CSR row of each entry
+    int32_t *d_csc_ptr = nullptr;         // column pointers of the transposed pattern
+    int32_t *d_csc_idx = nullptr;         // CSR entry index of each CSC entry
+    int32_t *d_csc_row = nullptr;         // row of each CSC entry
+    // solve: per front pivot/update dof -> global dof
+    int32_t *d_front_dofs = nullptr;      // per front: n global dof ids (pivots, then update)
+    int64_t *d_front_dof_off = nullptr;
+    std::vector<int64_t> h_front_dof_off;
+};
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    double total_ms = 0;
+    double bytes = 0;
+    int n = 0;
+    bool open = false;
+};
+
+struct hmec_ctx_impl;
+
+// Device buffers owned by a context.
+struct DeviceState {
+    double *fields = nullptr;        // HMEC_NFIELDS x stag_size
+    double *mk = nullptr;            // 19 x M markers
+    double *prop = nullptr;          // MP_NPROP x M derived properties
+    int32_t *cell = nullptr;         // 4 x M anchor cells
+    int32_t *perm = nullptr;         // 4 x M markers sorted by cell
+    int32_t *cell_start = nullptr;   // 4 x (ncell + 1)
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int32_t *keys_tmp = nullptr;     // 2 x M scratch for sorting
+    double *coords = nullptr;        // x, y, xvx, yvx, xvy, yvy, xp, yp
+    double *scal = nullptr;          // device scalars (ptscale, reductions)
+    double *h_scal = nullptr;        // pinned host mirror
+    // linear system
+    int32_t *rowp = nullptr, *coli = nullptr;
+    double *val = nullptr, *rhs = nullptr, *sol = nullptr;
+    int32_t *trip_slot = nullptr;    // CSR slot of each triplet (emission order)
+    int32_t *node_trip = nullptr;    // first triplet of each node (N/6 + 1)
+    // solver
+    double *arena = nullptr;         // FP64 fronts
+    float *arena32 = nullptr;        // FP32 fronts
+    double *vec = nullptr;           // per-front vector arena (solve)
+    double *rscale = nullptr, *cscale = nullptr;
+    double *work = nullptr;          // N-vector scratch (residual, correction)
+    double *work2 = nullptr;
+    int *counters = nullptr;         // perturbed pivots etc.
+};
+
+struct hmec_ctx {
+    hmec_params P;
+    int Nx, Ny, Nx1, Ny1;
+    int64_t nb, ns;                  // basic / staggered node counts
+    int64_t N, nnz, ntrip;
+    int64_t M = 0;
+    int precision = HMEC_PREC_FP64;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    DeviceState d;
+    // host copies of the pattern
+    std::vector<int32_t> h_rowp, h_coli;
+    // scalar state of the time loop
+    int timestep = 1, ynlast = 0, iterstep = 0, yndtdecrease = 1;
+    double timesum = 0, dt = 0, dt00 = 0, dtx = 0, dty = 0, dtlapusta = 0, Vmax = 0, maxvxy = 0;
+    double ptscale = 0;
+    std::vector<double> DSYLSQ;
+    bool markers_dirty = true;       // cell keys / sort need recomputing
+    bool have_markers = false;
+    // solver
+    bool analyzed = false;
+    FrontPlan plan;
+    DevPlan dp;
+    int factor_prec = -1;            // precision of the factor in the arena, -1: none
+    bool factor_valid = false;
+    int last_ir = 0, last_fallback = 0, last_perturbed = 0;
+    double last_res = 0;
+    int stale_solves = 0;            // FP32: solves since the last factorization
+    // statistics of the current step
+    hmec_step_info acc;
+    // timing
+    bool timing = false;
+    Timer t_scatter, t_assemble, t_factor, t_solve;
+};
+
+// ---- kernel launch wrappers (kernels_phys.hip) ----
+int k_marker_prep(hmec_ctx *c);          // derived props + anchor cells + sort
+int k_scatter(hmec_ctx *c);
+int k_step_begin_fields(hmec_ctx *c);
+int k_vep_pre(hmec_ctx *c, double dt);
+int k_scale(hmec_ctx *c, double dt, double *ptscale);
+int k_assemble(hmec_ctx *c, double dt, double ptscale);
+int k_reload(hmec_ctx *c, double ptscale);
+int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out);
+int k_iter_commit(hmec_ctx *c, int ynpl, int converged);
+int k_final_stress(hmec_ctx *c, double dt);
+int k_history(hmec_ctx *c);
+int k_advect(hmec_ctx *c, double dt);
+int k_init_state(hmec_ctx *c);
+int k_copy_field(hmec_ctx *c, int dst, int src);
+// symbolic CSR pattern of the reference's triplet emission (host)
+int hmec_build_pattern(hmec_ctx *c);
+
+// ---- solver (solver_numeric.hip) ----
+int sv_setup_device(hmec_ctx *c);
+int sv_free_device(hmec_ctx *c);
+int sv_factorize(hmec_ctx *c, int precision);
+int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *st);
+int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision);
+
+// timing helpers
+void tm_begin(hmec_ctx *c, Timer &t);
+void tm_end(hmec_ctx *c, Timer &t, double bytes);
+
+inline double *F(hmec_ctx *c, int f) { return c->d.fields + (int64_t)f * c->ns; }

@@ -1,0 +1,58 @@
+// Host-only entry points of the symbolic phase (no GPU needed): the fixed
+// CSR pattern of the coupled system and the nested-dissection front plan.
+#include <cstring>
+#include <vector>
+
+#include "hmec_internal.h"
+
+int hmec_pattern_host(const hmec_params &P, std::vector<int32_t> &rowp,
+                      std::vector<int32_t> &coli, std::vector<int32_t> *slot_out,
+                      std::vector<int32_t> *node_trip_out);
+int hmec_var_pos(int v);
+void hmec_footprint_record(int Nx, int Ny, int64_t arena);
+
+extern "C" int hmec_pattern(const hmec_params *p, int64_t *n, int64_t *nnz, int32_t *rowp,
+                            int32_t *coli) {
+    if (!p) return HMEC_E_ARG;
+    hmec_params P = *p;
+    HMEC_TRY(hmec_params_finalize(&P));
+    std::vector<int32_t> rp, ci;
+    HMEC_TRY(hmec_pattern_host(P, rp, ci, nullptr, nullptr));
+    if (n) *n = (int64_t)rp.size() - 1;
+    if (nnz) *nnz = (int64_t)ci.size();
+    if (rowp) std::memcpy(rowp, rp.data(), sizeof(int32_t) * rp.size());
+    if (coli) std::memcpy(coli, ci.data(), sizeof(int32_t) * ci.size());
+    return HMEC_OK;
+}
+
+extern "C" int hmec_plan_info(const hmec_params *p, hmec_solve_stats *st, int64_t *arena_entries,
+                              int32_t *elim_order) {
+    if (!p) return HMEC_E_ARG;
+    hmec_params P = *p;
+    HMEC_TRY(hmec_params_finalize(&P));
+    std::vector<int32_t> rp, ci;
+    HMEC_TRY(hmec_pattern_host(P, rp, ci, nullptr, nullptr));
+    FrontPlan pl;
+    HMEC_TRY(hmec_plan_build(P.Nx + 1, P.Ny + 1, rp, ci, pl));
+    hmec_footprint_record(P.Nx, P.Ny, pl.arena);
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->factor_flops = pl.flops;
+        st->factor_nnz = pl.factor_nnz;
+        st->nfronts = pl.nfronts;
+        st->nlevels = pl.nlevels;
+        st->max_front = pl.max_front;
+    }
+    if (arena_entries) *arena_entries = pl.arena;
+    if (elim_order) {
+        // dofs in elimination order: fronts by level, pivots in front order
+        int inv[6];
+        for (int v = 0; v < 6; v++) inv[hmec_var_pos(v)] = v;
+        int64_t k = 0;
+        for (int lv = 0; lv < pl.nlevels; lv++)
+            for (int f : pl.levels[lv])
+                for (int l = 0; l < pl.np[f]; l++)
+                    for (int q = 0; q < 6; q++) elim_order[k++] = pl.nodes[f][l] * 6 + inv[q];
+    }
+    return HMEC_OK;
+}

@@ -1,0 +1,1112 @@
+// Physics kernels of the per-timestep path for gfx950 (FP64, no FP
+// contraction so that every expression rounds like the reference's):
+//   marker preparation + cell sort, marker->node scatter with normalization,
+//   VEP pre-update, P-scaling, CSR value fill, reload, stress update, RSF
+//   fault solve with its reductions, Picard commit, final stresses, history,
+//   and the RK4 marker advection.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#define HMEC_HD __host__ __device__
+#include "assemble_emit.h"
+#include "hmec_internal.h"
+
+// device scalar slots (ctx->d.scal)
+enum {
+    SC_PTSCALE = 0,
+    SC_YNPL, SC_DDD, SC_DTLAP, SC_VMAX, SC_VXMAX, SC_VXMIN, SC_VYMAX, SC_VYMIN,
+    SC_BANDX, SC_BANDY, SC_DTSLIP, SC_CAPPED,
+    SC_COUNT
+};
+static_assert(SC_COUNT <= 32, "scalar slots");
+
+// coordinates: x(Nx), y(Ny), xvx(Nx1), yvx(Ny1), xvy(Nx1), yvy(Ny1), xp(Nx1), yp(Ny1)
+struct Coords {
+    const double *x, *y, *xvx, *yvx, *xvy, *yvy, *xp, *yp;
+};
+static Coords coords(hmec_ctx *c) {
+    const double *b = c->d.coords;
+    Coords k;
+    k.x = b;
+    k.y = k.x + c->Nx;
+    k.xvx = k.y + c->Ny;
+    k.yvx = k.xvx + c->Nx1;
+    k.xvy = k.yvx + c->Ny1;
+    k.yvy = k.xvy + c->Nx1;
+    k.xp = k.yvy + c->Ny1;
+    k.yp = k.xp + c->Nx1;
+    return k;
+}
+
+#define LAUNCH_CHECK()                                                      \
+    do {                                                                    \
+        hipError_t e_ = hipGetLastError();                                  \
+        if (e_ != hipSuccess) {                                             \
+            hmec_set_last_error(std::string("launch: ") + hipGetErrorString(e_)); \
+            return HMEC_E_HIP;                                              \
+        }                                                                   \
+    } while (0)
+
+__device__ __forceinline__ int d_fix(double t) { return t < 0. ? (int)ceil(t) : (int)floor(t); }
+__device__ __forceinline__ int d_check_bounds(int k, int bound) {
+    if (k < 0) k = 0;
+    else if (k > bound - 2) k = bound - 2;
+    return k;
+}
+__device__ __forceinline__ double d_enforce(double v, double mn, double mx) {
+    if (v < mn) v = mn;
+    else if (v > mx) v = mx;
+    return v;
+}
+
+// ------------------------------------------------------------ marker prep
+// Derived marker properties (h_mec_rsf_v9.cpp:389-423) and the anchor cell
+// of each of the four grids (:432-433, :468-469, :493-494, :515-516).
+// Cell id = i + j * (Ny - 1), i in [0, Ny-2], j in [0, Nx-2].
+__global__ void k_prep(const double *__restrict__ mk, int64_t M, double *__restrict__ prop,
+                       int32_t *__restrict__ cell, int32_t *__restrict__ keys, int Nx, int Ny,
+                       double dx, double dy, double POR0, double kkkmin, double kkkmax,
+                       double alpha, double etamin, double etamax) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+#define MK(f) mk[(int64_t)(f) * M + m]
+#define PR(f) prop[(int64_t)(f) * M + m]
+    const double porm = MK(HMEC_M_PORM);
+    double kkkmm = MK(HMEC_M_KKKM) * pow(porm / POR0, 3.0);
+    kkkmm = d_enforce(kkkmm, kkkmin, kkkmax);
+    double cohescmm, cohestmm, frictcmm, dilatcmm, fricttmm, etasmm0, etamm0, etamm, rhomm;
+    if (MK(HMEC_M_T_MARKER) != 0) {
+        cohescmm = MK(HMEC_M_COHESCM) * (1 - porm);
+        cohestmm = MK(HMEC_M_COHESTM) * (1 - porm);
+        frictcmm = MK(HMEC_M_FRICTCM);
+        dilatcmm = MK(HMEC_M_DILATCM);
+        fricttmm = MK(HMEC_M_FRICTTM);
+        etasmm0 = MK(HMEC_M_ETASM);
+        etamm0 = MK(HMEC_M_ETASM) * exp(-alpha * porm);
+        etamm = MK(HMEC_M_ETAM);
+        rhomm = MK(HMEC_M_RHOM) * (1 - porm) + MK(HMEC_M_RHOFM) * porm;
+    } else {
+        cohescmm = MK(HMEC_M_COHESCM);
+        cohestmm = MK(HMEC_M_COHESTM);
+        frictcmm = MK(HMEC_M_FRICTCM);
+        dilatcmm = 0;
+        fricttmm = MK(HMEC_M_FRICTTM);
+        etasmm0 = MK(HMEC_M_ETASM);
+        etamm0 = MK(HMEC_M_ETASM);
+        etamm = MK(HMEC_M_ETAM);
+        rhomm = MK(HMEC_M_RHOM);
+    }
+    etamm0 = d_enforce(etamm0, etamin, etamax);
+    etamm = d_enforce(etamm, etamin, etamax);
+    const double etadm = MK(HMEC_M_ETAFM) / kkkmm;
+    PR(MP_ETAMM) = etamm;
+    PR(MP_RHOMM) = rhomm;
+    PR(MP_KKKMM) = kkkmm;
+    PR(MP_TMARK) = MK(HMEC_M_T_MARKER);
+    PR(MP_INVG) = 1. / MK(HMEC_M_GM);
+    PR(MP_ETAMM0) = etamm0;
+    PR(MP_COHT) = cohestmm;
+    PR(MP_FRIT) = fricttmm;
+    PR(MP_COHC) = cohescmm;
+    PR(MP_FRIC) = frictcmm;
+    PR(MP_DILC) = dilatcmm;
+    PR(MP_PORM) = porm;
+    PR(MP_ETASMM0) = etasmm0;
+    PR(MP_INVETAD) = 1. / etadm;
+    PR(MP_RHOFM) = MK(HMEC_M_RHOFM);
+    const double xm = MK(HMEC_M_XM), ym = MK(HMEC_M_YM);
+    const int cy = Ny - 1;
+    int j, i;
+    j = d_check_bounds(d_fix(xm / dx), Nx);
+    i = d_check_bounds(d_fix(ym / dy), Ny);
+    cell[0 * M + m] = i + j * cy;
+    j = d_check_bounds(d_fix((xm + dx / 2.) / dx), Nx);
+    i = d_check_bounds(d_fix((ym + dy / 2.) / dy), Ny);
+    cell[1 * M + m] = i + j * cy;
+    j = d_check_bounds(d_fix((xm) / dx), Nx);
+    i = d_check_bounds(d_fix((ym + dy / 2.) / dy), Ny);
+    cell[2 * M + m] = i + j * cy;
+    j = d_check_bounds(d_fix((xm + dx / 2.) / dx), Nx);
+    i = d_check_bounds(d_fix((ym) / dy), Ny);
+    cell[3 * M + m] = i + j * cy;
+    keys[m] = (int32_t)m;
+#undef MK
+#undef PR
+}
+
+__global__ void k_iota(int32_t *v, int64_t M) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < M) v[m] = (int32_t)m;
+}
+
+// cell_start[c] = first sorted position with key >= c (lower bound)
+__global__ void k_cell_start(const int32_t *__restrict__ sorted_keys, int64_t M, int ncell,
+                             int32_t *__restrict__ start) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncell) return;
+    int64_t lo = 0, hi = M;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (sorted_keys[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    start[c] = (int32_t)lo;
+}
+
+int k_marker_prep(hmec_ctx *c) {
+    const int64_t M = c->M;
+    if (M <= 0) return HMEC_OK;
+    const hmec_params &P = c->P;
+    const int threads = 256;
+    const int blocks = (int)((M + threads - 1) / threads);
+    hipLaunchKernelGGL(k_prep, dim3(blocks), dim3(threads), 0, c->stream, c->d.mk, M, c->d.prop,
+                       c->d.cell, c->d.keys_tmp, c->Nx, c->Ny, P.dx, P.dy, P.POR0, P.kkkmin,
+                       P.kkkmax, P.alpha, P.etamin, P.etamax);
+    LAUNCH_CHECK();
+    const int ncell = (c->Nx - 1) * (c->Ny - 1);
+    int nbits = 1;
+    while ((1 << nbits) <= ncell) nbits++;
+    for (int g = 0; g < 4; g++) {
+        // cell ids are non-negative: sort them as unsigned keys
+        uint32_t *keys_in = reinterpret_cast<uint32_t *>(c->d.cell + (int64_t)g * M);
+        uint32_t *keys_out = reinterpret_cast<uint32_t *>(c->d.keys_tmp + M);   // sorted keys
+        int32_t *vals_in = c->d.keys_tmp;                                      // iota
+        int32_t *vals_out = c->d.perm + (int64_t)g * M;
+        hipLaunchKernelGGL(k_iota, dim3(blocks), dim3(threads), 0, c->stream, vals_in, M);
+        LAUNCH_CHECK();
+        size_t tmp = c->d.sort_tmp_bytes;
+        hipError_t e = rocprim::radix_sort_pairs(c->d.sort_tmp, tmp, keys_in, keys_out, vals_in,
+                                                 vals_out, (unsigned int)M, 0, nbits, c->stream);
+        if (e != hipSuccess) {
+            hmec_set_last_error(std::string("radix_sort_pairs: ") + hipGetErrorString(e));
+            return HMEC_E_HIP;
+        }
+        hipLaunchKernelGGL(k_cell_start, dim3((ncell + 1 + 255) / 256), dim3(256), 0, c->stream,
+                           reinterpret_cast<const int32_t *>(keys_out), M, ncell, c->d.cell_start + (int64_t)g * (ncell + 1));
+        LAUNCH_CHECK();
+    }
+    c->markers_dirty = false;
+    return HMEC_OK;
+}
+
+size_t hmec_sort_tmp_bytes(int64_t M, int nbits) {
+    size_t tmp = 0;
+    rocprim::radix_sort_pairs(nullptr, tmp, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                              (int32_t *)nullptr, (int32_t *)nullptr, (unsigned int)M, 0, nbits,
+                              (hipStream_t)0);
+    return tmp;
+}
+
+// ----------------------------------------------------------------- scatter
+// One thread per node of grid g.  The node gathers the markers of the (up to)
+// four anchor cells whose 2x2 block covers it, in fixed cell order and, inside
+// a cell, in marker order — deterministic, no atomics.  The sums then go
+// through the reference's normalization (h_mec_rsf_v9.cpp:531-583).
+template <int G>
+__global__ void k_scatter_grid(const double *__restrict__ mk, const double *__restrict__ prop,
+                               int64_t M, const int32_t *__restrict__ perm,
+                               const int32_t *__restrict__ start, const double *__restrict__ cx,
+                               const double *__restrict__ cyv, double *__restrict__ fields,
+                               int64_t ns, int Nx, int Ny, double dx, double dy) {
+    const int rows = (G == 0) ? Ny : Ny + 1;
+    const int cols = (G == 0) ? Nx : Nx + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * cols) return;
+    const int i = t % rows, j = t / rows;
+    const int cy = Ny - 1, cxn = Nx - 1;
+    constexpr int NF = (G == 0) ? 10 : (G == 1) ? 5 : 4;
+    double acc[NF + 1];
+#pragma unroll
+    for (int f = 0; f <= NF; f++) acc[f] = 0.;
+    const double *xm = mk + (int64_t)HMEC_M_XM * M;
+    const double *ym = mk + (int64_t)HMEC_M_YM * M;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int di = q & 1, dj = q >> 1;          // corner of the node in the cell
+        const int ci = i - di, cj = j - dj;         // cell (ci, cj)
+        if (ci < 0 || cj < 0 || ci >= cy || cj >= cxn) continue;
+        const int cid = ci + cj * cy;
+        const double x0 = cx[cj], y0 = cyv[ci];
+        for (int k = start[cid]; k < start[cid + 1]; k++) {
+            const int m = perm[k];
+            const double dxm = (xm[m] - x0) / dx;
+            const double dym = (ym[m] - y0) / dy;
+            const double w = (di ? dym : (1 - dym)) * (dj ? dxm : (1 - dxm));
+#define PR(f) prop[(int64_t)(f) * M + m]
+            if (G == 0) {
+                acc[0] += PR(MP_RHOMM) * w;
+                acc[1] += PR(MP_KKKMM) * w;
+                acc[2] += PR(MP_TMARK) * w;
+                acc[3] += PR(MP_INVG) * w;
+                acc[4] += PR(MP_ETAMM0) * w;
+                acc[5] += PR(MP_COHT) * w;
+                acc[6] += PR(MP_FRIT) * w;
+                acc[7] += PR(MP_COHC) * w;
+                acc[8] += PR(MP_FRIC) * w;
+                acc[9] += PR(MP_DILC) * w;
+            } else if (G == 1) {
+                acc[0] += PR(MP_ETAMM) * w;
+                acc[1] += PR(MP_PORM) * w;
+                acc[2] += PR(MP_INVG) * w;
+                acc[3] += PR(MP_ETAMM0) * w;
+                acc[4] += PR(MP_ETASMM0) * w;
+            } else {
+                acc[0] += PR(MP_RHOMM) * w;
+                acc[1] += PR(MP_INVETAD) * w;
+                acc[2] += PR(MP_RHOFM) * w;
+                acc[3] += PR(MP_PORM) * w;
+            }
+#undef PR
+            acc[NF] += w;
+        }
+    }
+    const double wt = acc[NF];
+    if (!(wt > 0)) return;
+    const int64_t idx = t;   // i + j * rows
+#define FLD(f) fields[(int64_t)(f) * ns + idx]
+    if (G == 0) {
+        FLD(HMEC_F_RHO) = acc[0] / wt;
+        FLD(HMEC_F_KKK) = acc[1] / wt;
+        FLD(HMEC_F_TTT) = acc[2] / wt;
+        FLD(HMEC_F_GGG) = 1. / (acc[3] / wt);
+        FLD(HMEC_F_ETA0) = acc[4] / wt;
+        FLD(HMEC_F_COHT) = acc[5] / wt;
+        FLD(HMEC_F_FRIT) = acc[6] / wt;
+        FLD(HMEC_F_COHC) = acc[7] / wt;
+        FLD(HMEC_F_FRIC) = acc[8] / wt;
+        FLD(HMEC_F_DILC) = acc[9] / wt;
+    } else if (G == 1) {
+        FLD(HMEC_F_ETAP) = acc[0] / wt;
+        const double por = acc[1] / wt;
+        FLD(HMEC_F_POR) = por;
+        FLD(HMEC_F_ETAB0) = acc[4] / wt / por;
+        FLD(HMEC_F_ETAP0) = acc[3] / wt;
+        FLD(HMEC_F_GGGP) = 1. / (acc[2] / wt);
+    } else if (G == 2) {
+        FLD(HMEC_F_RHOX) = acc[0] / wt;
+        FLD(HMEC_F_ETADX) = 1. / (acc[1] / wt);
+        FLD(HMEC_F_RHOFX) = acc[2] / wt;
+        FLD(HMEC_F_PORX) = acc[3] / wt;
+    } else {
+        FLD(HMEC_F_RHOY) = acc[0] / wt;
+        FLD(HMEC_F_ETADY) = 1. / (acc[1] / wt);
+        FLD(HMEC_F_RHOFY) = acc[2] / wt;
+        FLD(HMEC_F_PORY) = acc[3] / wt;
+    }
+#undef FLD
+}
+
+int k_scatter(hmec_ctx *c) {
+    if (!c->have_markers) return HMEC_E_ARG;
+    if (c->markers_dirty) HMEC_TRY(k_marker_prep(c));
+    Coords k = coords(c);
+    const int ncell = (c->Nx - 1) * (c->Ny - 1);
+    const int64_t M = c->M;
+    const hmec_params &P = c->P;
+    const int nbasic = c->Nx * c->Ny, nstag = c->Nx1 * c->Ny1;
+    hipLaunchKernelGGL(k_scatter_grid<0>, dim3((nbasic + 127) / 128), dim3(128), 0, c->stream,
+                       c->d.mk, c->d.prop, M, c->d.perm, c->d.cell_start, k.x, k.y, c->d.fields,
+                       c->ns, c->Nx, c->Ny, P.dx, P.dy);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scatter_grid<1>, dim3((nstag + 127) / 128), dim3(128), 0, c->stream,
+                       c->d.mk, c->d.prop, M, c->d.perm + M, c->d.cell_start + (ncell + 1), k.xp,
+                       k.yp, c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scatter_grid<2>, dim3((nstag + 127) / 128), dim3(128), 0, c->stream,
+                       c->d.mk, c->d.prop, M, c->d.perm + 2 * M, c->d.cell_start + 2 * (ncell + 1),
+                       k.xvx, k.yvx, c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scatter_grid<3>, dim3((nstag + 127) / 128), dim3(128), 0, c->stream,
+                       c->d.mk, c->d.prop, M, c->d.perm + 3 * M, c->d.cell_start + 3 * (ncell + 1),
+                       k.xvy, k.yvy, c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// --------------------------------------------------------- initial state
+// Fresh-start nodal state, h_mec_rsf_v9.cpp:283-338.
+__global__ void k_init(double *fields, int64_t ns, const double *x, const double *y, int Nx,
+                       int Ny, hmec_params P) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ny1 = Ny + 1, Nx1 = Nx + 1;
+    if (t < Nx1 * Ny1) {
+        fields[(int64_t)HMEC_F_PT0 * ns + t] = P.PCONF + P.PTFDIFF;
+        fields[(int64_t)HMEC_F_PF0 * ns + t] = P.PCONF;
+    }
+    if (t >= Nx * Ny) return;
+    const int i = t % Ny, j = t / Ny;
+    double om0 = P.omm[0], ar = P.arsfm[1], br = P.brsfm[1], lr = P.lrsfm[0];
+    if (y[i] > P.upper_block && y[i] < P.lower_block) {
+        const double xj = x[j];
+        const double TS_1 = P.TS_1, TS_2 = P.TS_2, TS_3 = P.TS_3, TS_4 = P.TS_4;
+        om0 = P.omm[1];
+        if (xj < TS_1) { br = P.brsfm[0]; ar = P.arsfm[0]; }
+        if (xj >= TS_1 && xj < TS_2) {
+            br = P.brsfm[0] - (P.brsfm[0] - P.brsfm[1]) * ((xj - TS_1) / (TS_2 - TS_1));
+            ar = P.arsfm[0] - (P.arsfm[0] - P.arsfm[1]) * ((xj - TS_1) / (TS_2 - TS_1));
+            lr = P.lrsfm[0] - (P.lrsfm[0] - P.lrsfm[1]) * ((xj - TS_1) / (TS_2 - TS_1));
+        }
+        if (xj >= TS_2 && xj <= TS_3) lr = P.lrsfm[1];
+        if (xj > TS_3 && xj <= TS_4) {
+            br = P.brsfm[1] - (P.brsfm[1] - P.brsfm[0]) * ((xj - TS_3) / (TS_4 - TS_3));
+            ar = P.arsfm[1] - (P.arsfm[1] - P.arsfm[0]) * ((xj - TS_3) / (TS_4 - TS_3));
+            lr = P.lrsfm[1] - (P.lrsfm[1] - P.lrsfm[0]) * ((xj - TS_3) / (TS_4 - TS_3));
+        }
+        if (xj > TS_4) { br = P.brsfm[0]; ar = P.arsfm[0]; }
+    }
+    fields[(int64_t)HMEC_F_OM0 * ns + t] = om0;
+    fields[(int64_t)HMEC_F_OM * ns + t] = om0;
+    fields[(int64_t)HMEC_F_ARSF * ns + t] = ar;
+    fields[(int64_t)HMEC_F_BRSF * ns + t] = br;
+    fields[(int64_t)HMEC_F_LRSF * ns + t] = lr;
+}
+
+int k_init_state(hmec_ctx *c) {
+    Coords k = coords(c);
+    const int n = c->Nx1 * c->Ny1;
+    hipLaunchKernelGGL(k_init, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.fields, c->ns,
+                       k.x, k.y, c->Nx, c->Ny, c->P);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+int k_copy_field(hmec_ctx *c, int dst, int src) {
+    const int64_t n = (dst < HMEC_NBASIC_FIELDS) ? c->nb : c->ns;
+    HMEC_HIP(hipMemcpyAsync(F(c, dst), F(c, src), n * sizeof(double), hipMemcpyDeviceToDevice,
+                            c->stream));
+    return HMEC_OK;
+}
+
+// ------------------------------------------------------------ VEP (a7)
+// copy_bounds(pt), copy_bounds(pf) (h_mec_rsf_v9.cpp:616-617, :154-159):
+// columns first, then rows, as in the reference.
+__global__ void k_copy_bounds_cols(double *a, double *b, int Nx, int Ny1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Ny1) return;
+    a[i] = a[i + Ny1];
+    a[i + (int64_t)Nx * Ny1] = a[i + (int64_t)(Nx - 1) * Ny1];
+    b[i] = b[i + Ny1];
+    b[i + (int64_t)Nx * Ny1] = b[i + (int64_t)(Nx - 1) * Ny1];
+}
+__global__ void k_copy_bounds_rows(double *a, double *b, int Nx1, int Ny, int Ny1) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= Nx1) return;
+    a[(int64_t)j * Ny1] = a[1 + (int64_t)j * Ny1];
+    a[Ny + (int64_t)j * Ny1] = a[Ny - 1 + (int64_t)j * Ny1];
+    b[(int64_t)j * Ny1] = b[1 + (int64_t)j * Ny1];
+    b[Ny + (int64_t)j * Ny1] = b[Ny - 1 + (int64_t)j * Ny1];
+}
+
+// basic nodes, h_mec_rsf_v9.cpp:620-636
+__global__ void k_vep_basic(double *f, int64_t ns, int Nx, int Ny, double etamincur,
+                            double dy_faultw) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Nx * Ny) return;
+    const int i = t % Ny, j = t / Ny, Ny1 = Ny + 1;
+    double *ETA = f + (int64_t)HMEC_F_ETA * ns;
+    const double *ETA0 = f + (int64_t)HMEC_F_ETA0 * ns;
+    double eta = ETA[t];
+    if (eta < etamincur) { eta = etamincur; ETA[t] = eta; }
+    if (eta < ETA0[t]) {
+        const double *SXX = f + (int64_t)HMEC_F_SXX * ns, *SYY = f + (int64_t)HMEC_F_SYY * ns;
+        const int64_t s = i + (int64_t)j * Ny1;
+        const double SXX_temp = ((SXX[s] + SXX[s + 1]) + SXX[s + Ny1]) + SXX[s + Ny1 + 1];
+        const double SYY_temp = ((SYY[s] + SYY[s + 1]) + SYY[s + Ny1]) + SYY[s + Ny1 + 1];
+        const double sxy = f[(int64_t)HMEC_F_SXY * ns + t];
+        const double siib = sqrt(sxy * sxy + (SXX_temp * SXX_temp + SYY_temp * SYY_temp +
+                                              (SXX_temp + SYY_temp) * (SXX_temp + SYY_temp)) / 32.);
+        f[(int64_t)HMEC_F_SIIB * ns + t] = siib;
+        const double ietapl = (1. / eta - 1. / ETA0[t]);
+        f[(int64_t)HMEC_F_IETAPLB * ns + t] = ietapl;
+        f[(int64_t)HMEC_F_EIIB * ns + t] = dy_faultw * siib * ietapl / 2.;
+    } else {
+        f[(int64_t)HMEC_F_EIIB * ns + t] = 0;
+        f[(int64_t)HMEC_F_IETAPLB * ns + t] = 0;
+    }
+}
+
+// pressure nodes, h_mec_rsf_v9.cpp:639-669
+__global__ void k_vep_p(double *f, int64_t ns, int Nx, int Ny, double etamincur,
+                        double dy_faultw) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (Nx - 1) * (Ny - 1)) return;
+    const int i = 1 + t % (Ny - 1), j = 1 + t / (Ny - 1), Ny1 = Ny + 1;
+    const double *IE = f + (int64_t)HMEC_F_IETAPLB * ns;
+    const double *YNY0 = f + (int64_t)HMEC_F_YNY0 * ns;
+    const int64_t b00 = (i - 1) + (int64_t)(j - 1) * Ny, b10 = i + (int64_t)(j - 1) * Ny;
+    const int64_t b01 = (i - 1) + (int64_t)j * Ny, b11 = i + (int64_t)j * Ny;
+    const double IETAPL = (((IE[b00] + IE[b10]) + IE[b01]) + IE[b11]) / 4.;
+    const int64_t s = i + (int64_t)j * Ny1;
+    const double POR = f[(int64_t)HMEC_F_POR * ns + s];
+    double etap, etab;
+    if (YNY0[b00] > 0 || YNY0[b10] > 0 || YNY0[b01] > 0 || YNY0[b11] > 0) {
+        etap = 1. / (1. / f[(int64_t)HMEC_F_ETAP0 * ns + s] + IETAPL);
+        etab = 1. / (1. / f[(int64_t)HMEC_F_ETAB0 * ns + s] + dy_faultw * IETAPL * POR);
+    } else {
+        etap = f[(int64_t)HMEC_F_ETAP0 * ns + s];
+        etab = f[(int64_t)HMEC_F_ETAB0 * ns + s];
+    }
+    if (etap < etamincur) etap = etamincur;
+    if (etab * POR < etamincur) etab = etamincur / POR;
+    f[(int64_t)HMEC_F_ETAP * ns + s] = etap;
+    f[(int64_t)HMEC_F_ETAB * ns + s] = etab;
+    f[(int64_t)HMEC_F_GGGB * ns + s] = f[(int64_t)HMEC_F_GGGP * ns + s] / POR;
+    f[(int64_t)HMEC_F_DILP * ns + s] = 0;
+}
+
+int k_vep_pre(hmec_ctx *c, double dt) {
+    const hmec_params &P = c->P;
+    const double etamincur = dt * P.shearmod * 1e-4;
+    const double dy_faultw = P.dy / P.faultwidth;
+    hipLaunchKernelGGL(k_copy_bounds_cols, dim3((c->Ny1 + 255) / 256), dim3(256), 0, c->stream,
+                       F(c, HMEC_F_PT), F(c, HMEC_F_PF), c->Nx, c->Ny1);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_copy_bounds_rows, dim3((c->Nx1 + 255) / 256), dim3(256), 0, c->stream,
+                       F(c, HMEC_F_PT), F(c, HMEC_F_PF), c->Nx1, c->Ny, c->Ny1);
+    LAUNCH_CHECK();
+    const int nb = c->Nx * c->Ny;
+    hipLaunchKernelGGL(k_vep_basic, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
+                       c->ns, c->Nx, c->Ny, etamincur, dy_faultw);
+    LAUNCH_CHECK();
+    const int np = (c->Nx - 1) * (c->Ny - 1);
+    hipLaunchKernelGGL(k_vep_p, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
+                       c->ns, c->Nx, c->Ny, etamincur, dy_faultw);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// ---------------------------------------------------------- scale (a8)
+// pfscale = ptscale = min(ETADX(1:Ny-1, 0:Nx-1)) * dx * (dt > 2e5 ? 1e17/dt^2 : 1)
+// (h_mec_rsf_v9.cpp:671-677); a min is order independent, so exact.
+__global__ void k_scale(const double *ETADX, int Nx, int Ny, double dx, double dt,
+                        double *scal) {
+    __shared__ double red[1024];
+    const int Ny1 = Ny + 1;
+    double mn = INFINITY;
+    for (int t = threadIdx.x; t < (Ny - 1) * Nx; t += blockDim.x) {
+        const int i = 1 + t % (Ny - 1), j = t / (Ny - 1);
+        mn = fmin(mn, ETADX[i + (int64_t)j * Ny1]);
+    }
+    red[threadIdx.x] = mn;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double m = red[0], pf;
+        if (dt > 2e5) pf = m * dx * 1e17 / (dt * dt);
+        else pf = m * dx;
+        scal[SC_PTSCALE] = pf;
+    }
+}
+
+int k_scale(hmec_ctx *c, double dt, double *ptscale) {
+    hipLaunchKernelGGL(k_scale, dim3(1), dim3(1024), 0, c->stream, F(c, HMEC_F_ETADX), c->Nx,
+                       c->Ny, c->P.dx, dt, c->d.scal);
+    LAUNCH_CHECK();
+    HMEC_HIP(hipMemcpyAsync(c->d.h_scal + SC_PTSCALE, c->d.scal + SC_PTSCALE, sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    *ptscale = c->d.h_scal[SC_PTSCALE];
+    return HMEC_OK;
+}
+
+// ------------------------------------------------------- assembly (a9)
+static AsmIn asm_in(hmec_ctx *c, const double *base, int64_t ns, double dt, double ptscale) {
+    const hmec_params &P = c->P;
+    AsmIn in;
+#define FP(f) (base + (int64_t)(f) * ns)
+    in.ETA = FP(HMEC_F_ETA); in.ETAP = FP(HMEC_F_ETAP); in.GGG = FP(HMEC_F_GGG);
+    in.GGGP = FP(HMEC_F_GGGP); in.SXY0 = FP(HMEC_F_SXY0); in.SXX0 = FP(HMEC_F_SXX0);
+    in.SYY0 = FP(HMEC_F_SYY0); in.RHOX = FP(HMEC_F_RHOX); in.RHO = FP(HMEC_F_RHO);
+    in.RHOY = FP(HMEC_F_RHOY); in.VX0 = FP(HMEC_F_VX0); in.VY0 = FP(HMEC_F_VY0);
+    in.POR = FP(HMEC_F_POR); in.ETAB = FP(HMEC_F_ETAB); in.GGGB = FP(HMEC_F_GGGB);
+    in.PT0 = FP(HMEC_F_PT0); in.PF0 = FP(HMEC_F_PF0); in.DILP = FP(HMEC_F_DILP);
+    in.RHOFX = FP(HMEC_F_RHOFX); in.ETADX = FP(HMEC_F_ETADX); in.PORX = FP(HMEC_F_PORX);
+    in.VXF0 = FP(HMEC_F_VXF0); in.RHOFY = FP(HMEC_F_RHOFY); in.ETADY = FP(HMEC_F_ETADY);
+    in.PORY = FP(HMEC_F_PORY); in.VYF0 = FP(HMEC_F_VYF0);
+#undef FP
+    in.Nx = c->Nx; in.Ny = c->Ny;
+    in.dt = dt; in.dx = P.dx; in.dy = P.dy; in.ptscale = ptscale;
+    in.gx = P.gx; in.gy = P.gy; in.inertia = P.inertia;
+    in.BETASOLID = P.BETASOLID; in.BETAFLUID = P.BETAFLUID;
+    in.bcupper = P.bcupper; in.bclower = P.bclower; in.bcvyflower = P.bcvyflower;
+    in.PTFDIFF = P.PTFDIFF;
+    return in;
+}
+
+struct PatternEmit {
+    std::vector<int32_t> *ti, *tj;
+    void add(int r, int cidx, double) { ti->push_back(r); tj->push_back(cidx); }
+    void rhs(int, double) {}
+};
+
+// Fixed CSR pattern of the system (rows sorted by column, duplicates merged)
+// and the triplet -> slot map, built on the host once per grid from the
+// emission order (what Eigen's setFromTriplets + makeCompressed produce,
+// h_mec_rsf_v9.cpp:1046-1049).
+int hmec_pattern_host(const hmec_params &P, std::vector<int32_t> &rowp,
+                      std::vector<int32_t> &coli, std::vector<int32_t> *slot_out,
+                      std::vector<int32_t> *node_trip_out) {
+    const int Nx1 = P.Nx + 1, Ny1 = P.Ny + 1;
+    const int64_t ns = (int64_t)Nx1 * Ny1, N = ns * 6;
+    std::vector<double> base((size_t)ns * HMEC_NFIELDS, 0.0);
+    AsmIn in;
+    {
+        const double *b = base.data();
+#define FP(f) (b + (int64_t)(f) * ns)
+        in.ETA = FP(HMEC_F_ETA); in.ETAP = FP(HMEC_F_ETAP); in.GGG = FP(HMEC_F_GGG);
+        in.GGGP = FP(HMEC_F_GGGP); in.SXY0 = FP(HMEC_F_SXY0); in.SXX0 = FP(HMEC_F_SXX0);
+        in.SYY0 = FP(HMEC_F_SYY0); in.RHOX = FP(HMEC_F_RHOX); in.RHO = FP(HMEC_F_RHO);
+        in.RHOY = FP(HMEC_F_RHOY); in.VX0 = FP(HMEC_F_VX0); in.VY0 = FP(HMEC_F_VY0);
+        in.POR = FP(HMEC_F_POR); in.ETAB = FP(HMEC_F_ETAB); in.GGGB = FP(HMEC_F_GGGB);
+        in.PT0 = FP(HMEC_F_PT0); in.PF0 = FP(HMEC_F_PF0); in.DILP = FP(HMEC_F_DILP);
+        in.RHOFX = FP(HMEC_F_RHOFX); in.ETADX = FP(HMEC_F_ETADX); in.PORX = FP(HMEC_F_PORX);
+        in.VXF0 = FP(HMEC_F_VXF0); in.RHOFY = FP(HMEC_F_RHOFY); in.ETADY = FP(HMEC_F_ETADY);
+        in.PORY = FP(HMEC_F_PORY); in.VYF0 = FP(HMEC_F_VYF0);
+#undef FP
+        in.Nx = P.Nx; in.Ny = P.Ny;
+        in.dt = 1; in.dx = P.dx; in.dy = P.dy; in.ptscale = 1;
+        in.gx = P.gx; in.gy = P.gy; in.inertia = P.inertia;
+        in.BETASOLID = P.BETASOLID; in.BETAFLUID = P.BETAFLUID;
+        in.bcupper = P.bcupper; in.bclower = P.bclower; in.bcvyflower = P.bcvyflower;
+        in.PTFDIFF = P.PTFDIFF;
+    }
+    std::vector<int32_t> ti, tj;
+    ti.reserve(N * 8);
+    tj.reserve(N * 8);
+    PatternEmit e{&ti, &tj};
+    std::vector<int32_t> node_trip((size_t)ns + 1, 0);
+    for (int j = 0; j < Nx1; j++)
+        for (int i = 0; i < Ny1; i++) {
+            node_trip[(size_t)j * Ny1 + i] = (int32_t)ti.size();
+            emit_node(i, j, in, e);
+        }
+    node_trip[(size_t)ns] = (int32_t)ti.size();
+    const int64_t nt = (int64_t)ti.size();
+    std::vector<int32_t> cnt(N + 1, 0);
+    for (int64_t t = 0; t < nt; t++) cnt[ti[t] + 1]++;
+    for (int64_t r = 0; r < N; r++) cnt[r + 1] += cnt[r];
+    std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+    std::vector<int64_t> ord(nt);
+    for (int64_t t = 0; t < nt; t++) ord[pos[ti[t]]++] = t;
+    for (int64_t r = 0; r < N; r++)
+        for (int a = cnt[r] + 1; a < cnt[r + 1]; a++) {
+            int64_t t = ord[a];
+            int b = a - 1;
+            while (b >= cnt[r] && tj[ord[b]] > tj[t]) { ord[b + 1] = ord[b]; b--; }
+            ord[b + 1] = t;
+        }
+    rowp.assign(N + 1, 0);
+    coli.clear();
+    std::vector<int32_t> slot(nt);
+    for (int64_t r = 0; r < N; r++) {
+        rowp[r] = (int32_t)coli.size();
+        int last = -1;
+        for (int a = cnt[r]; a < cnt[r + 1]; a++) {
+            const int64_t t = ord[a];
+            if (tj[t] != last) { coli.push_back(tj[t]); last = tj[t]; }
+            slot[t] = (int32_t)coli.size() - 1;
+        }
+    }
+    rowp[N] = (int32_t)coli.size();
+    if (slot_out) slot_out->swap(slot);
+    if (node_trip_out) node_trip_out->swap(node_trip);
+    return HMEC_OK;
+}
+
+int hmec_build_pattern(hmec_ctx *c) {
+    const int64_t N = c->N;
+    std::vector<int32_t> slot, node_trip;
+    HMEC_TRY(hmec_pattern_host(c->P, c->h_rowp, c->h_coli, &slot, &node_trip));
+    c->nnz = (int64_t)c->h_coli.size();
+    const int64_t nt = (int64_t)slot.size();
+    c->ntrip = nt;
+    HMEC_HIP(hipMalloc(&c->d.rowp, sizeof(int32_t) * (N + 1)));
+    HMEC_HIP(hipMalloc(&c->d.coli, sizeof(int32_t) * c->nnz));
+    HMEC_HIP(hipMalloc(&c->d.val, sizeof(double) * c->nnz));
+    HMEC_HIP(hipMalloc(&c->d.trip_slot, sizeof(int32_t) * nt));
+    HMEC_HIP(hipMalloc(&c->d.node_trip, sizeof(int32_t) * node_trip.size()));
+    HMEC_HIP(hipMemcpy(c->d.rowp, c->h_rowp.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
+    HMEC_HIP(hipMemcpy(c->d.coli, c->h_coli.data(), sizeof(int32_t) * c->nnz, hipMemcpyHostToDevice));
+    HMEC_HIP(hipMemcpy(c->d.trip_slot, slot.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice));
+    HMEC_HIP(hipMemcpy(c->d.node_trip, node_trip.data(), sizeof(int32_t) * node_trip.size(),
+                       hipMemcpyHostToDevice));
+    HMEC_HIP(hipMemset(c->d.val, 0, sizeof(double) * c->nnz));
+    return HMEC_OK;
+}
+
+struct ValueEmit {
+    double *val;
+    const int32_t *slot;
+    int t;
+    double *R;
+    __device__ void add(int, int, double v) { val[slot[t++]] += v; }
+    __device__ void rhs(int r, double v) { R[r] = v; }
+};
+
+// One thread per staggered node writes its six rows in place.
+__global__ void k_assemble(AsmIn in, const int32_t *__restrict__ rowp,
+                           const int32_t *__restrict__ node_trip,
+                           const int32_t *__restrict__ slot, double *__restrict__ val,
+                           double *__restrict__ R, int Nx1, int Ny1) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Nx1 * Ny1) return;
+    const int i = t % Ny1, j = t / Ny1;
+    const int kp = t * 6;
+    for (int p = rowp[kp]; p < rowp[kp + 6]; p++) val[p] = 0.;
+    for (int r = 0; r < 6; r++) R[kp + r] = 0.;
+    ValueEmit e{val, slot + node_trip[t], 0, R};
+    emit_node(i, j, in, e);
+}
+
+int k_assemble(hmec_ctx *c, double dt, double ptscale) {
+    AsmIn in = asm_in(c, c->d.fields, c->ns, dt, ptscale);
+    const int n = c->Nx1 * c->Ny1;
+    hipLaunchKernelGGL(k_assemble, dim3((n + 127) / 128), dim3(128), 0, c->stream, in, c->d.rowp,
+                       c->d.node_trip, c->d.trip_slot, c->d.val, c->d.rhs, c->Nx1, c->Ny1);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// ------------------------------------------------------- reload (a11)
+__global__ void k_reload(const double *S, double *f, int64_t ns, int n, const double *scal) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double sc = scal[SC_PTSCALE];
+    const int kp = t * 6;
+    f[(int64_t)HMEC_F_PT * ns + t] = S[kp] * sc;
+    f[(int64_t)HMEC_F_VXS * ns + t] = S[kp + 1];
+    f[(int64_t)HMEC_F_VYS * ns + t] = S[kp + 2];
+    f[(int64_t)HMEC_F_PF * ns + t] = S[kp + 3] * sc;
+    f[(int64_t)HMEC_F_VXD * ns + t] = S[kp + 4];
+    f[(int64_t)HMEC_F_VYD * ns + t] = S[kp + 5];
+}
+
+int k_reload(hmec_ctx *c, double ptscale) {
+    (void)ptscale;
+    const int n = c->Nx1 * c->Ny1;
+    hipLaunchKernelGGL(k_reload, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.sol,
+                       c->d.fields, c->ns, n, c->d.scal);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// ------------------------------------------------------ stress (a12)
+// h_mec_rsf_v9.cpp:1102-1127 (and :1480-1521 with ESP when final != 0)
+__global__ void k_stress(double *f, int64_t ns, int Nx, int Ny, double dx, double dy, double dt,
+                         int final_pass) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ny1 = Ny + 1;
+    if (t >= (Nx + 1) * Ny1) return;
+    const int i = t % Ny1, j = t / Ny1;
+    const double *vxs = f + (int64_t)HMEC_F_VXS * ns, *vys = f + (int64_t)HMEC_F_VYS * ns;
+    if (i < Ny && j < Nx) {
+        const int64_t b = i + (int64_t)j * Ny;
+        const int64_t s = i + (int64_t)j * Ny1;
+        const double dvxdy = (vxs[s + 1] - vxs[s]) / dy;
+        const double dvydx = (vys[s + Ny1] - vys[s]) / dx;
+        const double exy = .5 * (dvxdy + dvydx);
+        if (final_pass) f[(int64_t)HMEC_F_ESP * ns + b] = .5 * (dvydx - dvxdy);
+        f[(int64_t)HMEC_F_EXY * ns + b] = exy;
+        const double eta = f[(int64_t)HMEC_F_ETA * ns + b];
+        const double KXY = asm_divplus(dt * f[(int64_t)HMEC_F_GGG * ns + b], eta);
+        f[(int64_t)HMEC_F_SXY * ns + b] = 2 * eta * exy * KXY + f[(int64_t)HMEC_F_SXY0 * ns + b] * (1 - KXY);
+    }
+    if (i >= 1 && i < Ny && j >= 1 && j < Nx) {
+        const int64_t s = i + (int64_t)j * Ny1;
+        const double dvx = vxs[s] - vxs[s - Ny1];
+        const double dvy = vys[s] - vys[s - 1];
+        const double exx = (2 * dvx / dx - dvy / dy) / 3.;
+        const double eyy = (2 * dvy / dy - dvx / dx) / 3.;
+        f[(int64_t)HMEC_F_EXX * ns + s] = exx;
+        f[(int64_t)HMEC_F_EYY * ns + s] = eyy;
+        const double etap = f[(int64_t)HMEC_F_ETAP * ns + s];
+        const double KXX = asm_divplus(dt * f[(int64_t)HMEC_F_GGGP * ns + s], etap);
+        f[(int64_t)HMEC_F_SXX * ns + s] = 2 * etap * exx * KXX + f[(int64_t)HMEC_F_SXX0 * ns + s] * (1 - KXX);
+        f[(int64_t)HMEC_F_SYY * ns + s] = 2 * etap * eyy * KXX + f[(int64_t)HMEC_F_SYY0 * ns + s] * (1 - KXX);
+    }
+}
+
+// --------------------------------------------------------- RSF (a13)
+// One workgroup.  Vmax (before the solve's RSF, h_mec_rsf_v9.cpp:1077), then
+// the rate-and-state bisection per fault node (:1165-1308), then ynpl / ddd
+// (summed sequentially in node order like the reference) / dtlapusta.
+// Nodal rows other than line_fault keep ETA5 = ETA0, DSY = YNY = 0 etc.
+__global__ void __launch_bounds__(1024) k_rsf(double *f, int64_t ns, int Nx, int Ny,
+                                              hmec_params P, double dt, int active,
+                                              double *scal, double *dsy2, int *flag) {
+    __shared__ double red[1024];
+    const int nb = Nx * Ny, Ny1 = Ny + 1;
+    const int tid = threadIdx.x;
+    double *VS = f + (int64_t)HMEC_F_VSLIPB * ns;
+    double mx = -INFINITY;
+    for (int t = tid; t < nb; t += blockDim.x) mx = fmax(mx, VS[t]);
+    red[tid] = mx;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) scal[SC_VMAX] = red[0];
+    __syncthreads();
+    // ETA5 = ETA0, DSY = YNY = SigmaY = SII_fault = 0, OM5 = OM
+    for (int t = tid; t < nb; t += blockDim.x) {
+        f[(int64_t)HMEC_F_ETA5 * ns + t] = f[(int64_t)HMEC_F_ETA0 * ns + t];
+        f[(int64_t)HMEC_F_DSY * ns + t] = 0;
+        f[(int64_t)HMEC_F_YNY * ns + t] = 0;
+        f[(int64_t)HMEC_F_SIGMAY * ns + t] = 0;
+        f[(int64_t)HMEC_F_SII_FAULT * ns + t] = 0;
+        f[(int64_t)HMEC_F_OM5 * ns + t] = f[(int64_t)HMEC_F_OM * ns + t];
+    }
+    __syncthreads();
+    double dtl = 1e7;
+    int capped = 0;
+    if (active) {
+        const int i = P.line_fault;
+        const double V0 = P.V0, BETASOLID = P.BETASOLID, dx = P.dx;
+        for (int j = tid; j < Nx; j += blockDim.x) {
+            const int64_t b = i + (int64_t)j * Ny;
+            const int64_t s = i + (int64_t)j * Ny1;
+#define BF(n) f[(int64_t)(n) * ns + b]
+#define SB(n) (((f[(int64_t)(n) * ns + s] + f[(int64_t)(n) * ns + s + 1]) + f[(int64_t)(n) * ns + s + Ny1]) + f[(int64_t)(n) * ns + s + Ny1 + 1])
+            const double arsf = BF(HMEC_F_ARSF), brsf = BF(HMEC_F_BRSF), lrsf = BF(HMEC_F_LRSF);
+            const double fric = BF(HMEC_F_FRIC), eta0 = BF(HMEC_F_ETA0);
+            const double sxx = SB(HMEC_F_SXX), syy = SB(HMEC_F_SYY);
+            const double G = BF(HMEC_F_GGG), ETA = BF(HMEC_F_ETA), OMij = BF(HMEC_F_OM);
+            const double sxy = BF(HMEC_F_SXY);
+            const double SIIBij = sqrt(sxy * sxy + (sxx * sxx + syy * syy + (sxx + syy) * (sxx + syy)) / 32.);
+            BF(HMEC_F_SIIB) = SIIBij;
+            const double siiel = SIIBij / (ETA / (G * dt + ETA));
+            const double prB_temp = (SB(HMEC_F_PT) - SB(HMEC_F_PF)) / 4.;
+            double prB = prB_temp;
+            if (prB < 1e3) prB = 1e3;
+            const double expf_ = exp(-(brsf * OMij + fric) / arsf);
+            double SIIB1 = SIIBij;
+            double EIISLIP = V0 * sinh(fmax(SIIB1, 0.) / arsf / prB) * expf_ / dx;
+            double ETAPL = SIIB1 / 2. / EIISLIP;
+            double ETAVP = 1. / (1. / eta0 + 1. / ETAPL);
+            double SIIB2 = siiel * ETAVP / (G * dt + ETAVP);
+            const double DSIIB1 = SIIB2 - SIIB1;
+            double V = 2 * V0 * sinh(fmax(SIIB2, 0.) / arsf / prB) * expf_;
+            EIISLIP = V / dx / 2.;
+            ETAPL = SIIB2 / 2. / EIISLIP;
+            ETAVP = 1. / (1. / eta0 + 1. / ETAPL);
+            const double DSIIB2 = siiel * ETAVP / (G * dt + ETAVP) - SIIB2;
+            double SIIB4 = 0.;
+            if ((DSIIB1 >= 0 && DSIIB2 <= 0) || (DSIIB1 <= 0 && DSIIB2 >= 0)) {
+                double DSIIB = 1e9;
+                int it = 0;
+                while (fabs(DSIIB) > 1e-3) {
+                    if (it++ >= P.bisect_cap) { capped++; break; }
+                    SIIB4 = (SIIB1 + SIIB2) / 2.;
+                    V = 2 * V0 * sinh(fmax((SIIB4), 0.) / arsf / prB) * expf_;
+                    EIISLIP = V / dx / 2.;
+                    ETAPL = SIIB4 / 2. / EIISLIP;
+                    ETAVP = 1. / (1. / eta0 + 1. / ETAPL);
+                    DSIIB = siiel * ETAVP / (G * dt + ETAVP) - SIIB4;
+                    if ((DSIIB >= 0 && DSIIB1 >= 0) || (DSIIB <= 0 && DSIIB1 <= 0)) SIIB1 = SIIB4;
+                    else SIIB2 = SIIB4;
+                }
+            }
+            double om5;
+            if (V * dt / lrsf > 1e-6) om5 = log(V0 / V + (exp(BF(HMEC_F_OM0)) - V0 / V) * exp(-V * dt / lrsf));
+            else om5 = log(exp(BF(HMEC_F_OM0)) * (1 - V * dt / lrsf) + V0 * dt / lrsf);
+            BF(HMEC_F_OM5) = om5;
+            const double syield = fmax(P.syieldmin, prB_temp * arsf * asinh(V / 2. / V0 * exp((brsf * om5 + fric) / arsf)));
+            double etapl = eta0 * syield / (eta0 * V + syield);
+            BF(HMEC_F_SIGMAY) = syield;
+            BF(HMEC_F_VSLIPB) = V;
+            BF(HMEC_F_SII_FAULT) = SIIB4;
+            const double g_temp = 2 * G;
+            const double k = g_temp / (M_PI * (1 - ((3. / BETASOLID - g_temp) / (6. / BETASOLID + g_temp))) * dx);
+            double dTETAmax;
+            const double q = (k * lrsf / prB - brsf) / arsf - 1;
+            if ((q * q / 4. - k * lrsf / arsf / prB) < 0)
+                dTETAmax = fmin(1. - (brsf - arsf) * prB / (k * lrsf), .2);
+            else
+                dTETAmax = fmin(arsf * prB / (k * lrsf - (brsf - arsf) * prB), .2);
+            dtl = fmin(dtl, dTETAmax * lrsf / V);
+            int counted = 0;
+            double d2 = 0;
+            if (BF(HMEC_F_YNY0) > 0) {
+                counted = 1;
+                const double dsy = SIIBij - syield;
+                BF(HMEC_F_DSY) = dsy;
+                d2 = dsy * dsy;
+            }
+            const double A = syield / siiel;
+            if (A < 1) {
+                etapl = dt * G * A / (1 - A);
+                if (etapl < eta0) {
+                    BF(HMEC_F_ETA5) = pow(etapl, 1 - P.etawt) * pow(ETA, P.etawt);
+                    BF(HMEC_F_YNY) = 1;
+                    if (!counted) {
+                        counted = 1;
+                        const double dsy = SIIBij - syield;
+                        BF(HMEC_F_DSY) = dsy;
+                        d2 = dsy * dsy;
+                    }
+                } else {
+                    BF(HMEC_F_ETA5) = eta0;
+                }
+            } else {
+                BF(HMEC_F_ETA5) = eta0;
+            }
+            dsy2[j] = d2;
+            flag[j] = counted;
+#undef BF
+#undef SB
+        }
+    }
+    red[tid] = dtl;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = fmin(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) scal[SC_DTLAP] = red[0];
+    __syncthreads();
+    red[tid] = (double)capped;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        scal[SC_CAPPED] = red[0];
+        int ynpl = 0;
+        double ddd = 0;
+        if (active)
+            for (int j = 0; j < Nx; j++)
+                if (flag[j]) { ddd += dsy2[j]; ynpl++; }
+        scal[SC_YNPL] = ynpl;
+        scal[SC_DDD] = ddd;
+    }
+}
+
+// velocity extrema, fault-band maxima and dtslip (h_mec_rsf_v9.cpp:1344-1410)
+__global__ void __launch_bounds__(1024) k_reduce(const double *f, int64_t ns, int Nx, int Ny,
+                                                 const double *yvx, const double *yvy,
+                                                 double upper, double lower, double dxstp,
+                                                 double *scal) {
+    __shared__ double red[7][1024];
+    const int Ny1 = Ny + 1, tid = threadIdx.x;
+    const double *vxs = f + (int64_t)HMEC_F_VXS * ns, *vys = f + (int64_t)HMEC_F_VYS * ns;
+    const double *VS = f + (int64_t)HMEC_F_VSLIPB * ns;
+    double vxmax = -INFINITY, vxmin = INFINITY, vymax = -INFINITY, vymin = INFINITY, bx = 0, by = 0,
+           dts = 1e30;
+    for (int t = tid; t < (Nx + 1) * Ny1; t += blockDim.x) {
+        const int i = t % Ny1;
+        const double a = vxs[t], b = vys[t];
+        vxmax = fmax(vxmax, a);
+        vxmin = fmin(vxmin, a);
+        vymax = fmax(vymax, b);
+        vymin = fmin(vymin, b);
+        if (yvx[i] >= upper && yvx[i] <= lower) bx = fmax(bx, fabs(a));
+        if (yvy[i] >= upper && yvy[i] <= lower) by = fmax(by, fabs(b));
+    }
+    for (int t = tid; t < Nx * Ny; t += blockDim.x)
+        if (VS[t] > 0) dts = fmin(dts, dxstp / VS[t]);
+    red[0][tid] = vxmax; red[1][tid] = vxmin; red[2][tid] = vymax; red[3][tid] = vymin;
+    red[4][tid] = bx; red[5][tid] = by; red[6][tid] = dts;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[0][tid] = fmax(red[0][tid], red[0][tid + s]);
+            red[1][tid] = fmin(red[1][tid], red[1][tid + s]);
+            red[2][tid] = fmax(red[2][tid], red[2][tid + s]);
+            red[3][tid] = fmin(red[3][tid], red[3][tid + s]);
+            red[4][tid] = fmax(red[4][tid], red[4][tid + s]);
+            red[5][tid] = fmax(red[5][tid], red[5][tid + s]);
+            red[6][tid] = fmin(red[6][tid], red[6][tid + s]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        scal[SC_VXMAX] = red[0][0]; scal[SC_VXMIN] = red[1][0];
+        scal[SC_VYMAX] = red[2][0]; scal[SC_VYMIN] = red[3][0];
+        scal[SC_BANDX] = red[4][0]; scal[SC_BANDY] = red[5][0];
+        scal[SC_DTSLIP] = red[6][0];
+    }
+}
+
+int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out) {
+    const hmec_params &P = c->P;
+    const int n = c->Nx1 * c->Ny1;
+    hipLaunchKernelGGL(k_stress, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
+                       c->ns, c->Nx, c->Ny, P.dx, P.dy, dt, 0);
+    LAUNCH_CHECK();
+    const int active = (c->timestep > P.tyield) ? 1 : 0;
+    double *dsy2 = c->d.work2;
+    int *flag = c->d.counters + 16;
+    hipLaunchKernelGGL(k_rsf, dim3(1), dim3(1024), 0, c->stream, c->d.fields, c->ns, c->Nx, c->Ny,
+                       P, dt, active, c->d.scal, dsy2, flag);
+    LAUNCH_CHECK();
+    Coords k = coords(c);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, c->stream, c->d.fields, c->ns, c->Nx,
+                       c->Ny, k.yvx, k.yvy, P.upper_block, P.lower_block, P.dx * P.stpmax,
+                       c->d.scal);
+    LAUNCH_CHECK();
+    HMEC_HIP(hipMemcpyAsync(c->d.h_scal, c->d.scal, sizeof(double) * SC_COUNT,
+                            hipMemcpyDeviceToHost, c->stream));
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    const double *h = c->d.h_scal;
+    out->ynpl = (int)h[SC_YNPL];
+    out->ddd = h[SC_DDD];
+    out->dtlapusta = h[SC_DTLAP];
+    out->Vmax = h[SC_VMAX];
+    out->vxs_max = h[SC_VXMAX];
+    out->vxs_min = h[SC_VXMIN];
+    out->vys_max = h[SC_VYMAX];
+    out->vys_min = h[SC_VYMIN];
+    out->maxvxs_band = h[SC_BANDX];
+    out->maxvys_band = h[SC_BANDY];
+    out->dtslip = h[SC_DTSLIP];
+    out->bisect_capped = (int)h[SC_CAPPED];
+    return HMEC_OK;
+}
+
+// ------------------------------------------------ Picard commit (a14)
+// ETA = ETA0 when ynpl == 0 (h_mec_rsf_v9.cpp:1315-1317); when not
+// converged: ETA = max(min(ETA5, ETA0), etamin), ETA50 = ETA, YNY0 = YNY,
+// OM = OM5 (:1440-1448).
+__global__ void k_commit(double *f, int64_t ns, int nb, int ynpl, int converged, double etamin) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    double *ETA = f + (int64_t)HMEC_F_ETA * ns;
+    const double eta0 = f[(int64_t)HMEC_F_ETA0 * ns + t];
+    if (ynpl == 0) ETA[t] = eta0;
+    if (!converged) {
+        const double e = fmax(fmin(f[(int64_t)HMEC_F_ETA5 * ns + t], eta0), etamin);
+        ETA[t] = e;
+        f[(int64_t)HMEC_F_ETA50 * ns + t] = e;
+        f[(int64_t)HMEC_F_YNY0 * ns + t] = f[(int64_t)HMEC_F_YNY * ns + t];
+        f[(int64_t)HMEC_F_OM * ns + t] = f[(int64_t)HMEC_F_OM5 * ns + t];
+    }
+}
+
+int k_iter_commit(hmec_ctx *c, int ynpl, int converged) {
+    const int nb = c->Nx * c->Ny;
+    hipLaunchKernelGGL(k_commit, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
+                       c->ns, nb, ynpl, converged, c->P.etamin);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+int k_final_stress(hmec_ctx *c, double dt) {
+    const int n = c->Nx1 * c->Ny1;
+    hipLaunchKernelGGL(k_stress, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
+                       c->ns, c->Nx, c->Ny, c->P.dx, c->P.dy, dt, 1);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// ------------------------------------------------------ history (a16)
+// h_mec_rsf_v9.cpp:1672-1726
+__global__ void k_history(double *f, int64_t ns, int nstag, int nb) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nstag) return;
+#define FF(n) f[(int64_t)(n) * ns + t]
+    const double vx = FF(HMEC_F_VXS), vy = FF(HMEC_F_VYS);
+    FF(HMEC_F_VX0) = vx;
+    FF(HMEC_F_VY0) = vy;
+    if (FF(HMEC_F_PORX) > 0) FF(HMEC_F_VXF0) = vx + FF(HMEC_F_VXD) / FF(HMEC_F_PORX);
+    if (FF(HMEC_F_PORY) > 0) FF(HMEC_F_VYF0) = vy + FF(HMEC_F_VYD) / FF(HMEC_F_PORY);
+    FF(HMEC_F_SXX0) = FF(HMEC_F_SXX);
+    FF(HMEC_F_SYY0) = FF(HMEC_F_SYY);
+    if (t < nb) FF(HMEC_F_SXY0) = FF(HMEC_F_SXY);
+    const double pt = FF(HMEC_F_PT), pf = FF(HMEC_F_PF);
+    FF(HMEC_F_PTF0) = pt - pf;
+    FF(HMEC_F_PT0) = pt;
+    FF(HMEC_F_PF0) = pf;
+#undef FF
+}
+
+int k_history(hmec_ctx *c) {
+    const int n = c->Nx1 * c->Ny1, nb = c->Nx * c->Ny;
+    hipLaunchKernelGGL(k_history, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
+                       c->ns, n, nb);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// ---------------------------------------------------- advection (a15)
+// Node -> marker bilinear gather of vx, vy (vx / vy grids) and spin (basic
+// grid) with the reference's RK stages (h_mec_rsf_v9.cpp:1563-1656: stage 3
+// samples stage 2's position), stress rotation and periodic recycling in x.
+__global__ void k_advect(double *mk, int64_t M, const double *__restrict__ f, int64_t ns, Coords k,
+                         int Nx, int Ny, double dx, double dy, double dt, double xsize) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const int Ny1 = Ny + 1;
+    double *xm = mk + (int64_t)HMEC_M_XM * M, *ym = mk + (int64_t)HMEC_M_YM * M;
+    const double *vxs = f + (int64_t)HMEC_F_VXS * ns, *vys = f + (int64_t)HMEC_F_VYS * ns;
+    const double *ESP = f + (int64_t)HMEC_F_ESP * ns;
+    const double xold = xm[m], yold = ym[m];
+    double X = xold, Y = yold;
+    double vxm[4], vym[4], spm0 = 0;
+    for (int rk = 0; rk < 4; rk++) {
+        int j = d_check_bounds(d_fix(X / dx), Nx);
+        int i = d_check_bounds(d_fix((Y + dy / 2.) / dy), Ny);
+        double dxm = (X - k.xvx[j]) / dx, dym = (Y - k.yvx[i]) / dy;
+        int64_t s = i + (int64_t)j * Ny1;
+        vxm[rk] = vxs[s] * (1 - dxm) * (1 - dym) + vxs[s + 1] * (1 - dxm) * dym +
+                  vxs[s + Ny1] * dxm * (1 - dym) + vxs[s + Ny1 + 1] * dxm * dym;
+        j = d_check_bounds(d_fix((X + dx / 2.) / dx), Nx);
+        i = d_check_bounds(d_fix(Y / dy), Ny);
+        dxm = (X - k.xvy[j]) / dx;
+        dym = (Y - k.yvy[i]) / dy;
+        s = i + (int64_t)j * Ny1;
+        vym[rk] = vys[s] * (1 - dxm) * (1 - dym) + vys[s + 1] * (1 - dxm) * dym +
+                  vys[s + Ny1] * dxm * (1 - dym) + vys[s + Ny1 + 1] * dxm * dym;
+        if (rk == 0) {
+            j = d_check_bounds(d_fix((X) / dx), Nx);
+            i = d_check_bounds(d_fix((Y) / dy), Ny);
+            dxm = (X - k.x[j]) / dx;
+            dym = (Y - k.y[i]) / dy;
+            const int64_t b = i + (int64_t)j * Ny;
+            spm0 = ESP[b] * (1 - dxm) * (1 - dym) + ESP[b + 1] * (1 - dxm) * dym +
+                   ESP[b + Ny] * dxm * (1 - dym) + ESP[b + Ny + 1] * dxm * dym;
+        }
+        if (rk < 2) {
+            X = xold + vxm[rk] * dt / 2.;
+            Y = yold + vym[rk] * dt / 2.;
+        } else if (rk == 3) {
+            X = xold + vxm[rk] * dt;
+            Y = yold + vym[rk] * dt;
+        }
+    }
+    const double dspeff = spm0 * dt;
+    double *sxxm = mk + (int64_t)HMEC_M_SXXM * M, *syym = mk + (int64_t)HMEC_M_SYYM * M,
+           *sxym = mk + (int64_t)HMEC_M_SXYM * M;
+    const double msxxold = sxxm[m], msyyold = syym[m], msxyold = sxym[m];
+    const double s2 = sin(2 * dspeff), c2 = cos(2 * dspeff), cs = cos(dspeff), sn = sin(dspeff);
+    sxym[m] = .5 * (msxxold - msyyold) * s2 + msxyold * c2;
+    sxxm[m] = msxxold * (cs * cs) + msyyold * (sn * sn) - msxyold * s2;
+    syym[m] = msxxold * (sn * sn) + msyyold * (cs * cs) + msxyold * s2;
+    double xn = xold + (vxm[0] + 2 * vxm[1] + 2 * vxm[2] + vxm[3]) / 6. * dt;
+    const double yn = yold + (vym[0] + 2 * vym[1] + 2 * vym[2] + vym[3]) / 6. * dt;
+    if (xn < 0) xn = xn + xsize;
+    if (xn > xsize) xn = xn - xsize;
+    xm[m] = xn;
+    ym[m] = yn;
+}
+
+int k_advect(hmec_ctx *c, double dt) {
+    if (!c->have_markers || c->M == 0) return HMEC_OK;
+    const int64_t M = c->M;
+    hipLaunchKernelGGL(k_advect, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
+                       c->d.mk, M, c->d.fields, c->ns, coords(c), c->Nx, c->Ny, c->P.dx, c->P.dy,
+                       dt, c->P.xsize);
+    LAUNCH_CHECK();
+    c->markers_dirty = true;
+    return HMEC_OK;
+}

@@ -1,0 +1,780 @@
+// Numeric multifrontal LU and solve on gfx950.
+//
+// Replaces PardisoLU::factorize / solve (h_mec_rsf_v9.cpp:1051-1052).
+// The matrix is first equilibrated with power-of-two row and column scales
+// (exact in floating point), then assembled into dense column-major fronts.
+// Fronts are processed level by level (all fronts of a level in one batched
+// launch sequence):
+//   extend-add of the children's contribution blocks,
+//   blocked right-looking LU of the pivot columns / rows with static pivots
+//     (a pivot below tau * max|a| is perturbed to +-tau, as PARDISO does with
+//     iparm[9] = 13), restricted to the pivot rows / columns,
+//   one Schur-complement GEMM  CB -= L21 * U12  with K = #pivots.
+// The factor stays in the front arena for the triangular solves, which walk
+// the tree leaves->root (forward, contribution vectors extend-added like the
+// factor) and root->leaves (backward).  FP64 iterative refinement on the
+// original CSR closes the loop; the same kernels run with FP32 fronts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "hmec_internal.h"
+
+int hmec_var_pos(int v);
+
+namespace {
+
+constexpr int NB = 32;      // panel width
+constexpr int TM = 64;      // GEMM tile (rows / cols)
+constexpr int TK = 16;      // GEMM K chunk
+constexpr int PR = 256;     // rows / columns per panel-solve workgroup
+
+#define LAUNCH_CHECK()                                                      \
+    do {                                                                    \
+        hipError_t e_ = hipGetLastError();                                  \
+        if (e_ != hipSuccess) {                                             \
+            hmec_set_last_error(std::string("launch: ") + hipGetErrorString(e_)); \
+            return HMEC_E_HIP;                                              \
+        }                                                                   \
+    } while (0)
+
+// ------------------------------------------------------------ equilibration
+__device__ __forceinline__ double pow2_recip(double m) {
+    // 2^-e with m in [2^(e-1), 2^e): a power of two near 1/m (exact scaling)
+    if (!(m > 0) || !isfinite(m)) return 1.0;
+    int e;
+    frexp(m, &e);
+    return ldexp(1.0, -e + 1);
+}
+
+__global__ void k_row_scale(const int32_t *rowp, const double *val, int64_t N, double *rs) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    double m = 0;
+    for (int p = rowp[r]; p < rowp[r + 1]; p++) m = fmax(m, fabs(val[p]));
+    rs[r] = pow2_recip(m);
+}
+
+__global__ void k_col_scale(const int32_t *cptr, const int32_t *cidx, const int32_t *crow,
+                            const double *val, const double *rs, int64_t N, double *cs) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    double m = 0;
+    for (int q = cptr[c]; q < cptr[c + 1]; q++) m = fmax(m, fabs(val[cidx[q]] * rs[crow[q]]));
+    cs[c] = pow2_recip(m);
+}
+
+template <typename T>
+__global__ void k_fill_zero(T *a, int64_t n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) a[t] = (T)0;
+}
+
+template <typename T>
+__global__ void k_front_assemble(const double *val, const int32_t *row_of, const int32_t *coli,
+                                 const double *rs, const double *cs, const int64_t *dst,
+                                 int64_t nnz, T *arena) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    arena[dst[p]] = (T)(val[p] * rs[row_of[p]] * cs[coli[p]]);
+}
+
+// ------------------------------------------------------------ extend-add
+// parent(map(a), map(b)) += child CB(a, b); one launch per child slot so no
+// two workgroups of a launch touch the same parent entry.
+template <typename T>
+__global__ void k_extend_add(const EaTask *tasks, const int32_t *ea_map, T *arena) {
+    const EaTask tk = tasks[blockIdx.y];
+    const int u = tk.child_n - tk.child_p;
+    const int nt = (u + TM - 1) / TM;
+    const int tile = blockIdx.x;
+    if (tile >= nt * nt) return;
+    const int tr = tile % nt, tc = tile / nt;
+    const int32_t *map = ea_map + tk.map_off;
+    const T *ch = arena + tk.child_off;
+    T *pa = arena + tk.parent_off;
+    const int64_t cn = tk.child_n, pn = tk.parent_n;
+    for (int e = threadIdx.x; e < TM * TM; e += blockDim.x) {
+        const int a = tr * TM + (e % TM), b = tc * TM + (e / TM);
+        if (a >= u || b >= u) continue;
+        const int pa_r = map[a / 6] * 6 + a % 6, pa_c = map[b / 6] * 6 + b % 6;
+        const int ca = tk.child_p + a, cb = tk.child_p + b;
+        pa[(int64_t)pa_c * pn + pa_r] += ch[(int64_t)cb * cn + ca];
+    }
+}
+
+// ------------------------------------------------------------ diag LU
+// LU of the kb x kb diagonal block of panel k0 in LDS (no row exchanges).
+template <typename T>
+__global__ void __launch_bounds__(256) k_diag_lu(const FrontTask *tasks, int k0, T *arena,
+                                                 double tau, int *npert) {
+    __shared__ T D[NB][NB + 1];
+    const FrontTask tk = tasks[blockIdx.x];
+    if (tk.p <= k0) return;
+    const int kb = min(NB, tk.p - k0);
+    T *F = arena + tk.off;
+    const int64_t n = tk.n;
+    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+        const int r = e % kb, c = e / kb;
+        D[r][c] = F[(int64_t)(k0 + c) * n + k0 + r];
+    }
+    __syncthreads();
+    for (int k = 0; k < kb; k++) {
+        if (threadIdx.x == 0) {
+            T d = D[k][k];
+            if (!(fabs((double)d) >= tau)) {
+                d = (d < (T)0) ? (T)(-tau) : (T)tau;
+                D[k][k] = d;
+                atomicAdd(npert, 1);
+            }
+        }
+        __syncthreads();
+        const T piv = D[k][k];
+        for (int r = k + 1 + threadIdx.x; r < kb; r += blockDim.x) D[r][k] = D[r][k] / piv;
+        __syncthreads();
+        const int m = kb - k - 1;
+        for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+            const int r = k + 1 + e % m, c = k + 1 + e / m;
+            D[r][c] = D[r][c] - D[r][k] * D[k][c];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+        const int r = e % kb, c = e / kb;
+        F[(int64_t)(k0 + c) * n + k0 + r] = D[r][c];
+    }
+}
+
+// ------------------------------------------------------------ panel solves
+// L panel rows:  X U_D = A  for rows [k0+kb, n)   (one thread per row)
+// U panel cols:  L_D Y = A  for cols [k0+kb, n)   (one thread per column)
+template <typename T>
+__global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, T *arena) {
+    __shared__ T D[NB][NB + 1];
+    const FrontTask tk = tasks[blockIdx.y];
+    if (tk.p <= k0) return;
+    const int kb = min(NB, tk.p - k0);
+    const int64_t n = tk.n;
+    const int rest = tk.n - k0 - kb;
+    const int nch = (rest + PR - 1) / PR;
+    if ((int)blockIdx.x >= 2 * nch) return;
+    T *F = arena + tk.off;
+    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+        const int r = e % kb, c = e / kb;
+        D[r][c] = F[(int64_t)(k0 + c) * n + k0 + r];
+    }
+    __syncthreads();
+    T x[NB];
+    if ((int)blockIdx.x < nch) {
+        const int r = k0 + kb + blockIdx.x * PR + threadIdx.x;
+        if (r >= tk.n) return;
+        for (int c = 0; c < kb; c++) x[c] = F[(int64_t)(k0 + c) * n + r];
+        for (int c = 0; c < kb; c++) {
+            T s = x[c];
+            for (int q = 0; q < c; q++) s = s - x[q] * D[q][c];
+            x[c] = s / D[c][c];
+        }
+        for (int c = 0; c < kb; c++) F[(int64_t)(k0 + c) * n + r] = x[c];
+    } else {
+        const int col = k0 + kb + (blockIdx.x - nch) * PR + threadIdx.x;
+        if (col >= tk.n) return;
+        T *Fc = F + (int64_t)col * n + k0;
+        for (int r = 0; r < kb; r++) x[r] = Fc[r];
+        for (int r = 0; r < kb; r++) {
+            T s = x[r];
+            for (int q = 0; q < r; q++) s = s - D[r][q] * x[q];
+            x[r] = s;
+        }
+        for (int r = 0; r < kb; r++) Fc[r] = x[r];
+    }
+}
+
+// ------------------------------------------------------------ GEMM tile
+// C(m x nn) -= A(m x K) * B(K x nn); column-major with leading dimension ld.
+template <typename T>
+__device__ __forceinline__ void gemm_tile_sub(T *C, const T *A, const T *B, int64_t ld, int m,
+                                              int nn, int K) {
+    __shared__ T As[TK][TM + 1];
+    __shared__ T Bs[TK][TM + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;   // 16 x 16 threads, 4 x 4 each
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = (T)0;
+    for (int kk = 0; kk < K; kk += TK) {
+        const int kc = min(TK, K - kk);
+        for (int e = threadIdx.x; e < TK * TM; e += blockDim.x) {
+            const int r = e % TM, k = e / TM;
+            As[k][r] = (k < kc && r < m) ? A[(int64_t)(kk + k) * ld + r] : (T)0;
+            const int cc = e / TK, k2 = e % TK;
+            Bs[k2][cc] = (k2 < kc && cc < nn) ? B[(int64_t)cc * ld + kk + k2] : (T)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < TK; k++) {
+            T a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                a[q] = As[k][tx + 16 * q];
+                b[q] = Bs[k][ty + 16 * q];
+            }
+#pragma unroll
+            for (int qa = 0; qa < 4; qa++)
+#pragma unroll
+                for (int qb = 0; qb < 4; qb++) acc[qa][qb] = fma(a[qa], b[qb], acc[qa][qb]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int qa = 0; qa < 4; qa++)
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++) {
+            const int r = tx + 16 * qa, c = ty + 16 * qb;
+            if (r < m && c < nn) C[(int64_t)c * ld + r] -= acc[qa][qb];
+        }
+}
+
+// trailing update of panel k0 restricted to rows < p or columns < p
+template <typename T>
+__global__ void __launch_bounds__(256) k_update(const FrontTask *tasks, int k0, T *arena) {
+    const FrontTask tk = tasks[blockIdx.y];
+    if (tk.p <= k0) return;
+    const int kb = min(NB, tk.p - k0);
+    const int s = k0 + kb;                    // first trailing index
+    const int tp = (tk.p - s + TM - 1) / TM;  // pivot-part tiles
+    const int u = tk.n - tk.p;
+    const int tu = (u + TM - 1) / TM;
+    const int t = blockIdx.x;
+    int rt, ct;
+    if (t < tp * (tp + tu)) {
+        rt = t / (tp + tu);
+        ct = t % (tp + tu);
+    } else {
+        const int t2 = t - tp * (tp + tu);
+        if (t2 >= tu * tp) return;
+        rt = tp + t2 / tp;
+        ct = t2 % tp;
+    }
+    int r0, m, c0, nn;
+    if (rt < tp) { r0 = s + rt * TM; m = min(TM, tk.p - r0); }
+    else { r0 = tk.p + (rt - tp) * TM; m = min(TM, tk.n - r0); }
+    if (ct < tp) { c0 = s + ct * TM; nn = min(TM, tk.p - c0); }
+    else { c0 = tk.p + (ct - tp) * TM; nn = min(TM, tk.n - c0); }
+    T *F = arena + tk.off;
+    const int64_t n = tk.n;
+    gemm_tile_sub<T>(F + (int64_t)c0 * n + r0, F + (int64_t)k0 * n + r0, F + (int64_t)c0 * n + k0,
+                     n, m, nn, kb);
+}
+
+// Schur complement CB -= L21 U12 (K = p)
+template <typename T>
+__global__ void __launch_bounds__(256) k_schur(const FrontTask *tasks, T *arena) {
+    const FrontTask tk = tasks[blockIdx.y];
+    const int u = tk.n - tk.p;
+    if (u <= 0 || tk.p <= 0) return;
+    const int tu = (u + TM - 1) / TM;
+    const int t = blockIdx.x;
+    if (t >= tu * tu) return;
+    const int rt = t % tu, ct = t / tu;
+    const int r0 = tk.p + rt * TM, c0 = tk.p + ct * TM;
+    const int m = min(TM, tk.n - r0), nn = min(TM, tk.n - c0);
+    T *F = arena + tk.off;
+    const int64_t n = tk.n;
+    gemm_tile_sub<T>(F + (int64_t)c0 * n + r0, F + r0, F + (int64_t)c0 * n, n, m, nn, tk.p);
+}
+
+// ------------------------------------------------------------ solves
+// forward: w = [z(pivots); 0] + sum_children extend(cv_child);
+//          y = L11^-1 w_p (stored back to z), cv = w_u - L21 y.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd(const FrontTask *tasks, const int *fronts,
+                                             const int32_t *dofs, const int64_t *dof_off,
+                                             const int64_t *voff, const int *child_ptr,
+                                             const int *child_list, const int64_t *ea_off,
+                                             const int32_t *ea_map, const FrontTask *all_tasks,
+                                             const T *arena, double *z, double *cv) {
+    extern __shared__ double w[];
+    const int f = fronts[blockIdx.x];
+    const FrontTask tk = tasks[blockIdx.x];
+    const int n = tk.n, p = tk.p;
+    const int32_t *dof = dofs + dof_off[f];
+    for (int l = threadIdx.x; l < n; l += blockDim.x) w[l] = (l < p) ? z[dof[l]] : 0.0;
+    __syncthreads();
+    for (int q = child_ptr[f]; q < child_ptr[f + 1]; q++) {
+        const int ch = child_list[q];
+        const FrontTask ct = all_tasks[ch];
+        const int u = ct.n - ct.p;
+        const double *cvc = cv + voff[ch];
+        const int32_t *map = ea_map + ea_off[ch];
+        for (int a = threadIdx.x; a < u; a += blockDim.x) w[map[a / 6] * 6 + a % 6] += cvc[a];
+        __syncthreads();
+    }
+    const T *Fm = arena + tk.off;
+    for (int c = 0; c < p; c++) {
+        const double yc = w[c];
+        const T *col = Fm + (int64_t)c * n;
+        for (int r = c + 1 + threadIdx.x; r < n; r += blockDim.x) w[r] -= (double)col[r] * yc;
+        __syncthreads();
+    }
+    for (int l = threadIdx.x; l < n; l += blockDim.x) {
+        if (l < p) z[dof[l]] = w[l];
+        else cv[voff[f] + (l - p)] = w[l];
+    }
+}
+
+// backward: w_p = y_p - U12 x_u; x_p = U11^-1 w_p (stored to z)
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd(const FrontTask *tasks, const int *fronts,
+                                             const int32_t *dofs, const int64_t *dof_off,
+                                             const T *arena, double *z) {
+    extern __shared__ double w[];
+    const int f = fronts[blockIdx.x];
+    const FrontTask tk = tasks[blockIdx.x];
+    const int n = tk.n, p = tk.p;
+    const int32_t *dof = dofs + dof_off[f];
+    const T *Fm = arena + tk.off;
+    double *xu = w + p;
+    for (int l = threadIdx.x; l < n; l += blockDim.x) w[l] = z[dof[l]];
+    __syncthreads();
+    // w_p -= U12 x_u : row r of U12 is F[r, p:n]
+    for (int r = threadIdx.x; r < p; r += blockDim.x) {
+        double s = w[r];
+        for (int c = p; c < n; c++) s -= (double)Fm[(int64_t)c * n + r] * xu[c - p];
+        w[r] = s;
+    }
+    __syncthreads();
+    for (int c = p - 1; c >= 0; c--) {
+        if (threadIdx.x == 0) w[c] = w[c] / (double)Fm[(int64_t)c * n + c];
+        __syncthreads();
+        const double xc = w[c];
+        const T *col = Fm + (int64_t)c * n;
+        for (int r = threadIdx.x; r < c; r += blockDim.x) w[r] -= (double)col[r] * xc;
+        __syncthreads();
+    }
+    for (int l = threadIdx.x; l < p; l += blockDim.x) z[dof[l]] = w[l];
+}
+
+__global__ void k_scale_vec(const double *a, const double *s, double *out, int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) out[t] = a[t] * s[t];
+}
+// x += c * z  (first == 1: x = c * z)
+__global__ void k_axpy_scaled(double *x, const double *cs, const double *z, int64_t N, int first) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const double d = cs[t] * z[t];
+    x[t] = first ? d : x[t] + d;
+}
+// r = b - A x
+__global__ void k_residual(const int32_t *rowp, const int32_t *coli, const double *val,
+                           const double *x, const double *b, double *r, int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    double s = b[t];
+    for (int p = rowp[t]; p < rowp[t + 1]; p++) s -= val[p] * x[coli[p]];
+    r[t] = s;
+}
+// out[0] = max|a|, out[1] = max|b|
+__global__ void __launch_bounds__(1024) k_norms(const double *a, const double *b, int64_t N,
+                                                double *out) {
+    __shared__ double ra[1024], rb[1024];
+    double ma = 0, mb = 0;
+    for (int64_t t = threadIdx.x; t < N; t += blockDim.x) {
+        ma = fmax(ma, fabs(a[t]));
+        mb = fmax(mb, fabs(b[t]));
+    }
+    ra[threadIdx.x] = ma;
+    rb[threadIdx.x] = mb;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            ra[threadIdx.x] = fmax(ra[threadIdx.x], ra[threadIdx.x + s]);
+            rb[threadIdx.x] = fmax(rb[threadIdx.x], rb[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = ra[0];
+        out[1] = rb[0];
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ host side
+struct SolverDev {
+    int *fronts = nullptr;        // per level list of front ids (flattened)
+    std::vector<int64_t> lvl_fronts_off;
+    FrontTask *all_tasks = nullptr;
+    int *child_ptr = nullptr, *child_list = nullptr;
+    int64_t *voff = nullptr, *ea_off = nullptr;
+    double *norms = nullptr;
+    double *h_norms = nullptr;
+    std::vector<int> ea_cnt[2];
+};
+
+static SolverDev *sdev(hmec_ctx *c);
+
+#include <map>
+#include <mutex>
+static std::mutex g_sdev_mu;
+static std::map<hmec_ctx *, SolverDev *> g_sdev;
+static SolverDev *sdev(hmec_ctx *c) {
+    std::lock_guard<std::mutex> lk(g_sdev_mu);
+    auto it = g_sdev.find(c);
+    if (it != g_sdev.end()) return it->second;
+    SolverDev *s = new SolverDev();
+    g_sdev[c] = s;
+    return s;
+}
+
+int sv_free_device(hmec_ctx *c) {
+    SolverDev *s = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_sdev_mu);
+        auto it = g_sdev.find(c);
+        if (it != g_sdev.end()) {
+            s = it->second;
+            g_sdev.erase(it);
+        }
+    }
+    DevPlan &dp = c->dp;
+    for (auto p : dp.d_tasks) hipFree(p);
+    for (int k = 0; k < 2; k++)
+        for (auto p : dp.d_ea[k]) hipFree(p);
+    dp.d_tasks.clear();
+    hipFree(dp.d_nz_dst);
+    hipFree(dp.d_ea_map);
+    hipFree(dp.d_row_of);
+    hipFree(dp.d_csc_ptr);
+    hipFree(dp.d_csc_idx);
+    hipFree(dp.d_csc_row);
+    hipFree(dp.d_front_dofs);
+    hipFree(dp.d_front_dof_off);
+    dp = DevPlan();
+    if (s) {
+        hipFree(s->fronts);
+        hipFree(s->all_tasks);
+        hipFree(s->child_ptr);
+        hipFree(s->child_list);
+        hipFree(s->voff);
+        hipFree(s->ea_off);
+        hipFree(s->norms);
+        if (s->h_norms) hipHostFree(s->h_norms);
+        delete s;
+    }
+    hipFree(c->d.arena);
+    hipFree(c->d.arena32);
+    hipFree(c->d.vec);
+    c->d.arena = nullptr;
+    c->d.arena32 = nullptr;
+    c->d.vec = nullptr;
+    return HMEC_OK;
+}
+
+template <typename T>
+static int upload(T **dst, const std::vector<T> &v) {
+    HMEC_HIP(hipMalloc(dst, sizeof(T) * std::max<size_t>(1, v.size())));
+    if (!v.empty()) HMEC_HIP(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return HMEC_OK;
+}
+
+int sv_setup_device(hmec_ctx *c) {
+    const FrontPlan &pl = c->plan;
+    DevPlan &dp = c->dp;
+    SolverDev *s = sdev(c);
+    const int64_t N = c->N, nnz = c->nnz;
+    dp.nlevels = pl.nlevels;
+    // all tasks (indexed by front id)
+    std::vector<FrontTask> all(pl.nfronts);
+    for (int f = 0; f < pl.nfronts; f++)
+        all[f] = FrontTask{pl.off[f], 6 * (pl.np[f] + pl.nu[f]), 6 * pl.np[f]};
+    HMEC_TRY(upload(&s->all_tasks, all));
+    std::vector<int> flist;
+    dp.h_tasks.assign(pl.nlevels, {});
+    dp.h_ea[0].assign(pl.nlevels, {});
+    dp.h_ea[1].assign(pl.nlevels, {});
+    dp.maxp.assign(pl.nlevels, 0);
+    dp.maxn.assign(pl.nlevels, 0);
+    s->lvl_fronts_off.assign(pl.nlevels + 1, 0);
+    for (int lv = 0; lv < pl.nlevels; lv++) {
+        s->lvl_fronts_off[lv] = (int64_t)flist.size();
+        for (int f : pl.levels[lv]) {
+            flist.push_back(f);
+            dp.h_tasks[lv].push_back(all[f]);
+            dp.maxp[lv] = std::max(dp.maxp[lv], all[f].p);
+            dp.maxn[lv] = std::max(dp.maxn[lv], all[f].n);
+            const auto &kids = pl.children[f];
+            for (int q = 0; q < (int)kids.size() && q < 2; q++) {
+                const int k = kids[q];
+                dp.h_ea[q][lv].push_back(EaTask{all[k].off, all[f].off, all[k].n, all[k].p, all[f].n,
+                                                pl.ea_off[k]});
+            }
+            if (kids.size() > 2) {
+                hmec_set_last_error("front with more than two children");
+                return HMEC_E_ARG;
+            }
+        }
+    }
+    s->lvl_fronts_off[pl.nlevels] = (int64_t)flist.size();
+    HMEC_TRY(upload(&s->fronts, flist));
+    dp.d_tasks.assign(pl.nlevels, nullptr);
+    dp.d_ea[0].assign(pl.nlevels, nullptr);
+    dp.d_ea[1].assign(pl.nlevels, nullptr);
+    for (int lv = 0; lv < pl.nlevels; lv++) {
+        HMEC_TRY(upload(&dp.d_tasks[lv], dp.h_tasks[lv]));
+        for (int q = 0; q < 2; q++) HMEC_TRY(upload(&dp.d_ea[q][lv], dp.h_ea[q][lv]));
+    }
+    // children lists
+    std::vector<int> cptr(pl.nfronts + 1, 0), clist;
+    for (int f = 0; f < pl.nfronts; f++) {
+        cptr[f] = (int)clist.size();
+        for (int k : pl.children[f]) clist.push_back(k);
+    }
+    cptr[pl.nfronts] = (int)clist.size();
+    HMEC_TRY(upload(&s->child_ptr, cptr));
+    HMEC_TRY(upload(&s->child_list, clist));
+    HMEC_TRY(upload(&s->voff, pl.voff));
+    HMEC_TRY(upload(&s->ea_off, pl.ea_off));
+    HMEC_TRY(upload(&dp.d_nz_dst, pl.nz_dst));
+    HMEC_TRY(upload(&dp.d_ea_map, pl.ea_map));
+    // row of each CSR entry and the transposed pattern
+    std::vector<int32_t> row_of(nnz);
+    for (int64_t r = 0; r < N; r++)
+        for (int p = c->h_rowp[r]; p < c->h_rowp[r + 1]; p++) row_of[p] = (int32_t)r;
+    HMEC_TRY(upload(&dp.d_row_of, row_of));
+    std::vector<int32_t> cptr2(N + 1, 0), cidx(nnz), crow(nnz);
+    for (int64_t p = 0; p < nnz; p++) cptr2[c->h_coli[p] + 1]++;
+    for (int64_t k = 0; k < N; k++) cptr2[k + 1] += cptr2[k];
+    std::vector<int32_t> fill(cptr2.begin(), cptr2.end() - 1);
+    for (int64_t r = 0; r < N; r++)
+        for (int p = c->h_rowp[r]; p < c->h_rowp[r + 1]; p++) {
+            const int q = fill[c->h_coli[p]]++;
+            cidx[q] = p;
+            crow[q] = (int32_t)r;
+        }
+    HMEC_TRY(upload(&dp.d_csc_ptr, cptr2));
+    HMEC_TRY(upload(&dp.d_csc_idx, cidx));
+    HMEC_TRY(upload(&dp.d_csc_row, crow));
+    // per front global dofs
+    std::vector<int32_t> dofs;
+    dp.h_front_dof_off.assign(pl.nfronts + 1, 0);
+    int inv[6];
+    for (int v = 0; v < 6; v++) inv[hmec_var_pos(v)] = v;
+    for (int f = 0; f < pl.nfronts; f++) {
+        dp.h_front_dof_off[f] = (int64_t)dofs.size();
+        for (int node : pl.nodes[f])
+            for (int q = 0; q < 6; q++) dofs.push_back(node * 6 + inv[q]);
+    }
+    dp.h_front_dof_off[pl.nfronts] = (int64_t)dofs.size();
+    HMEC_TRY(upload(&dp.d_front_dofs, dofs));
+    HMEC_TRY(upload(&dp.d_front_dof_off, dp.h_front_dof_off));
+    // arenas
+    HMEC_HIP(hipMalloc(&c->d.vec, sizeof(double) * std::max<int64_t>(1, pl.varena)));
+    HMEC_HIP(hipMalloc(&s->norms, sizeof(double) * 4));
+    HMEC_HIP(hipHostMalloc(&s->h_norms, sizeof(double) * 4, hipHostMallocDefault));
+    return HMEC_OK;
+}
+
+static int ensure_arena(hmec_ctx *c, int precision) {
+    const int64_t n = c->plan.arena;
+    if (precision == HMEC_PREC_FP64 && !c->d.arena) {
+        if (hipMalloc(&c->d.arena, sizeof(double) * n) != hipSuccess) {
+            hipGetLastError();
+            hmec_set_last_error("front arena (FP64) allocation failed");
+            return HMEC_E_OOM;
+        }
+    }
+    if (precision == HMEC_PREC_FP32 && !c->d.arena32) {
+        if (hipMalloc(&c->d.arena32, sizeof(float) * n) != hipSuccess) {
+            hipGetLastError();
+            hmec_set_last_error("front arena (FP32) allocation failed");
+            return HMEC_E_OOM;
+        }
+    }
+    return HMEC_OK;
+}
+
+template <typename T>
+static int factor_levels(hmec_ctx *c, T *arena, double tau) {
+    const FrontPlan &pl = c->plan;
+    DevPlan &dp = c->dp;
+    hipStream_t st = c->stream;
+    const int64_t N = c->N, nnz = c->nnz;
+    // zero + assemble
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3(4096), dim3(256), 0, st, arena, pl.arena);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_front_assemble<T>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st,
+                       c->d.val, dp.d_row_of, c->d.coli, c->d.rscale, c->d.cscale, dp.d_nz_dst,
+                       nnz, arena);
+    LAUNCH_CHECK();
+    (void)N;
+    for (int lv = 0; lv < pl.nlevels; lv++) {
+        const int nfr = (int)dp.h_tasks[lv].size();
+        // extend-add children
+        for (int q = 0; q < 2; q++) {
+            const auto &ea = dp.h_ea[q][lv];
+            if (ea.empty()) continue;
+            int maxt = 0;
+            for (const auto &t : ea) {
+                const int u = t.child_n - t.child_p;
+                const int nt = (u + TM - 1) / TM;
+                maxt = std::max(maxt, nt * nt);
+            }
+            if (maxt == 0) continue;
+            hipLaunchKernelGGL(k_extend_add<T>, dim3(maxt, (unsigned)ea.size()), dim3(256), 0, st,
+                               dp.d_ea[q][lv], dp.d_ea_map, arena);
+            LAUNCH_CHECK();
+        }
+        // pivot panels
+        const int maxp = dp.maxp[lv], maxn = dp.maxn[lv];
+        for (int k0 = 0; k0 < maxp; k0 += NB) {
+            hipLaunchKernelGGL(k_diag_lu<T>, dim3(nfr), dim3(256), 0, st, dp.d_tasks[lv], k0,
+                               arena, tau, c->d.counters);
+            LAUNCH_CHECK();
+            const int rest = maxn - k0;
+            const int nch = std::max(1, (rest + PR - 1) / PR);
+            hipLaunchKernelGGL(k_panel<T>, dim3(2 * nch, nfr), dim3(PR), 0, st, dp.d_tasks[lv], k0,
+                               arena);
+            LAUNCH_CHECK();
+            int maxt = 0;
+            for (const auto &t : dp.h_tasks[lv]) {
+                if (t.p <= k0) continue;
+                const int kb = std::min(NB, t.p - k0);
+                const int tp = (t.p - k0 - kb + TM - 1) / TM;
+                const int tu = (t.n - t.p + TM - 1) / TM;
+                maxt = std::max(maxt, tp * (tp + tu) + tu * tp);
+            }
+            if (maxt > 0) {
+                hipLaunchKernelGGL(k_update<T>, dim3(maxt, nfr), dim3(256), 0, st, dp.d_tasks[lv],
+                                   k0, arena);
+                LAUNCH_CHECK();
+            }
+        }
+        int maxt = 0;
+        for (const auto &t : dp.h_tasks[lv]) {
+            const int tu = (t.n - t.p + TM - 1) / TM;
+            maxt = std::max(maxt, tu * tu);
+        }
+        if (maxt > 0) {
+            hipLaunchKernelGGL(k_schur<T>, dim3(maxt, nfr), dim3(256), 0, st, dp.d_tasks[lv], arena);
+            LAUNCH_CHECK();
+        }
+    }
+    return HMEC_OK;
+}
+
+int sv_factorize(hmec_ctx *c, int precision) {
+    if (!c->analyzed) return HMEC_E_ARG;
+    HMEC_TRY(ensure_arena(c, precision));
+    const int64_t N = c->N;
+    hipStream_t st = c->stream;
+    HMEC_HIP(hipMemsetAsync(c->d.counters, 0, sizeof(int) * 4, st));
+    hipLaunchKernelGGL(k_row_scale, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, c->d.rowp,
+                       c->d.val, N, c->d.rscale);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_col_scale, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
+                       c->dp.d_csc_ptr, c->dp.d_csc_idx, c->dp.d_csc_row, c->d.val, c->d.rscale, N,
+                       c->d.cscale);
+    LAUNCH_CHECK();
+    if (precision == HMEC_PREC_FP64) HMEC_TRY(factor_levels<double>(c, c->d.arena, 1e-13));
+    else HMEC_TRY(factor_levels<float>(c, c->d.arena32, 1e-6));
+    c->factor_prec = precision;
+    c->factor_valid = true;
+    c->stale_solves = 0;
+    return HMEC_OK;
+}
+
+template <typename T>
+static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
+    const FrontPlan &pl = c->plan;
+    DevPlan &dp = c->dp;
+    SolverDev *s = sdev(c);
+    hipStream_t st = c->stream;
+    for (int lv = 0; lv < pl.nlevels; lv++) {
+        const int nfr = (int)dp.h_tasks[lv].size();
+        const size_t sh = sizeof(double) * dp.maxn[lv];
+        hipLaunchKernelGGL(k_fwd<T>, dim3(nfr), dim3(256), sh, st, dp.d_tasks[lv],
+                           s->fronts + s->lvl_fronts_off[lv], dp.d_front_dofs, dp.d_front_dof_off,
+                           s->voff, s->child_ptr, s->child_list, s->ea_off, dp.d_ea_map,
+                           s->all_tasks, arena, z, c->d.vec);
+        LAUNCH_CHECK();
+    }
+    for (int lv = pl.nlevels - 1; lv >= 0; lv--) {
+        const int nfr = (int)dp.h_tasks[lv].size();
+        const size_t sh = sizeof(double) * dp.maxn[lv];
+        hipLaunchKernelGGL(k_bwd<T>, dim3(nfr), dim3(256), sh, st, dp.d_tasks[lv],
+                           s->fronts + s->lvl_fronts_off[lv], dp.d_front_dofs, dp.d_front_dof_off,
+                           arena, z);
+        LAUNCH_CHECK();
+    }
+    return HMEC_OK;
+}
+
+// x = A^-1 b by the factor + iterative refinement on the FP64 CSR.
+// tol: target ||b - A x||inf / ||b||inf; max_ir: refinement sweeps after the
+// first solve.  Returns HMEC_E_NOCONV when the target is not met.
+int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
+    if (!c->factor_valid) return HMEC_E_ARG;
+    SolverDev *s = sdev(c);
+    const int64_t N = c->N;
+    hipStream_t st = c->stream;
+    const unsigned g = (unsigned)((N + 255) / 256);
+    double *x = c->d.sol, *r = c->d.work, *z = c->d.work2;
+    // r = b
+    HMEC_HIP(hipMemcpyAsync(r, c->d.rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    double res = 0, bnorm = 0, prev = INFINITY;
+    int it = 0;
+    int stalls = 0;
+    for (;; it++) {
+        hipLaunchKernelGGL(k_scale_vec, dim3(g), dim3(256), 0, st, r, c->d.rscale, z, N);
+        LAUNCH_CHECK();
+        if (c->factor_prec == HMEC_PREC_FP64) HMEC_TRY(tri_solve<double>(c, c->d.arena, z));
+        else HMEC_TRY(tri_solve<float>(c, c->d.arena32, z));
+        hipLaunchKernelGGL(k_axpy_scaled, dim3(g), dim3(256), 0, st, x, c->d.cscale, z, N,
+                           it == 0 ? 1 : 0);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, x,
+                           c->d.rhs, r, N);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, st, r, c->d.rhs, N, s->norms);
+        LAUNCH_CHECK();
+        HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+        HMEC_HIP(hipStreamSynchronize(st));
+        bnorm = s->h_norms[1];
+        res = bnorm > 0 ? s->h_norms[0] / bnorm : s->h_norms[0];
+        if (res <= tol || it >= max_ir) break;
+        if (res > 0.5 * prev) {
+            if (++stalls >= 2) break;
+        } else {
+            stalls = 0;
+        }
+        prev = res;
+    }
+    int npert = 0;
+    HMEC_HIP(hipMemcpy(&npert, c->d.counters, sizeof(int), hipMemcpyDeviceToHost));
+    c->last_ir = it;
+    c->last_res = res;
+    c->last_perturbed = npert;
+    if (out) {
+        out->ir_iterations = it;
+        out->rel_residual = res;
+        out->perturbed_pivots = npert;
+        out->factor_flops = c->plan.flops;
+        out->factor_nnz = c->plan.factor_nnz;
+        out->nfronts = c->plan.nfronts;
+        out->nlevels = c->plan.nlevels;
+        out->max_front = c->plan.max_front;
+    }
+    c->stale_solves++;
+    return (res <= tol) ? HMEC_OK : HMEC_E_NOCONV;
+}
+
+int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision) {
+    for (int k = 0; k < n; k++) HMEC_TRY(sv_factorize(cs[k], precision));
+    return HMEC_OK;
+}

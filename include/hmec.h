@@ -269,6 +269,18 @@ int hmec_advect(hmec_ctx *ctx, double dt);
  * with the reference's Picard / dt controller around the kernels above. */
 int hmec_step(hmec_ctx *ctx, hmec_step_info *info);
 
+/* ------------------------------------------------- symbolic (host only, no GPU)
+ * The fixed CSR pattern of the coupled system (what setFromTriplets +
+ * makeCompressed give, h_mec_rsf_v9.cpp:1046-1049) and the statistics of the
+ * nested-dissection front plan used by hmec_analyze.  Call hmec_pattern with
+ * NULL arrays first to get n and nnz.  elim_order (N ints or NULL) receives the
+ * unknowns in elimination order. */
+int hmec_pattern(const hmec_params *p, int64_t *n, int64_t *nnz, int32_t *row_ptr, int32_t *col_idx);
+int hmec_plan_info(const hmec_params *p, hmec_solve_stats *stats, int64_t *arena_entries,
+                   int32_t *elim_order);
+/* text of the last error on this thread (HIP call, plan check, ...) */
+const char *hmec_last_error(void);
+
 /* ------------------------------------------------------------------ serving
  * Factor several same-size contexts' current CSR matrices in one batched
  * launch sequence (all contexts must share Nx, Ny and the current device). */
