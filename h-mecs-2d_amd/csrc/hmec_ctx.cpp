@@ -28,25 +28,53 @@ static void linspaced(double *v, int n, double low, double high) {
     }
 }
 
+static hipEvent_t tm_event(Timer &t) {
+    if (!t.pool.empty()) {
+        hipEvent_t e = t.pool.back();
+        t.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
 void tm_begin(hmec_ctx *c, Timer &t) {
     if (!c->timing) return;
-    if (!t.a) {
-        hipEventCreate(&t.a);
-        hipEventCreate(&t.b);
-    }
-    hipEventRecord(t.a, c->stream);
+    t.open_ev = tm_event(t);
+    hipEventRecord(t.open_ev, c->stream);
     t.open = true;
 }
-void tm_end(hmec_ctx *c, Timer &t, double bytes) {
+
+void tm_end(hmec_ctx *c, Timer &t, double work) {
     if (!c->timing || !t.open) return;
-    hipEventRecord(t.b, c->stream);
-    hipEventSynchronize(t.b);
-    float ms = 0;
-    hipEventElapsedTime(&ms, t.a, t.b);
-    t.total_ms += ms;
-    t.bytes += bytes;
-    t.n++;
+    hipEvent_t b = tm_event(t);
+    hipEventRecord(b, c->stream);
+    t.pending.emplace_back(t.open_ev, b);
+    t.pending_work.push_back(work);
     t.open = false;
+    if (t.pending.size() >= 4096) tm_resolve(t);
+}
+
+void tm_resolve(Timer &t) {
+    for (size_t k = 0; k < t.pending.size(); k++) {
+        hipEventSynchronize(t.pending[k].second);
+        float ms = 0;
+        hipEventElapsedTime(&ms, t.pending[k].first, t.pending[k].second);
+        t.total_ms += ms;
+        t.bytes += t.pending_work[k];
+        t.n++;
+        t.pool.push_back(t.pending[k].first);
+        t.pool.push_back(t.pending[k].second);
+    }
+    t.pending.clear();
+    t.pending_work.clear();
+}
+
+void tm_free(Timer &t) {
+    tm_resolve(t);
+    for (hipEvent_t e : t.pool) hipEventDestroy(e);
+    t.pool.clear();
 }
 
 static void free_ctx(hmec_ctx *c) {
@@ -60,10 +88,8 @@ static void free_ctx(hmec_ctx *c) {
     hipFree(d.rowp); hipFree(d.coli); hipFree(d.val); hipFree(d.rhs); hipFree(d.sol);
     hipFree(d.trip_slot); hipFree(d.node_trip); hipFree(d.rscale); hipFree(d.cscale);
     hipFree(d.work); hipFree(d.work2); hipFree(d.counters);
-    for (Timer *t : {&c->t_scatter, &c->t_assemble, &c->t_factor, &c->t_solve}) {
-        if (t->a) hipEventDestroy(t->a);
-        if (t->b) hipEventDestroy(t->b);
-    }
+    for (Timer *t : {&c->t_scatter, &c->t_assemble, &c->t_factor, &c->t_solve}) tm_free(*t);
+    for (auto &kv : c->kt) tm_free(kv.second);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -420,7 +446,10 @@ extern "C" int hmec_timing_enable(hmec_ctx *c, int on) {
 
 extern "C" int hmec_timing_reset(hmec_ctx *c) {
     if (!c) return HMEC_E_ARG;
-    for (Timer *t : {&c->t_scatter, &c->t_assemble, &c->t_factor, &c->t_solve}) {
+    std::vector<Timer *> all = {&c->t_scatter, &c->t_assemble, &c->t_factor, &c->t_solve};
+    for (auto &kv : c->kt) all.push_back(&kv.second);
+    for (Timer *t : all) {
+        tm_resolve(*t);
         t->total_ms = 0;
         t->bytes = 0;
         t->n = 0;
@@ -435,7 +464,12 @@ extern "C" int hmec_kernel_time(hmec_ctx *c, const char *name, double *avg_ms, d
     else if (!std::strcmp(name, "assemble")) t = &c->t_assemble;
     else if (!std::strcmp(name, "factor")) t = &c->t_factor;
     else if (!std::strcmp(name, "solve")) t = &c->t_solve;
+    else {
+        auto it = c->kt.find(name);
+        if (it != c->kt.end()) t = &it->second;
+    }
     if (!t) return -1;
+    tm_resolve(*t);
     if (avg_ms) *avg_ms = t->n ? t->total_ms / t->n : 0;
     if (bytes) *bytes = t->n ? t->bytes / t->n : 0;
     return t->n;

@@ -5,6 +5,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -109,10 +110,15 @@ struct DevPlan {
     std::vector<int64_t> h_front_dof_off;
 };
 
+// Named device timer: event pairs recorded around launches on the context's
+// stream, resolved lazily (no synchronization inside the timed region).
 struct Timer {
-    hipEvent_t a = nullptr, b = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<double> pending_work;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t open_ev = nullptr;
     double total_ms = 0;
-    double bytes = 0;
+    double bytes = 0;     // algorithmic work (bytes or flops) of resolved samples
     int n = 0;
     bool open = false;
 };
@@ -181,6 +187,7 @@ struct hmec_ctx {
     // timing
     bool timing = false;
     Timer t_scatter, t_assemble, t_factor, t_solve;
+    std::map<std::string, Timer> kt;    // per kernel class (schur, fwd, bwd, ...)
 };
 
 // ---- kernel launch wrappers (kernels_phys.hip) ----
@@ -211,5 +218,13 @@ int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision);
 // timing helpers
 void tm_begin(hmec_ctx *c, Timer &t);
 void tm_end(hmec_ctx *c, Timer &t, double bytes);
+void tm_resolve(Timer &t);
+void tm_free(Timer &t);
+inline void kt_begin(hmec_ctx *c, const char *name) {
+    if (c->timing) tm_begin(c, c->kt[name]);
+}
+inline void kt_end(hmec_ctx *c, const char *name, double work) {
+    if (c->timing) tm_end(c, c->kt[name], work);
+}
 
 inline double *F(hmec_ctx *c, int f) { return c->d.fields + (int64_t)f * c->ns; }

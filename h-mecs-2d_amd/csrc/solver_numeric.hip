@@ -106,53 +106,28 @@ __global__ void k_extend_add(const EaTask *tasks, const int32_t *ea_map, T *aren
     }
 }
 
-// ------------------------------------------------------------ diag LU
-// LU of the kb x kb diagonal block of panel k0 in LDS (no row exchanges).
-template <typename T>
-__global__ void __launch_bounds__(256) k_diag_lu(const FrontTask *tasks, int k0, T *arena,
-                                                 double tau, int *npert) {
-    __shared__ T D[NB][NB + 1];
-    const FrontTask tk = tasks[blockIdx.x];
-    if (tk.p <= k0) return;
-    const int kb = min(NB, tk.p - k0);
-    T *F = arena + tk.off;
-    const int64_t n = tk.n;
-    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
-        const int r = e % kb, c = e / kb;
-        D[r][c] = F[(int64_t)(k0 + c) * n + k0 + r];
-    }
-    __syncthreads();
-    for (int k = 0; k < kb; k++) {
-        if (threadIdx.x == 0) {
-            T d = D[k][k];
-            if (!(fabs((double)d) >= tau)) {
-                d = (d < (T)0) ? (T)(-tau) : (T)tau;
-                D[k][k] = d;
-                atomicAdd(npert, 1);
-            }
-        }
-        __syncthreads();
-        const T piv = D[k][k];
-        for (int r = k + 1 + threadIdx.x; r < kb; r += blockDim.x) D[r][k] = D[r][k] / piv;
-        __syncthreads();
-        const int m = kb - k - 1;
-        for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-            const int r = k + 1 + e % m, c = k + 1 + e / m;
-            D[r][c] = D[r][c] - D[r][k] * D[k][c];
-        }
-        __syncthreads();
-    }
-    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
-        const int r = e % kb, c = e / kb;
-        F[(int64_t)(k0 + c) * n + k0 + r] = D[r][c];
-    }
+// ------------------------------------------------------------ panel
+// wave-uniform broadcast of lane k's value
+__device__ __forceinline__ double bcast(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float bcast(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-// ------------------------------------------------------------ panel solves
-// L panel rows:  X U_D = A  for rows [k0+kb, n)   (one thread per row)
-// U panel cols:  L_D Y = A  for cols [k0+kb, n)   (one thread per column)
+// Panel k0 of every front of a level (no row exchanges):
+//   every workgroup loads the kb x kb diagonal block D, and wavefront 0
+//   factors it in registers (row per lane) with static-pivot perturbation;
+//   workgroup 0 stores the factored D in dscr (copied into the front by the
+//   following update launch, so no workgroup of this launch reads a half-
+//   written D);  row workgroups solve X U_D = A for rows [k0+kb, n),
+//   column workgroups solve L_D Y = A for columns [k0+kb, n).
 template <typename T>
-__global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, T *arena) {
+__global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, T *arena, T *dscr,
+                                              double tau, int *npert) {
     __shared__ T D[NB][NB + 1];
     const FrontTask tk = tasks[blockIdx.y];
     if (tk.p <= k0) return;
@@ -160,35 +135,79 @@ __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, T 
     const int64_t n = tk.n;
     const int rest = tk.n - k0 - kb;
     const int nch = (rest + PR - 1) / PR;
-    if ((int)blockIdx.x >= 2 * nch) return;
+    if ((int)blockIdx.x >= 1 + 2 * nch) return;
     T *F = arena + tk.off;
-    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
-        const int r = e % kb, c = e / kb;
-        D[r][c] = F[(int64_t)(k0 + c) * n + k0 + r];
+    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+        const int r = e % NB, c = e / NB;
+        D[r][c] = (r < kb && c < kb) ? F[(int64_t)(k0 + c) * n + k0 + r] : (r == c ? (T)1 : (T)0);
     }
     __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        T row[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) row[j] = (lane < NB) ? D[lane][j] : (T)0;
+        int pert = 0;
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            T piv = bcast(row[k], k);
+            if (!(fabs((double)piv) >= tau)) {
+                piv = (piv < (T)0) ? (T)(-tau) : (T)tau;
+                pert++;
+                if (lane == k) row[k] = piv;
+            }
+            const T l = row[k] / piv;
+            if (lane > k) row[k] = l;
+#pragma unroll
+            for (int j = k + 1; j < NB; j++) {
+                const T rk = bcast(row[j], k);
+                if (lane > k) row[j] = row[j] - l * rk;
+            }
+        }
+        if (lane < NB) {
+#pragma unroll
+            for (int j = 0; j < NB; j++) D[lane][j] = row[j];
+        }
+        if (lane == 0 && blockIdx.x == 0 && pert) atomicAdd(npert, pert);
+    }
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        T *ds = dscr + (int64_t)blockIdx.y * NB * NB;
+        for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) ds[e] = D[e % NB][e / NB];
+        return;
+    }
     T x[NB];
-    if ((int)blockIdx.x < nch) {
-        const int r = k0 + kb + blockIdx.x * PR + threadIdx.x;
+    if ((int)blockIdx.x <= nch) {
+        const int r = k0 + kb + (blockIdx.x - 1) * PR + threadIdx.x;
         if (r >= tk.n) return;
-        for (int c = 0; c < kb; c++) x[c] = F[(int64_t)(k0 + c) * n + r];
-        for (int c = 0; c < kb; c++) {
+#pragma unroll
+        for (int c = 0; c < NB; c++) x[c] = (c < kb) ? F[(int64_t)(k0 + c) * n + r] : (T)0;
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
             T s = x[c];
+#pragma unroll
             for (int q = 0; q < c; q++) s = s - x[q] * D[q][c];
             x[c] = s / D[c][c];
         }
-        for (int c = 0; c < kb; c++) F[(int64_t)(k0 + c) * n + r] = x[c];
+#pragma unroll
+        for (int c = 0; c < NB; c++)
+            if (c < kb) F[(int64_t)(k0 + c) * n + r] = x[c];
     } else {
-        const int col = k0 + kb + (blockIdx.x - nch) * PR + threadIdx.x;
+        const int col = k0 + kb + (blockIdx.x - 1 - nch) * PR + threadIdx.x;
         if (col >= tk.n) return;
         T *Fc = F + (int64_t)col * n + k0;
-        for (int r = 0; r < kb; r++) x[r] = Fc[r];
-        for (int r = 0; r < kb; r++) {
+#pragma unroll
+        for (int r = 0; r < NB; r++) x[r] = (r < kb) ? Fc[r] : (T)0;
+#pragma unroll
+        for (int r = 0; r < NB; r++) {
             T s = x[r];
+#pragma unroll
             for (int q = 0; q < r; q++) s = s - D[r][q] * x[q];
             x[r] = s;
         }
-        for (int r = 0; r < kb; r++) Fc[r] = x[r];
+#pragma unroll
+        for (int r = 0; r < NB; r++)
+            if (r < kb) Fc[r] = x[r];
     }
 }
 
@@ -240,10 +259,20 @@ __device__ __forceinline__ void gemm_tile_sub(T *C, const T *A, const T *B, int6
 
 // trailing update of panel k0 restricted to rows < p or columns < p
 template <typename T>
-__global__ void __launch_bounds__(256) k_update(const FrontTask *tasks, int k0, T *arena) {
+__global__ void __launch_bounds__(256) k_update(const FrontTask *tasks, int k0, T *arena,
+                                                const T *dscr) {
     const FrontTask tk = tasks[blockIdx.y];
     if (tk.p <= k0) return;
     const int kb = min(NB, tk.p - k0);
+    if (blockIdx.x == 0) {
+        // factored diagonal block of this panel back into the front
+        const T *ds = dscr + (int64_t)blockIdx.y * NB * NB;
+        T *F0 = arena + tk.off;
+        for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+            const int r = e % kb, c = e / kb;
+            F0[(int64_t)(k0 + c) * tk.n + k0 + r] = ds[c * NB + r];
+        }
+    }
     const int s = k0 + kb;                    // first trailing index
     const int tp = (tk.p - s + TM - 1) / TM;  // pivot-part tiles
     const int u = tk.n - tk.p;
@@ -288,19 +317,30 @@ __global__ void __launch_bounds__(256) k_schur(const FrontTask *tasks, T *arena)
 }
 
 // ------------------------------------------------------------ solves
-// forward: w = [z(pivots); 0] + sum_children extend(cv_child);
-//          y = L11^-1 w_p (stored back to z), cv = w_u - L21 y.
+// Forward sweep, leaves -> root.  Per level:
+//  k_fwd_piv (one workgroup per front): w = [z(pivots); 0] + sum of the
+//    children's contribution vectors (extend-add), y = L11^-1 w_p by blocks of
+//    SB columns (the SB x SB triangle in wavefront 0 with lane broadcasts, the
+//    rest of the pivot rows updated by the whole workgroup); y -> z, w -> wbuf.
+//  k_fwd_upd (row chunks of every front): cv = w_u - L21 y (in wbuf).
+// Backward sweep, root -> leaves.  Per level:
+//  k_bwd_upd (row chunks): w_p = y - U12 x_u (x_u final from ancestors).
+//  k_bwd_piv (one workgroup per front): x_p = U11^-1 w_p by blocks -> z.
+constexpr int SB = 32;
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd(const FrontTask *tasks, const int *fronts,
-                                             const int32_t *dofs, const int64_t *dof_off,
-                                             const int64_t *voff, const int *child_ptr,
-                                             const int *child_list, const int64_t *ea_off,
-                                             const int32_t *ea_map, const FrontTask *all_tasks,
-                                             const T *arena, double *z, double *cv) {
-    extern __shared__ double w[];
+__global__ void __launch_bounds__(256) k_fwd_piv(const FrontTask *tasks, const int *fronts,
+                                                 const int32_t *dofs, const int64_t *dof_off,
+                                                 const int64_t *voff, const int *child_ptr,
+                                                 const int *child_list, const int64_t *ea_off,
+                                                 const int32_t *ea_map, const FrontTask *all_tasks,
+                                                 const T *arena, double *z, double *wbuf) {
+    extern __shared__ double sm[];
     const int f = fronts[blockIdx.x];
     const FrontTask tk = tasks[blockIdx.x];
     const int n = tk.n, p = tk.p;
+    double *w = sm;
+    double *D = sm + ((n + 1) & ~1);   // SB x (SB + 1)
     const int32_t *dof = dofs + dof_off[f];
     for (int l = threadIdx.x; l < n; l += blockDim.x) w[l] = (l < p) ? z[dof[l]] : 0.0;
     __syncthreads();
@@ -308,52 +348,142 @@ __global__ void __launch_bounds__(256) k_fwd(const FrontTask *tasks, const int *
         const int ch = child_list[q];
         const FrontTask ct = all_tasks[ch];
         const int u = ct.n - ct.p;
-        const double *cvc = cv + voff[ch];
+        const double *cvc = wbuf + voff[ch] + ct.p;
         const int32_t *map = ea_map + ea_off[ch];
         for (int a = threadIdx.x; a < u; a += blockDim.x) w[map[a / 6] * 6 + a % 6] += cvc[a];
         __syncthreads();
     }
     const T *Fm = arena + tk.off;
-    for (int c = 0; c < p; c++) {
-        const double yc = w[c];
-        const T *col = Fm + (int64_t)c * n;
-        for (int r = c + 1 + threadIdx.x; r < n; r += blockDim.x) w[r] -= (double)col[r] * yc;
+    for (int c0 = 0; c0 < p; c0 += SB) {
+        const int cb = min(SB, p - c0);
+        for (int e = threadIdx.x; e < SB * SB; e += blockDim.x) {
+            const int r = e % SB, c = e / SB;
+            D[r * (SB + 1) + c] = (r < cb && c < cb) ? (double)Fm[(int64_t)(c0 + c) * n + c0 + r] : 0.0;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            double yr = (lane < cb) ? w[c0 + lane] : 0.0;
+#pragma unroll
+            for (int c = 0; c < SB; c++) {
+                const double yc = bcast(yr, c);
+                if (lane > c && lane < cb) yr -= D[lane * (SB + 1) + c] * yc;
+            }
+            if (lane < cb) w[c0 + lane] = yr;
+        }
+        __syncthreads();
+        for (int r = c0 + cb + threadIdx.x; r < p; r += blockDim.x) {
+            double s = w[r];
+            for (int c = 0; c < cb; c++) s -= (double)Fm[(int64_t)(c0 + c) * n + r] * w[c0 + c];
+            w[r] = s;
+        }
         __syncthreads();
     }
+    double *wb = wbuf + voff[f];
     for (int l = threadIdx.x; l < n; l += blockDim.x) {
+        wb[l] = w[l];
         if (l < p) z[dof[l]] = w[l];
-        else cv[voff[f] + (l - p)] = w[l];
     }
 }
 
-// backward: w_p = y_p - U12 x_u; x_p = U11^-1 w_p (stored to z)
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd(const FrontTask *tasks, const int *fronts,
-                                             const int32_t *dofs, const int64_t *dof_off,
-                                             const T *arena, double *z) {
-    extern __shared__ double w[];
+__global__ void __launch_bounds__(256) k_fwd_upd(const FrontTask *tasks, const int *fronts,
+                                                 const int64_t *voff, const T *arena, double *wbuf) {
+    extern __shared__ double y[];
+    const int f = fronts[blockIdx.y];
+    const FrontTask tk = tasks[blockIdx.y];
+    const int n = tk.n, p = tk.p, u = n - p;
+    const int r0 = blockIdx.x * 256;
+    if (r0 >= u) return;
+    double *wb = wbuf + voff[f];
+    for (int c = threadIdx.x; c < p; c += blockDim.x) y[c] = wb[c];
+    __syncthreads();
+    const int r = p + r0 + threadIdx.x;
+    if (r >= n) return;
+    const T *Fm = arena + tk.off;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int c = 0;
+    for (; c + 4 <= p; c += 4) {
+        s0 += (double)Fm[(int64_t)c * n + r] * y[c];
+        s1 += (double)Fm[(int64_t)(c + 1) * n + r] * y[c + 1];
+        s2 += (double)Fm[(int64_t)(c + 2) * n + r] * y[c + 2];
+        s3 += (double)Fm[(int64_t)(c + 3) * n + r] * y[c + 3];
+    }
+    for (; c < p; c++) s0 += (double)Fm[(int64_t)c * n + r] * y[c];
+    wb[r] = wb[r] - ((s0 + s1) + (s2 + s3));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_upd(const FrontTask *tasks, const int *fronts,
+                                                 const int32_t *dofs, const int64_t *dof_off,
+                                                 const int64_t *voff, const T *arena,
+                                                 const double *z, double *wbuf) {
+    extern __shared__ double xu[];
+    const int f = fronts[blockIdx.y];
+    const FrontTask tk = tasks[blockIdx.y];
+    const int n = tk.n, p = tk.p, u = n - p;
+    const int r0 = blockIdx.x * 256;
+    if (r0 >= p) return;
+    const int32_t *dof = dofs + dof_off[f];
+    for (int c = threadIdx.x; c < u; c += blockDim.x) xu[c] = z[dof[p + c]];
+    __syncthreads();
+    const int r = r0 + threadIdx.x;
+    if (r >= p) return;
+    const T *Fm = arena + tk.off + (int64_t)p * n;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int c = 0;
+    for (; c + 4 <= u; c += 4) {
+        s0 += (double)Fm[(int64_t)c * n + r] * xu[c];
+        s1 += (double)Fm[(int64_t)(c + 1) * n + r] * xu[c + 1];
+        s2 += (double)Fm[(int64_t)(c + 2) * n + r] * xu[c + 2];
+        s3 += (double)Fm[(int64_t)(c + 3) * n + r] * xu[c + 3];
+    }
+    for (; c < u; c++) s0 += (double)Fm[(int64_t)c * n + r] * xu[c];
+    wbuf[voff[f] + r] = z[dof[r]] - ((s0 + s1) + (s2 + s3));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_piv(const FrontTask *tasks, const int *fronts,
+                                                 const int32_t *dofs, const int64_t *dof_off,
+                                                 const int64_t *voff, const T *arena,
+                                                 const double *wbuf, double *z) {
+    extern __shared__ double sm[];
     const int f = fronts[blockIdx.x];
     const FrontTask tk = tasks[blockIdx.x];
     const int n = tk.n, p = tk.p;
+    double *w = sm;
+    double *D = sm + ((p + 1) & ~1);
     const int32_t *dof = dofs + dof_off[f];
+    const double *wb = wbuf + voff[f];
+    for (int l = threadIdx.x; l < p; l += blockDim.x) w[l] = wb[l];
+    __syncthreads();
     const T *Fm = arena + tk.off;
-    double *xu = w + p;
-    for (int l = threadIdx.x; l < n; l += blockDim.x) w[l] = z[dof[l]];
-    __syncthreads();
-    // w_p -= U12 x_u : row r of U12 is F[r, p:n]
-    for (int r = threadIdx.x; r < p; r += blockDim.x) {
-        double s = w[r];
-        for (int c = p; c < n; c++) s -= (double)Fm[(int64_t)c * n + r] * xu[c - p];
-        w[r] = s;
-    }
-    __syncthreads();
-    for (int c = p - 1; c >= 0; c--) {
-        if (threadIdx.x == 0) w[c] = w[c] / (double)Fm[(int64_t)c * n + c];
+    for (int c1 = p; c1 > 0;) {
+        const int c0 = max(0, c1 - SB), cb = c1 - c0;
+        for (int e = threadIdx.x; e < SB * SB; e += blockDim.x) {
+            const int r = e % SB, c = e / SB;
+            D[r * (SB + 1) + c] = (r < cb && c < cb) ? (double)Fm[(int64_t)(c0 + c) * n + c0 + r] : 1.0;
+        }
         __syncthreads();
-        const double xc = w[c];
-        const T *col = Fm + (int64_t)c * n;
-        for (int r = threadIdx.x; r < c; r += blockDim.x) w[r] -= (double)col[r] * xc;
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            double xr = (lane < cb) ? w[c0 + lane] : 0.0;
+#pragma unroll
+            for (int c = SB - 1; c >= 0; c--) {
+                if (lane == c) xr = xr / D[lane * (SB + 1) + c];
+                const double xc = bcast(xr, c);
+                if (lane < c) xr -= D[lane * (SB + 1) + c] * xc;
+            }
+            if (lane < cb) w[c0 + lane] = xr;
+        }
         __syncthreads();
+        for (int r = threadIdx.x; r < c0; r += blockDim.x) {
+            double s = w[r];
+            for (int c = 0; c < cb; c++) s -= (double)Fm[(int64_t)(c0 + c) * n + r] * w[c0 + c];
+            w[r] = s;
+        }
+        __syncthreads();
+        c1 = c0;
     }
     for (int l = threadIdx.x; l < p; l += blockDim.x) z[dof[l]] = w[l];
 }
@@ -414,7 +544,10 @@ struct SolverDev {
     int64_t *voff = nullptr, *ea_off = nullptr;
     double *norms = nullptr;
     double *h_norms = nullptr;
-    std::vector<int> ea_cnt[2];
+    double *dscr = nullptr;        // factored diagonal blocks of a panel step
+    int max_level_fronts = 0;
+    std::vector<int> maxu;         // per level max update size
+    std::vector<int> maxt_fwd, maxt_bwd;   // per level row-chunk counts
 };
 
 static SolverDev *sdev(hmec_ctx *c);
@@ -464,6 +597,7 @@ int sv_free_device(hmec_ctx *c) {
         hipFree(s->voff);
         hipFree(s->ea_off);
         hipFree(s->norms);
+        hipFree(s->dscr);
         if (s->h_norms) hipHostFree(s->h_norms);
         delete s;
     }
@@ -521,6 +655,19 @@ int sv_setup_device(hmec_ctx *c) {
         }
     }
     s->lvl_fronts_off[pl.nlevels] = (int64_t)flist.size();
+    s->max_level_fronts = 0;
+    s->maxu.assign(pl.nlevels, 0);
+    s->maxt_fwd.assign(pl.nlevels, 0);
+    s->maxt_bwd.assign(pl.nlevels, 0);
+    for (int lv = 0; lv < pl.nlevels; lv++) {
+        s->max_level_fronts = std::max<int>(s->max_level_fronts, (int)pl.levels[lv].size());
+        for (const auto &t : dp.h_tasks[lv]) {
+            s->maxu[lv] = std::max(s->maxu[lv], t.n - t.p);
+            s->maxt_fwd[lv] = std::max(s->maxt_fwd[lv], (t.n - t.p + 255) / 256);
+            s->maxt_bwd[lv] = std::max(s->maxt_bwd[lv], (t.p + 255) / 256);
+        }
+    }
+    HMEC_HIP(hipMalloc(&s->dscr, sizeof(double) * NB * NB * std::max(1, s->max_level_fronts)));
     HMEC_TRY(upload(&s->fronts, flist));
     dp.d_tasks.assign(pl.nlevels, nullptr);
     dp.d_ea[0].assign(pl.nlevels, nullptr);
@@ -632,14 +779,13 @@ static int factor_levels(hmec_ctx *c, T *arena, double tau) {
         }
         // pivot panels
         const int maxp = dp.maxp[lv], maxn = dp.maxn[lv];
+        SolverDev *s = sdev(c);
+        T *dscr = reinterpret_cast<T *>(s->dscr);
         for (int k0 = 0; k0 < maxp; k0 += NB) {
-            hipLaunchKernelGGL(k_diag_lu<T>, dim3(nfr), dim3(256), 0, st, dp.d_tasks[lv], k0,
-                               arena, tau, c->d.counters);
-            LAUNCH_CHECK();
             const int rest = maxn - k0;
             const int nch = std::max(1, (rest + PR - 1) / PR);
-            hipLaunchKernelGGL(k_panel<T>, dim3(2 * nch, nfr), dim3(PR), 0, st, dp.d_tasks[lv], k0,
-                               arena);
+            hipLaunchKernelGGL(k_panel<T>, dim3(1 + 2 * nch, nfr), dim3(PR), 0, st, dp.d_tasks[lv],
+                               k0, arena, dscr, tau, c->d.counters);
             LAUNCH_CHECK();
             int maxt = 0;
             for (const auto &t : dp.h_tasks[lv]) {
@@ -649,11 +795,9 @@ static int factor_levels(hmec_ctx *c, T *arena, double tau) {
                 const int tu = (t.n - t.p + TM - 1) / TM;
                 maxt = std::max(maxt, tp * (tp + tu) + tu * tp);
             }
-            if (maxt > 0) {
-                hipLaunchKernelGGL(k_update<T>, dim3(maxt, nfr), dim3(256), 0, st, dp.d_tasks[lv],
-                                   k0, arena);
-                LAUNCH_CHECK();
-            }
+            hipLaunchKernelGGL(k_update<T>, dim3(std::max(1, maxt), nfr), dim3(256), 0, st,
+                               dp.d_tasks[lv], k0, arena, (const T *)dscr);
+            LAUNCH_CHECK();
         }
         int maxt = 0;
         for (const auto &t : dp.h_tasks[lv]) {
@@ -695,22 +839,38 @@ static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
     DevPlan &dp = c->dp;
     SolverDev *s = sdev(c);
     hipStream_t st = c->stream;
+    double *wbuf = c->d.vec;
+    const size_t dsz = sizeof(double) * SB * (SB + 1);
     for (int lv = 0; lv < pl.nlevels; lv++) {
         const int nfr = (int)dp.h_tasks[lv].size();
-        const size_t sh = sizeof(double) * dp.maxn[lv];
-        hipLaunchKernelGGL(k_fwd<T>, dim3(nfr), dim3(256), sh, st, dp.d_tasks[lv],
-                           s->fronts + s->lvl_fronts_off[lv], dp.d_front_dofs, dp.d_front_dof_off,
-                           s->voff, s->child_ptr, s->child_list, s->ea_off, dp.d_ea_map,
-                           s->all_tasks, arena, z, c->d.vec);
+        const int *fr = s->fronts + s->lvl_fronts_off[lv];
+        kt_begin(c, "fwd");
+        hipLaunchKernelGGL(k_fwd_piv<T>, dim3(nfr), dim3(256),
+                           sizeof(double) * ((dp.maxn[lv] + 1) & ~1) + dsz, st, dp.d_tasks[lv], fr,
+                           dp.d_front_dofs, dp.d_front_dof_off, s->voff, s->child_ptr,
+                           s->child_list, s->ea_off, dp.d_ea_map, s->all_tasks, arena, z, wbuf);
         LAUNCH_CHECK();
+        if (s->maxt_fwd[lv] > 0) {
+            hipLaunchKernelGGL(k_fwd_upd<T>, dim3(s->maxt_fwd[lv], nfr), dim3(256),
+                               sizeof(double) * std::max(1, dp.maxp[lv]), st, dp.d_tasks[lv], fr,
+                               s->voff, arena, wbuf);
+            LAUNCH_CHECK();
+        }
+        kt_end(c, "fwd", 0);
     }
     for (int lv = pl.nlevels - 1; lv >= 0; lv--) {
         const int nfr = (int)dp.h_tasks[lv].size();
-        const size_t sh = sizeof(double) * dp.maxn[lv];
-        hipLaunchKernelGGL(k_bwd<T>, dim3(nfr), dim3(256), sh, st, dp.d_tasks[lv],
-                           s->fronts + s->lvl_fronts_off[lv], dp.d_front_dofs, dp.d_front_dof_off,
-                           arena, z);
+        const int *fr = s->fronts + s->lvl_fronts_off[lv];
+        kt_begin(c, "bwd");
+        hipLaunchKernelGGL(k_bwd_upd<T>, dim3(std::max(1, s->maxt_bwd[lv]), nfr), dim3(256),
+                           sizeof(double) * std::max(1, s->maxu[lv]), st, dp.d_tasks[lv], fr,
+                           dp.d_front_dofs, dp.d_front_dof_off, s->voff, arena, z, wbuf);
         LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_bwd_piv<T>, dim3(nfr), dim3(256),
+                           sizeof(double) * ((dp.maxp[lv] + 1) & ~1) + dsz, st, dp.d_tasks[lv], fr,
+                           dp.d_front_dofs, dp.d_front_dof_off, s->voff, arena, wbuf, z);
+        LAUNCH_CHECK();
+        kt_end(c, "bwd", 0);
     }
     return HMEC_OK;
 }

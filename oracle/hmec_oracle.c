@@ -193,10 +193,32 @@ static int pardiso_solve(orc *o) {
     return err;
 }
 
-/* Banded LU with partial pivoting (LAPACK dgbtrf/dgbtrs algorithm), used when
- * MKL is absent.  Band storage: ab[(kl+ku+1+kl) x n], column-major. */
+/* power of two nearest below 1/m (exact scaling) */
+static double pow2_recip(double m) {
+    if (!(m > 0) || !isfinite(m)) return 1.0;
+    int e;
+    frexp(m, &e);
+    return ldexp(1.0, -e + 1);
+}
+
+/* Banded LU with partial pivoting (LAPACK dgbtrf/dgbtrs algorithm) on the
+ * power-of-two equilibrated matrix, used when MKL is absent.  Without the
+ * equilibration partial pivoting's growth destroys the solution of the
+ * 101x51 system (LAPACK dgbsv fails the same way).  Band storage:
+ * ab[(kl+ku+1+kl) x n], column-major. */
 static int band_solve(orc *o) {
     const int n = o->N;
+    double *rs = malloc(sizeof(double) * n), *cs = malloc(sizeof(double) * n);
+    for (int r = 0; r < n; r++) {
+        double m = 0;
+        for (int p = o->rowp[r]; p < o->rowp[r + 1]; p++) m = dmax(m, fabs(o->val[p]));
+        rs[r] = pow2_recip(m);
+    }
+    for (int c = 0; c < n; c++) cs[c] = 0;
+    for (int r = 0; r < n; r++)
+        for (int p = o->rowp[r]; p < o->rowp[r + 1]; p++)
+            cs[o->coli[p]] = dmax(cs[o->coli[p]], fabs(o->val[p] * rs[r]));
+    for (int c = 0; c < n; c++) cs[c] = pow2_recip(cs[c]);
     int kl = 0, ku = 0;
     for (int r = 0; r < n; r++)
         for (int p = o->rowp[r]; p < o->rowp[r + 1]; p++) {
@@ -207,10 +229,11 @@ static int band_solve(orc *o) {
     const int ldab = 2 * kl + ku + 1, kv = kl + ku;
     double *ab = (double *)calloc((size_t)ldab * n, sizeof(double));
     int *ipiv = (int *)malloc(sizeof(int) * n);
-    if (!ab || !ipiv) { free(ab); free(ipiv); return -99; }
+    if (!ab || !ipiv) { free(ab); free(ipiv); free(rs); free(cs); return -99; }
 #define AB(r, c) ab[(size_t)(kv + (r) - (c)) + (size_t)(c) * ldab]
     for (int r = 0; r < n; r++)
-        for (int p = o->rowp[r]; p < o->rowp[r + 1]; p++) AB(r, o->coli[p]) += o->val[p];
+        for (int p = o->rowp[r]; p < o->rowp[r + 1]; p++)
+            AB(r, o->coli[p]) += o->val[p] * rs[r] * cs[o->coli[p]];
     int ju = 0, info = 0;
     for (int j = 0; j < n; j++) {
         int km = kl < n - 1 - j ? kl : n - 1 - j;
@@ -234,7 +257,7 @@ static int band_solve(orc *o) {
         }
     }
     double *b = o->S;
-    memcpy(b, o->R, sizeof(double) * n);
+    for (int r = 0; r < n; r++) b[r] = o->R[r] * rs[r];
     for (int j = 0; j < n; j++) {
         int km = kl < n - 1 - j ? kl : n - 1 - j;
         int l = ipiv[j];
@@ -247,8 +270,11 @@ static int band_solve(orc *o) {
         for (int r = top; r < j; r++) b[r] -= AB(r, j) * b[j];
     }
 #undef AB
+    for (int c = 0; c < n; c++) b[c] *= cs[c];
     free(ab);
     free(ipiv);
+    free(rs);
+    free(cs);
     return info ? -98 : 0;
 }
 
