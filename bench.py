@@ -1,0 +1,179 @@
+"""Benchmark of the H-MEC 2D per-timestep hot path on one MI355X.
+
+Workload: the 801x201-node precision configuration (972,024 unknowns,
+2,560,000 markers, 16 per cell; fault viscosity contrast up to 1e6,
+log-uniform permeability), synthetic seeded input generated on the host and
+uploaded once.  A step is one full timestep: marker->node scatter, then the
+Picard loop of VEP update, CSR assembly, sparse LU solve, RSF fault solve and
+timestep control, then the final stresses and history.
+
+value = timesteps/s of the FP32-factor + FP64-iterative-refinement path
+        (relative residual <= 1e-12 every solve); the plain FP64-factor path
+        (refactored every Picard iteration) is timed beside it.
+roofline = the dominant kernel of the FP32 path, measured with HIP events
+        on the context's stream over the timed region.
+cpu_baseline = the CPU oracle (plain-C restatement + MKL PARDISO, sequential)
+        on one timestep of the same workload, on this host.
+"""
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PEAK_HBM = 8.0e12          # B/s
+PEAK_FP64 = 78.6e12        # FLOP/s
+PEAK_FP32 = 157.3e12       # FLOP/s
+METRIC = "timesteps/s, FP32-LU+FP64 IR vs FP64-LU at rel.res ≤1e-10; p99 step latency"
+
+
+def load_hmec():
+    spec = importlib.util.spec_from_file_location("hmec", os.path.join(ROOT, "h-mecs-2d_amd", "hmec.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["hmec"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def run_path(hm, p, soa, prec, warmup, steps):
+    ctx = hm.Context(p, soa, precision=prec)
+    for _ in range(warmup):
+        ctx.step()
+    ctx.timing(True)
+    ctx.timing_reset()
+    lat = []
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        infos.append(ctx.step())      # returns after the step's last stream sync
+        lat.append(time.perf_counter() - a)
+    total = time.perf_counter() - t0
+    kt = {}
+    for name in ("scatter", "assemble", "factor", "solve", "schur", "fwd", "bwd", "fwd_upd",
+                 "bwd_upd", "extend"):
+        n, ms, work = ctx.kernel_time(name)
+        if n > 0:
+            kt[name] = {"n": n, "avg_ms": ms, "work": work}
+    return ctx, total, np.array(lat), infos, kt
+
+
+def cpu_baseline(hm, p, soa):
+    """One steady-state timestep of the oracle on the host (MKL sequential)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+    import oracle
+    oracle.build()
+    o = oracle.Oracle(p, soa, hm)
+    o.step()                       # untimed: first step (two Picard iterations, MKL init)
+    t = time.perf_counter()
+    info = o.step()
+    dt = time.perf_counter() - t
+    return {"value": 1.0 / dt, "unit": "timesteps/s", "cores": 1, "kind": "port",
+            "sample": "one steady-state timestep (%d Picard iteration) of the 801x201 workload, "
+                      "oracle restatement + MKL PARDISO sequential, after one untimed step"
+                      % info.iterations}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cfg", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each ensemble member "
+              "or grid runs on one device); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    hm = load_hmec()
+    p = hm.params(args.cfg)
+    soa = hm.generate_markers(p, cfg=args.cfg, seed=args.cfg)
+    ctx32, t32, lat32, inf32, kt32 = run_path(hm, p, soa, hm.PREC_FP32, args.warmup, args.steps)
+    res = {}
+    if not args.no_fp64:
+        ctx64, t64, lat64, inf64, kt64 = run_path(hm, p, soa, hm.PREC_FP64, args.warmup, args.steps)
+        errs = {}
+        for f in ("vxs", "vys", "pt", "pf", "vxD", "vyD", "SXX", "SYY", "SXY", "VSLIPB", "ETA", "OM"):
+            a, b = ctx32.get(f), ctx64.get(f)
+            nb = np.linalg.norm(b)
+            errs[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a - b))
+        res["fp64"] = {"timesteps_per_s": args.steps / t64,
+                       "p50_ms": float(np.percentile(lat64, 50) * 1e3),
+                       "p99_ms": float(np.percentile(lat64, 99) * 1e3),
+                       "factorizations": int(sum(i.factorizations for i in inf64)),
+                       "max_rel_residual": float(max(i.max_rel_residual for i in inf64))}
+        res["error_vs_fp64"] = errs
+        res["speedup_vs_fp64"] = (args.steps / t32) / (args.steps / t64)
+        iters_same = all(a.iterations == b.iterations for a, b in zip(inf32, inf64))
+        res["same_picard_iterations"] = bool(iters_same)
+    # roofline of the dominant kernel class of the FP32 path
+    cands = []
+    fac = kt32.get("schur")
+    if fac:
+        ach = fac["work"] / (fac["avg_ms"] * 1e-3)
+        cands.append((fac["n"] * fac["avg_ms"], {"kernel": "k_schur<float>", "bound": "mfma",
+                                                   "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
+                                                   "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
+                                                   "traffic": None}))
+    for nm, kn in (("fwd_upd", "k_fwd_upd<float>"), ("bwd_upd", "k_bwd_upd<float>")):
+        k = kt32.get(nm)
+        if k:
+            ach = k["work"] / (k["avg_ms"] * 1e-3)
+            cands.append((k["n"] * k["avg_ms"], {"kernel": kn, "bound": "hbm", "achieved": ach / 1e9,
+                                                  "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                                                  "frac": ach / PEAK_HBM, "traffic": None}))
+    sc = kt32.get("scatter")
+    if sc:
+        ach = sc["work"] / (sc["avg_ms"] * 1e-3)
+        cands.append((sc["n"] * sc["avg_ms"], {"kernel": "k_scatter_grid (4 grids)", "bound": "hbm",
+                                                "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9,
+                                                "unit": "GB/s", "frac": ach / PEAK_HBM,
+                                                "traffic": None}))
+    roof = max(cands, key=lambda c: c[0])[1] if cands else None
+    value = args.steps / t32
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-factor+f64",
+        "data": "synthetic (seeded host generator, uploaded once)",
+        "config": {"workload": "801x201 nodes, 972024 unknowns, 2560000 markers (precision config)",
+                   "grid": [p.Nx, p.Ny], "unknowns": (p.Nx + 1) * (p.Ny + 1) * 6,
+                   "markers": int(soa.shape[1]), "parallelism": "1 GPU"},
+        "p50_ms": float(np.percentile(lat32, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat32, 99) * 1e3),
+        "fp32_ir": {"factorizations": int(sum(i.factorizations for i in inf32)),
+                    "ir_iterations": int(sum(i.ir_iterations for i in inf32)),
+                    "fallbacks": int(sum(i.fallbacks for i in inf32)),
+                    "max_rel_residual": float(max(i.max_rel_residual for i in inf32)),
+                    "picard_iterations": int(sum(i.iterations for i in inf32))},
+        "kernels_fp32": {k: {"n": v["n"], "avg_ms": v["avg_ms"]} for k, v in kt32.items()},
+        "roofline": roof,
+    }
+    out.update(res)
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(hm, p, soa)
+        except Exception as e:     # reported, never fatal for the GPU number
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -89,15 +89,18 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         if (rc != HMEC_OK && rc != HMEC_E_NOCONV) return rc;
     } else {
         bool fresh = false;
-        if (!c->factor_valid || c->factor_prec != HMEC_PREC_FP32) {
+        // dt enters every viscoelastic coefficient and the pressure scaling:
+        // a factor of another dt is a poor preconditioner, refactor at once
+        if (!c->factor_valid || c->factor_prec != HMEC_PREC_FP32 || c->factor_dt != c->dt) {
             tm_begin(c, c->t_factor);
             HMEC_TRY(sv_factorize(c, HMEC_PREC_FP32));
             tm_end(c, c->t_factor, c->plan.flops);
             acc.factorizations++;
+            c->factor_dt = c->dt;
             fresh = true;
         }
         tm_begin(c, c->t_solve);
-        int rc = sv_solve(c, fresh ? 30 : 12, 1e-12, &st);
+        int rc = sv_solve(c, fresh ? 30 : 8, 1e-12, &st);
         tm_end(c, c->t_solve, 0);
         acc.ir_iterations += st.ir_iterations;
         if (rc == HMEC_E_NOCONV && !fresh) {
@@ -105,6 +108,7 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
             HMEC_TRY(sv_factorize(c, HMEC_PREC_FP32));
             tm_end(c, c->t_factor, c->plan.flops);
             acc.factorizations++;
+            c->factor_dt = c->dt;
             tm_begin(c, c->t_solve);
             rc = sv_solve(c, 30, 1e-12, &st);
             tm_end(c, c->t_solve, 0);

@@ -80,6 +80,8 @@ struct FrontTask {
     int64_t off;    // arena offset of the front
     int n;          // front order
     int p;          // pivots
+    int64_t doff;   // offset of the inverted diagonal blocks (2 x NB x NB per NB pivots)
+    int64_t poff;   // offset of the front's partial sums in the solve scratch (per level)
 };
 
 struct EaTask {       // extend-add of one child CB into its parent
@@ -182,6 +184,7 @@ struct hmec_ctx {
     int last_ir = 0, last_fallback = 0, last_perturbed = 0;
     double last_res = 0;
     int stale_solves = 0;            // FP32: solves since the last factorization
+    double factor_dt = -1;           // dt of the system the factor was computed for
     // statistics of the current step
     hmec_step_info acc;
     // timing

@@ -1,5 +1,6 @@
 // Host-only entry points of the symbolic phase (no GPU needed): the fixed
 // CSR pattern of the coupled system and the nested-dissection front plan.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -53,6 +54,39 @@ extern "C" int hmec_plan_info(const hmec_params *p, hmec_solve_stats *st, int64_
             for (int f : pl.levels[lv])
                 for (int l = 0; l < pl.np[f]; l++)
                     for (int q = 0; q < 6; q++) elim_order[k++] = pl.nodes[f][l] * 6 + inv[q];
+    }
+    return HMEC_OK;
+}
+
+// Per level of the front plan: number of fronts, largest front order and
+// pivot count, and the dense flops of the level (arrays of nlevels entries;
+// call with NULL arrays to get nlevels).
+extern "C" int hmec_plan_levels(const hmec_params *p, int *nlevels, int *nfronts, int *maxn,
+                                int *maxp, double *flops) {
+    if (!p || !nlevels) return HMEC_E_ARG;
+    hmec_params P = *p;
+    HMEC_TRY(hmec_params_finalize(&P));
+    std::vector<int32_t> rp, ci;
+    HMEC_TRY(hmec_pattern_host(P, rp, ci, nullptr, nullptr));
+    FrontPlan pl;
+    HMEC_TRY(hmec_plan_build(P.Nx + 1, P.Ny + 1, rp, ci, pl));
+    *nlevels = pl.nlevels;
+    for (int lv = 0; lv < pl.nlevels; lv++) {
+        int mn = 0, mp = 0;
+        double fl = 0;
+        for (int f : pl.levels[lv]) {
+            const int n = 6 * (pl.np[f] + pl.nu[f]), pp = 6 * pl.np[f];
+            mn = std::max(mn, n);
+            mp = std::max(mp, pp);
+            for (int k = 0; k < pp; k++) {
+                const double r = (double)(n - k - 1);
+                fl += r + 2.0 * r * r;
+            }
+        }
+        if (nfronts) nfronts[lv] = (int)pl.levels[lv].size();
+        if (maxn) maxn[lv] = mn;
+        if (maxp) maxp[lv] = mp;
+        if (flops) flops[lv] = fl;
     }
     return HMEC_OK;
 }

@@ -118,17 +118,40 @@ __device__ __forceinline__ float bcast(float v, int k) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
+// Inverse of the upper (U, with diagonal) or unit-lower (L) triangle of the
+// factored NB x NB block D, one column per thread of the first NB threads.
+template <typename T>
+__device__ __forceinline__ void invert_tri(T (*D)[NB + 1], T (*Inv)[NB + 1], bool upper) {
+    if (threadIdx.x >= NB) return;
+    const int c = threadIdx.x;
+    if (upper) {
+        for (int r = NB - 1; r >= 0; r--) {
+            T s = (r == c) ? (T)1 : (T)0;
+            for (int q = r + 1; q <= c; q++) s = s - D[r][q] * Inv[q][c];
+            Inv[r][c] = (r <= c) ? s / D[r][r] : (T)0;
+        }
+    } else {
+        for (int r = 0; r < NB; r++) {
+            T s = (r == c) ? (T)1 : (T)0;
+            for (int q = c; q < r; q++) s = s - D[r][q] * Inv[q][c];
+            Inv[r][c] = (r >= c) ? s : (T)0;
+        }
+    }
+}
+
 // Panel k0 of every front of a level (no row exchanges):
-//   every workgroup loads the kb x kb diagonal block D, and wavefront 0
-//   factors it in registers (row per lane) with static-pivot perturbation;
-//   workgroup 0 stores the factored D in dscr (copied into the front by the
-//   following update launch, so no workgroup of this launch reads a half-
-//   written D);  row workgroups solve X U_D = A for rows [k0+kb, n),
-//   column workgroups solve L_D Y = A for columns [k0+kb, n).
+//   every workgroup loads the kb x kb diagonal block D and factors it in LDS
+//   (static-pivot perturbation), then inverts its triangles; workgroup 0
+//   stores the factored D in dscr (copied into the front by the following
+//   update launch, so no workgroup of this launch reads a half-written D);
+//   row workgroups form X = A U_D^-1 for rows [k0+kb, n), column workgroups
+//   Y = L_D^-1 A for columns [k0+kb, n) (products with the inverted
+//   triangles: no serial chain per row).
 template <typename T>
 __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, T *arena, T *dscr,
-                                              double tau, int *npert) {
+                                              T *dinv, double tau, int *npert) {
     __shared__ T D[NB][NB + 1];
+    __shared__ T Inv[NB][NB + 1];     // U_D^-1 (upper) for row blocks, L_D^-1 (unit lower) for columns
     const FrontTask tk = tasks[blockIdx.y];
     if (tk.p <= k0) return;
     const int kb = min(NB, tk.p - k0);
@@ -137,93 +160,116 @@ __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, T 
     const int nch = (rest + PR - 1) / PR;
     if ((int)blockIdx.x >= 1 + 2 * nch) return;
     T *F = arena + tk.off;
-    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+    const int tid = threadIdx.x;
+    for (int e = tid; e < NB * NB; e += blockDim.x) {
         const int r = e % NB, c = e / NB;
         D[r][c] = (r < kb && c < kb) ? F[(int64_t)(k0 + c) * n + k0 + r] : (r == c ? (T)1 : (T)0);
     }
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        T row[NB];
-#pragma unroll
-        for (int j = 0; j < NB; j++) row[j] = (lane < NB) ? D[lane][j] : (T)0;
-        int pert = 0;
-#pragma unroll
-        for (int k = 0; k < NB; k++) {
-            T piv = bcast(row[k], k);
-            if (!(fabs((double)piv) >= tau)) {
-                piv = (piv < (T)0) ? (T)(-tau) : (T)tau;
-                pert++;
-                if (lane == k) row[k] = piv;
-            }
-            const T l = row[k] / piv;
-            if (lane > k) row[k] = l;
-#pragma unroll
-            for (int j = k + 1; j < NB; j++) {
-                const T rk = bcast(row[j], k);
-                if (lane > k) row[j] = row[j] - l * rk;
+    for (int k = 0; k < kb; k++) {
+        if (tid == 0) {
+            T d = D[k][k];
+            if (!(fabs((double)d) >= tau)) {
+                D[k][k] = (d < (T)0) ? (T)(-tau) : (T)tau;
+                if (blockIdx.x == 0) atomicAdd(npert, 1);
             }
         }
-        if (lane < NB) {
-#pragma unroll
-            for (int j = 0; j < NB; j++) D[lane][j] = row[j];
+        __syncthreads();
+        const T piv = D[k][k];
+        if (tid > k && tid < kb) D[tid][k] = D[tid][k] / piv;
+        __syncthreads();
+        const int m = kb - k - 1;
+        for (int e = tid; e < m * m; e += blockDim.x) {
+            const int r = k + 1 + e % m, c = k + 1 + e / m;
+            D[r][c] = D[r][c] - D[r][k] * D[k][c];
         }
-        if (lane == 0 && blockIdx.x == 0 && pert) atomicAdd(npert, pert);
+        __syncthreads();
     }
-    __syncthreads();
     if (blockIdx.x == 0) {
+        // factored D for the update launch, and both inverted triangles of
+        // this diagonal block for the solves (block k0/NB of the front:
+        // [0] = L^-1 unit lower, [1] = U^-1 upper)
         T *ds = dscr + (int64_t)blockIdx.y * NB * NB;
-        for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) ds[e] = D[e % NB][e / NB];
+        for (int e = tid; e < NB * NB; e += blockDim.x) ds[e] = D[e % NB][e / NB];
+        T *dst = dinv + tk.doff + (int64_t)(k0 / NB) * 2 * NB * NB;
+        invert_tri<T>(D, Inv, true);
+        __syncthreads();
+        for (int e = tid; e < NB * NB; e += blockDim.x) dst[NB * NB + e] = Inv[e % NB][e / NB];
+        __syncthreads();
+        invert_tri<T>(D, Inv, false);
+        __syncthreads();
+        for (int e = tid; e < NB * NB; e += blockDim.x) dst[e] = Inv[e % NB][e / NB];
         return;
     }
-    T x[NB];
-    if ((int)blockIdx.x <= nch) {
-        const int r = k0 + kb + (blockIdx.x - 1) * PR + threadIdx.x;
+    const bool rows = (int)blockIdx.x <= nch;
+    invert_tri<T>(D, Inv, rows);
+    __syncthreads();
+    T a[NB];
+    if (rows) {
+        const int r = k0 + kb + (blockIdx.x - 1) * PR + tid;
         if (r >= tk.n) return;
 #pragma unroll
-        for (int c = 0; c < NB; c++) x[c] = (c < kb) ? F[(int64_t)(k0 + c) * n + r] : (T)0;
+        for (int q = 0; q < NB; q++) a[q] = (q < kb) ? F[(int64_t)(k0 + q) * n + r] : (T)0;
+        for (int c = 0; c < kb; c++) {
+            T s = (T)0;
 #pragma unroll
-        for (int c = 0; c < NB; c++) {
-            T s = x[c];
-#pragma unroll
-            for (int q = 0; q < c; q++) s = s - x[q] * D[q][c];
-            x[c] = s / D[c][c];
+            for (int q = 0; q < NB; q++) s = fma(a[q], Inv[q][c], s);
+            F[(int64_t)(k0 + c) * n + r] = s;
         }
-#pragma unroll
-        for (int c = 0; c < NB; c++)
-            if (c < kb) F[(int64_t)(k0 + c) * n + r] = x[c];
     } else {
-        const int col = k0 + kb + (blockIdx.x - 1 - nch) * PR + threadIdx.x;
+        const int col = k0 + kb + (blockIdx.x - 1 - nch) * PR + tid;
         if (col >= tk.n) return;
         T *Fc = F + (int64_t)col * n + k0;
 #pragma unroll
-        for (int r = 0; r < NB; r++) x[r] = (r < kb) ? Fc[r] : (T)0;
+        for (int q = 0; q < NB; q++) a[q] = (q < kb) ? Fc[q] : (T)0;
+        for (int r = 0; r < kb; r++) {
+            T s = (T)0;
 #pragma unroll
-        for (int r = 0; r < NB; r++) {
-            T s = x[r];
-#pragma unroll
-            for (int q = 0; q < r; q++) s = s - D[r][q] * x[q];
-            x[r] = s;
+            for (int q = 0; q < NB; q++) s = fma(Inv[r][q], a[q], s);
+            Fc[r] = s;
         }
-#pragma unroll
-        for (int r = 0; r < NB; r++)
-            if (r < kb) Fc[r] = x[r];
     }
 }
 
 // ------------------------------------------------------------ GEMM tile
 // C(m x nn) -= A(m x K) * B(K x nn); column-major with leading dimension ld.
+// 64 x 64 output tile per workgroup of 4 wavefronts (2 x 2 of 32 x 32), each
+// wavefront a 2 x 2 grid of 16 x 16 MFMA blocks (v_mfma_*_16x16x4: lane l
+// supplies A(l%16, l/16) and B(l/16, l%16); element r of the result holds
+// D(Mfma<T>::row(l, r), l%16): row 4*(l/16)+r for f32, 4*r+l/16 for f64, as
+// measured on gfx950).  K is streamed through LDS in chunks of TK.
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef v4d acc_t;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+template <> struct Mfma<float> {
+    typedef v4f acc_t;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
 template <typename T>
 __device__ __forceinline__ void gemm_tile_sub(T *C, const T *A, const T *B, int64_t ld, int m,
                                               int nn, int K) {
-    __shared__ T As[TK][TM + 1];
-    __shared__ T Bs[TK][TM + 1];
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;   // 16 x 16 threads, 4 x 4 each
-    T acc[4][4];
+    typedef typename Mfma<T>::acc_t acc_t;
+    __shared__ T As[TK][TM + 4];
+    __shared__ T Bs[TK][TM + 4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    acc_t acc[2][2];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = (T)0;
+        for (int b = 0; b < 2; b++) acc[a][b] = acc_t{0, 0, 0, 0};
     for (int kk = 0; kk < K; kk += TK) {
         const int kc = min(TK, K - kk);
         for (int e = threadIdx.x; e < TK * TM; e += blockDim.x) {
@@ -234,27 +280,25 @@ __device__ __forceinline__ void gemm_tile_sub(T *C, const T *A, const T *B, int6
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < TK; k++) {
-            T a[4], b[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                a[q] = As[k][tx + 16 * q];
-                b[q] = Bs[k][ty + 16 * q];
-            }
-#pragma unroll
-            for (int qa = 0; qa < 4; qa++)
-#pragma unroll
-                for (int qb = 0; qb < 4; qb++) acc[qa][qb] = fma(a[qa], b[qb], acc[qa][qb]);
+        for (int k4 = 0; k4 < TK; k4 += 4) {
+            const T a0 = As[k4 + lk][wr + li], a1 = As[k4 + lk][wr + 16 + li];
+            const T b0 = Bs[k4 + lk][wc + li], b1 = Bs[k4 + lk][wc + 16 + li];
+            acc[0][0] = Mfma<T>::mma(a0, b0, acc[0][0]);
+            acc[0][1] = Mfma<T>::mma(a0, b1, acc[0][1]);
+            acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
+            acc[1][1] = Mfma<T>::mma(a1, b1, acc[1][1]);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int qa = 0; qa < 4; qa++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int qb = 0; qb < 4; qb++) {
-            const int r = tx + 16 * qa, c = ty + 16 * qb;
-            if (r < m && c < nn) C[(int64_t)c * ld + r] -= acc[qa][qb];
-        }
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = wr + a * 16 + Mfma<T>::row(lane, r), col = wc + b * 16 + li;
+                if (row < m && col < nn) C[(int64_t)col * ld + row] -= acc[a][b][r];
+            }
 }
 
 // trailing update of panel k0 restricted to rows < p or columns < p
@@ -299,7 +343,7 @@ __global__ void __launch_bounds__(256) k_update(const FrontTask *tasks, int k0, 
                      n, m, nn, kb);
 }
 
-// Schur complement CB -= L21 U12 (K = p)
+// Schur complement CB -= L21 U12 (K = p), 64 x 64 MFMA tiles
 template <typename T>
 __global__ void __launch_bounds__(256) k_schur(const FrontTask *tasks, T *arena) {
     const FrontTask tk = tasks[blockIdx.y];
@@ -318,31 +362,35 @@ __global__ void __launch_bounds__(256) k_schur(const FrontTask *tasks, T *arena)
 
 // ------------------------------------------------------------ solves
 // Forward sweep, leaves -> root.  Per level:
-//  k_fwd_piv (one workgroup per front): w = [z(pivots); 0] + sum of the
-//    children's contribution vectors (extend-add), y = L11^-1 w_p by blocks of
-//    SB columns (the SB x SB triangle in wavefront 0 with lane broadcasts, the
-//    rest of the pivot rows updated by the whole workgroup); y -> z, w -> wbuf.
-//  k_fwd_upd (row chunks of every front): cv = w_u - L21 y (in wbuf).
+//  k_fwd_piv (one 1024-thread workgroup per front): w = [z(pivots); 0] plus
+//    the children's contribution vectors (extend-add); y = L11^-1 w_p block
+//    by block with the inverted NB x NB diagonal blocks kept by the panel
+//    factorization, the rest of the pivot rows updated after each block;
+//    y -> z, w -> wbuf.
+//  k_gemv_part + k_fwd_reduce: cv = w_u - L21 y, split over row and column
+//    chunks, partial sums reduced in a fixed order (deterministic).
 // Backward sweep, root -> leaves.  Per level:
-//  k_bwd_upd (row chunks): w_p = y - U12 x_u (x_u final from ancestors).
-//  k_bwd_piv (one workgroup per front): x_p = U11^-1 w_p by blocks -> z.
-constexpr int SB = 32;
+//  k_gemv_part + k_bwd_reduce: w_p = y - U12 x_u (x_u final from ancestors).
+//  k_bwd_piv: x_p = U11^-1 w_p block by block -> z.
+constexpr int GR = 256;    // rows per GEMV workgroup
+constexpr int GC = 256;    // columns per GEMV workgroup
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_piv(const FrontTask *tasks, const int *fronts,
-                                                 const int32_t *dofs, const int64_t *dof_off,
-                                                 const int64_t *voff, const int *child_ptr,
-                                                 const int *child_list, const int64_t *ea_off,
-                                                 const int32_t *ea_map, const FrontTask *all_tasks,
-                                                 const T *arena, double *z, double *wbuf) {
+__global__ void __launch_bounds__(1024) k_fwd_piv(const FrontTask *tasks, const int *fronts,
+                                                  const int32_t *dofs, const int64_t *dof_off,
+                                                  const int64_t *voff, const int *child_ptr,
+                                                  const int *child_list, const int64_t *ea_off,
+                                                  const int32_t *ea_map, const FrontTask *all_tasks,
+                                                  const T *arena, const T *dinv, double *z,
+                                                  double *wbuf) {
     extern __shared__ double sm[];
+    __shared__ double yb[NB];
     const int f = fronts[blockIdx.x];
     const FrontTask tk = tasks[blockIdx.x];
-    const int n = tk.n, p = tk.p;
+    const int n = tk.n, p = tk.p, tid = threadIdx.x;
     double *w = sm;
-    double *D = sm + ((n + 1) & ~1);   // SB x (SB + 1)
     const int32_t *dof = dofs + dof_off[f];
-    for (int l = threadIdx.x; l < n; l += blockDim.x) w[l] = (l < p) ? z[dof[l]] : 0.0;
+    for (int l = tid; l < n; l += blockDim.x) w[l] = (l < p) ? z[dof[l]] : 0.0;
     __syncthreads();
     for (int q = child_ptr[f]; q < child_ptr[f + 1]; q++) {
         const int ch = child_list[q];
@@ -350,142 +398,131 @@ __global__ void __launch_bounds__(256) k_fwd_piv(const FrontTask *tasks, const i
         const int u = ct.n - ct.p;
         const double *cvc = wbuf + voff[ch] + ct.p;
         const int32_t *map = ea_map + ea_off[ch];
-        for (int a = threadIdx.x; a < u; a += blockDim.x) w[map[a / 6] * 6 + a % 6] += cvc[a];
+        for (int a = tid; a < u; a += blockDim.x) w[map[a / 6] * 6 + a % 6] += cvc[a];
         __syncthreads();
     }
     const T *Fm = arena + tk.off;
-    for (int c0 = 0; c0 < p; c0 += SB) {
-        const int cb = min(SB, p - c0);
-        for (int e = threadIdx.x; e < SB * SB; e += blockDim.x) {
-            const int r = e % SB, c = e / SB;
-            D[r * (SB + 1) + c] = (r < cb && c < cb) ? (double)Fm[(int64_t)(c0 + c) * n + c0 + r] : 0.0;
+    const T *Di = dinv + tk.doff;
+    for (int c0 = 0; c0 < p; c0 += NB) {
+        const int cb = min(NB, p - c0);
+        const T *Li = Di + (int64_t)(c0 / NB) * 2 * NB * NB;    // L^-1, column-major NB x NB
+        if (tid < cb) {
+            double y = 0;
+            for (int q = 0; q <= tid; q++) y += (double)Li[q * NB + tid] * w[c0 + q];
+            yb[tid] = y;
         }
         __syncthreads();
-        if (threadIdx.x < 64) {
-            const int lane = threadIdx.x;
-            double yr = (lane < cb) ? w[c0 + lane] : 0.0;
-#pragma unroll
-            for (int c = 0; c < SB; c++) {
-                const double yc = bcast(yr, c);
-                if (lane > c && lane < cb) yr -= D[lane * (SB + 1) + c] * yc;
-            }
-            if (lane < cb) w[c0 + lane] = yr;
-        }
-        __syncthreads();
-        for (int r = c0 + cb + threadIdx.x; r < p; r += blockDim.x) {
+        if (tid < cb) w[c0 + tid] = yb[tid];
+        for (int r = c0 + cb + tid; r < p; r += blockDim.x) {
             double s = w[r];
-            for (int c = 0; c < cb; c++) s -= (double)Fm[(int64_t)(c0 + c) * n + r] * w[c0 + c];
+            for (int c = 0; c < cb; c++) s -= (double)Fm[(int64_t)(c0 + c) * n + r] * yb[c];
             w[r] = s;
         }
         __syncthreads();
     }
     double *wb = wbuf + voff[f];
-    for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    for (int l = tid; l < n; l += blockDim.x) {
         wb[l] = w[l];
         if (l < p) z[dof[l]] = w[l];
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_upd(const FrontTask *tasks, const int *fronts,
-                                                 const int64_t *voff, const T *arena, double *wbuf) {
-    extern __shared__ double y[];
+// partial[cc][r] = sum over the column chunk cc of M[r, c] x[c], M a
+// rows x cols block of the front at (row0, col0).  fwd: rows = update rows,
+// cols = pivots, x = y (wbuf); bwd: rows = pivots, cols = update, x = z(dofs).
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(GR) k_gemv_part(const FrontTask *tasks, const int *fronts,
+                                                  const int32_t *dofs, const int64_t *dof_off,
+                                                  const int64_t *voff, const T *arena,
+                                                  const double *z, const double *wbuf,
+                                                  double *part) {
+    __shared__ double xs[GC];
     const int f = fronts[blockIdx.y];
     const FrontTask tk = tasks[blockIdx.y];
     const int n = tk.n, p = tk.p, u = n - p;
-    const int r0 = blockIdx.x * 256;
-    if (r0 >= u) return;
-    double *wb = wbuf + voff[f];
-    for (int c = threadIdx.x; c < p; c += blockDim.x) y[c] = wb[c];
+    const int rows = FWD ? u : p, cols = FWD ? p : u;
+    if (rows <= 0 || cols <= 0) return;
+    const int nrc = (rows + GR - 1) / GR, ncc = (cols + GC - 1) / GC;
+    if ((int)blockIdx.x >= nrc * ncc) return;
+    const int rc = blockIdx.x % nrc, cc = blockIdx.x / nrc;
+    const int c0 = cc * GC, cn = min(GC, cols - c0);
+    const int tid = threadIdx.x;
+    if (tid < cn) {
+        if (FWD) xs[tid] = wbuf[voff[f] + c0 + tid];
+        else xs[tid] = z[dofs[dof_off[f] + p + c0 + tid]];
+    }
     __syncthreads();
-    const int r = p + r0 + threadIdx.x;
-    if (r >= n) return;
-    const T *Fm = arena + tk.off;
+    const int r = rc * GR + tid;
+    if (r >= rows) return;
+    const int row = FWD ? p + r : r, col0 = FWD ? c0 : p + c0;
+    const T *M = arena + tk.off + (int64_t)col0 * n + row;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     int c = 0;
-    for (; c + 4 <= p; c += 4) {
-        s0 += (double)Fm[(int64_t)c * n + r] * y[c];
-        s1 += (double)Fm[(int64_t)(c + 1) * n + r] * y[c + 1];
-        s2 += (double)Fm[(int64_t)(c + 2) * n + r] * y[c + 2];
-        s3 += (double)Fm[(int64_t)(c + 3) * n + r] * y[c + 3];
+    for (; c + 4 <= cn; c += 4) {
+        s0 += (double)M[(int64_t)c * n] * xs[c];
+        s1 += (double)M[(int64_t)(c + 1) * n] * xs[c + 1];
+        s2 += (double)M[(int64_t)(c + 2) * n] * xs[c + 2];
+        s3 += (double)M[(int64_t)(c + 3) * n] * xs[c + 3];
     }
-    for (; c < p; c++) s0 += (double)Fm[(int64_t)c * n + r] * y[c];
-    wb[r] = wb[r] - ((s0 + s1) + (s2 + s3));
+    for (; c < cn; c++) s0 += (double)M[(int64_t)c * n] * xs[c];
+    part[tk.poff + (int64_t)cc * rows + r] = (s0 + s1) + (s2 + s3);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_upd(const FrontTask *tasks, const int *fronts,
-                                                 const int32_t *dofs, const int64_t *dof_off,
-                                                 const int64_t *voff, const T *arena,
-                                                 const double *z, double *wbuf) {
-    extern __shared__ double xu[];
+// fwd: cv = w_u - sum_cc partial  (in wbuf);  bwd: w_p = z(pivots) - sum_cc partial
+template <bool FWD>
+__global__ void __launch_bounds__(256) k_gemv_reduce(const FrontTask *tasks, const int *fronts,
+                                                     const int32_t *dofs, const int64_t *dof_off,
+                                                     const int64_t *voff, const double *z,
+                                                     double *wbuf, const double *part) {
     const int f = fronts[blockIdx.y];
     const FrontTask tk = tasks[blockIdx.y];
     const int n = tk.n, p = tk.p, u = n - p;
-    const int r0 = blockIdx.x * 256;
-    if (r0 >= p) return;
-    const int32_t *dof = dofs + dof_off[f];
-    for (int c = threadIdx.x; c < u; c += blockDim.x) xu[c] = z[dof[p + c]];
-    __syncthreads();
-    const int r = r0 + threadIdx.x;
-    if (r >= p) return;
-    const T *Fm = arena + tk.off + (int64_t)p * n;
-    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    int c = 0;
-    for (; c + 4 <= u; c += 4) {
-        s0 += (double)Fm[(int64_t)c * n + r] * xu[c];
-        s1 += (double)Fm[(int64_t)(c + 1) * n + r] * xu[c + 1];
-        s2 += (double)Fm[(int64_t)(c + 2) * n + r] * xu[c + 2];
-        s3 += (double)Fm[(int64_t)(c + 3) * n + r] * xu[c + 3];
-    }
-    for (; c < u; c++) s0 += (double)Fm[(int64_t)c * n + r] * xu[c];
-    wbuf[voff[f] + r] = z[dof[r]] - ((s0 + s1) + (s2 + s3));
+    const int rows = FWD ? u : p, cols = FWD ? p : u;
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
+    double s = 0;
+    const int ncc = (cols + GC - 1) / GC;
+    for (int cc = 0; cc < ncc; cc++) s += part[tk.poff + (int64_t)cc * rows + r];
+    if (FWD) wbuf[voff[f] + p + r] -= s;
+    else wbuf[voff[f] + r] = z[dofs[dof_off[f] + r]] - s;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_piv(const FrontTask *tasks, const int *fronts,
-                                                 const int32_t *dofs, const int64_t *dof_off,
-                                                 const int64_t *voff, const T *arena,
-                                                 const double *wbuf, double *z) {
+__global__ void __launch_bounds__(1024) k_bwd_piv(const FrontTask *tasks, const int *fronts,
+                                                  const int32_t *dofs, const int64_t *dof_off,
+                                                  const int64_t *voff, const T *arena,
+                                                  const T *dinv, const double *wbuf, double *z) {
     extern __shared__ double sm[];
+    __shared__ double xb[NB];
     const int f = fronts[blockIdx.x];
     const FrontTask tk = tasks[blockIdx.x];
-    const int n = tk.n, p = tk.p;
+    const int n = tk.n, p = tk.p, tid = threadIdx.x;
     double *w = sm;
-    double *D = sm + ((p + 1) & ~1);
     const int32_t *dof = dofs + dof_off[f];
     const double *wb = wbuf + voff[f];
-    for (int l = threadIdx.x; l < p; l += blockDim.x) w[l] = wb[l];
+    for (int l = tid; l < p; l += blockDim.x) w[l] = wb[l];
     __syncthreads();
     const T *Fm = arena + tk.off;
-    for (int c1 = p; c1 > 0;) {
-        const int c0 = max(0, c1 - SB), cb = c1 - c0;
-        for (int e = threadIdx.x; e < SB * SB; e += blockDim.x) {
-            const int r = e % SB, c = e / SB;
-            D[r * (SB + 1) + c] = (r < cb && c < cb) ? (double)Fm[(int64_t)(c0 + c) * n + c0 + r] : 1.0;
+    const T *Di = dinv + tk.doff;
+    const int nblk = (p + NB - 1) / NB;
+    for (int b = nblk - 1; b >= 0; b--) {
+        const int c0 = b * NB, cb = min(NB, p - c0);
+        const T *Ui = Di + (int64_t)b * 2 * NB * NB + NB * NB;   // U^-1, column-major NB x NB
+        if (tid < cb) {
+            double x = 0;
+            for (int q = tid; q < cb; q++) x += (double)Ui[q * NB + tid] * w[c0 + q];
+            xb[tid] = x;
         }
         __syncthreads();
-        if (threadIdx.x < 64) {
-            const int lane = threadIdx.x;
-            double xr = (lane < cb) ? w[c0 + lane] : 0.0;
-#pragma unroll
-            for (int c = SB - 1; c >= 0; c--) {
-                if (lane == c) xr = xr / D[lane * (SB + 1) + c];
-                const double xc = bcast(xr, c);
-                if (lane < c) xr -= D[lane * (SB + 1) + c] * xc;
-            }
-            if (lane < cb) w[c0 + lane] = xr;
-        }
-        __syncthreads();
-        for (int r = threadIdx.x; r < c0; r += blockDim.x) {
+        if (tid < cb) w[c0 + tid] = xb[tid];
+        for (int r = tid; r < c0; r += blockDim.x) {
             double s = w[r];
-            for (int c = 0; c < cb; c++) s -= (double)Fm[(int64_t)(c0 + c) * n + r] * w[c0 + c];
+            for (int c = 0; c < cb; c++) s -= (double)Fm[(int64_t)(c0 + c) * n + r] * xb[c];
             w[r] = s;
         }
         __syncthreads();
-        c1 = c0;
     }
-    for (int l = threadIdx.x; l < p; l += blockDim.x) z[dof[l]] = w[l];
+    for (int l = tid; l < p; l += blockDim.x) z[dof[l]] = w[l];
 }
 
 __global__ void k_scale_vec(const double *a, const double *s, double *out, int64_t N) {
@@ -545,6 +582,12 @@ struct SolverDev {
     double *norms = nullptr;
     double *h_norms = nullptr;
     double *dscr = nullptr;        // factored diagonal blocks of a panel step
+    double *dinv64 = nullptr;      // inverted diagonal blocks (FP64 factor)
+    float *dinv32 = nullptr;       // inverted diagonal blocks (FP32 factor)
+    double *part = nullptr;        // GEMV partial sums of one level
+    int64_t dinv_size = 0;
+    std::vector<int> gemv_fwd_grid, gemv_bwd_grid;   // per level grid.x
+    std::vector<double> schur_flops, fwdupd_entries, bwdupd_entries;   // per level work
     int max_level_fronts = 0;
     std::vector<int> maxu;         // per level max update size
     std::vector<int> maxt_fwd, maxt_bwd;   // per level row-chunk counts
@@ -598,6 +641,9 @@ int sv_free_device(hmec_ctx *c) {
         hipFree(s->ea_off);
         hipFree(s->norms);
         hipFree(s->dscr);
+        hipFree(s->dinv64);
+        hipFree(s->dinv32);
+        hipFree(s->part);
         if (s->h_norms) hipHostFree(s->h_norms);
         delete s;
     }
@@ -625,8 +671,31 @@ int sv_setup_device(hmec_ctx *c) {
     dp.nlevels = pl.nlevels;
     // all tasks (indexed by front id)
     std::vector<FrontTask> all(pl.nfronts);
-    for (int f = 0; f < pl.nfronts; f++)
-        all[f] = FrontTask{pl.off[f], 6 * (pl.np[f] + pl.nu[f]), 6 * pl.np[f]};
+    int64_t doff = 0;
+    for (int f = 0; f < pl.nfronts; f++) {
+        const int p = 6 * pl.np[f];
+        all[f] = FrontTask{pl.off[f], 6 * (pl.np[f] + pl.nu[f]), p, doff, 0};
+        doff += (int64_t)((p + NB - 1) / NB) * 2 * NB * NB;
+    }
+    s->dinv_size = doff;
+    // per level partial-sum offsets of the split GEMVs (scratch reused by levels)
+    int64_t part_max = 0;
+    s->gemv_fwd_grid.assign(pl.nlevels, 0);
+    s->gemv_bwd_grid.assign(pl.nlevels, 0);
+    for (int lv = 0; lv < pl.nlevels; lv++) {
+        int64_t po = 0;
+        for (int f : pl.levels[lv]) {
+            const int n = all[f].n, p = all[f].p, u = n - p;
+            all[f].poff = po;
+            po += (int64_t)std::max((p + GC - 1) / GC * u, (u + GC - 1) / GC * p);
+            if (u > 0 && p > 0) {
+                s->gemv_fwd_grid[lv] = std::max(s->gemv_fwd_grid[lv], ((u + GR - 1) / GR) * ((p + GC - 1) / GC));
+                s->gemv_bwd_grid[lv] = std::max(s->gemv_bwd_grid[lv], ((p + GR - 1) / GR) * ((u + GC - 1) / GC));
+            }
+        }
+        part_max = std::max(part_max, po);
+    }
+    HMEC_HIP(hipMalloc(&s->part, sizeof(double) * std::max<int64_t>(1, part_max)));
     HMEC_TRY(upload(&s->all_tasks, all));
     std::vector<int> flist;
     dp.h_tasks.assign(pl.nlevels, {});
@@ -659,7 +728,16 @@ int sv_setup_device(hmec_ctx *c) {
     s->maxu.assign(pl.nlevels, 0);
     s->maxt_fwd.assign(pl.nlevels, 0);
     s->maxt_bwd.assign(pl.nlevels, 0);
+    s->schur_flops.assign(pl.nlevels, 0);
+    s->fwdupd_entries.assign(pl.nlevels, 0);
+    s->bwdupd_entries.assign(pl.nlevels, 0);
     for (int lv = 0; lv < pl.nlevels; lv++) {
+        for (const auto &t : dp.h_tasks[lv]) {
+            const double u = t.n - t.p, pp = t.p;
+            s->schur_flops[lv] += 2.0 * u * u * pp;
+            s->fwdupd_entries[lv] += u * pp;
+            s->bwdupd_entries[lv] += u * pp;
+        }
         s->max_level_fronts = std::max<int>(s->max_level_fronts, (int)pl.levels[lv].size());
         for (const auto &t : dp.h_tasks[lv]) {
             s->maxu[lv] = std::max(s->maxu[lv], t.n - t.p);
@@ -729,6 +807,11 @@ int sv_setup_device(hmec_ctx *c) {
 
 static int ensure_arena(hmec_ctx *c, int precision) {
     const int64_t n = c->plan.arena;
+    SolverDev *s = sdev(c);
+    if (precision == HMEC_PREC_FP64 && !s->dinv64)
+        HMEC_HIP(hipMalloc(&s->dinv64, sizeof(double) * std::max<int64_t>(1, s->dinv_size)));
+    if (precision == HMEC_PREC_FP32 && !s->dinv32)
+        HMEC_HIP(hipMalloc(&s->dinv32, sizeof(float) * std::max<int64_t>(1, s->dinv_size)));
     if (precision == HMEC_PREC_FP64 && !c->d.arena) {
         if (hipMalloc(&c->d.arena, sizeof(double) * n) != hipSuccess) {
             hipGetLastError();
@@ -747,7 +830,7 @@ static int ensure_arena(hmec_ctx *c, int precision) {
 }
 
 template <typename T>
-static int factor_levels(hmec_ctx *c, T *arena, double tau) {
+static int factor_levels(hmec_ctx *c, T *arena, T *dinv, double tau) {
     const FrontPlan &pl = c->plan;
     DevPlan &dp = c->dp;
     hipStream_t st = c->stream;
@@ -785,7 +868,7 @@ static int factor_levels(hmec_ctx *c, T *arena, double tau) {
             const int rest = maxn - k0;
             const int nch = std::max(1, (rest + PR - 1) / PR);
             hipLaunchKernelGGL(k_panel<T>, dim3(1 + 2 * nch, nfr), dim3(PR), 0, st, dp.d_tasks[lv],
-                               k0, arena, dscr, tau, c->d.counters);
+                               k0, arena, dscr, dinv, tau, c->d.counters);
             LAUNCH_CHECK();
             int maxt = 0;
             for (const auto &t : dp.h_tasks[lv]) {
@@ -805,8 +888,10 @@ static int factor_levels(hmec_ctx *c, T *arena, double tau) {
             maxt = std::max(maxt, tu * tu);
         }
         if (maxt > 0) {
+            kt_begin(c, "schur");
             hipLaunchKernelGGL(k_schur<T>, dim3(maxt, nfr), dim3(256), 0, st, dp.d_tasks[lv], arena);
             LAUNCH_CHECK();
+            kt_end(c, "schur", sdev(c)->schur_flops[lv]);
         }
     }
     return HMEC_OK;
@@ -825,8 +910,9 @@ int sv_factorize(hmec_ctx *c, int precision) {
                        c->dp.d_csc_ptr, c->dp.d_csc_idx, c->dp.d_csc_row, c->d.val, c->d.rscale, N,
                        c->d.cscale);
     LAUNCH_CHECK();
-    if (precision == HMEC_PREC_FP64) HMEC_TRY(factor_levels<double>(c, c->d.arena, 1e-13));
-    else HMEC_TRY(factor_levels<float>(c, c->d.arena32, 1e-6));
+    SolverDev *s = sdev(c);
+    if (precision == HMEC_PREC_FP64) HMEC_TRY(factor_levels<double>(c, c->d.arena, s->dinv64, 1e-13));
+    else HMEC_TRY(factor_levels<float>(c, c->d.arena32, s->dinv32, 1e-6));
     c->factor_prec = precision;
     c->factor_valid = true;
     c->stale_solves = 0;
@@ -834,26 +920,31 @@ int sv_factorize(hmec_ctx *c, int precision) {
 }
 
 template <typename T>
-static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
+static int tri_solve(hmec_ctx *c, const T *arena, const T *dinv, double *z) {
     const FrontPlan &pl = c->plan;
     DevPlan &dp = c->dp;
     SolverDev *s = sdev(c);
     hipStream_t st = c->stream;
     double *wbuf = c->d.vec;
-    const size_t dsz = sizeof(double) * SB * (SB + 1);
     for (int lv = 0; lv < pl.nlevels; lv++) {
         const int nfr = (int)dp.h_tasks[lv].size();
         const int *fr = s->fronts + s->lvl_fronts_off[lv];
         kt_begin(c, "fwd");
-        hipLaunchKernelGGL(k_fwd_piv<T>, dim3(nfr), dim3(256),
-                           sizeof(double) * ((dp.maxn[lv] + 1) & ~1) + dsz, st, dp.d_tasks[lv], fr,
-                           dp.d_front_dofs, dp.d_front_dof_off, s->voff, s->child_ptr,
-                           s->child_list, s->ea_off, dp.d_ea_map, s->all_tasks, arena, z, wbuf);
+        hipLaunchKernelGGL(k_fwd_piv<T>, dim3(nfr), dim3(1024), sizeof(double) * dp.maxn[lv], st,
+                           dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff,
+                           s->child_ptr, s->child_list, s->ea_off, dp.d_ea_map, s->all_tasks, arena,
+                           dinv, z, wbuf);
         LAUNCH_CHECK();
-        if (s->maxt_fwd[lv] > 0) {
-            hipLaunchKernelGGL(k_fwd_upd<T>, dim3(s->maxt_fwd[lv], nfr), dim3(256),
-                               sizeof(double) * std::max(1, dp.maxp[lv]), st, dp.d_tasks[lv], fr,
-                               s->voff, arena, wbuf);
+        if (s->gemv_fwd_grid[lv] > 0) {
+            kt_begin(c, "fwd_upd");
+            hipLaunchKernelGGL((k_gemv_part<T, true>), dim3(s->gemv_fwd_grid[lv], nfr), dim3(GR), 0,
+                               st, dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff,
+                               arena, z, wbuf, s->part);
+            LAUNCH_CHECK();
+            kt_end(c, "fwd_upd", s->fwdupd_entries[lv] * sizeof(T));
+            hipLaunchKernelGGL(k_gemv_reduce<true>, dim3((s->maxu[lv] + 255) / 256, nfr), dim3(256),
+                               0, st, dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off,
+                               s->voff, z, wbuf, s->part);
             LAUNCH_CHECK();
         }
         kt_end(c, "fwd", 0);
@@ -862,13 +953,22 @@ static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
         const int nfr = (int)dp.h_tasks[lv].size();
         const int *fr = s->fronts + s->lvl_fronts_off[lv];
         kt_begin(c, "bwd");
-        hipLaunchKernelGGL(k_bwd_upd<T>, dim3(std::max(1, s->maxt_bwd[lv]), nfr), dim3(256),
-                           sizeof(double) * std::max(1, s->maxu[lv]), st, dp.d_tasks[lv], fr,
-                           dp.d_front_dofs, dp.d_front_dof_off, s->voff, arena, z, wbuf);
+        if (s->gemv_bwd_grid[lv] > 0) {
+            kt_begin(c, "bwd_upd");
+            hipLaunchKernelGGL((k_gemv_part<T, false>), dim3(s->gemv_bwd_grid[lv], nfr), dim3(GR), 0,
+                               st, dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff,
+                               arena, z, wbuf, s->part);
+            LAUNCH_CHECK();
+            kt_end(c, "bwd_upd", s->bwdupd_entries[lv] * sizeof(T));
+        }
+        // w_p = z(pivots) - partials (plain copy where the front has no update set)
+        hipLaunchKernelGGL(k_gemv_reduce<false>, dim3((dp.maxp[lv] + 255) / 256, nfr), dim3(256), 0,
+                           st, dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, z,
+                           wbuf, s->part);
         LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bwd_piv<T>, dim3(nfr), dim3(256),
-                           sizeof(double) * ((dp.maxp[lv] + 1) & ~1) + dsz, st, dp.d_tasks[lv], fr,
-                           dp.d_front_dofs, dp.d_front_dof_off, s->voff, arena, wbuf, z);
+        hipLaunchKernelGGL(k_bwd_piv<T>, dim3(nfr), dim3(1024), sizeof(double) * dp.maxp[lv], st,
+                           dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, arena,
+                           dinv, wbuf, z);
         LAUNCH_CHECK();
         kt_end(c, "bwd", 0);
     }
@@ -893,8 +993,8 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
     for (;; it++) {
         hipLaunchKernelGGL(k_scale_vec, dim3(g), dim3(256), 0, st, r, c->d.rscale, z, N);
         LAUNCH_CHECK();
-        if (c->factor_prec == HMEC_PREC_FP64) HMEC_TRY(tri_solve<double>(c, c->d.arena, z));
-        else HMEC_TRY(tri_solve<float>(c, c->d.arena32, z));
+        if (c->factor_prec == HMEC_PREC_FP64) HMEC_TRY(tri_solve<double>(c, c->d.arena, s->dinv64, z));
+        else HMEC_TRY(tri_solve<float>(c, c->d.arena32, s->dinv32, z));
         hipLaunchKernelGGL(k_axpy_scaled, dim3(g), dim3(256), 0, st, x, c->d.cscale, z, N,
                            it == 0 ? 1 : 0);
         LAUNCH_CHECK();

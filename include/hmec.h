@@ -278,6 +278,10 @@ int hmec_step(hmec_ctx *ctx, hmec_step_info *info);
 int hmec_pattern(const hmec_params *p, int64_t *n, int64_t *nnz, int32_t *row_ptr, int32_t *col_idx);
 int hmec_plan_info(const hmec_params *p, hmec_solve_stats *stats, int64_t *arena_entries,
                    int32_t *elim_order);
+/* per level of the plan: fronts, largest front order, largest pivot count and
+ * dense flops (arrays of *nlevels entries; NULL arrays return nlevels only) */
+int hmec_plan_levels(const hmec_params *p, int *nlevels, int *nfronts, int *max_front,
+                     int *max_pivots, double *flops);
 /* text of the last error on this thread (HIP call, plan check, ...) */
 const char *hmec_last_error(void);
 
