@@ -158,7 +158,10 @@ __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, T 
     const int64_t n = tk.n;
     const int rest = tk.n - k0 - kb;
     const int nch = (rest + PR - 1) / PR;
-    if ((int)blockIdx.x >= 1 + 2 * nch) return;
+    // a front whose trailing rows fit one workgroup is done by workgroup 0
+    // alone (one diagonal factorization instead of three)
+    const bool solo = nch <= 1;
+    if ((int)blockIdx.x >= (solo ? 1 : 1 + 2 * nch)) return;
     T *F = arena + tk.off;
     const int tid = threadIdx.x;
     for (int e = tid; e < NB * NB; e += blockDim.x) {
@@ -199,6 +202,37 @@ __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, T 
         invert_tri<T>(D, Inv, false);
         __syncthreads();
         for (int e = tid; e < NB * NB; e += blockDim.x) dst[e] = Inv[e % NB][e / NB];
+        if (!solo) return;
+        // solo: columns with L^-1 (in Inv now), then rows with U^-1
+        __syncthreads();
+        const int col = k0 + kb + tid;
+        if (col < tk.n) {
+            T a[NB];
+            T *Fc = F + (int64_t)col * n + k0;
+#pragma unroll
+            for (int q = 0; q < NB; q++) a[q] = (q < kb) ? Fc[q] : (T)0;
+            for (int r = 0; r < kb; r++) {
+                T s = (T)0;
+#pragma unroll
+                for (int q = 0; q < NB; q++) s = fma(Inv[r][q], a[q], s);
+                Fc[r] = s;
+            }
+        }
+        __syncthreads();
+        invert_tri<T>(D, Inv, true);
+        __syncthreads();
+        const int r = k0 + kb + tid;
+        if (r < tk.n) {
+            T a[NB];
+#pragma unroll
+            for (int q = 0; q < NB; q++) a[q] = (q < kb) ? F[(int64_t)(k0 + q) * n + r] : (T)0;
+            for (int c = 0; c < kb; c++) {
+                T s = (T)0;
+#pragma unroll
+                for (int q = 0; q < NB; q++) s = fma(a[q], Inv[q][c], s);
+                F[(int64_t)(k0 + c) * n + r] = s;
+            }
+        }
         return;
     }
     const bool rows = (int)blockIdx.x <= nch;
@@ -376,7 +410,7 @@ constexpr int GR = 256;    // rows per GEMV workgroup
 constexpr int GC = 256;    // columns per GEMV workgroup
 
 template <typename T>
-__global__ void __launch_bounds__(1024) k_fwd_piv(const FrontTask *tasks, const int *fronts,
+__global__ void __launch_bounds__(256) k_fwd_piv(const FrontTask *tasks, const int *fronts,
                                                   const int32_t *dofs, const int64_t *dof_off,
                                                   const int64_t *voff, const int *child_ptr,
                                                   const int *child_list, const int64_t *ea_off,
@@ -406,17 +440,37 @@ __global__ void __launch_bounds__(1024) k_fwd_piv(const FrontTask *tasks, const 
     for (int c0 = 0; c0 < p; c0 += NB) {
         const int cb = min(NB, p - c0);
         const T *Li = Di + (int64_t)(c0 / NB) * 2 * NB * NB;    // L^-1, column-major NB x NB
+        if (tid >= cb && tid < NB) yb[tid] = 0.0;
         if (tid < cb) {
             double y = 0;
-            for (int q = 0; q <= tid; q++) y += (double)Li[q * NB + tid] * w[c0 + q];
+#pragma unroll
+            for (int q = 0; q < NB; q++)
+                if (q <= tid) y += (double)Li[q * NB + tid] * w[c0 + q];
             yb[tid] = y;
         }
         __syncthreads();
         if (tid < cb) w[c0 + tid] = yb[tid];
         for (int r = c0 + cb + tid; r < p; r += blockDim.x) {
-            double s = w[r];
-            for (int c = 0; c < cb; c++) s -= (double)Fm[(int64_t)(c0 + c) * n + r] * yb[c];
-            w[r] = s;
+            // all NB loads issued before the dependent sums
+            double s0 = 0, s1 = 0;
+            const T *col = Fm + (int64_t)c0 * n + r;
+            if (cb == NB) {
+                // full block: NB independent loads, converted at use
+#pragma unroll
+                for (int c8 = 0; c8 < NB; c8 += 16) {
+                    T l[16];
+#pragma unroll
+                    for (int c = 0; c < 16; c++) l[c] = col[(int64_t)(c8 + c) * n];
+#pragma unroll
+                    for (int c = 0; c < 16; c += 2) {
+                        s0 += (double)l[c] * yb[c8 + c];
+                        s1 += (double)l[c + 1] * yb[c8 + c + 1];
+                    }
+                }
+            } else {
+                for (int c = 0; c < cb; c++) s0 += (double)col[(int64_t)c * n] * yb[c];
+            }
+            w[r] -= s0 + s1;
         }
         __syncthreads();
     }
@@ -458,11 +512,14 @@ __global__ void __launch_bounds__(GR) k_gemv_part(const FrontTask *tasks, const 
     const T *M = arena + tk.off + (int64_t)col0 * n + row;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     int c = 0;
-    for (; c + 4 <= cn; c += 4) {
-        s0 += (double)M[(int64_t)c * n] * xs[c];
-        s1 += (double)M[(int64_t)(c + 1) * n] * xs[c + 1];
-        s2 += (double)M[(int64_t)(c + 2) * n] * xs[c + 2];
-        s3 += (double)M[(int64_t)(c + 3) * n] * xs[c + 3];
+    for (; c + 8 <= cn; c += 8) {
+        T l[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) l[q] = M[(int64_t)(c + q) * n];
+        s0 += (double)l[0] * xs[c] + (double)l[4] * xs[c + 4];
+        s1 += (double)l[1] * xs[c + 1] + (double)l[5] * xs[c + 5];
+        s2 += (double)l[2] * xs[c + 2] + (double)l[6] * xs[c + 6];
+        s3 += (double)l[3] * xs[c + 3] + (double)l[7] * xs[c + 7];
     }
     for (; c < cn; c++) s0 += (double)M[(int64_t)c * n] * xs[c];
     part[tk.poff + (int64_t)cc * rows + r] = (s0 + s1) + (s2 + s3);
@@ -488,7 +545,7 @@ __global__ void __launch_bounds__(256) k_gemv_reduce(const FrontTask *tasks, con
 }
 
 template <typename T>
-__global__ void __launch_bounds__(1024) k_bwd_piv(const FrontTask *tasks, const int *fronts,
+__global__ void __launch_bounds__(256) k_bwd_piv(const FrontTask *tasks, const int *fronts,
                                                   const int32_t *dofs, const int64_t *dof_off,
                                                   const int64_t *voff, const T *arena,
                                                   const T *dinv, const double *wbuf, double *z) {
@@ -508,17 +565,36 @@ __global__ void __launch_bounds__(1024) k_bwd_piv(const FrontTask *tasks, const 
     for (int b = nblk - 1; b >= 0; b--) {
         const int c0 = b * NB, cb = min(NB, p - c0);
         const T *Ui = Di + (int64_t)b * 2 * NB * NB + NB * NB;   // U^-1, column-major NB x NB
+        if (tid >= cb && tid < NB) xb[tid] = 0.0;
         if (tid < cb) {
             double x = 0;
-            for (int q = tid; q < cb; q++) x += (double)Ui[q * NB + tid] * w[c0 + q];
+#pragma unroll
+            for (int q = 0; q < NB; q++)
+                if (q >= tid && q < cb) x += (double)Ui[q * NB + tid] * w[c0 + q];
             xb[tid] = x;
         }
         __syncthreads();
         if (tid < cb) w[c0 + tid] = xb[tid];
         for (int r = tid; r < c0; r += blockDim.x) {
-            double s = w[r];
-            for (int c = 0; c < cb; c++) s -= (double)Fm[(int64_t)(c0 + c) * n + r] * xb[c];
-            w[r] = s;
+            double s0 = 0, s1 = 0;
+            const T *col = Fm + (int64_t)c0 * n + r;
+            if (cb == NB) {
+                // full block: NB independent loads, converted at use
+#pragma unroll
+                for (int c8 = 0; c8 < NB; c8 += 16) {
+                    T l[16];
+#pragma unroll
+                    for (int c = 0; c < 16; c++) l[c] = col[(int64_t)(c8 + c) * n];
+#pragma unroll
+                    for (int c = 0; c < 16; c += 2) {
+                        s0 += (double)l[c] * xb[c8 + c];
+                        s1 += (double)l[c + 1] * xb[c8 + c + 1];
+                    }
+                }
+            } else {
+                for (int c = 0; c < cb; c++) s0 += (double)col[(int64_t)c * n] * xb[c];
+            }
+            w[r] -= s0 + s1;
         }
         __syncthreads();
     }
@@ -865,9 +941,14 @@ static int factor_levels(hmec_ctx *c, T *arena, T *dinv, double tau) {
         SolverDev *s = sdev(c);
         T *dscr = reinterpret_cast<T *>(s->dscr);
         for (int k0 = 0; k0 < maxp; k0 += NB) {
-            const int rest = maxn - k0;
-            const int nch = std::max(1, (rest + PR - 1) / PR);
-            hipLaunchKernelGGL(k_panel<T>, dim3(1 + 2 * nch, nfr), dim3(PR), 0, st, dp.d_tasks[lv],
+            int gx = 1;
+            for (const auto &t : dp.h_tasks[lv]) {
+                if (t.p <= k0) continue;
+                const int kb = std::min(NB, t.p - k0);
+                const int nch = (t.n - k0 - kb + PR - 1) / PR;
+                gx = std::max(gx, nch <= 1 ? 1 : 1 + 2 * nch);
+            }
+            hipLaunchKernelGGL(k_panel<T>, dim3(gx, nfr), dim3(PR), 0, st, dp.d_tasks[lv],
                                k0, arena, dscr, dinv, tau, c->d.counters);
             LAUNCH_CHECK();
             int maxt = 0;
@@ -929,8 +1010,9 @@ static int tri_solve(hmec_ctx *c, const T *arena, const T *dinv, double *z) {
     for (int lv = 0; lv < pl.nlevels; lv++) {
         const int nfr = (int)dp.h_tasks[lv].size();
         const int *fr = s->fronts + s->lvl_fronts_off[lv];
+        const int bs = dp.maxp[lv] <= 96 ? 64 : 256;
         kt_begin(c, "fwd");
-        hipLaunchKernelGGL(k_fwd_piv<T>, dim3(nfr), dim3(1024), sizeof(double) * dp.maxn[lv], st,
+        hipLaunchKernelGGL(k_fwd_piv<T>, dim3(nfr), dim3(bs), sizeof(double) * dp.maxn[lv], st,
                            dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff,
                            s->child_ptr, s->child_list, s->ea_off, dp.d_ea_map, s->all_tasks, arena,
                            dinv, z, wbuf);
@@ -966,7 +1048,8 @@ static int tri_solve(hmec_ctx *c, const T *arena, const T *dinv, double *z) {
                            st, dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, z,
                            wbuf, s->part);
         LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bwd_piv<T>, dim3(nfr), dim3(1024), sizeof(double) * dp.maxp[lv], st,
+        const int bs = dp.maxp[lv] <= 96 ? 64 : 256;
+        hipLaunchKernelGGL(k_bwd_piv<T>, dim3(nfr), dim3(bs), sizeof(double) * dp.maxp[lv], st,
                            dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, arena,
                            dinv, wbuf, z);
         LAUNCH_CHECK();

@@ -152,3 +152,63 @@ def test_steps_cfg1(hmec, orc, gpu, nsteps):
         assert abs(a.dt - b.dt) <= 1e-12 * b.dt, s
         for name, (e, tol) in group_errors(ctx, o).items():
             assert e < tol, (s, name, e)
+
+
+def prestress(obj, p, frac=0.98, mu=0.6):
+    """Uniform pre-stress at `frac` of mu * (PT0 - PF0): the fault goes
+    coseismic on the second step (plastic Picard iterations, dt collapse)."""
+    tau = frac * mu * p.PTFDIFF
+    obj.set("SXY0", np.full((p.Ny, p.Nx), tau))
+    obj.set("SXY", np.full((p.Ny, p.Nx), tau))
+
+
+def test_prestressed_fault_cfg1(hmec, orc, gpu):
+    """Coseismic steps (10 plastic Picard iterations, dt from 1e7 s to 6e-4 s):
+    the GPU path against the oracle with PARDISO, bounded by the oracle's own
+    solver noise (PARDISO vs the oracle's banded LU on the same input).  The
+    plastic viscosity dt*G*A/(1-A) with A -> 1 amplifies rounding, so the two
+    CPU solvers already differ by ~1e-6 there; every group must stay within
+    max(1e-10, 10 x that floor), iteration counts and dt must agree."""
+    p, soa, ctx, o = make(hmec, orc, 1)
+    ob = orc.Oracle(p, soa, hmec, use_pardiso=False)
+    for x in (ctx, o, ob):
+        prestress(x, p)
+    plastic = 0
+    for s in range(4):
+        a = ctx.step()
+        b = o.step()
+        c = ob.step()
+        assert a.iterations == b.iterations == c.iterations, (s, a.iterations, b.iterations)
+        assert abs(a.dt - b.dt) <= 1e-9 * b.dt, (s, a.dt, b.dt)
+        plastic += b.iterations > 1
+        floor = group_errors_pair(o, ob)
+        for name, (e, tol) in group_errors(ctx, o).items():
+            assert e <= max(tol, 10 * floor[name]), (s, name, e, floor[name])
+    assert plastic >= 3
+
+
+def group_errors_pair(x, y):
+    out = {}
+    for name, (fields, tol) in GROUPS.items():
+        ga = np.concatenate([x.get(f).ravel() for f in fields])
+        gb = np.concatenate([y.get(f).ravel() for f in fields])
+        out[name] = rel(ga, gb)
+    return out
+
+
+def test_advection_step(hmec, orc, gpu):
+    """RK4 marker advection switched on (the reference's loop body with
+    markers moving): markers, marker stresses and the next scatter agree."""
+    p = hmec.params(1, advect_markers=1)
+    soa = hmec.generate_markers(p, cfg=1, seed=7)
+    ctx = hmec.Context(p, soa)
+    o = orc.Oracle(p, soa, hmec)
+    for _ in range(2):
+        ctx.step()
+        o.step()
+    mg = ctx.get_markers()
+    for f in (0, 1, 16, 17, 18):
+        mo = o.marker(f)
+        scale = max(np.abs(mo).max(), 1e-300)
+        assert np.abs(mg[f] - mo).max() <= 1e-9 * scale, f
+    assert np.array_equal(ctx.marker_cells(), o.marker_cells())
