@@ -88,10 +88,12 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         tm_end(c, c->t_solve, 0);
         if (rc != HMEC_OK && rc != HMEC_E_NOCONV) return rc;
     } else {
+        // FP32: GMRES preconditioned by the FP32 factor, reused across Picard
+        // iterations and timesteps (also across dt changes) while GMRES
+        // converges within its budget; otherwise refactor in FP32 and retry;
+        // a fresh FP32 factor that still fails falls back to an FP64 factor.
         bool fresh = false;
-        // dt enters every viscoelastic coefficient and the pressure scaling:
-        // a factor of another dt is a poor preconditioner, refactor at once
-        if (!c->factor_valid || c->factor_prec != HMEC_PREC_FP32 || c->factor_dt != c->dt) {
+        if (!c->factor_valid || c->factor_prec != HMEC_PREC_FP32) {
             tm_begin(c, c->t_factor);
             HMEC_TRY(sv_factorize(c, HMEC_PREC_FP32));
             tm_end(c, c->t_factor, c->plan.flops);
@@ -100,7 +102,7 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
             fresh = true;
         }
         tm_begin(c, c->t_solve);
-        int rc = sv_solve(c, fresh ? 30 : 8, 1e-12, &st);
+        int rc = sv_gmres(c, fresh ? 40 : 12, 1e-12, &st);
         tm_end(c, c->t_solve, 0);
         acc.ir_iterations += st.ir_iterations;
         if (rc == HMEC_E_NOCONV && !fresh) {
@@ -110,7 +112,7 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
             acc.factorizations++;
             c->factor_dt = c->dt;
             tm_begin(c, c->t_solve);
-            rc = sv_solve(c, 30, 1e-12, &st);
+            rc = sv_gmres(c, 40, 1e-12, &st);
             tm_end(c, c->t_solve, 0);
             acc.ir_iterations += st.ir_iterations;
         }
@@ -124,7 +126,6 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
             c->factor_valid = false;
         }
         if (rc != HMEC_OK) return rc;
-        acc.ir_iterations -= 0;
     }
     acc.solves++;
     if (c->precision == HMEC_PREC_FP64) acc.ir_iterations += st.ir_iterations;

@@ -82,6 +82,7 @@ struct FrontTask {
     int p;          // pivots
     int64_t doff;   // offset of the inverted diagonal blocks (2 x NB x NB per NB pivots)
     int64_t poff;   // offset of the front's partial sums in the solve scratch (per level)
+    int64_t coff;   // offset of the front's pivot-block copy in the inversion scratch (per level)
 };
 
 struct EaTask {       // extend-add of one child CB into its parent
@@ -217,6 +218,8 @@ int sv_free_device(hmec_ctx *c);
 int sv_factorize(hmec_ctx *c, int precision);
 int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *st);
 int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision);
+int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *st);
+void sv_gmres_free(hmec_ctx *c);
 
 // timing helpers
 void tm_begin(hmec_ctx *c, Timer &t);

@@ -19,6 +19,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <vector>
 
 #include "hmec_internal.h"
 
@@ -394,37 +396,153 @@ __global__ void __launch_bounds__(256) k_schur(const FrontTask *tasks, T *arena)
     gemm_tile_sub<T>(F + (int64_t)c0 * n + r0, F + r0, F + (int64_t)c0 * n, n, m, nn, tk.p);
 }
 
+// ------------------------------------------------------------ triangle inverses
+// After a level's panels, L11 (unit lower) and U11 (upper) of every front are
+// replaced in place by their inverses, so the solves are plain GEMVs.
+// Blocked, right to left / bottom to top: with X the already inverted
+// trailing block and the original triangles read from a copy,
+//   L: X[r0:p, blk] = -X[r0:p, r0:p] L[r0:p, blk] inv(L_blk)
+//   U: Y[blk, r0:p] = -inv(U_blk) U[blk, r0:p] Y[r0:p, r0:p]
+// (inv(L_blk), inv(U_blk) are the panel's NB x NB inverted diagonal blocks).
+template <typename T>
+__global__ void k_pivot_copy(const FrontTask *tasks, const T *arena, T *copy) {
+    const FrontTask tk = tasks[blockIdx.y];
+    const int p = tk.p;
+    const int64_t tot = (int64_t)p * p;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % p), c = (int)(e / p);
+        copy[tk.coff + e] = arena[tk.off + (int64_t)c * tk.n + r];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_tri_inv(const FrontTask *tasks, int j, T *arena,
+                                                 const T *copy, const T *dinv) {
+    __shared__ T As[64][NB + 1];
+    __shared__ T Bs[NB][64 + 1];
+    __shared__ T Ts[64][NB + 1];
+    __shared__ T Iv[NB][NB + 1];
+    const FrontTask tk = tasks[blockIdx.y];
+    const int p = tk.p;
+    const int c0 = j * NB;
+    if (c0 >= p) return;
+    const int jw = min(NB, p - c0), r0 = c0 + jw;
+    const bool lower = blockIdx.z == 0;
+    const int tile = blockIdx.x;
+    const int64_t n = tk.n;
+    T *F = arena + tk.off;
+    const T *Cp = copy + tk.coff;
+    const T *Dv = dinv + tk.doff + (int64_t)j * 2 * NB * NB + (lower ? 0 : NB * NB);
+    const int tid = threadIdx.x;
+    if (tile == 0) {
+        // diagonal block: strict lower part of inv(L_blk) / upper part of inv(U_blk)
+        for (int e = tid; e < jw * jw; e += blockDim.x) {
+            const int r = e % jw, c = e / jw;
+            if (lower ? (r > c) : (r <= c)) F[(int64_t)(c0 + c) * n + c0 + r] = Dv[c * NB + r];
+        }
+    }
+    const int S0 = r0 + tile * 64;
+    if (S0 >= p) return;
+    const int w = min(64, p - S0);
+    for (int e = tid; e < NB * NB; e += blockDim.x) Iv[e % NB][e / NB] = Dv[(e / NB) * NB + e % NB];
+    // thread layout: 64 "long" indices (rows for L, columns for U) x 4 groups of 8 block indices
+    const int li = tid & 63, g = tid >> 6;
+    T acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc[q] = (T)0;
+    const int kend = min(p, S0 + w);
+    for (int kk = r0; kk < kend; kk += NB) {
+        // index order per operand so that consecutive threads read
+        // consecutive addresses (column-major storage)
+        for (int e = tid; e < 64 * NB; e += blockDim.x) {
+            const int a = lower ? e % 64 : e / NB, kl = lower ? e / 64 : e % NB;
+            const int k = kk + kl, sidx = S0 + a;
+            T v = (T)0;
+            if (k < kend && sidx < p && k <= sidx) {
+                if (lower) v = (k == sidx) ? (T)1 : F[(int64_t)k * n + sidx];      // X[sidx, k]
+                else v = F[(int64_t)sidx * n + k];                                 // Y[k, sidx]
+            }
+            if (lower) As[a][kl] = v;
+            else Bs[kl][a] = v;
+        }
+        for (int e = tid; e < NB * NB; e += blockDim.x) {
+            const int b = lower ? e / NB : e % NB, kl = lower ? e % NB : e / NB, k = kk + kl;
+            T v = (T)0;
+            if (k < kend && b < jw) {
+                if (lower) v = Cp[(int64_t)(c0 + b) * p + k];     // L[k, c0 + b]
+                else v = Cp[(int64_t)k * p + c0 + b];             // U[c0 + b, k]
+            }
+            if (lower) Bs[kl][b] = v;
+            else As[b][kl] = v;
+        }
+        __syncthreads();
+        if (lower) {
+            // T[li, b] += sum_k X[li, k] L[k, b]
+#pragma unroll 8
+            for (int kl = 0; kl < NB; kl++) {
+                const T x = As[li][kl];
+#pragma unroll
+                for (int q = 0; q < 8; q++) acc[q] += x * Bs[kl][g * 8 + q];
+            }
+        } else {
+            // T[b, li] += sum_k U[b, k] Y[k, li]
+#pragma unroll 8
+            for (int kl = 0; kl < NB; kl++) {
+                const T y = Bs[kl][li];
+#pragma unroll
+                for (int q = 0; q < 8; q++) acc[q] += As[g * 8 + q][kl] * y;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) Ts[li][g * 8 + q] = acc[q];
+    __syncthreads();
+    if (li < w) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int b = g * 8 + q;
+            if (b >= jw) continue;
+            T r = (T)0;
+            if (lower) {
+                // (T inv(L_blk))[li, b]
+                for (int c = 0; c < jw; c++) r += Ts[li][c] * Iv[c][b];
+                F[(int64_t)(c0 + b) * n + S0 + li] = -r;
+            } else {
+                // (inv(U_blk) T)[b, li]
+                for (int c = 0; c < jw; c++) r += Iv[b][c] * Ts[li][c];
+                F[(int64_t)(S0 + li) * n + c0 + b] = -r;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------ solves
-// Forward sweep, leaves -> root.  Per level:
-//  k_fwd_piv (one 1024-thread workgroup per front): w = [z(pivots); 0] plus
-//    the children's contribution vectors (extend-add); y = L11^-1 w_p block
-//    by block with the inverted NB x NB diagonal blocks kept by the panel
-//    factorization, the rest of the pivot rows updated after each block;
-//    y -> z, w -> wbuf.
-//  k_gemv_part + k_fwd_reduce: cv = w_u - L21 y, split over row and column
-//    chunks, partial sums reduced in a fixed order (deterministic).
-// Backward sweep, root -> leaves.  Per level:
-//  k_gemv_part + k_bwd_reduce: w_p = y - U12 x_u (x_u final from ancestors).
-//  k_bwd_piv: x_p = U11^-1 w_p block by block -> z.
+// With L11 and U11 replaced by their inverses every sweep is a sequence of
+// parallel GEMVs per level (HBM bound):
+//  forward, leaves -> root:  k_fwd_gather  w = [z(pivots); 0] + children's cv
+//                            y  = L11^-1 w_p        (MODE 2)      -> z, wbuf
+//                            cv = w_u - L21 y       (MODE 0)      -> wbuf
+//  backward, root -> leaves: t  = y - U12 x_u       (MODE 1)      -> wbuf
+//                            x_p = U11^-1 t         (MODE 3)      -> z
+// Each GEMV is split over row and column chunks; the partial sums are
+// reduced in a fixed order (deterministic).
 constexpr int GR = 256;    // rows per GEMV workgroup
 constexpr int GC = 256;    // columns per GEMV workgroup
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_piv(const FrontTask *tasks, const int *fronts,
-                                                  const int32_t *dofs, const int64_t *dof_off,
-                                                  const int64_t *voff, const int *child_ptr,
-                                                  const int *child_list, const int64_t *ea_off,
-                                                  const int32_t *ea_map, const FrontTask *all_tasks,
-                                                  const T *arena, const T *dinv, double *z,
-                                                  double *wbuf) {
-    extern __shared__ double sm[];
-    __shared__ double yb[NB];
+__global__ void __launch_bounds__(256) k_fwd_gather(const FrontTask *tasks, const int *fronts,
+                                                    const int32_t *dofs, const int64_t *dof_off,
+                                                    const int64_t *voff, const int *child_ptr,
+                                                    const int *child_list, const int64_t *ea_off,
+                                                    const int32_t *ea_map, const FrontTask *all_tasks,
+                                                    const double *z, double *wbuf) {
     const int f = fronts[blockIdx.x];
     const FrontTask tk = tasks[blockIdx.x];
-    const int n = tk.n, p = tk.p, tid = threadIdx.x;
-    double *w = sm;
+    const int n = tk.n, p = tk.p;
     const int32_t *dof = dofs + dof_off[f];
-    for (int l = tid; l < n; l += blockDim.x) w[l] = (l < p) ? z[dof[l]] : 0.0;
+    double *w = wbuf + voff[f];
+    for (int l = threadIdx.x; l < n; l += blockDim.x) w[l] = (l < p) ? z[dof[l]] : 0.0;
     __syncthreads();
     for (int q = child_ptr[f]; q < child_ptr[f + 1]; q++) {
         const int ch = child_list[q];
@@ -432,59 +550,16 @@ __global__ void __launch_bounds__(256) k_fwd_piv(const FrontTask *tasks, const i
         const int u = ct.n - ct.p;
         const double *cvc = wbuf + voff[ch] + ct.p;
         const int32_t *map = ea_map + ea_off[ch];
-        for (int a = tid; a < u; a += blockDim.x) w[map[a / 6] * 6 + a % 6] += cvc[a];
+        for (int a = threadIdx.x; a < u; a += blockDim.x) w[map[a / 6] * 6 + a % 6] += cvc[a];
         __syncthreads();
-    }
-    const T *Fm = arena + tk.off;
-    const T *Di = dinv + tk.doff;
-    for (int c0 = 0; c0 < p; c0 += NB) {
-        const int cb = min(NB, p - c0);
-        const T *Li = Di + (int64_t)(c0 / NB) * 2 * NB * NB;    // L^-1, column-major NB x NB
-        if (tid >= cb && tid < NB) yb[tid] = 0.0;
-        if (tid < cb) {
-            double y = 0;
-#pragma unroll
-            for (int q = 0; q < NB; q++)
-                if (q <= tid) y += (double)Li[q * NB + tid] * w[c0 + q];
-            yb[tid] = y;
-        }
-        __syncthreads();
-        if (tid < cb) w[c0 + tid] = yb[tid];
-        for (int r = c0 + cb + tid; r < p; r += blockDim.x) {
-            // all NB loads issued before the dependent sums
-            double s0 = 0, s1 = 0;
-            const T *col = Fm + (int64_t)c0 * n + r;
-            if (cb == NB) {
-                // full block: NB independent loads, converted at use
-#pragma unroll
-                for (int c8 = 0; c8 < NB; c8 += 16) {
-                    T l[16];
-#pragma unroll
-                    for (int c = 0; c < 16; c++) l[c] = col[(int64_t)(c8 + c) * n];
-#pragma unroll
-                    for (int c = 0; c < 16; c += 2) {
-                        s0 += (double)l[c] * yb[c8 + c];
-                        s1 += (double)l[c + 1] * yb[c8 + c + 1];
-                    }
-                }
-            } else {
-                for (int c = 0; c < cb; c++) s0 += (double)col[(int64_t)c * n] * yb[c];
-            }
-            w[r] -= s0 + s1;
-        }
-        __syncthreads();
-    }
-    double *wb = wbuf + voff[f];
-    for (int l = tid; l < n; l += blockDim.x) {
-        wb[l] = w[l];
-        if (l < p) z[dof[l]] = w[l];
     }
 }
 
-// partial[cc][r] = sum over the column chunk cc of M[r, c] x[c], M a
-// rows x cols block of the front at (row0, col0).  fwd: rows = update rows,
-// cols = pivots, x = y (wbuf); bwd: rows = pivots, cols = update, x = z(dofs).
-template <typename T, bool FWD>
+// MODE 0: L21 (rows u @p, cols p @0, x = wbuf[0:p])
+// MODE 1: U12 (rows p @0, cols u @p, x = z[dofs[p:n]])
+// MODE 2: L11^-1 unit lower (rows p, cols p, x = wbuf[0:p])
+// MODE 3: U11^-1 upper (rows p, cols p, x = wbuf[0:p])
+template <typename T, int MODE>
 __global__ void __launch_bounds__(GR) k_gemv_part(const FrontTask *tasks, const int *fronts,
                                                   const int32_t *dofs, const int64_t *dof_off,
                                                   const int64_t *voff, const T *arena,
@@ -494,25 +569,33 @@ __global__ void __launch_bounds__(GR) k_gemv_part(const FrontTask *tasks, const 
     const int f = fronts[blockIdx.y];
     const FrontTask tk = tasks[blockIdx.y];
     const int n = tk.n, p = tk.p, u = n - p;
-    const int rows = FWD ? u : p, cols = FWD ? p : u;
+    const int rows = (MODE == 0) ? u : p, cols = (MODE == 1) ? u : p;
     if (rows <= 0 || cols <= 0) return;
     const int nrc = (rows + GR - 1) / GR, ncc = (cols + GC - 1) / GC;
     if ((int)blockIdx.x >= nrc * ncc) return;
     const int rc = blockIdx.x % nrc, cc = blockIdx.x / nrc;
     const int c0 = cc * GC, cn = min(GC, cols - c0);
+    const int r0 = rc * GR;
+    // triangular blocks: skip tiles entirely outside the triangle
+    if (MODE == 2 && c0 > r0 + GR - 1) return;
+    if (MODE == 3 && c0 + cn - 1 < r0) return;
     const int tid = threadIdx.x;
     if (tid < cn) {
-        if (FWD) xs[tid] = wbuf[voff[f] + c0 + tid];
-        else xs[tid] = z[dofs[dof_off[f] + p + c0 + tid]];
+        if (MODE == 1) xs[tid] = z[dofs[dof_off[f] + p + c0 + tid]];
+        else xs[tid] = wbuf[voff[f] + c0 + tid];
     }
     __syncthreads();
-    const int r = rc * GR + tid;
+    const int r = r0 + tid;
     if (r >= rows) return;
-    const int row = FWD ? p + r : r, col0 = FWD ? c0 : p + c0;
+    const int row = (MODE == 0) ? p + r : r, col0 = (MODE == 1) ? p + c0 : c0;
     const T *M = arena + tk.off + (int64_t)col0 * n + row;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    int c = 0;
-    for (; c + 8 <= cn; c += 8) {
+    // column range inside the triangle
+    int cb = 0, ce = cn;
+    if (MODE == 2) ce = max(0, min(cn, r - c0));     // strictly below the diagonal
+    if (MODE == 3) cb = min(cn, max(0, r - c0));     // on and above the diagonal
+    int c = cb;
+    for (; c + 8 <= ce; c += 8) {
         T l[8];
 #pragma unroll
         for (int q = 0; q < 8; q++) l[q] = M[(int64_t)(c + q) * n];
@@ -521,84 +604,39 @@ __global__ void __launch_bounds__(GR) k_gemv_part(const FrontTask *tasks, const 
         s2 += (double)l[2] * xs[c + 2] + (double)l[6] * xs[c + 6];
         s3 += (double)l[3] * xs[c + 3] + (double)l[7] * xs[c + 7];
     }
-    for (; c < cn; c++) s0 += (double)M[(int64_t)c * n] * xs[c];
-    part[tk.poff + (int64_t)cc * rows + r] = (s0 + s1) + (s2 + s3);
+    for (; c < ce; c++) s0 += (double)M[(int64_t)c * n] * xs[c];
+    double s = (s0 + s1) + (s2 + s3);
+    // unit diagonal of L11^-1 (the stored diagonal belongs to U11^-1)
+    if (MODE == 2 && r >= c0 && r < c0 + cn) s += xs[r - c0];
+    part[tk.poff + (int64_t)cc * rows + r] = s;
 }
 
-// fwd: cv = w_u - sum_cc partial  (in wbuf);  bwd: w_p = z(pivots) - sum_cc partial
-template <bool FWD>
+template <int MODE>
 __global__ void __launch_bounds__(256) k_gemv_reduce(const FrontTask *tasks, const int *fronts,
                                                      const int32_t *dofs, const int64_t *dof_off,
-                                                     const int64_t *voff, const double *z,
-                                                     double *wbuf, const double *part) {
+                                                     const int64_t *voff, double *z, double *wbuf,
+                                                     const double *part) {
     const int f = fronts[blockIdx.y];
     const FrontTask tk = tasks[blockIdx.y];
     const int n = tk.n, p = tk.p, u = n - p;
-    const int rows = FWD ? u : p, cols = FWD ? p : u;
+    const int rows = (MODE == 0) ? u : p, cols = (MODE == 1) ? u : p;
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= rows) return;
-    double s = 0;
     const int ncc = (cols + GC - 1) / GC;
-    for (int cc = 0; cc < ncc; cc++) s += part[tk.poff + (int64_t)cc * rows + r];
-    if (FWD) wbuf[voff[f] + p + r] -= s;
-    else wbuf[voff[f] + r] = z[dofs[dof_off[f] + r]] - s;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_piv(const FrontTask *tasks, const int *fronts,
-                                                  const int32_t *dofs, const int64_t *dof_off,
-                                                  const int64_t *voff, const T *arena,
-                                                  const T *dinv, const double *wbuf, double *z) {
-    extern __shared__ double sm[];
-    __shared__ double xb[NB];
-    const int f = fronts[blockIdx.x];
-    const FrontTask tk = tasks[blockIdx.x];
-    const int n = tk.n, p = tk.p, tid = threadIdx.x;
-    double *w = sm;
-    const int32_t *dof = dofs + dof_off[f];
-    const double *wb = wbuf + voff[f];
-    for (int l = tid; l < p; l += blockDim.x) w[l] = wb[l];
-    __syncthreads();
-    const T *Fm = arena + tk.off;
-    const T *Di = dinv + tk.doff;
-    const int nblk = (p + NB - 1) / NB;
-    for (int b = nblk - 1; b >= 0; b--) {
-        const int c0 = b * NB, cb = min(NB, p - c0);
-        const T *Ui = Di + (int64_t)b * 2 * NB * NB + NB * NB;   // U^-1, column-major NB x NB
-        if (tid >= cb && tid < NB) xb[tid] = 0.0;
-        if (tid < cb) {
-            double x = 0;
-#pragma unroll
-            for (int q = 0; q < NB; q++)
-                if (q >= tid && q < cb) x += (double)Ui[q * NB + tid] * w[c0 + q];
-            xb[tid] = x;
-        }
-        __syncthreads();
-        if (tid < cb) w[c0 + tid] = xb[tid];
-        for (int r = tid; r < c0; r += blockDim.x) {
-            double s0 = 0, s1 = 0;
-            const T *col = Fm + (int64_t)c0 * n + r;
-            if (cb == NB) {
-                // full block: NB independent loads, converted at use
-#pragma unroll
-                for (int c8 = 0; c8 < NB; c8 += 16) {
-                    T l[16];
-#pragma unroll
-                    for (int c = 0; c < 16; c++) l[c] = col[(int64_t)(c8 + c) * n];
-#pragma unroll
-                    for (int c = 0; c < 16; c += 2) {
-                        s0 += (double)l[c] * xb[c8 + c];
-                        s1 += (double)l[c + 1] * xb[c8 + c + 1];
-                    }
-                }
-            } else {
-                for (int c = 0; c < cb; c++) s0 += (double)col[(int64_t)c * n] * xb[c];
-            }
-            w[r] -= s0 + s1;
-        }
-        __syncthreads();
+    const int r0 = (r / GR) * GR;
+    double s = 0;
+    for (int cc = 0; cc < ncc; cc++) {
+        const int c0 = cc * GC;
+        if (MODE == 2 && c0 > r0 + GR - 1) break;                     // skipped tiles
+        if (MODE == 3 && c0 + min(GC, cols - c0) - 1 < r0) continue;
+        s += part[tk.poff + (int64_t)cc * rows + r];
     }
-    for (int l = tid; l < p; l += blockDim.x) z[dof[l]] = w[l];
+    double *w = wbuf + voff[f];
+    const int32_t *dof = dofs + dof_off[f];
+    if (MODE == 0) w[p + r] -= s;
+    else if (MODE == 1) w[r] = z[dof[r]] - s;
+    else if (MODE == 2) { w[r] = s; z[dof[r]] = s; }
+    else z[dof[r]] = s;
 }
 
 __global__ void k_scale_vec(const double *a, const double *s, double *out, int64_t N) {
@@ -646,6 +684,73 @@ __global__ void __launch_bounds__(1024) k_norms(const double *a, const double *b
     }
 }
 
+
+// ------------------------------------------------------------ GMRES helpers
+constexpr int RB = 512;      // blocks of the two-stage reductions
+// y = A x
+__global__ void k_spmv(const int32_t *rowp, const int32_t *coli, const double *val,
+                       const double *x, double *y, int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    double s = 0;
+    for (int p = rowp[t]; p < rowp[t + 1]; p++) s += val[p] * x[coli[p]];
+    y[t] = s;
+}
+// partial[b * nv + i] = sum over block b's slice of V_i . w   (nv vectors)
+__global__ void __launch_bounds__(256) k_dots_part(const double *V, int64_t ldv, int nv,
+                                                   const double *w, int64_t N, double *partial) {
+    __shared__ double red[256];
+    const int64_t chunk = (N + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = (int64_t)blockIdx.x * chunk, b1 = min(N, b0 + chunk);
+    for (int i = 0; i < nv; i++) {
+        const double *v = V + (int64_t)i * ldv;
+        double s = 0;
+        for (int64_t t = b0 + threadIdx.x; t < b1; t += blockDim.x) s += v[t] * w[t];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int k = 128; k > 0; k >>= 1) {
+            if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nv + i] = red[0];
+        __syncthreads();
+    }
+}
+// out[i] = sum_b partial[b * nv + i] in block order
+__global__ void k_dots_final(const double *partial, int nb, int nv, double *out) {
+    const int i = threadIdx.x;
+    if (i >= nv) return;
+    double s = 0;
+    for (int b = 0; b < nb; b++) s += partial[(int64_t)b * nv + i];
+    out[i] = s;
+}
+// coefficients passed by value (no upload before the kernel)
+struct GmCoef {
+    double v[32];
+};
+// w -= sum_i h_i V_i
+__global__ void k_gs_update(const double *V, int64_t ldv, int nv, GmCoef h, double *w,
+                            int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    double s = w[t];
+    for (int i = 0; i < nv; i++) s -= h.v[i] * V[(int64_t)i * ldv + t];
+    w[t] = s;
+}
+// y = a * x
+__global__ void k_vscale(const double *x, double a, double *y, int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) y[t] = a * x[t];
+}
+// x += sum_j c_j Z_j
+__global__ void k_combine(const double *Z, int64_t ldz, int nz, GmCoef c, double *x,
+                          int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    double s = x[t];
+    for (int j = 0; j < nz; j++) s += c.v[j] * Z[(int64_t)j * ldz + t];
+    x[t] = s;
+}
 }  // namespace
 
 // ------------------------------------------------------------ host side
@@ -661,8 +766,9 @@ struct SolverDev {
     double *dinv64 = nullptr;      // inverted diagonal blocks (FP64 factor)
     float *dinv32 = nullptr;       // inverted diagonal blocks (FP32 factor)
     double *part = nullptr;        // GEMV partial sums of one level
+    double *icopy = nullptr;       // pivot-block copies of one level (inversion)
     int64_t dinv_size = 0;
-    std::vector<int> gemv_fwd_grid, gemv_bwd_grid;   // per level grid.x
+    std::vector<int> gemv_fwd_grid, gemv_bwd_grid, gemv_tri_grid;   // per level grid.x
     std::vector<double> schur_flops, fwdupd_entries, bwdupd_entries;   // per level work
     int max_level_fronts = 0;
     std::vector<int> maxu;         // per level max update size
@@ -720,6 +826,7 @@ int sv_free_device(hmec_ctx *c) {
         hipFree(s->dinv64);
         hipFree(s->dinv32);
         hipFree(s->part);
+        hipFree(s->icopy);
         if (s->h_norms) hipHostFree(s->h_norms);
         delete s;
     }
@@ -750,7 +857,7 @@ int sv_setup_device(hmec_ctx *c) {
     int64_t doff = 0;
     for (int f = 0; f < pl.nfronts; f++) {
         const int p = 6 * pl.np[f];
-        all[f] = FrontTask{pl.off[f], 6 * (pl.np[f] + pl.nu[f]), p, doff, 0};
+        all[f] = FrontTask{pl.off[f], 6 * (pl.np[f] + pl.nu[f]), p, doff, 0, 0};
         doff += (int64_t)((p + NB - 1) / NB) * 2 * NB * NB;
     }
     s->dinv_size = doff;
@@ -758,19 +865,26 @@ int sv_setup_device(hmec_ctx *c) {
     int64_t part_max = 0;
     s->gemv_fwd_grid.assign(pl.nlevels, 0);
     s->gemv_bwd_grid.assign(pl.nlevels, 0);
+    s->gemv_tri_grid.assign(pl.nlevels, 0);
+    int64_t copy_max = 0;
     for (int lv = 0; lv < pl.nlevels; lv++) {
-        int64_t po = 0;
+        int64_t po = 0, co = 0;
         for (int f : pl.levels[lv]) {
             const int n = all[f].n, p = all[f].p, u = n - p;
             all[f].poff = po;
-            po += (int64_t)std::max((p + GC - 1) / GC * u, (u + GC - 1) / GC * p);
+            all[f].coff = co;
+            co += (int64_t)p * p;
+            po += (int64_t)std::max({(p + GC - 1) / GC * u, (u + GC - 1) / GC * p, (p + GC - 1) / GC * p});
             if (u > 0 && p > 0) {
                 s->gemv_fwd_grid[lv] = std::max(s->gemv_fwd_grid[lv], ((u + GR - 1) / GR) * ((p + GC - 1) / GC));
                 s->gemv_bwd_grid[lv] = std::max(s->gemv_bwd_grid[lv], ((p + GR - 1) / GR) * ((u + GC - 1) / GC));
             }
+            s->gemv_tri_grid[lv] = std::max(s->gemv_tri_grid[lv], ((p + GR - 1) / GR) * ((p + GC - 1) / GC));
         }
         part_max = std::max(part_max, po);
+        copy_max = std::max(copy_max, co);
     }
+    HMEC_HIP(hipMalloc(&s->icopy, sizeof(double) * std::max<int64_t>(1, copy_max)));
     HMEC_HIP(hipMalloc(&s->part, sizeof(double) * std::max<int64_t>(1, part_max)));
     HMEC_TRY(upload(&s->all_tasks, all));
     std::vector<int> flist;
@@ -974,6 +1088,20 @@ static int factor_levels(hmec_ctx *c, T *arena, T *dinv, double tau) {
             LAUNCH_CHECK();
             kt_end(c, "schur", sdev(c)->schur_flops[lv]);
         }
+        // L11, U11 -> inverses (pivot block only: disjoint from L21, U12, CB)
+        {
+            T *cp = reinterpret_cast<T *>(s->icopy);
+            hipLaunchKernelGGL(k_pivot_copy<T>, dim3(std::max(1, std::min(1024, maxp * maxp / 256 + 1)), nfr),
+                               dim3(256), 0, st, dp.d_tasks[lv], (const T *)arena, cp);
+            LAUNCH_CHECK();
+            const int nblk = (maxp + NB - 1) / NB;
+            const int ntile = std::max(1, (maxp + 63) / 64);
+            for (int j = nblk - 1; j >= 0; j--) {
+                hipLaunchKernelGGL(k_tri_inv<T>, dim3(ntile, nfr, 2), dim3(256), 0, st, dp.d_tasks[lv],
+                                   j, arena, (const T *)cp, (const T *)dinv);
+                LAUNCH_CHECK();
+            }
+        }
     }
     return HMEC_OK;
 }
@@ -1001,7 +1129,7 @@ int sv_factorize(hmec_ctx *c, int precision) {
 }
 
 template <typename T>
-static int tri_solve(hmec_ctx *c, const T *arena, const T *dinv, double *z) {
+static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
     const FrontPlan &pl = c->plan;
     DevPlan &dp = c->dp;
     SolverDev *s = sdev(c);
@@ -1010,23 +1138,30 @@ static int tri_solve(hmec_ctx *c, const T *arena, const T *dinv, double *z) {
     for (int lv = 0; lv < pl.nlevels; lv++) {
         const int nfr = (int)dp.h_tasks[lv].size();
         const int *fr = s->fronts + s->lvl_fronts_off[lv];
-        const int bs = dp.maxp[lv] <= 96 ? 64 : 256;
+        const int bs = dp.maxn[lv] <= 128 ? 64 : 256;
         kt_begin(c, "fwd");
-        hipLaunchKernelGGL(k_fwd_piv<T>, dim3(nfr), dim3(bs), sizeof(double) * dp.maxn[lv], st,
-                           dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff,
-                           s->child_ptr, s->child_list, s->ea_off, dp.d_ea_map, s->all_tasks, arena,
-                           dinv, z, wbuf);
+        hipLaunchKernelGGL(k_fwd_gather, dim3(nfr), dim3(bs), 0, st, dp.d_tasks[lv], fr,
+                           dp.d_front_dofs, dp.d_front_dof_off, s->voff, s->child_ptr, s->child_list,
+                           s->ea_off, dp.d_ea_map, s->all_tasks, z, wbuf);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_gemv_part<T, 2>), dim3(s->gemv_tri_grid[lv], nfr), dim3(GR), 0, st,
+                           dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, arena, z,
+                           wbuf, s->part);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_gemv_reduce<2>, dim3((dp.maxp[lv] + 255) / 256, nfr), dim3(256), 0, st,
+                           dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, z, wbuf,
+                           s->part);
         LAUNCH_CHECK();
         if (s->gemv_fwd_grid[lv] > 0) {
             kt_begin(c, "fwd_upd");
-            hipLaunchKernelGGL((k_gemv_part<T, true>), dim3(s->gemv_fwd_grid[lv], nfr), dim3(GR), 0,
-                               st, dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff,
+            hipLaunchKernelGGL((k_gemv_part<T, 0>), dim3(s->gemv_fwd_grid[lv], nfr), dim3(GR), 0, st,
+                               dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff,
                                arena, z, wbuf, s->part);
             LAUNCH_CHECK();
             kt_end(c, "fwd_upd", s->fwdupd_entries[lv] * sizeof(T));
-            hipLaunchKernelGGL(k_gemv_reduce<true>, dim3((s->maxu[lv] + 255) / 256, nfr), dim3(256),
-                               0, st, dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off,
-                               s->voff, z, wbuf, s->part);
+            hipLaunchKernelGGL(k_gemv_reduce<0>, dim3((s->maxu[lv] + 255) / 256, nfr), dim3(256), 0,
+                               st, dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, z,
+                               wbuf, s->part);
             LAUNCH_CHECK();
         }
         kt_end(c, "fwd", 0);
@@ -1037,21 +1172,24 @@ static int tri_solve(hmec_ctx *c, const T *arena, const T *dinv, double *z) {
         kt_begin(c, "bwd");
         if (s->gemv_bwd_grid[lv] > 0) {
             kt_begin(c, "bwd_upd");
-            hipLaunchKernelGGL((k_gemv_part<T, false>), dim3(s->gemv_bwd_grid[lv], nfr), dim3(GR), 0,
-                               st, dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff,
+            hipLaunchKernelGGL((k_gemv_part<T, 1>), dim3(s->gemv_bwd_grid[lv], nfr), dim3(GR), 0, st,
+                               dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff,
                                arena, z, wbuf, s->part);
             LAUNCH_CHECK();
             kt_end(c, "bwd_upd", s->bwdupd_entries[lv] * sizeof(T));
         }
-        // w_p = z(pivots) - partials (plain copy where the front has no update set)
-        hipLaunchKernelGGL(k_gemv_reduce<false>, dim3((dp.maxp[lv] + 255) / 256, nfr), dim3(256), 0,
-                           st, dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, z,
+        // t = y - U12 x_u (a plain copy of y where the front has no update set)
+        hipLaunchKernelGGL(k_gemv_reduce<1>, dim3((dp.maxp[lv] + 255) / 256, nfr), dim3(256), 0, st,
+                           dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, z, wbuf,
+                           s->part);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_gemv_part<T, 3>), dim3(s->gemv_tri_grid[lv], nfr), dim3(GR), 0, st,
+                           dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, arena, z,
                            wbuf, s->part);
         LAUNCH_CHECK();
-        const int bs = dp.maxp[lv] <= 96 ? 64 : 256;
-        hipLaunchKernelGGL(k_bwd_piv<T>, dim3(nfr), dim3(bs), sizeof(double) * dp.maxp[lv], st,
-                           dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, arena,
-                           dinv, wbuf, z);
+        hipLaunchKernelGGL(k_gemv_reduce<3>, dim3((dp.maxp[lv] + 255) / 256, nfr), dim3(256), 0, st,
+                           dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, z, wbuf,
+                           s->part);
         LAUNCH_CHECK();
         kt_end(c, "bwd", 0);
     }
@@ -1076,8 +1214,8 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
     for (;; it++) {
         hipLaunchKernelGGL(k_scale_vec, dim3(g), dim3(256), 0, st, r, c->d.rscale, z, N);
         LAUNCH_CHECK();
-        if (c->factor_prec == HMEC_PREC_FP64) HMEC_TRY(tri_solve<double>(c, c->d.arena, s->dinv64, z));
-        else HMEC_TRY(tri_solve<float>(c, c->d.arena32, s->dinv32, z));
+        if (c->factor_prec == HMEC_PREC_FP64) HMEC_TRY(tri_solve<double>(c, c->d.arena, z));
+        else HMEC_TRY(tri_solve<float>(c, c->d.arena32, z));
         hipLaunchKernelGGL(k_axpy_scaled, dim3(g), dim3(256), 0, st, x, c->d.cscale, z, N,
                            it == 0 ? 1 : 0);
         LAUNCH_CHECK();
@@ -1120,4 +1258,200 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
 int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision) {
     for (int k = 0; k < n; k++) HMEC_TRY(sv_factorize(cs[k], precision));
     return HMEC_OK;
+}
+
+
+// ---------------------------------------------------------------- GMRES
+// Right-preconditioned restarted GMRES in FP64 on the original CSR, with the
+// (possibly stale) factor as preconditioner: M^-1 v = Dc LU^-1 Dr v.  The
+// initial guess is the context's last solution (warm start across Picard
+// iterations and timesteps).  Classical Gram-Schmidt with one
+// re-orthogonalization; every reduction is two-stage in a fixed order.
+struct GmresBuf {
+    double *V = nullptr, *Z = nullptr, *w = nullptr, *part = nullptr, *hdev = nullptr;
+    double *hhost = nullptr;
+    int m = 0;
+    int64_t N = 0;
+};
+static std::mutex g_gm_mu;
+static std::map<hmec_ctx *, GmresBuf> g_gm;
+
+void sv_gmres_free(hmec_ctx *c) {
+    std::lock_guard<std::mutex> lk(g_gm_mu);
+    auto it = g_gm.find(c);
+    if (it == g_gm.end()) return;
+    GmresBuf &g = it->second;
+    hipFree(g.V); hipFree(g.Z); hipFree(g.w); hipFree(g.part); hipFree(g.hdev);
+    if (g.hhost) hipHostFree(g.hhost);
+    g_gm.erase(it);
+}
+
+static int gm_buf(hmec_ctx *c, int m, GmresBuf **out) {
+    std::lock_guard<std::mutex> lk(g_gm_mu);
+    GmresBuf &g = g_gm[c];
+    if (g.m < m || g.N != c->N) {
+        hipFree(g.V); hipFree(g.Z); hipFree(g.w); hipFree(g.part); hipFree(g.hdev);
+        if (g.hhost) hipHostFree(g.hhost);
+        g = GmresBuf();
+        g.m = m;
+        g.N = c->N;
+        HMEC_HIP(hipMalloc(&g.V, sizeof(double) * (m + 1) * c->N));
+        HMEC_HIP(hipMalloc(&g.Z, sizeof(double) * m * c->N));
+        HMEC_HIP(hipMalloc(&g.w, sizeof(double) * c->N));
+        HMEC_HIP(hipMalloc(&g.part, sizeof(double) * RB * (m + 2)));
+        HMEC_HIP(hipMalloc(&g.hdev, sizeof(double) * (m + 2)));
+        HMEC_HIP(hipHostMalloc(&g.hhost, sizeof(double) * (m + 2), hipHostMallocDefault));
+    }
+    *out = &g;
+    return HMEC_OK;
+}
+
+// dot products of w with V_0..V_{nv-1} -> host h[0..nv)
+static int gm_dots(hmec_ctx *c, GmresBuf &g, const double *V, int nv, const double *w,
+                   double *h) {
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_dots_part, dim3(RB), dim3(256), 0, st, V, c->N, nv, w, c->N, g.part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(64), 0, st, g.part, RB, nv, g.hdev);
+    LAUNCH_CHECK();
+    HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
+    HMEC_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nv; i++) h[i] = g.hhost[i];
+    return HMEC_OK;
+}
+
+static int gm_precond(hmec_ctx *c, const double *v, double *zout) {
+    SolverDev *s = sdev(c);
+    const int64_t N = c->N;
+    const unsigned gN = (unsigned)((N + 255) / 256);
+    hipStream_t st = c->stream;
+    double *z = c->d.work2;
+    hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, v, c->d.rscale, z, N);
+    LAUNCH_CHECK();
+    if (c->factor_prec == HMEC_PREC_FP64) HMEC_TRY(tri_solve<double>(c, c->d.arena, z));
+    else HMEC_TRY(tri_solve<float>(c, c->d.arena32, z));
+    hipLaunchKernelGGL(k_axpy_scaled, dim3(gN), dim3(256), 0, st, zout, c->d.cscale, z, N, 1);
+    LAUNCH_CHECK();
+    (void)s;
+    return HMEC_OK;
+}
+
+int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
+    if (!c->factor_valid) return HMEC_E_ARG;
+    const int m = 20;               // restart length (<= 31: GmCoef holds the coefficients)
+    GmresBuf *gp;
+    HMEC_TRY(gm_buf(c, m, &gp));
+    GmresBuf &g = *gp;
+    SolverDev *s = sdev(c);
+    const int64_t N = c->N;
+    const unsigned gN = (unsigned)((N + 255) / 256);
+    hipStream_t st = c->stream;
+    double *x = c->d.sol, *r = c->d.work;
+    // ||b||
+    double bnorm2, tmp[2];
+    HMEC_TRY(gm_dots(c, g, c->d.rhs, 1, c->d.rhs, &bnorm2));
+    const double bnorm = std::sqrt(bnorm2);
+    if (!(bnorm > 0)) {
+        HMEC_HIP(hipMemsetAsync(x, 0, sizeof(double) * N, st));
+        if (out) std::memset(out, 0, sizeof(*out));
+        return HMEC_OK;
+    }
+    int total_it = 0;
+    double relres = 1;
+    bool done = false;
+    std::vector<double> H((m + 1) * m), cs(m), sn(m), gv(m + 1), h(m + 2), h2(m + 2), y(m);
+    for (int cycle = 0; !done && total_it < max_it; cycle++) {
+        // r = b - A x
+        hipLaunchKernelGGL(k_residual, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, x,
+                           c->d.rhs, r, N);
+        LAUNCH_CHECK();
+        double rr;
+        HMEC_TRY(gm_dots(c, g, r, 1, r, &rr));
+        const double beta = std::sqrt(rr);
+        relres = beta / bnorm;
+        if (relres <= tol) break;
+        hipLaunchKernelGGL(k_vscale, dim3(gN), dim3(256), 0, st, r, 1.0 / beta, g.V, N);
+        LAUNCH_CHECK();
+        std::fill(gv.begin(), gv.end(), 0.0);
+        gv[0] = beta;
+        int j = 0;
+        for (; j < m && total_it < max_it; j++) {
+            double *Zj = g.Z + (int64_t)j * N;
+            HMEC_TRY(gm_precond(c, g.V + (int64_t)j * N, Zj));
+            hipLaunchKernelGGL(k_spmv, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, Zj,
+                               g.w, N);
+            LAUNCH_CHECK();
+            // CGS2
+            GmCoef co;
+            HMEC_TRY(gm_dots(c, g, g.V, j + 1, g.w, h.data()));
+            for (int i = 0; i <= j; i++) co.v[i] = h[i];
+            hipLaunchKernelGGL(k_gs_update, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, co, g.w, N);
+            LAUNCH_CHECK();
+            HMEC_TRY(gm_dots(c, g, g.V, j + 1, g.w, h2.data()));
+            for (int i = 0; i <= j; i++) co.v[i] = h2[i];
+            hipLaunchKernelGGL(k_gs_update, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, co, g.w, N);
+            LAUNCH_CHECK();
+            for (int i = 0; i <= j; i++) h[i] += h2[i];
+            double ww;
+            HMEC_TRY(gm_dots(c, g, g.w, 1, g.w, &ww));
+            h[j + 1] = std::sqrt(ww);
+            total_it++;
+            // apply previous rotations, form the new one
+            for (int i = 0; i < j; i++) {
+                const double t1 = cs[i] * h[i] + sn[i] * h[i + 1];
+                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+                h[i] = t1;
+            }
+            const double den = std::hypot(h[j], h[j + 1]);
+            cs[j] = den > 0 ? h[j] / den : 1.0;
+            sn[j] = den > 0 ? h[j + 1] / den : 0.0;
+            h[j] = den;
+            h[j + 1] = 0;
+            gv[j + 1] = -sn[j] * gv[j];
+            gv[j] = cs[j] * gv[j];
+            for (int i = 0; i <= j; i++) H[i + j * (m + 1)] = h[i];
+            relres = std::fabs(gv[j + 1]) / bnorm;
+            const double hn = std::sqrt(ww);
+            if (relres <= tol || !(hn > 0)) { j++; done = relres <= tol; break; }
+            hipLaunchKernelGGL(k_vscale, dim3(gN), dim3(256), 0, st, g.w, 1.0 / hn,
+                               g.V + (int64_t)(j + 1) * N, N);
+            LAUNCH_CHECK();
+        }
+        // y = R^-1 g ; x += Z y
+        for (int i = j - 1; i >= 0; i--) {
+            double t1 = gv[i];
+            for (int k = i + 1; k < j; k++) t1 -= H[i + k * (m + 1)] * y[k];
+            y[i] = t1 / H[i + i * (m + 1)];
+        }
+        GmCoef cy;
+        for (int i = 0; i < j; i++) cy.v[i] = y[i];
+        hipLaunchKernelGGL(k_combine, dim3(gN), dim3(256), 0, st, g.Z, N, j, cy, x, N);
+        LAUNCH_CHECK();
+    }
+    // true residual (inf-norm, as the refinement reports it)
+    hipLaunchKernelGGL(k_residual, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, x,
+                       c->d.rhs, r, N);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, st, r, c->d.rhs, N, s->norms);
+    LAUNCH_CHECK();
+    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+    HMEC_HIP(hipStreamSynchronize(st));
+    const double res = s->h_norms[1] > 0 ? s->h_norms[0] / s->h_norms[1] : s->h_norms[0];
+    (void)tmp;
+    int npert = 0;
+    HMEC_HIP(hipMemcpy(&npert, c->d.counters, sizeof(int), hipMemcpyDeviceToHost));
+    c->last_ir = total_it;
+    c->last_res = res;
+    if (out) {
+        out->ir_iterations = total_it;
+        out->rel_residual = res;
+        out->perturbed_pivots = npert;
+        out->factor_flops = c->plan.flops;
+        out->factor_nnz = c->plan.factor_nnz;
+        out->nfronts = c->plan.nfronts;
+        out->nlevels = c->plan.nlevels;
+        out->max_front = c->plan.max_front;
+    }
+    c->stale_solves++;
+    return (res <= tol * 10) ? HMEC_OK : HMEC_E_NOCONV;
 }
