@@ -80,6 +80,7 @@ void tm_free(Timer &t) {
 static void free_ctx(hmec_ctx *c) {
     if (!c) return;
     sv_gmres_free(c);
+    sv_sweep_free(c);
     sv_free_device(c);
     DeviceState &d = c->d;
     hipFree(d.fields); hipFree(d.mk); hipFree(d.prop); hipFree(d.cell); hipFree(d.perm);

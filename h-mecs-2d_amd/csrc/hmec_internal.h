@@ -220,6 +220,12 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *st);
 int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision);
 int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *st);
 void sv_gmres_free(hmec_ctx *c);
+// ---- sweep form of the factor (solver_sweep.hip) ----
+int sv_sweep_build(hmec_ctx *c, int precision);
+bool sv_sweep_ready(hmec_ctx *c, int precision);
+void sv_sweep_invalidate(hmec_ctx *c);
+int sv_sweep_solve(hmec_ctx *c, double *z);
+void sv_sweep_free(hmec_ctx *c);
 
 // timing helpers
 void tm_begin(hmec_ctx *c, Timer &t);

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "hmec_internal.h"
+#include "mfma_traits.h"
 
 int hmec_var_pos(int v);
 
@@ -274,24 +275,6 @@ __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, T 
 // supplies A(l%16, l/16) and B(l/16, l%16); element r of the result holds
 // D(Mfma<T>::row(l, r), l%16): row 4*(l/16)+r for f32, 4*r+l/16 for f64, as
 // measured on gfx950).  K is streamed through LDS in chunks of TK.
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-template <typename T> struct Mfma;
-template <> struct Mfma<double> {
-    typedef v4d acc_t;
-    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
-};
-template <> struct Mfma<float> {
-    typedef v4f acc_t;
-    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
-
 template <typename T>
 __device__ __forceinline__ void gemm_tile_sub(T *C, const T *A, const T *B, int64_t ld, int m,
                                               int nn, int K) {
@@ -716,13 +699,21 @@ __global__ void __launch_bounds__(256) k_dots_part(const double *V, int64_t ldv,
         __syncthreads();
     }
 }
-// out[i] = sum_b partial[b * nv + i] in block order
-__global__ void k_dots_final(const double *partial, int nb, int nv, double *out) {
-    const int i = threadIdx.x;
-    if (i >= nv) return;
+// out[i] = sum_b partial[b * nv + i]: one workgroup per vector, fixed
+// two-level order (strided per thread, then a tree)
+__global__ void __launch_bounds__(256) k_dots_final(const double *partial, int nb, int nv,
+                                                    double *out) {
+    __shared__ double red[256];
+    const int i = blockIdx.x;
     double s = 0;
-    for (int b = 0; b < nb; b++) s += partial[(int64_t)b * nv + i];
-    out[i] = s;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(int64_t)b * nv + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = red[0];
 }
 // coefficients passed by value (no upload before the kernel)
 struct GmCoef {
@@ -1120,8 +1111,12 @@ int sv_factorize(hmec_ctx *c, int precision) {
                        c->d.cscale);
     LAUNCH_CHECK();
     SolverDev *s = sdev(c);
+    sv_sweep_invalidate(c);
     if (precision == HMEC_PREC_FP64) HMEC_TRY(factor_levels<double>(c, c->d.arena, s->dinv64, 1e-13));
     else HMEC_TRY(factor_levels<float>(c, c->d.arena32, s->dinv32, 1e-6));
+    // FP32 factors are reused over many solves: form the one-GEMV-per-front
+    // sweep form once
+    if (precision == HMEC_PREC_FP32) HMEC_TRY(sv_sweep_build(c, precision));
     c->factor_prec = precision;
     c->factor_valid = true;
     c->stale_solves = 0;
@@ -1199,6 +1194,14 @@ static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
 // x = A^-1 b by the factor + iterative refinement on the FP64 CSR.
 // tol: target ||b - A x||inf / ||b||inf; max_ir: refinement sweeps after the
 // first solve.  Returns HMEC_E_NOCONV when the target is not met.
+// z <- (LU)^-1 z with the current factor: the sweep form when it has been
+// formed for this factor, else the level-by-level inverted-triangle solves
+static int apply_factor(hmec_ctx *c, double *z) {
+    if (sv_sweep_ready(c, c->factor_prec)) return sv_sweep_solve(c, z);
+    if (c->factor_prec == HMEC_PREC_FP64) return tri_solve<double>(c, c->d.arena, z);
+    return tri_solve<float>(c, c->d.arena32, z);
+}
+
 int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
     if (!c->factor_valid) return HMEC_E_ARG;
     SolverDev *s = sdev(c);
@@ -1214,8 +1217,7 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
     for (;; it++) {
         hipLaunchKernelGGL(k_scale_vec, dim3(g), dim3(256), 0, st, r, c->d.rscale, z, N);
         LAUNCH_CHECK();
-        if (c->factor_prec == HMEC_PREC_FP64) HMEC_TRY(tri_solve<double>(c, c->d.arena, z));
-        else HMEC_TRY(tri_solve<float>(c, c->d.arena32, z));
+        HMEC_TRY(apply_factor(c, z));
         hipLaunchKernelGGL(k_axpy_scaled, dim3(g), dim3(256), 0, st, x, c->d.cscale, z, N,
                            it == 0 ? 1 : 0);
         LAUNCH_CHECK();
@@ -1312,7 +1314,7 @@ static int gm_dots(hmec_ctx *c, GmresBuf &g, const double *V, int nv, const doub
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(k_dots_part, dim3(RB), dim3(256), 0, st, V, c->N, nv, w, c->N, g.part);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(64), 0, st, g.part, RB, nv, g.hdev);
+    hipLaunchKernelGGL(k_dots_final, dim3(nv), dim3(256), 0, st, g.part, RB, nv, g.hdev);
     LAUNCH_CHECK();
     HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * nv, hipMemcpyDeviceToHost, st));
     HMEC_HIP(hipStreamSynchronize(st));
@@ -1328,48 +1330,56 @@ static int gm_precond(hmec_ctx *c, const double *v, double *zout) {
     double *z = c->d.work2;
     hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, v, c->d.rscale, z, N);
     LAUNCH_CHECK();
-    if (c->factor_prec == HMEC_PREC_FP64) HMEC_TRY(tri_solve<double>(c, c->d.arena, z));
-    else HMEC_TRY(tri_solve<float>(c, c->d.arena32, z));
+    HMEC_TRY(apply_factor(c, z));
     hipLaunchKernelGGL(k_axpy_scaled, dim3(gN), dim3(256), 0, st, zout, c->d.cscale, z, N, 1);
     LAUNCH_CHECK();
     (void)s;
     return HMEC_OK;
 }
 
+// ||r||_inf / ||b||_inf of r = b - A x (r left in c->d.work)
+static int gm_true_residual(hmec_ctx *c, const double *x, double *relinf) {
+    SolverDev *s = sdev(c);
+    const int64_t N = c->N;
+    const unsigned gN = (unsigned)((N + 255) / 256);
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_residual, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, x,
+                       c->d.rhs, c->d.work, N);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, st, c->d.work, c->d.rhs, N, s->norms);
+    LAUNCH_CHECK();
+    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+    HMEC_HIP(hipStreamSynchronize(st));
+    *relinf = s->h_norms[1] > 0 ? s->h_norms[0] / s->h_norms[1] : s->h_norms[0];
+    return HMEC_OK;
+}
+
+// Converged when ||b - A x||_inf <= tol ||b||_inf (the residual every solve
+// reports).  Each restart cycle starts from the true residual; its Arnoldi
+// loop stops once the estimated 2-norm has dropped by the factor the
+// inf-norm still needs (with a margin of 2), so the norms' ratio on this
+// system is accounted for.
 int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     if (!c->factor_valid) return HMEC_E_ARG;
     const int m = 20;               // restart length (<= 31: GmCoef holds the coefficients)
     GmresBuf *gp;
     HMEC_TRY(gm_buf(c, m, &gp));
     GmresBuf &g = *gp;
-    SolverDev *s = sdev(c);
     const int64_t N = c->N;
     const unsigned gN = (unsigned)((N + 255) / 256);
     hipStream_t st = c->stream;
     double *x = c->d.sol, *r = c->d.work;
-    // ||b||
-    double bnorm2, tmp[2];
-    HMEC_TRY(gm_dots(c, g, c->d.rhs, 1, c->d.rhs, &bnorm2));
-    const double bnorm = std::sqrt(bnorm2);
-    if (!(bnorm > 0)) {
-        HMEC_HIP(hipMemsetAsync(x, 0, sizeof(double) * N, st));
-        if (out) std::memset(out, 0, sizeof(*out));
-        return HMEC_OK;
-    }
     int total_it = 0;
-    double relres = 1;
-    bool done = false;
+    double relinf = 0;
     std::vector<double> H((m + 1) * m), cs(m), sn(m), gv(m + 1), h(m + 2), h2(m + 2), y(m);
-    for (int cycle = 0; !done && total_it < max_it; cycle++) {
-        // r = b - A x
-        hipLaunchKernelGGL(k_residual, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, x,
-                           c->d.rhs, r, N);
-        LAUNCH_CHECK();
+    for (;;) {
+        HMEC_TRY(gm_true_residual(c, x, &relinf));
+        if (!(relinf > tol) || total_it >= max_it) break;
         double rr;
         HMEC_TRY(gm_dots(c, g, r, 1, r, &rr));
         const double beta = std::sqrt(rr);
-        relres = beta / bnorm;
-        if (relres <= tol) break;
+        if (!(beta > 0)) break;
+        const double target = beta * (tol / relinf) * 0.5;
         hipLaunchKernelGGL(k_vscale, dim3(gN), dim3(256), 0, st, r, 1.0 / beta, g.V, N);
         LAUNCH_CHECK();
         std::fill(gv.begin(), gv.end(), 0.0);
@@ -1381,7 +1391,7 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
             hipLaunchKernelGGL(k_spmv, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, Zj,
                                g.w, N);
             LAUNCH_CHECK();
-            // CGS2
+            // classical Gram-Schmidt, twice
             GmCoef co;
             HMEC_TRY(gm_dots(c, g, g.V, j + 1, g.w, h.data()));
             for (int i = 0; i <= j; i++) co.v[i] = h[i];
@@ -1394,9 +1404,10 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
             for (int i = 0; i <= j; i++) h[i] += h2[i];
             double ww;
             HMEC_TRY(gm_dots(c, g, g.w, 1, g.w, &ww));
-            h[j + 1] = std::sqrt(ww);
+            const double hn = std::sqrt(ww);
+            h[j + 1] = hn;
             total_it++;
-            // apply previous rotations, form the new one
+            // previous rotations, then the new one
             for (int i = 0; i < j; i++) {
                 const double t1 = cs[i] * h[i] + sn[i] * h[i + 1];
                 h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
@@ -1410,13 +1421,15 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
             gv[j + 1] = -sn[j] * gv[j];
             gv[j] = cs[j] * gv[j];
             for (int i = 0; i <= j; i++) H[i + j * (m + 1)] = h[i];
-            relres = std::fabs(gv[j + 1]) / bnorm;
-            const double hn = std::sqrt(ww);
-            if (relres <= tol || !(hn > 0)) { j++; done = relres <= tol; break; }
+            if (std::fabs(gv[j + 1]) <= target || !(hn > 0)) {
+                j++;
+                break;
+            }
             hipLaunchKernelGGL(k_vscale, dim3(gN), dim3(256), 0, st, g.w, 1.0 / hn,
                                g.V + (int64_t)(j + 1) * N, N);
             LAUNCH_CHECK();
         }
+        if (j == 0) break;
         // y = R^-1 g ; x += Z y
         for (int i = j - 1; i >= 0; i--) {
             double t1 = gv[i];
@@ -1428,23 +1441,13 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
         hipLaunchKernelGGL(k_combine, dim3(gN), dim3(256), 0, st, g.Z, N, j, cy, x, N);
         LAUNCH_CHECK();
     }
-    // true residual (inf-norm, as the refinement reports it)
-    hipLaunchKernelGGL(k_residual, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, x,
-                       c->d.rhs, r, N);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, st, r, c->d.rhs, N, s->norms);
-    LAUNCH_CHECK();
-    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
-    HMEC_HIP(hipStreamSynchronize(st));
-    const double res = s->h_norms[1] > 0 ? s->h_norms[0] / s->h_norms[1] : s->h_norms[0];
-    (void)tmp;
     int npert = 0;
     HMEC_HIP(hipMemcpy(&npert, c->d.counters, sizeof(int), hipMemcpyDeviceToHost));
     c->last_ir = total_it;
-    c->last_res = res;
+    c->last_res = relinf;
     if (out) {
         out->ir_iterations = total_it;
-        out->rel_residual = res;
+        out->rel_residual = relinf;
         out->perturbed_pivots = npert;
         out->factor_flops = c->plan.flops;
         out->factor_nnz = c->plan.factor_nnz;
@@ -1453,5 +1456,5 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
         out->max_front = c->plan.max_front;
     }
     c->stale_solves++;
-    return (res <= tol * 10) ? HMEC_OK : HMEC_E_NOCONV;
+    return relinf <= tol ? HMEC_OK : HMEC_E_NOCONV;
 }

@@ -1,0 +1,540 @@
+// Sweep form of the multifrontal factor: the triangular solves of
+// PardisoLU::solve (h_mec_rsf_v9.cpp:1052) as one matrix-vector product per
+// front and direction.
+//
+// For a front with n = p + u rows, factored pivot block L11 U11 (unit lower /
+// upper), L21 (u x p) and U12 (p x u), the substitutions of one front are
+//   forward   y1 = L11^-1 b1,      c = b2 - L21 y1          (c -> parent)
+//   backward  x1 = U11^-1 (y1 - U12 x2)
+// With W = L21 L11^-1 and V = U11^-1 U12 (the off-diagonal blocks of the
+// inverses of the unit-extended n x n triangles) both become ONE product
+// whose input is complete before the front starts:
+//   [y1; c] = [b1; b2] + Fm b1      Fm = [strict_lower(L11^-1); -W]   n x p
+//   x1      = Bm [y1; x2]           Bm = [U11^-1 | -V]                 p x n
+// Fm and Bm of every front are formed once per factorization by one MFMA
+// launch that reads the triangle inverses (kept in place in the pivot block
+// by the factorization), L21 and U12, and writes a separate sweep arena with
+// both blocks ROW-major, rows padded to 16 bytes: every row is one contiguous
+// dot product, so a sweep parallelizes over rows without splitting K (no
+// partial sums, no inter-workgroup synchronization) and reads every stored
+// nonzero entry once with 16-byte vector loads.
+//
+// A sweep is one launch per level and direction; a workgroup owns a block of
+// rows of one front, gathers the front's input vector into LDS once, and a
+// group of L lanes (L chosen per level from the row length) forms each row's
+// dot product, reduced by a fixed butterfly.  The forward gathers the
+// children's contributions through a precomputed parent-row -> child-entry
+// map (children in a fixed order).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "hmec_internal.h"
+#include "mfma_traits.h"
+
+#define SW_LAUNCH_CHECK()                                                   \
+    do {                                                                    \
+        hipError_t e_ = hipGetLastError();                                  \
+        if (e_ != hipSuccess) {                                             \
+            hmec_set_last_error(std::string("launch: ") + hipGetErrorString(e_)); \
+            return HMEC_E_HIP;                                              \
+        }                                                                   \
+    } while (0)
+
+namespace {
+
+constexpr int SW_T = 256;      // threads per sweep workgroup
+constexpr int SW_RPG = 4;      // rows per lane group and workgroup block
+constexpr int SW_TILE = 64;    // sweep-form GEMM tile
+constexpr int SW_TK = 16;      // its K chunk
+
+struct SwFront {
+    int64_t aoff;    // front offset in the factor arena (n x n, ld n)
+    int64_t soff;    // Fm offset in the sweep arena (n rows x ldf)
+    int64_t boff;    // Bm offset in the sweep arena (p rows x ldb)
+    int64_t voff;    // front vector offset in the per-front vector arena
+    int64_t dofo;    // offset of the front's global dof list
+    int64_t goff;    // offset of the front's gather row pointers (n + 1)
+    int n, p;
+    int ldf, ldb;    // padded row lengths (multiples of 4 entries)
+};
+
+struct SwTask {
+    int f;           // front
+    int r0, nr;      // row block
+};
+
+struct SwTile {
+    int f;
+    int kind;        // 0: -W tile, 1: -V tile, 2: pivot-block copy tile
+    int ti, tj;
+};
+
+// ------------------------------------------------------------ sweep form
+template <typename T>
+__global__ void __launch_bounds__(256) k_sweep_form(const SwTile *tiles, const SwFront *fronts,
+                                                    const T *arena, T *sa) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    __shared__ T As[SW_TK][SW_TILE + 4];
+    __shared__ T Bs[SW_TK][SW_TILE + 4];
+    const SwTile tl = tiles[blockIdx.x];
+    const SwFront F = fronts[tl.f];
+    const int n = F.n, p = F.p, u = n - p;
+    const T *A = arena + F.aoff;
+    T *Fm = sa + F.soff;
+    T *Bm = sa + F.boff;
+    const int64_t ldf = F.ldf, ldb = F.ldb;
+    const int tid = threadIdx.x;
+    if (tl.kind == 2) {
+        // strict lower part of L11^-1 -> Fm rows < p; U11^-1 -> Bm columns < p
+        const int r0 = tl.ti * SW_TILE, c0 = tl.tj * SW_TILE;
+        for (int e = tid; e < SW_TILE * SW_TILE; e += blockDim.x) {
+            const int r = r0 + e % SW_TILE, k = c0 + e / SW_TILE;
+            if (r >= p || k >= p) continue;
+            const T v = A[(int64_t)k * n + r];
+            Fm[(int64_t)r * ldf + k] = r > k ? v : (T)0;
+            Bm[(int64_t)r * ldb + k] = r <= k ? v : (T)0;
+        }
+        return;
+    }
+    // kind 0: C = -(L21 * L11^-1)   rows of L21, K from the tile's first column
+    // kind 1: C = -(U11^-1 * U12)   rows of U11^-1, K from the tile's first row
+    const bool w = tl.kind == 0;
+    const int m = w ? min(SW_TILE, u - tl.ti * SW_TILE) : min(SW_TILE, p - tl.ti * SW_TILE);
+    const int nn = w ? min(SW_TILE, p - tl.tj * SW_TILE) : min(SW_TILE, u - tl.tj * SW_TILE);
+    const int kb = w ? tl.tj * SW_TILE : tl.ti * SW_TILE;
+    const int rbase = tl.ti * SW_TILE, cbase = tl.tj * SW_TILE;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    acc_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = acc_t{0, 0, 0, 0};
+    for (int kk = kb; kk < p; kk += SW_TK) {
+        for (int e = tid; e < SW_TK * SW_TILE; e += blockDim.x) {
+            // A operand, row index fastest (column-major source)
+            const int r = e % SW_TILE, k = e / SW_TILE, kg = kk + k;
+            T va = (T)0;
+            if (r < m && kg < p) {
+                if (w) {
+                    va = A[(int64_t)kg * n + p + rbase + r];                    // L21[r, kg]
+                } else {
+                    const int i = rbase + r;
+                    va = kg >= i ? A[(int64_t)kg * n + i] : (T)0;                // U11^-1[i, kg]
+                }
+            }
+            As[k][r] = va;
+            // B operand, K index fastest
+            const int cc = e / SW_TK, k2 = e % SW_TK, kg2 = kk + k2;
+            T vb = (T)0;
+            if (cc < nn && kg2 < p) {
+                if (w) {
+                    const int j = cbase + cc;                                    // L11^-1[kg2, j]
+                    vb = kg2 > j ? A[(int64_t)j * n + kg2] : (kg2 == j ? (T)1 : (T)0);
+                } else {
+                    vb = A[(int64_t)(p + cbase + cc) * n + kg2];                 // U12[kg2, cc]
+                }
+            }
+            Bs[k2][cc] = vb;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k4 = 0; k4 < SW_TK; k4 += 4) {
+            const T a0 = As[k4 + lk][wr + li], a1 = As[k4 + lk][wr + 16 + li];
+            const T b0 = Bs[k4 + lk][wc + li], b1 = Bs[k4 + lk][wc + 16 + li];
+            acc[0][0] = Mfma<T>::mma(a0, b0, acc[0][0]);
+            acc[0][1] = Mfma<T>::mma(a0, b1, acc[0][1]);
+            acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
+            acc[1][1] = Mfma<T>::mma(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = wr + a * 16 + Mfma<T>::row(lane, r), col = wc + b * 16 + li;
+                if (row >= m || col >= nn) continue;
+                if (w) Fm[(int64_t)(p + rbase + row) * ldf + cbase + col] = -acc[a][b][r];
+                else Bm[(int64_t)(rbase + row) * ldb + p + cbase + col] = -acc[a][b][r];
+            }
+}
+
+// ------------------------------------------------------------ sweep
+template <typename T> struct Vec16;
+template <> struct Vec16<float> {
+    typedef float4 type;
+    static constexpr int W = 4;
+    static __device__ __forceinline__ double dot(const type &v, const double *x) {
+        return (double)v.x * x[0] + (double)v.y * x[1] + (double)v.z * x[2] + (double)v.w * x[3];
+    }
+};
+template <> struct Vec16<double> {
+    typedef double2 type;
+    static constexpr int W = 2;
+    static __device__ __forceinline__ double dot(const type &v, const double *x) {
+        return v.x * x[0] + v.y * x[1];
+    }
+};
+
+// DIR 0 (forward, leaves -> root): x = b1 gathered from z (scaled right-hand
+// side at the front's pivot dofs) and the children's contributions;
+// out[r] = b[r] + Fm[r,:] x -> per-front vector (y1 rows < p, contribution c
+// rows >= p).
+// DIR 1 (backward, root -> leaves): x = [y1; z(update dofs)],
+// out[r] = Bm[r,:] x -> z at the front's pivot dofs.
+// Dynamic LDS: xs (K entries padded to a multiple of 4, zero padded) then,
+// forward only, the gathered b of the block's rows.
+template <typename T, int DIR, int L>
+__global__ void __launch_bounds__(SW_T) k_sweep(const SwTask *tasks, const SwFront *fronts,
+                                                const T *sa, const int32_t *dofs,
+                                                const int32_t *gptr, const int32_t *gidx,
+                                                double *z, double *wbuf) {
+    extern __shared__ double xs[];
+    typedef Vec16<T> V;
+    const SwTask t = tasks[blockIdx.x];
+    const SwFront F = fronts[t.f];
+    const int tid = threadIdx.x;
+    const int32_t *dof = dofs + F.dofo;
+    const int K = DIR == 0 ? F.p : F.n;
+    const int Kp = (K + 3) & ~3;
+    for (int k = tid; k < Kp; k += SW_T) {
+        double v = 0.0;
+        if (k < K) {
+            if (DIR == 0) {
+                v = z[dof[k]];
+                for (int e = gptr[F.goff + k]; e < gptr[F.goff + k + 1]; e++) v += wbuf[gidx[e]];
+            } else {
+                v = k < F.p ? wbuf[F.voff + k] : z[dof[k]];
+            }
+        }
+        xs[k] = v;
+    }
+    double *bi = xs + Kp;      // forward: b of rows >= p of this block
+    if (DIR == 0) {
+        for (int q = tid; q < t.nr; q += SW_T) {
+            const int r = t.r0 + q;
+            double v = 0.0;
+            if (r >= F.p)
+                for (int e = gptr[F.goff + r]; e < gptr[F.goff + r + 1]; e++) v += wbuf[gidx[e]];
+            bi[q] = v;
+        }
+    }
+    __syncthreads();
+    const int lane = tid % L, grp = tid / L;
+    constexpr int NG = SW_T / L;
+    const T *base = sa + (DIR == 0 ? F.soff : F.boff);
+    const int64_t ld = DIR == 0 ? F.ldf : F.ldb;
+    for (int q = grp; q < t.nr; q += NG) {
+        const int r = t.r0 + q;
+        int cb, ce;
+        if (DIR == 0) {
+            cb = 0;
+            ce = r < F.p ? r : F.p;      // Fm rows < p: strictly lower
+        } else {
+            cb = r & ~3;                 // Bm: upper in the pivot columns
+            ce = F.n;
+        }
+        const typename V::type *row = reinterpret_cast<const typename V::type *>(base + (int64_t)r * ld);
+        double acc = 0.0;
+        int c = cb + V::W * lane;
+        constexpr int STEP = V::W * L;
+        for (; c + 3 * STEP < ce; c += 4 * STEP) {
+            typename V::type v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = row[(c + u * STEP) / V::W];
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc += V::dot(v[u], xs + c + u * STEP);
+        }
+        for (; c < ce; c += STEP) acc += V::dot(row[c / V::W], xs + c);
+#pragma unroll
+        for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
+        if (lane == 0) {
+            if (DIR == 0) {
+                const double b = r < F.p ? xs[r] : bi[q];
+                wbuf[F.voff + r] = b + acc;
+            } else {
+                z[dof[r]] = acc;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ host state
+struct SweepDev {
+    bool ready = false;
+    SwFront *fronts = nullptr;
+    std::vector<SwFront> h_fronts;
+    int32_t *gptr = nullptr, *gidx = nullptr;
+    SwTile *tiles = nullptr;
+    int ntiles = 0;
+    std::vector<SwTask *> d_fwd, d_bwd;
+    std::vector<int> n_fwd, n_bwd;
+    std::vector<int> l_fwd, l_bwd;               // lanes per row
+    std::vector<int> lds_fwd, lds_bwd;           // dynamic LDS bytes
+    std::vector<double> bytes_fwd, bytes_bwd;    // stored entries read per level
+    int64_t sa_entries = 0;
+    void *sa[2] = {nullptr, nullptr};            // [0] FP32, [1] FP64 sweep arena
+    int built = -1;                              // precision of the current sweep form
+    double form_flops = 0;
+};
+
+std::mutex g_sw_mu;
+std::map<hmec_ctx *, SweepDev *> g_sw;
+
+SweepDev *swdev(hmec_ctx *c) {
+    std::lock_guard<std::mutex> lk(g_sw_mu);
+    auto it = g_sw.find(c);
+    if (it != g_sw.end()) return it->second;
+    SweepDev *s = new SweepDev();
+    g_sw[c] = s;
+    return s;
+}
+
+template <typename V>
+int upload(V **dst, const std::vector<V> &v) {
+    HMEC_HIP(hipMalloc(dst, sizeof(V) * std::max<size_t>(1, v.size())));
+    if (!v.empty()) HMEC_HIP(hipMemcpy(*dst, v.data(), sizeof(V) * v.size(), hipMemcpyHostToDevice));
+    return HMEC_OK;
+}
+
+// row-block tasks of one level and direction; lanes per row from the
+// level's largest row length
+void level_tasks(const std::vector<SwFront> &fr, const std::vector<int> &fronts, int dir,
+                 int vw, std::vector<SwTask> &tasks, int &lanes, int &lds, double &entries) {
+    tasks.clear();
+    entries = 0;
+    int kmax = 0;
+    for (int f : fronts) kmax = std::max(kmax, dir == 0 ? fr[f].p : fr[f].n);
+    lanes = 8;
+    while (lanes < 64 && lanes * vw * 2 < kmax) lanes *= 2;
+    const int rows_per_block = (SW_T / lanes) * SW_RPG;
+    int nrmax = 0;
+    for (int f : fronts) {
+        const int n = fr[f].n, p = fr[f].p;
+        const int m = dir == 0 ? n : p;
+        for (int r0 = 0; r0 < m; r0 += rows_per_block) {
+            const int nr = std::min(rows_per_block, m - r0);
+            tasks.push_back(SwTask{f, r0, nr});
+            nrmax = std::max(nrmax, nr);
+        }
+        // stored nonzero entries read: Fm = strict lower p x p + u x p,
+        // Bm = upper p x p + p x u
+        entries += (double)p * (p + (dir == 0 ? -1 : 1)) / 2 + (double)(n - p) * p;
+    }
+    const int kp = (kmax + 3) & ~3;
+    lds = (int)sizeof(double) * (kp + (dir == 0 ? nrmax : 0));
+}
+
+}  // namespace
+
+// Per-context sweep plan: fronts, gather map, tasks per level, form tiles.
+static int sweep_setup(hmec_ctx *c, SweepDev *s) {
+    const FrontPlan &pl = c->plan;
+    const DevPlan &dp = c->dp;
+    const int nf = pl.nfronts;
+    s->h_fronts.resize(nf);
+    int64_t soff = 0, goff = 0;
+    for (int f = 0; f < nf; f++) {
+        const int n = 6 * (pl.np[f] + pl.nu[f]), p = 6 * pl.np[f];
+        const int ldf = (p + 3) & ~3, ldb = (n + 3) & ~3;
+        const int64_t fo = soff, bo = soff + (int64_t)n * ldf;
+        s->h_fronts[f] = SwFront{pl.off[f], fo, bo, pl.voff[f], dp.h_front_dof_off[f], goff, n, p,
+                                 ldf, ldb};
+        soff = bo + (int64_t)p * ldb;
+        goff += n + 1;
+    }
+    s->sa_entries = soff;
+    if (pl.varena > INT32_MAX) {
+        hmec_set_last_error("sweep: vector arena exceeds 32-bit gather indices");
+        return HMEC_E_ARG;
+    }
+    // parent row -> child contribution entries, children in plan order
+    std::vector<int32_t> gptr(goff), gidx;
+    std::vector<std::vector<int32_t>> rows;
+    for (int f = 0; f < nf; f++) {
+        const SwFront &F = s->h_fronts[f];
+        rows.assign(F.n, {});
+        for (int ch : pl.children[f]) {
+            const int cn = 6 * (pl.np[ch] + pl.nu[ch]), cp = 6 * pl.np[ch];
+            const int32_t *map = pl.ea_map.data() + pl.ea_off[ch];
+            for (int a = 0; a < cn - cp; a++) {
+                const int loc = map[a / 6] * 6 + a % 6;
+                rows[loc].push_back((int32_t)(pl.voff[ch] + cp + a));
+            }
+        }
+        for (int r = 0; r < F.n; r++) {
+            gptr[F.goff + r] = (int32_t)gidx.size();
+            gidx.insert(gidx.end(), rows[r].begin(), rows[r].end());
+        }
+        gptr[F.goff + F.n] = (int32_t)gidx.size();
+    }
+    // sweep-form tiles
+    std::vector<SwTile> tiles;
+    double flops = 0;
+    for (int f = 0; f < nf; f++) {
+        const int n = s->h_fronts[f].n, p = s->h_fronts[f].p, u = n - p;
+        const int tp = (p + SW_TILE - 1) / SW_TILE, tu = (u + SW_TILE - 1) / SW_TILE;
+        for (int i = 0; i < tp; i++)
+            for (int j = 0; j < tp; j++) tiles.push_back(SwTile{f, 2, i, j});
+        for (int i = 0; i < tu; i++)
+            for (int j = 0; j < tp; j++) tiles.push_back(SwTile{f, 0, i, j});
+        for (int i = 0; i < tp; i++)
+            for (int j = 0; j < tu; j++) tiles.push_back(SwTile{f, 1, i, j});
+        flops += 2.0 * (double)u * p * p;     // two triangular products, u p^2 each
+    }
+    s->form_flops = flops;
+    s->ntiles = (int)tiles.size();
+    HMEC_TRY(upload(&s->fronts, s->h_fronts));
+    HMEC_TRY(upload(&s->gptr, gptr));
+    HMEC_TRY(upload(&s->gidx, gidx));
+    HMEC_TRY(upload(&s->tiles, tiles));
+    // tasks per level (the LDS need is the same for both precisions)
+    s->d_fwd.assign(pl.nlevels, nullptr);
+    s->d_bwd.assign(pl.nlevels, nullptr);
+    s->n_fwd.assign(pl.nlevels, 0);
+    s->n_bwd.assign(pl.nlevels, 0);
+    s->l_fwd.assign(pl.nlevels, 0);
+    s->l_bwd.assign(pl.nlevels, 0);
+    s->lds_fwd.assign(pl.nlevels, 0);
+    s->lds_bwd.assign(pl.nlevels, 0);
+    s->bytes_fwd.assign(pl.nlevels, 0);
+    s->bytes_bwd.assign(pl.nlevels, 0);
+    std::vector<SwTask> tk;
+    for (int lv = 0; lv < pl.nlevels; lv++) {
+        for (int dir = 0; dir < 2; dir++) {
+            int lanes, lds;
+            double by;
+            level_tasks(s->h_fronts, pl.levels[lv], dir, 4, tk, lanes, lds, by);
+            if (lds > 64 * 1024) {
+                hmec_set_last_error("sweep: front too large for the LDS input vector");
+                return HMEC_E_ARG;
+            }
+            HMEC_TRY(upload(dir == 0 ? &s->d_fwd[lv] : &s->d_bwd[lv], tk));
+            (dir == 0 ? s->n_fwd : s->n_bwd)[lv] = (int)tk.size();
+            (dir == 0 ? s->l_fwd : s->l_bwd)[lv] = lanes;
+            (dir == 0 ? s->lds_fwd : s->lds_bwd)[lv] = lds;
+            (dir == 0 ? s->bytes_fwd : s->bytes_bwd)[lv] = by;
+        }
+    }
+    s->ready = true;
+    return HMEC_OK;
+}
+
+// Form Fm / Bm of every front from the factor in the arena of the given
+// precision (triangle inverses in place, L21, U12).
+int sv_sweep_build(hmec_ctx *c, int precision) {
+    SweepDev *s = swdev(c);
+    if (!s->ready) HMEC_TRY(sweep_setup(c, s));
+    const int q = precision == HMEC_PREC_FP64 ? 1 : 0;
+    const size_t es = q ? sizeof(double) : sizeof(float);
+    if (!s->sa[q]) {
+        if (hipMalloc(&s->sa[q], es * std::max<int64_t>(1, s->sa_entries)) != hipSuccess) {
+            hipGetLastError();
+            hmec_set_last_error("sweep arena allocation failed");
+            return HMEC_E_OOM;
+        }
+        // row padding and the unused triangles are never written: zero once
+        HMEC_HIP(hipMemset(s->sa[q], 0, es * std::max<int64_t>(1, s->sa_entries)));
+    }
+    hipStream_t st = c->stream;
+    kt_begin(c, "sweep_form");
+    if (s->ntiles > 0) {
+        if (q)
+            hipLaunchKernelGGL(k_sweep_form<double>, dim3(s->ntiles), dim3(256), 0, st, s->tiles,
+                               s->fronts, (const double *)c->d.arena, (double *)s->sa[q]);
+        else
+            hipLaunchKernelGGL(k_sweep_form<float>, dim3(s->ntiles), dim3(256), 0, st, s->tiles,
+                               s->fronts, (const float *)c->d.arena32, (float *)s->sa[q]);
+        SW_LAUNCH_CHECK();
+    }
+    kt_end(c, "sweep_form", s->form_flops);
+    s->built = precision;
+    return HMEC_OK;
+}
+
+bool sv_sweep_ready(hmec_ctx *c, int precision) {
+    SweepDev *s = swdev(c);
+    return s->built == precision;
+}
+
+void sv_sweep_invalidate(hmec_ctx *c) { swdev(c)->built = -1; }
+
+template <typename T, int DIR>
+static int launch_sweep(hmec_ctx *c, SweepDev *s, int lv, const T *sa, double *z) {
+    const int lanes = DIR == 0 ? s->l_fwd[lv] : s->l_bwd[lv];
+    const int n = DIR == 0 ? s->n_fwd[lv] : s->n_bwd[lv];
+    const int lds = DIR == 0 ? s->lds_fwd[lv] : s->lds_bwd[lv];
+    const SwTask *tk = DIR == 0 ? s->d_fwd[lv] : s->d_bwd[lv];
+    hipStream_t st = c->stream;
+    const int32_t *dofs = c->dp.d_front_dofs;
+    double *wbuf = c->d.vec;
+#define SW_GO(LN)                                                                           \
+    hipLaunchKernelGGL((k_sweep<T, DIR, LN>), dim3(n), dim3(SW_T), lds, st, tk, s->fronts, sa, \
+                       dofs, s->gptr, s->gidx, z, wbuf)
+    switch (lanes) {
+        case 8: SW_GO(8); break;
+        case 16: SW_GO(16); break;
+        case 32: SW_GO(32); break;
+        default: SW_GO(64); break;
+    }
+#undef SW_GO
+    SW_LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// z (scaled right-hand side) -> solution of the scaled system
+int sv_sweep_solve(hmec_ctx *c, double *z) {
+    SweepDev *s = swdev(c);
+    if (s->built < 0) return HMEC_E_ARG;
+    const FrontPlan &pl = c->plan;
+    const bool q = s->built == HMEC_PREC_FP64;
+    const double es = q ? 8.0 : 4.0;
+    for (int lv = 0; lv < pl.nlevels; lv++) {
+        kt_begin(c, "sweep_fwd");
+        if (q) HMEC_TRY((launch_sweep<double, 0>(c, s, lv, (const double *)s->sa[1], z)));
+        else HMEC_TRY((launch_sweep<float, 0>(c, s, lv, (const float *)s->sa[0], z)));
+        kt_end(c, "sweep_fwd", s->bytes_fwd[lv] * es);
+    }
+    for (int lv = pl.nlevels - 1; lv >= 0; lv--) {
+        kt_begin(c, "sweep_bwd");
+        if (q) HMEC_TRY((launch_sweep<double, 1>(c, s, lv, (const double *)s->sa[1], z)));
+        else HMEC_TRY((launch_sweep<float, 1>(c, s, lv, (const float *)s->sa[0], z)));
+        kt_end(c, "sweep_bwd", s->bytes_bwd[lv] * es);
+    }
+    return HMEC_OK;
+}
+
+void sv_sweep_free(hmec_ctx *c) {
+    SweepDev *s = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_sw_mu);
+        auto it = g_sw.find(c);
+        if (it == g_sw.end()) return;
+        s = it->second;
+        g_sw.erase(it);
+    }
+    hipFree(s->fronts);
+    hipFree(s->gptr);
+    hipFree(s->gidx);
+    hipFree(s->tiles);
+    for (auto *p : s->d_fwd) hipFree(p);
+    for (auto *p : s->d_bwd) hipFree(p);
+    hipFree(s->sa[0]);
+    hipFree(s->sa[1]);
+    delete s;
+}
+
+int64_t sv_sweep_bytes(hmec_ctx *c, int precision) {
+    SweepDev *s = swdev(c);
+    return s->sa_entries * (precision == HMEC_PREC_FP64 ? 8 : 4);
+}
