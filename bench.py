@@ -7,13 +7,24 @@ uploaded once.  A step is one full timestep: marker->node scatter, then the
 Picard loop of VEP update, CSR assembly, sparse LU solve, RSF fault solve and
 timestep control, then the final stresses and history.
 
-value = timesteps/s of the FP32-factor + FP64-iterative-refinement path
-        (relative residual <= 1e-12 every solve); the plain FP64-factor path
-        (refactored every Picard iteration) is timed beside it.
+value = timesteps/s of the mixed-precision path: an FP32 multifrontal factor
+        reused across Picard iterations and timesteps as the preconditioner of
+        FP64 GMRES on the original system (relative inf-norm residual
+        <= 1e-12 every solve; refactored in FP32 when GMRES needs more than
+        its budget).  The plain FP64-factor path (refactored every Picard
+        iteration, PardisoLU semantics) is timed beside it.
 roofline = the dominant kernel of the FP32 path, measured with HIP events
         on the context's stream over the timed region.
 cpu_baseline = the CPU oracle (plain-C restatement + MKL PARDISO, sequential)
         on one timestep of the same workload, on this host.
+
+--serve runs the serving workload instead and prints its own JSON line: a
+Poisson-arrival stream of ensemble members with grids drawn uniformly from
+{101x51, 201x51, 401x101, 801x201}, a ~ U[0.008, 0.012], b ~ U[0.012, 0.020],
+L ~ U[0.01, 0.05] m on the fault and one log-uniform permeability in
+[1e-16, 1e-14] m^2 per member, served by the ensemble server (admission by
+device footprint, continuous batching, batched same-size factorizations);
+value = member-timesteps/s with p50/p99 step latency.
 """
 import argparse
 import ctypes as C
@@ -56,8 +67,8 @@ def run_path(hm, p, soa, prec, warmup, steps):
         lat.append(time.perf_counter() - a)
     total = time.perf_counter() - t0
     kt = {}
-    for name in ("scatter", "assemble", "factor", "solve", "schur", "fwd", "bwd", "fwd_upd",
-                 "bwd_upd", "extend"):
+    for name in ("scatter", "assemble", "factor", "solve", "schur", "sweep_form", "sweep_fwd",
+                 "sweep_bwd", "fwd_upd", "bwd_upd"):
         n, ms, work = ctx.kernel_time(name)
         if n > 0:
             kt[name] = {"n": n, "avg_ms": ms, "work": work}
@@ -81,6 +92,55 @@ def cpu_baseline(hm, p, soa):
                       % info.iterations}
 
 
+def serve(hm, args):
+    rng = np.random.default_rng(args.serve_seed)
+    grids = [(101, 51), (201, 51), (401, 101), (801, 201)]
+    prec = hm.PREC_FP64 if args.serve_fp64 else hm.PREC_FP32
+    srv = hm.Server(precision=prec, max_active=args.serve_slots, batch_window_ms=2.0)
+    t = 0.0
+    mix = {}
+    for k in range(args.serve_members):
+        t += rng.exponential(args.serve_interarrival_ms)
+        nx, ny = grids[int(rng.integers(len(grids)))]
+        a = rng.uniform(0.008, 0.012)
+        b = rng.uniform(0.012, 0.020)
+        L = rng.uniform(0.01, 0.05)
+        p = hm.params(1, Nx=nx, Ny=ny, arsfm=(0.011, a), brsfm=(0.007, b), lrsfm=(0.02, L))
+        srv.submit(p, args.serve_steps, cfg=5, seed=1000 + k, arrival_ms=t)
+        mix["%dx%d" % (nx, ny)] = mix.get("%dx%d" % (nx, ny), 0) + 1
+    srv.run()
+    st = srv.stats()
+    res = [srv.result(i) for i in range(args.serve_members)]
+    out = {
+        "metric": "member-timesteps/s, serving mix (continuous batching)",
+        "value": st.member_steps_per_s,
+        "unit": "member-timesteps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "p50_ms": st.p50_ms,
+        "p99_ms": st.p99_ms,
+        "wall_s": st.wall_ms * 1e-3,
+        "member_steps": int(st.member_steps),
+        "members_done": st.members_done,
+        "members_failed": st.members_failed,
+        "max_concurrent": st.max_concurrent,
+        "peak_admitted_GB": st.peak_bytes / 1e9,
+        "factor_batches": st.factor_batches,
+        "factor_batch_members": st.factor_batch_members,
+        "factorizations": int(sum(r.factorizations for r in res)),
+        "fallbacks": int(sum(r.fallbacks for r in res)),
+        "max_rel_residual": float(max(r.max_rel_residual for r in res)),
+        "dtype": "f32-factor+f64" if prec == hm.PREC_FP32 else "f64",
+        "data": "synthetic (seeded host generator per member)",
+        "config": {"workload": "serving mix", "members": args.serve_members,
+                   "steps_per_member": args.serve_steps, "grids": mix,
+                   "mean_interarrival_ms": args.serve_interarrival_ms,
+                   "slots": args.serve_slots},
+    }
+    srv.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +149,13 @@ def main():
     ap.add_argument("--cfg", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--serve", action="store_true", help="serving workload instead")
+    ap.add_argument("--serve-members", type=int, default=64)
+    ap.add_argument("--serve-steps", type=int, default=100)
+    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-interarrival-ms", type=float, default=50.0)
+    ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--serve-fp64", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -107,6 +174,9 @@ def main():
               "or grid runs on one device); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     hm = load_hmec()
+    if args.serve:
+        serve(hm, args)
+        return
     p = hm.params(args.cfg)
     soa = hm.generate_markers(p, cfg=args.cfg, seed=args.cfg)
     ctx32, t32, lat32, inf32, kt32 = run_path(hm, p, soa, hm.PREC_FP32, args.warmup, args.steps)
@@ -136,7 +206,8 @@ def main():
                                                    "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
                                                    "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
                                                    "traffic": None}))
-    for nm, kn in (("fwd_upd", "k_fwd_upd<float>"), ("bwd_upd", "k_bwd_upd<float>")):
+    for nm, kn in (("sweep_fwd", "k_sweep<float,0> (forward, per level)"),
+                   ("sweep_bwd", "k_sweep<float,1> (backward, per level)")):
         k = kt32.get(nm)
         if k:
             ach = k["work"] / (k["avg_ms"] * 1e-3)

@@ -22,7 +22,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"), "-I
 # the physics kernels (the solver's inner products use explicit fma / MFMA).
 HIPFLAGS = ["-x", "hip", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
             "-packed-fp32-ops", "-ffp-contract=off", "-fno-gpu-rdc", "-munsafe-fp-atomics"]
-HOSTFLAGS = ["-ffp-contract=off"]
+HOSTFLAGS = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", "-ffp-contract=off"]
 
 
 def _compile(src):

@@ -70,6 +70,11 @@ bool controller(hmec_ctx *c, const hmec_iter_scalars &sc) {
                                  std::fabs(vratio) < P.vratiomax));
 }
 
+// Numeric factorization, through the server's batcher for server members
+int factor(hmec_ctx *c, int precision) {
+    return c->batcher ? batcher_factorize(c->batcher, c, precision) : sv_factorize(c, precision);
+}
+
 // Linear solve of the freshly assembled system under the context's precision
 // policy.  FP64: numeric refactorization every Picard iteration, like
 // PardisoLU (h_mec_rsf_v9.cpp:1050-1052).  FP32: the equilibrated FP32 factor
@@ -80,7 +85,7 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
     std::memset(&st, 0, sizeof(st));
     if (c->precision == HMEC_PREC_FP64) {
         tm_begin(c, c->t_factor);
-        HMEC_TRY(sv_factorize(c, HMEC_PREC_FP64));
+        HMEC_TRY(factor(c, HMEC_PREC_FP64));
         tm_end(c, c->t_factor, c->plan.flops);
         acc.factorizations++;
         tm_begin(c, c->t_solve);
@@ -95,7 +100,7 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         bool fresh = false;
         if (!c->factor_valid || c->factor_prec != HMEC_PREC_FP32) {
             tm_begin(c, c->t_factor);
-            HMEC_TRY(sv_factorize(c, HMEC_PREC_FP32));
+            HMEC_TRY(factor(c, HMEC_PREC_FP32));
             tm_end(c, c->t_factor, c->plan.flops);
             acc.factorizations++;
             c->factor_dt = c->dt;
@@ -107,7 +112,7 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         acc.ir_iterations += st.ir_iterations;
         if (rc == HMEC_E_NOCONV && !fresh) {
             tm_begin(c, c->t_factor);
-            HMEC_TRY(sv_factorize(c, HMEC_PREC_FP32));
+            HMEC_TRY(factor(c, HMEC_PREC_FP32));
             tm_end(c, c->t_factor, c->plan.flops);
             acc.factorizations++;
             c->factor_dt = c->dt;
@@ -118,7 +123,7 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         }
         if (rc == HMEC_E_NOCONV) {
             acc.fallbacks++;
-            HMEC_TRY(sv_factorize(c, HMEC_PREC_FP64));
+            HMEC_TRY(factor(c, HMEC_PREC_FP64));
             acc.factorizations++;
             rc = sv_solve(c, 3, 1e-14, &st);
             if (rc == HMEC_E_NOCONV) rc = HMEC_OK;

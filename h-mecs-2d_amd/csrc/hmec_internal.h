@@ -127,6 +127,7 @@ struct Timer {
 };
 
 struct hmec_ctx_impl;
+struct hmec_batcher;
 
 // Device buffers owned by a context.
 struct DeviceState {
@@ -186,6 +187,8 @@ struct hmec_ctx {
     double last_res = 0;
     int stale_solves = 0;            // FP32: solves since the last factorization
     double factor_dt = -1;           // dt of the system the factor was computed for
+    hmec_batcher *batcher = nullptr; // server member: factorizations go through the batcher
+    int batched_facs = 0;            // factorizations done inside a batch of > 1 members
     // statistics of the current step
     hmec_step_info acc;
     // timing
@@ -216,12 +219,15 @@ int hmec_build_pattern(hmec_ctx *c);
 int sv_setup_device(hmec_ctx *c);
 int sv_free_device(hmec_ctx *c);
 int sv_factorize(hmec_ctx *c, int precision);
+int sv_factorize_batch(hmec_ctx *const *cs, int n, int precision);
 int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *st);
 int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision);
 int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *st);
 void sv_gmres_free(hmec_ctx *c);
+// ---- serving (hmec_serve.cpp) ----
+int batcher_factorize(hmec_batcher *b, hmec_ctx *c, int precision);
 // ---- sweep form of the factor (solver_sweep.hip) ----
-int sv_sweep_build(hmec_ctx *c, int precision);
+int sv_sweep_build_batch(hmec_ctx *const *cs, int n, int precision);
 bool sv_sweep_ready(hmec_ctx *c, int precision);
 void sv_sweep_invalidate(hmec_ctx *c);
 int sv_sweep_solve(hmec_ctx *c, double *z);

@@ -127,6 +127,8 @@ extern "C" int hmec_generate_markers(const hmec_params *p, int cfg, uint64_t see
     const double dyms = p->ysize / (double)Nym;      // :84
     const int64_t M = (int64_t)Nxm * Nym;
     const double lnsig = std::log(1.05);
+    // serving mix: one permeability per member, log-uniform in [1e-16, 1e-14]
+    const double kmember = std::pow(10.0, -16.0 + 2.0 * uni(seed, -1, 9));
     for (int64_t m = 0; m < M; m++) {
         // h_mec_rsf_v9.cpp:190-191 lattice, optional uniform jitter in the sub-cell
         const double jx = jitter ? uni(seed, m, 0) : 0.0;
@@ -157,6 +159,8 @@ extern "C" int hmec_generate_markers(const hmec_params *p, int cfg, uint64_t see
                 etasm = 1e23 * std::pow(10.0, -c);
             }
             kkkm = std::pow(10.0, -16.0 + 2.0 * uni(seed, m, 5));
+        } else if (cfg == 5) {
+            kkkm = kmember;
         }
         double etam = host ? etasm : etasm * std::exp(-p->alpha * porm);   // :194,199
         soa[HMEC_M_XM][m] = xm;

@@ -2,6 +2,8 @@
 // CSR pattern of the coupled system and the nested-dissection front plan.
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "hmec_internal.h"
@@ -89,4 +91,64 @@ extern "C" int hmec_plan_levels(const hmec_params *p, int *nlevels, int *nfronts
         if (flops) flops[lv] = fl;
     }
     return HMEC_OK;
+}
+
+// Device bytes a context of this grid allocates under the given precision
+// policy, from the exact symbolic plan (cached per grid): markers and their
+// derived streams and sort buffers, nodal fields, CSR and its triplet map,
+// solver plan arrays and vectors, then the factor storage of the policy
+// (FP64: front arena + inverted diagonal blocks; FP32: FP32 front arena,
+// inverted blocks, sweep arena and the GMRES basis).
+namespace {
+struct PlanSizes {
+    int64_t arena = 0, varena = 0, nnz = 0, ntrip = 0, sweep = 0, dinv = 0, nfronts = 0;
+};
+std::mutex g_ps_mu;
+std::map<std::pair<int, int>, PlanSizes> g_ps;
+}  // namespace
+
+extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
+    if (!p) return -1;
+    hmec_params P = *p;
+    if (hmec_params_finalize(&P) != HMEC_OK) return -1;
+    PlanSizes z;
+    bool have = false;
+    {
+        std::lock_guard<std::mutex> lk(g_ps_mu);
+        auto it = g_ps.find({P.Nx, P.Ny});
+        if (it != g_ps.end()) {
+            z = it->second;
+            have = true;
+        }
+    }
+    if (!have) {
+        std::vector<int32_t> rp, ci, slot, nt;
+        if (hmec_pattern_host(P, rp, ci, &slot, &nt) != HMEC_OK) return -1;
+        FrontPlan pl;
+        if (hmec_plan_build(P.Nx + 1, P.Ny + 1, rp, ci, pl) != HMEC_OK) return -1;
+        z.arena = pl.arena;
+        z.varena = pl.varena;
+        z.nnz = (int64_t)ci.size();
+        z.ntrip = (int64_t)slot.size();
+        z.nfronts = pl.nfronts;
+        for (int f = 0; f < pl.nfronts; f++) {
+            const int64_t n = 6 * (pl.np[f] + pl.nu[f]), pv = 6 * pl.np[f];
+            z.sweep += n * ((pv + 3) & ~3) + pv * ((n + 3) & ~3);
+            z.dinv += ((pv + 31) / 32) * 2 * 32 * 32;
+        }
+        std::lock_guard<std::mutex> lk(g_ps_mu);
+        g_ps[{P.Nx, P.Ny}] = z;
+    }
+    const int64_t ns = (int64_t)(P.Nx + 1) * (P.Ny + 1), N = 6 * ns;
+    const int64_t M = hmec_marker_count(&P);
+    int64_t b = M * 8 * (HMEC_NMARKER_FIELDS + MP_NPROP) + M * 4 * (4 + 4 + 2) + 16 * ns;
+    b += ns * 8 * HMEC_NFIELDS;
+    b += z.nnz * (4 + 8 + 8 + 4 + 3 * 4) + (N + 1) * 8 + z.ntrip * 4;   // CSR, plan maps, CSC
+    b += 6 * N * 8 + z.varena * 8 + z.nfronts * 256;                    // vectors, plan arrays
+    if (precision == HMEC_PREC_FP64) {
+        b += z.arena * 8 + z.dinv * 8;
+    } else {
+        b += z.arena * 4 + z.dinv * 4 + z.sweep * 4 + (2 * 20 + 2) * N * 8;
+    }
+    return b;
 }

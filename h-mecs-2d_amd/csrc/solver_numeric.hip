@@ -43,6 +43,21 @@ constexpr int PR = 256;     // rows / columns per panel-solve workgroup
         }                                                                   \
     } while (0)
 
+// Per-member device pointers of a (possibly batched) factorization: every
+// dense kernel of the factorization takes the table and blockIdx.z selects
+// the member, so same-size contexts share one launch per kernel (they share
+// the symbolic plan and its task arrays).
+template <typename T>
+struct FacPtrs {
+    T *arena;              // front arena
+    T *dinv;               // inverted NB x NB diagonal blocks
+    T *dscr;               // factored diagonal blocks of the current panel
+    T *icopy;              // pivot-block copies of the current level (inversion)
+    int *npert;            // perturbed-pivot counter
+    const double *val;     // CSR values
+    const double *rs, *cs; // equilibration scales
+};
+
 // ------------------------------------------------------------ equilibration
 __device__ __forceinline__ double pow2_recip(double m) {
     // 2^-e with m in [2^(e-1), 2^e): a power of two near 1/m (exact scaling)
@@ -70,26 +85,28 @@ __global__ void k_col_scale(const int32_t *cptr, const int32_t *cidx, const int3
 }
 
 template <typename T>
-__global__ void k_fill_zero(T *a, int64_t n) {
+__global__ void k_fill_zero(const FacPtrs<T> *mem, int64_t n) {
+    T *a = mem[blockIdx.z].arena;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; t < n; t += stride) a[t] = (T)0;
 }
 
 template <typename T>
-__global__ void k_front_assemble(const double *val, const int32_t *row_of, const int32_t *coli,
-                                 const double *rs, const double *cs, const int64_t *dst,
-                                 int64_t nnz, T *arena) {
+__global__ void k_front_assemble(const FacPtrs<T> *mem, const int32_t *row_of, const int32_t *coli,
+                                 const int64_t *dst, int64_t nnz) {
+    const FacPtrs<T> M = mem[blockIdx.z];
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nnz) return;
-    arena[dst[p]] = (T)(val[p] * rs[row_of[p]] * cs[coli[p]]);
+    M.arena[dst[p]] = (T)(M.val[p] * M.rs[row_of[p]] * M.cs[coli[p]]);
 }
 
 // ------------------------------------------------------------ extend-add
 // parent(map(a), map(b)) += child CB(a, b); one launch per child slot so no
 // two workgroups of a launch touch the same parent entry.
 template <typename T>
-__global__ void k_extend_add(const EaTask *tasks, const int32_t *ea_map, T *arena) {
+__global__ void k_extend_add(const EaTask *tasks, const int32_t *ea_map, const FacPtrs<T> *mem) {
+    T *arena = mem[blockIdx.z].arena;
     const EaTask tk = tasks[blockIdx.y];
     const int u = tk.child_n - tk.child_p;
     const int nt = (u + TM - 1) / TM;
@@ -151,8 +168,12 @@ __device__ __forceinline__ void invert_tri(T (*D)[NB + 1], T (*Inv)[NB + 1], boo
 //   Y = L_D^-1 A for columns [k0+kb, n) (products with the inverted
 //   triangles: no serial chain per row).
 template <typename T>
-__global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, T *arena, T *dscr,
-                                              T *dinv, double tau, int *npert) {
+__global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, const FacPtrs<T> *mem,
+                                              double tau) {
+    T *const arena = mem[blockIdx.z].arena;
+    T *const dscr = mem[blockIdx.z].dscr;
+    T *const dinv = mem[blockIdx.z].dinv;
+    int *const npert = mem[blockIdx.z].npert;
     __shared__ T D[NB][NB + 1];
     __shared__ T Inv[NB][NB + 1];     // U_D^-1 (upper) for row blocks, L_D^-1 (unit lower) for columns
     const FrontTask tk = tasks[blockIdx.y];
@@ -322,8 +343,10 @@ __device__ __forceinline__ void gemm_tile_sub(T *C, const T *A, const T *B, int6
 
 // trailing update of panel k0 restricted to rows < p or columns < p
 template <typename T>
-__global__ void __launch_bounds__(256) k_update(const FrontTask *tasks, int k0, T *arena,
-                                                const T *dscr) {
+__global__ void __launch_bounds__(256) k_update(const FrontTask *tasks, int k0,
+                                                const FacPtrs<T> *mem) {
+    T *const arena = mem[blockIdx.z].arena;
+    const T *const dscr = mem[blockIdx.z].dscr;
     const FrontTask tk = tasks[blockIdx.y];
     if (tk.p <= k0) return;
     const int kb = min(NB, tk.p - k0);
@@ -364,7 +387,8 @@ __global__ void __launch_bounds__(256) k_update(const FrontTask *tasks, int k0, 
 
 // Schur complement CB -= L21 U12 (K = p), 64 x 64 MFMA tiles
 template <typename T>
-__global__ void __launch_bounds__(256) k_schur(const FrontTask *tasks, T *arena) {
+__global__ void __launch_bounds__(256) k_schur(const FrontTask *tasks, const FacPtrs<T> *mem) {
+    T *const arena = mem[blockIdx.z].arena;
     const FrontTask tk = tasks[blockIdx.y];
     const int u = tk.n - tk.p;
     if (u <= 0 || tk.p <= 0) return;
@@ -388,7 +412,9 @@ __global__ void __launch_bounds__(256) k_schur(const FrontTask *tasks, T *arena)
 //   U: Y[blk, r0:p] = -inv(U_blk) U[blk, r0:p] Y[r0:p, r0:p]
 // (inv(L_blk), inv(U_blk) are the panel's NB x NB inverted diagonal blocks).
 template <typename T>
-__global__ void k_pivot_copy(const FrontTask *tasks, const T *arena, T *copy) {
+__global__ void k_pivot_copy(const FrontTask *tasks, const FacPtrs<T> *mem) {
+    const T *const arena = mem[blockIdx.z].arena;
+    T *const copy = mem[blockIdx.z].icopy;
     const FrontTask tk = tasks[blockIdx.y];
     const int p = tk.p;
     const int64_t tot = (int64_t)p * p;
@@ -400,8 +426,11 @@ __global__ void k_pivot_copy(const FrontTask *tasks, const T *arena, T *copy) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_tri_inv(const FrontTask *tasks, int j, T *arena,
-                                                 const T *copy, const T *dinv) {
+__global__ void __launch_bounds__(256) k_tri_inv(const FrontTask *tasks, int j,
+                                                 const FacPtrs<T> *mem) {
+    T *const arena = mem[blockIdx.z].arena;
+    const T *const copy = mem[blockIdx.z].icopy;
+    const T *const dinv = mem[blockIdx.z].dinv;
     __shared__ T As[64][NB + 1];
     __shared__ T Bs[NB][64 + 1];
     __shared__ T Ts[64][NB + 1];
@@ -411,8 +440,8 @@ __global__ void __launch_bounds__(256) k_tri_inv(const FrontTask *tasks, int j, 
     const int c0 = j * NB;
     if (c0 >= p) return;
     const int jw = min(NB, p - c0), r0 = c0 + jw;
-    const bool lower = blockIdx.z == 0;
-    const int tile = blockIdx.x;
+    const bool lower = (blockIdx.x & 1) == 0;
+    const int tile = blockIdx.x >> 1;
     const int64_t n = tk.n;
     T *F = arena + tk.off;
     const T *Cp = copy + tk.coff;
@@ -764,6 +793,9 @@ struct SolverDev {
     int max_level_fronts = 0;
     std::vector<int> maxu;         // per level max update size
     std::vector<int> maxt_fwd, maxt_bwd;   // per level row-chunk counts
+    void *mem = nullptr;           // member pointer table of the last factorization
+    int64_t mem_cap = 0;           // bytes
+    std::vector<unsigned char> mem_host;   // its host copy
 };
 
 static SolverDev *sdev(hmec_ctx *c);
@@ -818,6 +850,7 @@ int sv_free_device(hmec_ctx *c) {
         hipFree(s->dinv32);
         hipFree(s->part);
         hipFree(s->icopy);
+        hipFree(s->mem);
         if (s->h_norms) hipHostFree(s->h_norms);
         delete s;
     }
@@ -1011,19 +1044,20 @@ static int ensure_arena(hmec_ctx *c, int precision) {
 }
 
 template <typename T>
-static int factor_levels(hmec_ctx *c, T *arena, T *dinv, double tau) {
+static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipStream_t st,
+                         double tau) {
+    hmec_ctx *c = cs[0];                   // plan and task arrays shared by the batch
     const FrontPlan &pl = c->plan;
     DevPlan &dp = c->dp;
-    hipStream_t st = c->stream;
-    const int64_t N = c->N, nnz = c->nnz;
+    SolverDev *s = sdev(c);
+    const int64_t nnz = c->nnz;
+    const unsigned Bz = (unsigned)B;
     // zero + assemble
-    hipLaunchKernelGGL(k_fill_zero<T>, dim3(4096), dim3(256), 0, st, arena, pl.arena);
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3(4096, 1, Bz), dim3(256), 0, st, mem, pl.arena);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_front_assemble<T>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st,
-                       c->d.val, dp.d_row_of, c->d.coli, c->d.rscale, c->d.cscale, dp.d_nz_dst,
-                       nnz, arena);
+    hipLaunchKernelGGL(k_front_assemble<T>, dim3((unsigned)((nnz + 255) / 256), 1, Bz), dim3(256), 0,
+                       st, mem, dp.d_row_of, c->d.coli, dp.d_nz_dst, nnz);
     LAUNCH_CHECK();
-    (void)N;
     for (int lv = 0; lv < pl.nlevels; lv++) {
         const int nfr = (int)dp.h_tasks[lv].size();
         // extend-add children
@@ -1037,14 +1071,12 @@ static int factor_levels(hmec_ctx *c, T *arena, T *dinv, double tau) {
                 maxt = std::max(maxt, nt * nt);
             }
             if (maxt == 0) continue;
-            hipLaunchKernelGGL(k_extend_add<T>, dim3(maxt, (unsigned)ea.size()), dim3(256), 0, st,
-                               dp.d_ea[q][lv], dp.d_ea_map, arena);
+            hipLaunchKernelGGL(k_extend_add<T>, dim3(maxt, (unsigned)ea.size(), Bz), dim3(256), 0, st,
+                               dp.d_ea[q][lv], dp.d_ea_map, mem);
             LAUNCH_CHECK();
         }
         // pivot panels
-        const int maxp = dp.maxp[lv], maxn = dp.maxn[lv];
-        SolverDev *s = sdev(c);
-        T *dscr = reinterpret_cast<T *>(s->dscr);
+        const int maxp = dp.maxp[lv];
         for (int k0 = 0; k0 < maxp; k0 += NB) {
             int gx = 1;
             for (const auto &t : dp.h_tasks[lv]) {
@@ -1053,8 +1085,8 @@ static int factor_levels(hmec_ctx *c, T *arena, T *dinv, double tau) {
                 const int nch = (t.n - k0 - kb + PR - 1) / PR;
                 gx = std::max(gx, nch <= 1 ? 1 : 1 + 2 * nch);
             }
-            hipLaunchKernelGGL(k_panel<T>, dim3(gx, nfr), dim3(PR), 0, st, dp.d_tasks[lv],
-                               k0, arena, dscr, dinv, tau, c->d.counters);
+            hipLaunchKernelGGL(k_panel<T>, dim3(gx, nfr, Bz), dim3(PR), 0, st, dp.d_tasks[lv], k0, mem,
+                               tau);
             LAUNCH_CHECK();
             int maxt = 0;
             for (const auto &t : dp.h_tasks[lv]) {
@@ -1064,8 +1096,8 @@ static int factor_levels(hmec_ctx *c, T *arena, T *dinv, double tau) {
                 const int tu = (t.n - t.p + TM - 1) / TM;
                 maxt = std::max(maxt, tp * (tp + tu) + tu * tp);
             }
-            hipLaunchKernelGGL(k_update<T>, dim3(std::max(1, maxt), nfr), dim3(256), 0, st,
-                               dp.d_tasks[lv], k0, arena, (const T *)dscr);
+            hipLaunchKernelGGL(k_update<T>, dim3(std::max(1, maxt), nfr, Bz), dim3(256), 0, st,
+                               dp.d_tasks[lv], k0, mem);
             LAUNCH_CHECK();
         }
         int maxt = 0;
@@ -1075,53 +1107,113 @@ static int factor_levels(hmec_ctx *c, T *arena, T *dinv, double tau) {
         }
         if (maxt > 0) {
             kt_begin(c, "schur");
-            hipLaunchKernelGGL(k_schur<T>, dim3(maxt, nfr), dim3(256), 0, st, dp.d_tasks[lv], arena);
+            hipLaunchKernelGGL(k_schur<T>, dim3(maxt, nfr, Bz), dim3(256), 0, st, dp.d_tasks[lv], mem);
             LAUNCH_CHECK();
-            kt_end(c, "schur", sdev(c)->schur_flops[lv]);
+            kt_end(c, "schur", s->schur_flops[lv] * B);
         }
         // L11, U11 -> inverses (pivot block only: disjoint from L21, U12, CB)
-        {
-            T *cp = reinterpret_cast<T *>(s->icopy);
-            hipLaunchKernelGGL(k_pivot_copy<T>, dim3(std::max(1, std::min(1024, maxp * maxp / 256 + 1)), nfr),
-                               dim3(256), 0, st, dp.d_tasks[lv], (const T *)arena, cp);
+        hipLaunchKernelGGL(k_pivot_copy<T>, dim3(std::max(1, std::min(1024, maxp * maxp / 256 + 1)), nfr, Bz),
+                           dim3(256), 0, st, dp.d_tasks[lv], mem);
+        LAUNCH_CHECK();
+        const int nblk = (maxp + NB - 1) / NB;
+        const int ntile = std::max(1, (maxp + 63) / 64);
+        for (int j = nblk - 1; j >= 0; j--) {
+            hipLaunchKernelGGL(k_tri_inv<T>, dim3(2 * ntile, nfr, Bz), dim3(256), 0, st, dp.d_tasks[lv], j,
+                               mem);
             LAUNCH_CHECK();
-            const int nblk = (maxp + NB - 1) / NB;
-            const int ntile = std::max(1, (maxp + 63) / 64);
-            for (int j = nblk - 1; j >= 0; j--) {
-                hipLaunchKernelGGL(k_tri_inv<T>, dim3(ntile, nfr, 2), dim3(256), 0, st, dp.d_tasks[lv],
-                                   j, arena, (const T *)cp, (const T *)dinv);
-                LAUNCH_CHECK();
-            }
         }
     }
     return HMEC_OK;
 }
 
-int sv_factorize(hmec_ctx *c, int precision) {
-    if (!c->analyzed) return HMEC_E_ARG;
-    HMEC_TRY(ensure_arena(c, precision));
-    const int64_t N = c->N;
-    hipStream_t st = c->stream;
-    HMEC_HIP(hipMemsetAsync(c->d.counters, 0, sizeof(int) * 4, st));
-    hipLaunchKernelGGL(k_row_scale, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, c->d.rowp,
-                       c->d.val, N, c->d.rscale);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_col_scale, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
-                       c->dp.d_csc_ptr, c->dp.d_csc_idx, c->dp.d_csc_row, c->d.val, c->d.rscale, N,
-                       c->d.cscale);
-    LAUNCH_CHECK();
-    SolverDev *s = sdev(c);
-    sv_sweep_invalidate(c);
-    if (precision == HMEC_PREC_FP64) HMEC_TRY(factor_levels<double>(c, c->d.arena, s->dinv64, 1e-13));
-    else HMEC_TRY(factor_levels<float>(c, c->d.arena32, s->dinv32, 1e-6));
+// Numeric factorization of n same-size contexts' current matrices with one
+// launch per kernel (n = 1: a single context).  The batch runs on the first
+// context's stream; the other members' streams are joined before and after
+// with events, so each member's own stream order is kept.
+int sv_factorize_batch(hmec_ctx *const *cs, int n, int precision) {
+    if (!cs || n <= 0) return HMEC_E_ARG;
+    hmec_ctx *c0 = cs[0];
+    for (int k = 0; k < n; k++) {
+        if (!cs[k]->analyzed || cs[k]->Nx != c0->Nx || cs[k]->Ny != c0->Ny ||
+            cs[k]->plan.arena != c0->plan.arena || cs[k]->device != c0->device)
+            return HMEC_E_ARG;
+        HMEC_TRY(ensure_arena(cs[k], precision));
+    }
+    hipStream_t st = c0->stream;
+    std::vector<hipEvent_t> evs;
+    auto join = [&](hipStream_t from, hipStream_t to) -> int {
+        hipEvent_t e;
+        HMEC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HMEC_HIP(hipEventRecord(e, from));
+        HMEC_HIP(hipStreamWaitEvent(to, e, 0));
+        evs.push_back(e);
+        return HMEC_OK;
+    };
+    for (int k = 1; k < n; k++) HMEC_TRY(join(cs[k]->stream, st));
+    // equilibration per member (small kernels), counters cleared
+    for (int k = 0; k < n; k++) {
+        hmec_ctx *c = cs[k];
+        const int64_t N = c->N;
+        HMEC_HIP(hipMemsetAsync(c->d.counters, 0, sizeof(int) * 4, st));
+        hipLaunchKernelGGL(k_row_scale, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, c0->d.rowp,
+                           c->d.val, N, c->d.rscale);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_col_scale, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
+                           c0->dp.d_csc_ptr, c0->dp.d_csc_idx, c0->dp.d_csc_row, c->d.val, c->d.rscale,
+                           N, c->d.cscale);
+        LAUNCH_CHECK();
+        sv_sweep_invalidate(c);
+    }
+    // member pointer table (uploaded with the stream's order)
+    SolverDev *s0 = sdev(c0);
+    const bool fp64 = precision == HMEC_PREC_FP64;
+    const size_t psz = fp64 ? sizeof(FacPtrs<double>) : sizeof(FacPtrs<float>);
+    std::vector<unsigned char> host(psz * n);
+    for (int k = 0; k < n; k++) {
+        hmec_ctx *c = cs[k];
+        SolverDev *s = sdev(c);
+        if (fp64) {
+            FacPtrs<double> f{c->d.arena, s->dinv64, s->dscr, s->icopy, c->d.counters, c->d.val,
+                              c->d.rscale, c->d.cscale};
+            std::memcpy(host.data() + psz * k, &f, psz);
+        } else {
+            FacPtrs<float> f{c->d.arena32, s->dinv32, reinterpret_cast<float *>(s->dscr),
+                             reinterpret_cast<float *>(s->icopy), c->d.counters, c->d.val, c->d.rscale,
+                             c->d.cscale};
+            std::memcpy(host.data() + psz * k, &f, psz);
+        }
+    }
+    // the table only changes with the member set or the precision: upload it
+    // then, after the stream has drained any launch that still reads it
+    if (s0->mem_host != host) {
+        HMEC_HIP(hipStreamSynchronize(st));
+        if (s0->mem_cap < (int64_t)host.size()) {
+            hipFree(s0->mem);
+            s0->mem = nullptr;
+            HMEC_HIP(hipMalloc(&s0->mem, host.size()));
+            s0->mem_cap = (int64_t)host.size();
+        }
+        HMEC_HIP(hipMemcpy(s0->mem, host.data(), host.size(), hipMemcpyHostToDevice));
+        s0->mem_host = host;
+    }
+    if (fp64)
+        HMEC_TRY(factor_levels<double>(cs, n, (const FacPtrs<double> *)s0->mem, st, 1e-13));
+    else
+        HMEC_TRY(factor_levels<float>(cs, n, (const FacPtrs<float> *)s0->mem, st, 1e-6));
     // FP32 factors are reused over many solves: form the one-GEMV-per-front
     // sweep form once
-    if (precision == HMEC_PREC_FP32) HMEC_TRY(sv_sweep_build(c, precision));
-    c->factor_prec = precision;
-    c->factor_valid = true;
-    c->stale_solves = 0;
+    if (!fp64) HMEC_TRY(sv_sweep_build_batch(cs, n, precision));
+    for (int k = 0; k < n; k++) {
+        cs[k]->factor_prec = precision;
+        cs[k]->factor_valid = true;
+        cs[k]->stale_solves = 0;
+    }
+    for (int k = 1; k < n; k++) HMEC_TRY(join(st, cs[k]->stream));
+    for (hipEvent_t e : evs) hipEventDestroy(e);
     return HMEC_OK;
 }
+
+int sv_factorize(hmec_ctx *c, int precision) { return sv_factorize_batch(&c, 1, precision); }
 
 template <typename T>
 static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
@@ -1258,8 +1350,7 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
 }
 
 int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision) {
-    for (int k = 0; k < n; k++) HMEC_TRY(sv_factorize(cs[k], precision));
-    return HMEC_OK;
+    return sv_factorize_batch(cs, n, precision);
 }
 
 

@@ -75,10 +75,18 @@ struct SwTile {
     int ti, tj;
 };
 
+template <typename T>
+struct SwMem {       // one member of a (batched) sweep-form launch
+    const T *arena;  // factor arena
+    T *sa;           // sweep arena
+};
+
 // ------------------------------------------------------------ sweep form
 template <typename T>
 __global__ void __launch_bounds__(256) k_sweep_form(const SwTile *tiles, const SwFront *fronts,
-                                                    const T *arena, T *sa) {
+                                                    const SwMem<T> *mem) {
+    const T *const arena = mem[blockIdx.z].arena;
+    T *const sa = mem[blockIdx.z].sa;
     typedef typename Mfma<T>::acc_t acc_t;
     __shared__ T As[SW_TK][SW_TILE + 4];
     __shared__ T Bs[SW_TK][SW_TILE + 4];
@@ -286,6 +294,9 @@ struct SweepDev {
     void *sa[2] = {nullptr, nullptr};            // [0] FP32, [1] FP64 sweep arena
     int built = -1;                              // precision of the current sweep form
     double form_flops = 0;
+    void *mem = nullptr;                         // member table of the last form launch
+    int64_t mem_cap = 0;
+    std::vector<unsigned char> mem_host;
 };
 
 std::mutex g_sw_mu;
@@ -430,35 +441,61 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
     return HMEC_OK;
 }
 
-// Form Fm / Bm of every front from the factor in the arena of the given
-// precision (triangle inverses in place, L21, U12).
-int sv_sweep_build(hmec_ctx *c, int precision) {
-    SweepDev *s = swdev(c);
-    if (!s->ready) HMEC_TRY(sweep_setup(c, s));
-    const int q = precision == HMEC_PREC_FP64 ? 1 : 0;
-    const size_t es = q ? sizeof(double) : sizeof(float);
-    if (!s->sa[q]) {
-        if (hipMalloc(&s->sa[q], es * std::max<int64_t>(1, s->sa_entries)) != hipSuccess) {
-            hipGetLastError();
-            hmec_set_last_error("sweep arena allocation failed");
-            return HMEC_E_OOM;
-        }
-        // row padding and the unused triangles are never written: zero once
-        HMEC_HIP(hipMemset(s->sa[q], 0, es * std::max<int64_t>(1, s->sa_entries)));
+template <typename T>
+static int form_batch(hmec_ctx *const *cs, int n, int q, SweepDev *s0) {
+    std::vector<SwMem<T>> h(n);
+    for (int k = 0; k < n; k++) {
+        const T *ar = q ? (const T *)(const void *)cs[k]->d.arena : (const T *)(const void *)cs[k]->d.arena32;
+        h[k] = SwMem<T>{ar, (T *)swdev(cs[k])->sa[q]};
     }
-    hipStream_t st = c->stream;
-    kt_begin(c, "sweep_form");
-    if (s->ntiles > 0) {
-        if (q)
-            hipLaunchKernelGGL(k_sweep_form<double>, dim3(s->ntiles), dim3(256), 0, st, s->tiles,
-                               s->fronts, (const double *)c->d.arena, (double *)s->sa[q]);
-        else
-            hipLaunchKernelGGL(k_sweep_form<float>, dim3(s->ntiles), dim3(256), 0, st, s->tiles,
-                               s->fronts, (const float *)c->d.arena32, (float *)s->sa[q]);
+    hipStream_t st = cs[0]->stream;
+    const size_t bytes = sizeof(SwMem<T>) * n;
+    std::vector<unsigned char> hb(bytes);
+    std::memcpy(hb.data(), h.data(), bytes);
+    if (s0->mem_host != hb) {
+        HMEC_HIP(hipStreamSynchronize(st));
+        if (s0->mem_cap < (int64_t)bytes) {
+            hipFree(s0->mem);
+            s0->mem = nullptr;
+            HMEC_HIP(hipMalloc(&s0->mem, bytes));
+            s0->mem_cap = (int64_t)bytes;
+        }
+        HMEC_HIP(hipMemcpy(s0->mem, hb.data(), bytes, hipMemcpyHostToDevice));
+        s0->mem_host = hb;
+    }
+    if (s0->ntiles > 0) {
+        hipLaunchKernelGGL(k_sweep_form<T>, dim3(s0->ntiles, 1, n), dim3(256), 0, st, s0->tiles,
+                           s0->fronts, (const SwMem<T> *)s0->mem);
         SW_LAUNCH_CHECK();
     }
-    kt_end(c, "sweep_form", s->form_flops);
-    s->built = precision;
+    return HMEC_OK;
+}
+
+// Form Fm / Bm of every front of n same-size contexts (one launch) from the
+// factors in their arenas of the given precision (triangle inverses in
+// place, L21, U12).  Runs on the first context's stream.
+int sv_sweep_build_batch(hmec_ctx *const *cs, int n, int precision) {
+    const int q = precision == HMEC_PREC_FP64 ? 1 : 0;
+    const size_t es = q ? sizeof(double) : sizeof(float);
+    for (int k = 0; k < n; k++) {
+        SweepDev *s = swdev(cs[k]);
+        if (!s->ready) HMEC_TRY(sweep_setup(cs[k], s));
+        if (!s->sa[q]) {
+            if (hipMalloc(&s->sa[q], es * std::max<int64_t>(1, s->sa_entries)) != hipSuccess) {
+                hipGetLastError();
+                hmec_set_last_error("sweep arena allocation failed");
+                return HMEC_E_OOM;
+            }
+            // row padding and the unused triangles are never written: zero once
+            HMEC_HIP(hipMemset(s->sa[q], 0, es * std::max<int64_t>(1, s->sa_entries)));
+        }
+    }
+    SweepDev *s0 = swdev(cs[0]);
+    kt_begin(cs[0], "sweep_form");
+    if (q) HMEC_TRY(form_batch<double>(cs, n, q, s0));
+    else HMEC_TRY(form_batch<float>(cs, n, q, s0));
+    kt_end(cs[0], "sweep_form", s0->form_flops * n);
+    for (int k = 0; k < n; k++) swdev(cs[k])->built = precision;
     return HMEC_OK;
 }
 
@@ -531,6 +568,7 @@ void sv_sweep_free(hmec_ctx *c) {
     for (auto *p : s->d_bwd) hipFree(p);
     hipFree(s->sa[0]);
     hipFree(s->sa[1]);
+    hipFree(s->mem);
     delete s;
 }
 

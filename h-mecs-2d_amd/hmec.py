@@ -110,6 +110,37 @@ class SolveStats(C.Structure):
 _lib = None
 
 
+class ServerConfig(C.Structure):
+    """hmec_server_config (include/hmec.h)."""
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("max_active", C.c_int),
+                ("hbm_budget", C.c_int64), ("batch_window_ms", C.c_double), ("keep_fields", C.c_int)]
+
+
+class Request(C.Structure):
+    """hmec_request (include/hmec.h)."""
+    _fields_ = [("params", Params), ("cfg", C.c_int), ("seed", C.c_uint64), ("jitter", C.c_int),
+                ("nsteps", C.c_int), ("arrival_ms", C.c_double)]
+
+
+class MemberResult(C.Structure):
+    """hmec_member_result (include/hmec.h)."""
+    _fields_ = [("status", C.c_int), ("steps", C.c_int), ("admit_ms", C.c_double),
+                ("finish_ms", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("footprint", C.c_int64), ("factorizations", C.c_int),
+                ("batched_factorizations", C.c_int), ("ir_iterations", C.c_int),
+                ("fallbacks", C.c_int), ("max_rel_residual", C.c_double), ("timesum", C.c_double),
+                ("dt", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    """hmec_server_stats (include/hmec.h)."""
+    _fields_ = [("member_steps", C.c_int64), ("wall_ms", C.c_double),
+                ("member_steps_per_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("members_done", C.c_int), ("members_failed", C.c_int), ("max_concurrent", C.c_int),
+                ("peak_bytes", C.c_int64), ("factor_batches", C.c_int),
+                ("factor_batch_members", C.c_int)]
+
+
 class HmecError(RuntimeError):
     pass
 
@@ -192,6 +223,17 @@ def _declare(L):
     L.hmec_step.argtypes = [vp, P(StepInfo)]
     L.hmec_batch_factorize.argtypes = [P(vp), C.c_int, C.c_int]
     L.hmec_batch_step.argtypes = [P(vp), C.c_int, P(StepInfo)]
+    L.hmec_member_footprint.restype = C.c_int64
+    L.hmec_member_footprint.argtypes = [P(Params), C.c_int]
+    L.hmec_server_create.argtypes = [P(ServerConfig), P(vp)]
+    L.hmec_server_destroy.argtypes = [vp]
+    L.hmec_server_destroy.restype = None
+    L.hmec_server_submit.argtypes = [vp, P(Request), P(C.c_int)]
+    L.hmec_server_run.argtypes = [vp]
+    L.hmec_server_result.argtypes = [vp, C.c_int, P(MemberResult)]
+    L.hmec_server_get_stats.argtypes = [vp, P(ServerStats)]
+    L.hmec_server_latencies.argtypes = [vp, C.c_int, P(C.c_double), C.c_int]
+    L.hmec_server_get_field.argtypes = [vp, C.c_int, C.c_int, P(C.c_double), C.c_int]
     L.hmec_kernel_time.argtypes = [vp, C.c_char_p, P(C.c_double), P(C.c_double)]
     L.hmec_timing_enable.argtypes = [vp, C.c_int]
     L.hmec_timing_reset.argtypes = [vp]
@@ -410,3 +452,68 @@ def batch_factorize(ctxs, prec=PREC_FP64):
     n = len(ctxs)
     arr = (C.c_void_p * n)(*[c.h for c in ctxs])
     check(lib().hmec_batch_factorize(arr, n, int(prec)), "batch_factorize")
+
+
+def member_footprint(p, prec=PREC_FP64):
+    """Device bytes a context of this grid allocates under a precision policy."""
+    return int(lib().hmec_member_footprint(C.byref(p), int(prec)))
+
+
+class Server:
+    """Ensemble server (hmec_server_*): continuous batching of independent
+    simulations, admission by device footprint, batched same-size
+    factorizations.  Not in the reference (one simulation per process)."""
+
+    def __init__(self, precision=PREC_FP32, max_active=8, hbm_budget=0, batch_window_ms=2.0,
+                 keep_fields=False, device=0):
+        self.cfg = ServerConfig(device, int(precision), int(max_active), int(hbm_budget),
+                                float(batch_window_ms), 1 if keep_fields else 0)
+        h = C.c_void_p()
+        check(lib().hmec_server_create(C.byref(self.cfg), C.byref(h)), "server_create")
+        self.h = h
+        self.params = {}
+
+    def submit(self, p, nsteps, cfg=1, seed=1, jitter=1, arrival_ms=0.0):
+        r = Request(p, int(cfg), int(seed), int(jitter), int(nsteps), float(arrival_ms))
+        i = C.c_int()
+        check(lib().hmec_server_submit(self.h, C.byref(r), C.byref(i)), "server_submit")
+        self.params[i.value] = p
+        return i.value
+
+    def run(self):
+        check(lib().hmec_server_run(self.h), "server_run")
+
+    def result(self, i):
+        r = MemberResult()
+        check(lib().hmec_server_result(self.h, int(i), C.byref(r)), "server_result")
+        return r
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().hmec_server_get_stats(self.h, C.byref(st)), "server_stats")
+        return st
+
+    def latencies(self, i):
+        n = lib().hmec_server_latencies(self.h, int(i), None, 0)
+        buf = (C.c_double * max(1, n))()
+        lib().hmec_server_latencies(self.h, int(i), buf, n)
+        return np.array(buf[:n])
+
+    def get(self, i, name):
+        """A finished member's final field (keep_fields=True), numpy (rows, cols)."""
+        r, c = field_shape(self.params[i], name)
+        a = np.zeros((c, r), dtype=np.float64)   # column-major storage
+        check(lib().hmec_server_get_field(self.h, int(i), FIELD_ID[name], dptr(a), 0),
+              "server_get_field " + name)
+        return a.T
+
+    def close(self):
+        if self.h:
+            lib().hmec_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

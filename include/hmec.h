@@ -111,7 +111,9 @@ int64_t hmec_marker_count(const hmec_params *p);
  * m = ix * Ny_markers + iy as in h_mec_rsf_v9.cpp:188-191; each marker is
  * jittered uniformly inside its marker sub-cell when jitter != 0 (jitter == 0
  * reproduces the reference's regular lattice).  Materials follow
- * h_mec_rsf_v9.cpp:192-208 with the per-config perturbations of the presets.
+ * h_mec_rsf_v9.cpp:192-208 with the per-config perturbations of the presets
+ * (cfg 5, the serving mix: one log-uniform permeability in [1e-16, 1e-14] m^2
+ * per member, drawn from the seed).
  * soa[f] must hold hmec_marker_count(p) doubles for each of the 19 fields. */
 int hmec_generate_markers(const hmec_params *p, int cfg, uint64_t seed,
                           int jitter, double *const soa[HMEC_NMARKER_FIELDS]);
@@ -292,6 +294,75 @@ int hmec_batch_factorize(hmec_ctx *const *ctxs, int n, int precision);
 /* Advance n contexts by one timestep each, overlapping their work on the
  * device; same-size members share batched factorizations. */
 int hmec_batch_step(hmec_ctx *const *ctxs, int n, hmec_step_info *infos);
+
+/* Device bytes a context of this grid allocates under a precision policy
+ * (exact, from the symbolic plan; the admission measure of the server). */
+int64_t hmec_member_footprint(const hmec_params *p, int precision);
+
+/* Ensemble server: a request loop that advances independent simulations
+ * (members) one timestep at a time with continuous batching.  Requests carry
+ * an arrival time; a member is admitted once it has arrived and its
+ * footprint fits the HBM budget (and a slot is free), runs on its own HIP
+ * stream and host thread, and frees its memory when done so later arrivals
+ * are admitted while others run.  Same-size members that need a
+ * factorization at the same time share one batched launch sequence
+ * (hmec_batch_factorize).  Markers and factors stay device resident for the
+ * member's lifetime.  No reference counterpart: the reference runs one
+ * simulation per process. */
+typedef struct hmec_server hmec_server;
+typedef struct {
+    int device;                /* HIP device of every member                   */
+    int precision;             /* HMEC_PREC_FP64 / HMEC_PREC_FP32              */
+    int max_active;            /* member slots (concurrent host threads)       */
+    int64_t hbm_budget;        /* admission budget in bytes (<= 0: 90% of the
+                                  device memory free at creation)             */
+    double batch_window_ms;    /* how long a factorization waits for same-size
+                                  partners                                     */
+    int keep_fields;           /* keep the members' final fields for
+                                  hmec_server_get_field                         */
+} hmec_server_config;
+typedef struct {
+    hmec_params params;        /* grid, friction, permeability, ...            */
+    int cfg;                   /* marker material layout (hmec_generate_markers) */
+    uint64_t seed;             /* marker generator seed                        */
+    int jitter;                /* marker jitter on / off                       */
+    int nsteps;                /* timesteps to run                             */
+    double arrival_ms;         /* arrival time after hmec_server_run starts    */
+} hmec_request;
+typedef struct {
+    int status;                /* 0 done, HMEC_E_* on failure, 1 not finished   */
+    int steps;
+    double admit_ms, finish_ms;          /* since hmec_server_run started       */
+    double p50_ms, p99_ms, mean_ms;      /* step latency of this member         */
+    int64_t footprint;
+    int factorizations, batched_factorizations;   /* own / inside a batch > 1  */
+    int ir_iterations, fallbacks;
+    double max_rel_residual;
+    double timesum, dt;
+} hmec_member_result;
+typedef struct {
+    int64_t member_steps;
+    double wall_ms;
+    double member_steps_per_s;
+    double p50_ms, p99_ms;               /* over all member steps               */
+    int members_done, members_failed;
+    int max_concurrent;
+    int64_t peak_bytes;                  /* peak admitted footprint              */
+    int factor_batches;                  /* batched launches (> 1 member)       */
+    int factor_batch_members;            /* members factored inside them        */
+} hmec_server_stats;
+int hmec_server_create(const hmec_server_config *cfg, hmec_server **srv);
+void hmec_server_destroy(hmec_server *srv);
+/* queue a request (before or while hmec_server_run runs); *id >= 0 */
+int hmec_server_submit(hmec_server *srv, const hmec_request *req, int *id);
+/* serve until every submitted request has finished */
+int hmec_server_run(hmec_server *srv);
+int hmec_server_result(hmec_server *srv, int id, hmec_member_result *res);
+int hmec_server_get_stats(hmec_server *srv, hmec_server_stats *st);
+/* per-step latencies (ms) of a member: NULL buf returns the count */
+int hmec_server_latencies(hmec_server *srv, int id, double *buf, int cap);
+/* a finished member's final field (keep_fields), as hmec_get_field */
+int hmec_server_get_field(hmec_server *srv, int id, int field, double *buf, int layout);
 
 /* ------------------------------------------------------------------ timing
  * Average device duration (ms, HIP events on the context's stream) of the
