@@ -173,6 +173,53 @@ extern "C" int hmec_set_precision(hmec_ctx *c, int precision) {
     return HMEC_OK;
 }
 
+extern "C" int hmec_set_refinement(hmec_ctx *c, int max_stale, int max_fresh, double tol) {
+    if (!c || max_stale < 0 || max_fresh < 0 || !(tol > 0)) return HMEC_E_ARG;
+    c->ir_max_stale = max_stale;
+    c->ir_max_fresh = max_fresh;
+    c->ir_tol = tol;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_set_factor_cache(hmec_ctx *c, int slots) {
+    if (!c) return HMEC_E_ARG;
+    return sv_slots_set(c, slots);
+}
+
+extern "C" int hmec_init_coseismic(hmec_ctx *c, double frac, double noise, double pf_ratio,
+                                   uint64_t seed) {
+    if (!c || !(frac >= 0) || !(noise >= 0) || !(pf_ratio >= 0 && pf_ratio < 1)) return HMEC_E_ARG;
+    const hmec_params &P = c->P;
+    const int r1 = c->Ny1, c1 = c->Nx1, rb = c->Ny, cb = c->Nx;
+    std::vector<double> pt0((size_t)r1 * c1), pf0(pt0.size()), ptf0(pt0.size());
+    for (int j = 0; j < c1; j++)
+        for (int i = 0; i < r1; i++) {
+            const size_t k = (size_t)i + (size_t)j * r1;     // column-major
+            const double lit = (P.PCONF + P.PTFDIFF) * (1.0 + noise * hmec_seeded_normal(seed, (int64_t)k, 20));
+            pt0[k] = lit;
+            pf0[k] = pf_ratio * lit;
+            ptf0[k] = lit - pf0[k];
+        }
+    // basic node (i, j) sits between pressure nodes (i, j) .. (i + 1, j + 1)
+    const double mu = 0.6;
+    std::vector<double> sxy((size_t)rb * cb);
+    for (int j = 0; j < cb; j++)
+        for (int i = 0; i < rb; i++) {
+            double se = 0;
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++) se += ptf0[(size_t)(i + a) + (size_t)(j + b) * r1];
+            sxy[(size_t)i + (size_t)j * rb] = frac * mu * 0.25 * se;
+        }
+    HMEC_TRY(hmec_set_field(c, HMEC_F_PT0, pt0.data(), HMEC_COLMAJOR));
+    HMEC_TRY(hmec_set_field(c, HMEC_F_PF0, pf0.data(), HMEC_COLMAJOR));
+    HMEC_TRY(hmec_set_field(c, HMEC_F_PTF0, ptf0.data(), HMEC_COLMAJOR));
+    HMEC_TRY(hmec_set_field(c, HMEC_F_PT, pt0.data(), HMEC_COLMAJOR));
+    HMEC_TRY(hmec_set_field(c, HMEC_F_PF, pf0.data(), HMEC_COLMAJOR));
+    HMEC_TRY(hmec_set_field(c, HMEC_F_SXY0, sxy.data(), HMEC_COLMAJOR));
+    HMEC_TRY(hmec_set_field(c, HMEC_F_SXY, sxy.data(), HMEC_COLMAJOR));
+    return HMEC_OK;
+}
+
 extern "C" int hmec_set_markers(hmec_ctx *c, const double *const soa[HMEC_NMARKER_FIELDS], int64_t M) {
     if (!c || !soa || M <= 0 || M > INT32_MAX) return HMEC_E_ARG;
     DeviceState &d = c->d;

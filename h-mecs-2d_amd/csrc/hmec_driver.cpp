@@ -93,31 +93,39 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         tm_end(c, c->t_solve, 0);
         if (rc != HMEC_OK && rc != HMEC_E_NOCONV) return rc;
     } else {
-        // FP32: GMRES preconditioned by the FP32 factor, reused across Picard
-        // iterations and timesteps (also across dt changes) while GMRES
-        // converges within its budget; otherwise refactor in FP32 and retry;
-        // a fresh FP32 factor that still fails falls back to an FP64 factor.
+        // FP32: GMRES preconditioned by a cached FP32 factor.  The cache keeps
+        // a few factors keyed by the dt they were computed at (the
+        // controller alternates between an increased dt on a step's first
+        // Picard iteration and the reduced one after it), so each solve
+        // starts from the factor of the nearest dt.  If GMRES needs more than
+        // its budget the system has moved: refactor in FP32, into the same
+        // slot when dt is close to the slot's, else into an empty or the
+        // least recently used slot; a fresh factor that still fails falls
+        // back to an FP64 factor.
+        double dist = 0;
+        int k = sv_slot_nearest(c, c->dt, &dist);
         bool fresh = false;
-        if (!c->factor_valid || c->factor_prec != HMEC_PREC_FP32) {
+        auto refactor = [&](int slot) -> int {
+            sv_slot_target(c, slot);
             tm_begin(c, c->t_factor);
             HMEC_TRY(factor(c, HMEC_PREC_FP32));
             tm_end(c, c->t_factor, c->plan.flops);
             acc.factorizations++;
             c->factor_dt = c->dt;
             fresh = true;
-        }
+            return HMEC_OK;
+        };
+        if (k < 0) HMEC_TRY(refactor(sv_slot_victim(c, -1)));
+        else HMEC_TRY(sv_slot_activate(c, k));
         tm_begin(c, c->t_solve);
-        int rc = sv_gmres(c, fresh ? 40 : 12, 1e-12, &st);
+        int rc = sv_gmres(c, fresh ? c->ir_max_fresh : c->ir_max_stale, c->ir_tol, &st);
         tm_end(c, c->t_solve, 0);
         acc.ir_iterations += st.ir_iterations;
         if (rc == HMEC_E_NOCONV && !fresh) {
-            tm_begin(c, c->t_factor);
-            HMEC_TRY(factor(c, HMEC_PREC_FP32));
-            tm_end(c, c->t_factor, c->plan.flops);
-            acc.factorizations++;
-            c->factor_dt = c->dt;
+            const int slot = dist <= 0.05 ? k : sv_slot_victim(c, k);
+            HMEC_TRY(refactor(slot));
             tm_begin(c, c->t_solve);
-            rc = sv_gmres(c, 40, 1e-12, &st);
+            rc = sv_gmres(c, c->ir_max_fresh, c->ir_tol, &st);
             tm_end(c, c->t_solve, 0);
             acc.ir_iterations += st.ir_iterations;
         }
@@ -127,8 +135,6 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
             acc.factorizations++;
             rc = sv_solve(c, 3, 1e-14, &st);
             if (rc == HMEC_E_NOCONV) rc = HMEC_OK;
-            // the next iteration starts again from a fresh FP32 factor
-            c->factor_valid = false;
         }
         if (rc != HMEC_OK) return rc;
     }

@@ -27,6 +27,7 @@
     } while (0)
 
 void hmec_set_last_error(const std::string &s);
+double hmec_seeded_normal(uint64_t seed, int64_t idx, unsigned stream);
 
 // Number of live scatter accumulators per grid and the marker property
 // streams the scatter reads (derived once per marker set).
@@ -187,6 +188,9 @@ struct hmec_ctx {
     double last_res = 0;
     int stale_solves = 0;            // FP32: solves since the last factorization
     double factor_dt = -1;           // dt of the system the factor was computed for
+    int ir_max_stale = 12;           // FP32 policy: GMRES budget with a reused factor
+    int ir_max_fresh = 40;           //              ... with a fresh factor
+    double ir_tol = 1e-12;           //              target relative inf-norm residual
     hmec_batcher *batcher = nullptr; // server member: factorizations go through the batcher
     int batched_facs = 0;            // factorizations done inside a batch of > 1 members
     // statistics of the current step
@@ -229,7 +233,13 @@ int batcher_factorize(hmec_batcher *b, hmec_ctx *c, int precision);
 // ---- sweep form of the factor (solver_sweep.hip) ----
 int sv_sweep_build_batch(hmec_ctx *const *cs, int n, int precision);
 bool sv_sweep_ready(hmec_ctx *c, int precision);
-void sv_sweep_invalidate(hmec_ctx *c);
+void sv_sweep_invalidate(hmec_ctx *c, int precision);
+int sv_slots_set(hmec_ctx *c, int n);
+int sv_slots_count(hmec_ctx *c);
+int sv_slot_nearest(hmec_ctx *c, double dt, double *dist);
+int sv_slot_victim(hmec_ctx *c, int keep);
+void sv_slot_target(hmec_ctx *c, int k);
+int sv_slot_activate(hmec_ctx *c, int k);
 int sv_sweep_solve(hmec_ctx *c, double *z);
 void sv_sweep_free(hmec_ctx *c);
 

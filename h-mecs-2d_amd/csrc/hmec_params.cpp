@@ -116,6 +116,12 @@ inline double normal(uint64_t seed, int64_t m, unsigned stream) {
 }
 }  // namespace
 
+// seeded standard normal (clipped at 3 sigma) of the marker generator's
+// counter-based stream, shared with the other seeded initializers
+double hmec_seeded_normal(uint64_t seed, int64_t idx, unsigned stream) {
+    return normal(seed, idx, stream);
+}
+
 extern "C" int hmec_generate_markers(const hmec_params *p, int cfg, uint64_t seed, int jitter,
                                      double *const soa[HMEC_NMARKER_FIELDS]) {
     if (!p || !soa) return HMEC_E_ARG;

@@ -98,7 +98,8 @@ extern "C" int hmec_plan_levels(const hmec_params *p, int *nlevels, int *nfronts
 // derived streams and sort buffers, nodal fields, CSR and its triplet map,
 // solver plan arrays and vectors, then the factor storage of the policy
 // (FP64: front arena + inverted diagonal blocks; FP32: FP32 front arena,
-// inverted blocks, sweep arena and the GMRES basis).
+// inverted blocks, two factor slots and the GMRES basis; an FP64 fallback
+// allocates the FP64 arena on top).
 namespace {
 struct PlanSizes {
     int64_t arena = 0, varena = 0, nnz = 0, ntrip = 0, sweep = 0, dinv = 0, nfronts = 0;
@@ -148,7 +149,8 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     if (precision == HMEC_PREC_FP64) {
         b += z.arena * 8 + z.dinv * 8;
     } else {
-        b += z.arena * 4 + z.dinv * 4 + z.sweep * 4 + (2 * 20 + 2) * N * 8;
+        // two FP32 factor slots (sweep form + scales each)
+        b += z.arena * 4 + z.dinv * 4 + 2 * (z.sweep * 4 + 2 * N * 8) + (2 * 20 + 3) * N * 8;
     }
     return b;
 }

@@ -708,6 +708,15 @@ __global__ void k_spmv(const int32_t *rowp, const int32_t *coli, const double *v
     for (int p = rowp[t]; p < rowp[t + 1]; p++) s += val[p] * x[coli[p]];
     y[t] = s;
 }
+// y = Dr (A x)   (row-equilibrated operator)
+__global__ void k_spmv_rs(const int32_t *rowp, const int32_t *coli, const double *val,
+                          const double *rs, const double *x, double *y, int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    double s = 0;
+    for (int p = rowp[t]; p < rowp[t + 1]; p++) s += val[p] * x[coli[p]];
+    y[t] = rs[t] * s;
+}
 // partial[b * nv + i] = sum over block b's slice of V_i . w   (nv vectors)
 __global__ void __launch_bounds__(256) k_dots_part(const double *V, int64_t ldv, int nv,
                                                    const double *w, int64_t N, double *partial) {
@@ -1162,7 +1171,7 @@ int sv_factorize_batch(hmec_ctx *const *cs, int n, int precision) {
                            c0->dp.d_csc_ptr, c0->dp.d_csc_idx, c0->dp.d_csc_row, c->d.val, c->d.rscale,
                            N, c->d.cscale);
         LAUNCH_CHECK();
-        sv_sweep_invalidate(c);
+        sv_sweep_invalidate(c, precision);
     }
     // member pointer table (uploaded with the stream's order)
     SolverDev *s0 = sdev(c0);
@@ -1362,6 +1371,7 @@ int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision) {
 // re-orthogonalization; every reduction is two-stage in a fixed order.
 struct GmresBuf {
     double *V = nullptr, *Z = nullptr, *w = nullptr, *part = nullptr, *hdev = nullptr;
+    double *bs = nullptr, *rsd = nullptr;   // Dr b, Dr (b - A x)
     double *hhost = nullptr;
     int m = 0;
     int64_t N = 0;
@@ -1375,6 +1385,7 @@ void sv_gmres_free(hmec_ctx *c) {
     if (it == g_gm.end()) return;
     GmresBuf &g = it->second;
     hipFree(g.V); hipFree(g.Z); hipFree(g.w); hipFree(g.part); hipFree(g.hdev);
+    hipFree(g.bs); hipFree(g.rsd);
     if (g.hhost) hipHostFree(g.hhost);
     g_gm.erase(it);
 }
@@ -1384,6 +1395,7 @@ static int gm_buf(hmec_ctx *c, int m, GmresBuf **out) {
     GmresBuf &g = g_gm[c];
     if (g.m < m || g.N != c->N) {
         hipFree(g.V); hipFree(g.Z); hipFree(g.w); hipFree(g.part); hipFree(g.hdev);
+        hipFree(g.bs); hipFree(g.rsd);
         if (g.hhost) hipHostFree(g.hhost);
         g = GmresBuf();
         g.m = m;
@@ -1391,6 +1403,8 @@ static int gm_buf(hmec_ctx *c, int m, GmresBuf **out) {
         HMEC_HIP(hipMalloc(&g.V, sizeof(double) * (m + 1) * c->N));
         HMEC_HIP(hipMalloc(&g.Z, sizeof(double) * m * c->N));
         HMEC_HIP(hipMalloc(&g.w, sizeof(double) * c->N));
+        HMEC_HIP(hipMalloc(&g.bs, sizeof(double) * c->N));
+        HMEC_HIP(hipMalloc(&g.rsd, sizeof(double) * c->N));
         HMEC_HIP(hipMalloc(&g.part, sizeof(double) * RB * (m + 2)));
         HMEC_HIP(hipMalloc(&g.hdev, sizeof(double) * (m + 2)));
         HMEC_HIP(hipHostMalloc(&g.hhost, sizeof(double) * (m + 2), hipHostMallocDefault));
@@ -1413,23 +1427,25 @@ static int gm_dots(hmec_ctx *c, GmresBuf &g, const double *V, int nv, const doub
     return HMEC_OK;
 }
 
+// z = Dc (LU)^-1 v: right preconditioner of the row-equilibrated operator
+// Dr A (the factor is of Dr A Dc)
 static int gm_precond(hmec_ctx *c, const double *v, double *zout) {
-    SolverDev *s = sdev(c);
     const int64_t N = c->N;
-    const unsigned gN = (unsigned)((N + 255) / 256);
     hipStream_t st = c->stream;
     double *z = c->d.work2;
-    hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, v, c->d.rscale, z, N);
-    LAUNCH_CHECK();
+    HMEC_HIP(hipMemcpyAsync(z, v, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
     HMEC_TRY(apply_factor(c, z));
+    const unsigned gN = (unsigned)((N + 255) / 256);
     hipLaunchKernelGGL(k_axpy_scaled, dim3(gN), dim3(256), 0, st, zout, c->d.cscale, z, N, 1);
     LAUNCH_CHECK();
-    (void)s;
     return HMEC_OK;
 }
 
-// ||r||_inf / ||b||_inf of r = b - A x (r left in c->d.work)
-static int gm_true_residual(hmec_ctx *c, const double *x, double *relinf) {
+// Relative inf-norm residuals of x: unscaled ||b - A x|| / ||b|| (the
+// reported one) and row-equilibrated ||Dr (b - A x)|| / ||Dr b||; leaves
+// Dr (b - A x) in g.rsd (the GMRES residual)
+static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *rel_u,
+                            double *rel_s) {
     SolverDev *s = sdev(c);
     const int64_t N = c->N;
     const unsigned gN = (unsigned)((N + 255) / 256);
@@ -1437,19 +1453,28 @@ static int gm_true_residual(hmec_ctx *c, const double *x, double *relinf) {
     hipLaunchKernelGGL(k_residual, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, x,
                        c->d.rhs, c->d.work, N);
     LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, c->d.work, c->d.rscale, g.rsd, N);
+    LAUNCH_CHECK();
     hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, st, c->d.work, c->d.rhs, N, s->norms);
     LAUNCH_CHECK();
-    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, st, g.rsd, g.bs, N, s->norms + 2);
+    LAUNCH_CHECK();
+    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 4, hipMemcpyDeviceToHost, st));
     HMEC_HIP(hipStreamSynchronize(st));
-    *relinf = s->h_norms[1] > 0 ? s->h_norms[0] / s->h_norms[1] : s->h_norms[0];
+    const double *h = s->h_norms;
+    *rel_u = h[1] > 0 ? h[0] / h[1] : h[0];
+    *rel_s = h[3] > 0 ? h[2] / h[3] : h[2];
     return HMEC_OK;
 }
 
-// Converged when ||b - A x||_inf <= tol ||b||_inf (the residual every solve
-// reports).  Each restart cycle starts from the true residual; its Arnoldi
-// loop stops once the estimated 2-norm has dropped by the factor the
-// inf-norm still needs (with a margin of 2), so the norms' ratio on this
-// system is accounted for.
+// GMRES on the row-equilibrated system Dr A x = Dr b (every row weighted by
+// its own scale, so small-magnitude rows such as the Darcy equations are
+// resolved like the others), right-preconditioned by the FP32 factor.
+// Converged when both the reported ||b - A x||_inf <= tol ||b||_inf and the
+// equilibrated ||Dr (b - A x)||_inf <= tol ||Dr b||_inf hold.  Each restart
+// cycle starts from the true residual; its Arnoldi loop stops once the
+// estimated 2-norm has dropped by the factor the inf-norms still need (with
+// a margin of 2).
 int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     if (!c->factor_valid) return HMEC_E_ARG;
     const int m = 20;               // restart length (<= 31: GmCoef holds the coefficients)
@@ -1459,18 +1484,24 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     const int64_t N = c->N;
     const unsigned gN = (unsigned)((N + 255) / 256);
     hipStream_t st = c->stream;
-    double *x = c->d.sol, *r = c->d.work;
+    double *x = c->d.sol;
+    double *r = g.rsd;
     int total_it = 0;
-    double relinf = 0;
+    double relinf = 0, last_rel_s = 0;
     std::vector<double> H((m + 1) * m), cs(m), sn(m), gv(m + 1), h(m + 2), h2(m + 2), y(m);
+    hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, c->d.rhs, c->d.rscale, g.bs, N);
+    LAUNCH_CHECK();
     for (;;) {
-        HMEC_TRY(gm_true_residual(c, x, &relinf));
-        if (!(relinf > tol) || total_it >= max_it) break;
+        double rel_s = 0;
+        HMEC_TRY(gm_true_residual(c, g, x, &relinf, &rel_s));
+        last_rel_s = rel_s;
+        const double worst = std::max(relinf, rel_s);
+        if (!(worst > tol) || total_it >= max_it) break;
         double rr;
         HMEC_TRY(gm_dots(c, g, r, 1, r, &rr));
         const double beta = std::sqrt(rr);
         if (!(beta > 0)) break;
-        const double target = beta * (tol / relinf) * 0.5;
+        const double target = beta * (tol / worst) * 0.5;
         hipLaunchKernelGGL(k_vscale, dim3(gN), dim3(256), 0, st, r, 1.0 / beta, g.V, N);
         LAUNCH_CHECK();
         std::fill(gv.begin(), gv.end(), 0.0);
@@ -1479,8 +1510,8 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
         for (; j < m && total_it < max_it; j++) {
             double *Zj = g.Z + (int64_t)j * N;
             HMEC_TRY(gm_precond(c, g.V + (int64_t)j * N, Zj));
-            hipLaunchKernelGGL(k_spmv, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, Zj,
-                               g.w, N);
+            hipLaunchKernelGGL(k_spmv_rs, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val,
+                               c->d.rscale, Zj, g.w, N);
             LAUNCH_CHECK();
             // classical Gram-Schmidt, twice
             GmCoef co;
@@ -1547,5 +1578,5 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
         out->max_front = c->plan.max_front;
     }
     c->stale_solves++;
-    return relinf <= tol ? HMEC_OK : HMEC_E_NOCONV;
+    return (relinf <= tol && last_rel_s <= tol) ? HMEC_OK : HMEC_E_NOCONV;
 }

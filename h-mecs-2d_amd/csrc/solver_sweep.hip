@@ -291,8 +291,21 @@ struct SweepDev {
     std::vector<int> lds_fwd, lds_bwd;           // dynamic LDS bytes
     std::vector<double> bytes_fwd, bytes_bwd;    // stored entries read per level
     int64_t sa_entries = 0;
-    void *sa[2] = {nullptr, nullptr};            // [0] FP32, [1] FP64 sweep arena
+    void *sa[2] = {nullptr, nullptr};            // [1]: FP64 sweep arena (when formed)
     int built = -1;                              // precision of the current sweep form
+    // FP32 factor cache: each slot keeps the sweep form and the equilibration
+    // scales of one FP32 factorization and the dt it was computed at
+    struct Slot {
+        void *sa = nullptr;
+        double *rs = nullptr, *cs = nullptr;
+        double dt = 0;
+        int64_t used = 0;
+        bool valid = false;
+    };
+    std::vector<Slot> slots = std::vector<Slot>(2);
+    int cur = 0;                                 // slot of the active factor
+    int target = 0;                              // slot the next FP32 factorization fills
+    int64_t clock = 0;
     double form_flops = 0;
     void *mem = nullptr;                         // member table of the last form launch
     int64_t mem_cap = 0;
@@ -445,8 +458,10 @@ template <typename T>
 static int form_batch(hmec_ctx *const *cs, int n, int q, SweepDev *s0) {
     std::vector<SwMem<T>> h(n);
     for (int k = 0; k < n; k++) {
+        SweepDev *sk = swdev(cs[k]);
         const T *ar = q ? (const T *)(const void *)cs[k]->d.arena : (const T *)(const void *)cs[k]->d.arena32;
-        h[k] = SwMem<T>{ar, (T *)swdev(cs[k])->sa[q]};
+        T *dst = q ? (T *)sk->sa[1] : (T *)sk->slots[sk->target].sa;
+        h[k] = SwMem<T>{ar, dst};
     }
     hipStream_t st = cs[0]->stream;
     const size_t bytes = sizeof(SwMem<T>) * n;
@@ -477,17 +492,28 @@ static int form_batch(hmec_ctx *const *cs, int n, int q, SweepDev *s0) {
 int sv_sweep_build_batch(hmec_ctx *const *cs, int n, int precision) {
     const int q = precision == HMEC_PREC_FP64 ? 1 : 0;
     const size_t es = q ? sizeof(double) : sizeof(float);
+    auto alloc_arena = [&](void **pp) -> int {
+        if (*pp) return HMEC_OK;
+        if (hipMalloc(pp, es * std::max<int64_t>(1, swdev(cs[0])->sa_entries)) != hipSuccess) {
+            hipGetLastError();
+            hmec_set_last_error("sweep arena allocation failed");
+            return HMEC_E_OOM;
+        }
+        // row padding and the unused triangles are never written: zero once
+        HMEC_HIP(hipMemset(*pp, 0, es * std::max<int64_t>(1, swdev(cs[0])->sa_entries)));
+        return HMEC_OK;
+    };
     for (int k = 0; k < n; k++) {
         SweepDev *s = swdev(cs[k]);
         if (!s->ready) HMEC_TRY(sweep_setup(cs[k], s));
-        if (!s->sa[q]) {
-            if (hipMalloc(&s->sa[q], es * std::max<int64_t>(1, s->sa_entries)) != hipSuccess) {
-                hipGetLastError();
-                hmec_set_last_error("sweep arena allocation failed");
-                return HMEC_E_OOM;
-            }
-            // row padding and the unused triangles are never written: zero once
-            HMEC_HIP(hipMemset(s->sa[q], 0, es * std::max<int64_t>(1, s->sa_entries)));
+        if (q) {
+            HMEC_TRY(alloc_arena(&s->sa[1]));
+        } else {
+            SweepDev::Slot &sl = s->slots[s->target];
+            HMEC_TRY(alloc_arena(&sl.sa));
+            if (!sl.rs) HMEC_HIP(hipMalloc(&sl.rs, sizeof(double) * cs[k]->N));
+            if (!sl.cs) HMEC_HIP(hipMalloc(&sl.cs, sizeof(double) * cs[k]->N));
+            sl.valid = false;
         }
     }
     SweepDev *s0 = swdev(cs[0]);
@@ -495,16 +521,105 @@ int sv_sweep_build_batch(hmec_ctx *const *cs, int n, int precision) {
     if (q) HMEC_TRY(form_batch<double>(cs, n, q, s0));
     else HMEC_TRY(form_batch<float>(cs, n, q, s0));
     kt_end(cs[0], "sweep_form", s0->form_flops * n);
-    for (int k = 0; k < n; k++) swdev(cs[k])->built = precision;
+    for (int k = 0; k < n; k++) {
+        SweepDev *s = swdev(cs[k]);
+        s->built = precision;
+        if (!q) {
+            // keep this factorization's scales with its slot (same stream as
+            // the factorization that computed them)
+            SweepDev::Slot &sl = s->slots[s->target];
+            const size_t nb = sizeof(double) * cs[k]->N;
+            HMEC_HIP(hipMemcpyAsync(sl.rs, cs[k]->d.rscale, nb, hipMemcpyDeviceToDevice, cs[0]->stream));
+            HMEC_HIP(hipMemcpyAsync(sl.cs, cs[k]->d.cscale, nb, hipMemcpyDeviceToDevice, cs[0]->stream));
+            sl.valid = true;
+            sl.dt = cs[k]->dt;
+            sl.used = ++s->clock;
+            s->cur = s->target;
+        }
+    }
+    return HMEC_OK;
+}
+
+// ---- FP32 factor cache (slots) ----
+int sv_slots_set(hmec_ctx *c, int n) {
+    if (n < 1 || n > 8) return HMEC_E_ARG;
+    SweepDev *s = swdev(c);
+    for (size_t k = n; k < s->slots.size(); k++) {
+        hipFree(s->slots[k].sa);
+        hipFree(s->slots[k].rs);
+        hipFree(s->slots[k].cs);
+    }
+    s->slots.resize(n);
+    if (s->cur >= n) s->cur = 0;
+    if (s->target >= n) s->target = 0;
+    if (!s->slots[s->cur].valid && s->built == HMEC_PREC_FP32) s->built = -1;
+    return HMEC_OK;
+}
+
+int sv_slots_count(hmec_ctx *c) { return (int)swdev(c)->slots.size(); }
+
+// valid slot with the dt nearest to dt (|log| distance), -1 if none
+int sv_slot_nearest(hmec_ctx *c, double dt, double *dist) {
+    SweepDev *s = swdev(c);
+    int best = -1;
+    double bd = 0;
+    for (int k = 0; k < (int)s->slots.size(); k++) {
+        const SweepDev::Slot &sl = s->slots[k];
+        if (!sl.valid) continue;
+        const double d = std::fabs(std::log(dt / sl.dt));
+        if (best < 0 || d < bd) {
+            best = k;
+            bd = d;
+        }
+    }
+    if (dist) *dist = bd;
+    return best;
+}
+
+// an invalid slot, else the least recently used one other than `keep`
+int sv_slot_victim(hmec_ctx *c, int keep) {
+    SweepDev *s = swdev(c);
+    int v = -1;
+    for (int k = 0; k < (int)s->slots.size(); k++) {
+        if (k == keep && s->slots.size() > 1) continue;
+        if (!s->slots[k].valid) return k;
+        if (v < 0 || s->slots[k].used < s->slots[v].used) v = k;
+    }
+    return v < 0 ? 0 : v;
+}
+
+void sv_slot_target(hmec_ctx *c, int k) { swdev(c)->target = k; }
+
+// make slot k the active FP32 factor: its scales become the context's
+int sv_slot_activate(hmec_ctx *c, int k) {
+    SweepDev *s = swdev(c);
+    if (k < 0 || k >= (int)s->slots.size() || !s->slots[k].valid) return HMEC_E_ARG;
+    SweepDev::Slot &sl = s->slots[k];
+    const size_t nb = sizeof(double) * c->N;
+    HMEC_HIP(hipMemcpyAsync(c->d.rscale, sl.rs, nb, hipMemcpyDeviceToDevice, c->stream));
+    HMEC_HIP(hipMemcpyAsync(c->d.cscale, sl.cs, nb, hipMemcpyDeviceToDevice, c->stream));
+    s->cur = k;
+    s->built = HMEC_PREC_FP32;
+    sl.used = ++s->clock;
+    c->factor_prec = HMEC_PREC_FP32;
+    c->factor_valid = true;
+    c->factor_dt = sl.dt;
     return HMEC_OK;
 }
 
 bool sv_sweep_ready(hmec_ctx *c, int precision) {
     SweepDev *s = swdev(c);
-    return s->built == precision;
+    if (s->built != precision) return false;
+    return precision != HMEC_PREC_FP32 || s->slots[s->cur].valid;
 }
 
-void sv_sweep_invalidate(hmec_ctx *c) { swdev(c)->built = -1; }
+// an FP32 factorization is about to overwrite the target slot; an FP64 one
+// leaves the FP32 slots alone
+void sv_sweep_invalidate(hmec_ctx *c, int precision) {
+    SweepDev *s = swdev(c);
+    if (precision == HMEC_PREC_FP32) s->slots[s->target].valid = false;
+    s->built = -1;
+}
 
 template <typename T, int DIR>
 static int launch_sweep(hmec_ctx *c, SweepDev *s, int lv, const T *sa, double *z) {
@@ -539,13 +654,13 @@ int sv_sweep_solve(hmec_ctx *c, double *z) {
     for (int lv = 0; lv < pl.nlevels; lv++) {
         kt_begin(c, "sweep_fwd");
         if (q) HMEC_TRY((launch_sweep<double, 0>(c, s, lv, (const double *)s->sa[1], z)));
-        else HMEC_TRY((launch_sweep<float, 0>(c, s, lv, (const float *)s->sa[0], z)));
+        else HMEC_TRY((launch_sweep<float, 0>(c, s, lv, (const float *)s->slots[s->cur].sa, z)));
         kt_end(c, "sweep_fwd", s->bytes_fwd[lv] * es);
     }
     for (int lv = pl.nlevels - 1; lv >= 0; lv--) {
         kt_begin(c, "sweep_bwd");
         if (q) HMEC_TRY((launch_sweep<double, 1>(c, s, lv, (const double *)s->sa[1], z)));
-        else HMEC_TRY((launch_sweep<float, 1>(c, s, lv, (const float *)s->sa[0], z)));
+        else HMEC_TRY((launch_sweep<float, 1>(c, s, lv, (const float *)s->slots[s->cur].sa, z)));
         kt_end(c, "sweep_bwd", s->bytes_bwd[lv] * es);
     }
     return HMEC_OK;
@@ -568,6 +683,11 @@ void sv_sweep_free(hmec_ctx *c) {
     for (auto *p : s->d_bwd) hipFree(p);
     hipFree(s->sa[0]);
     hipFree(s->sa[1]);
+    for (auto &sl : s->slots) {
+        hipFree(sl.sa);
+        hipFree(sl.rs);
+        hipFree(sl.cs);
+    }
     hipFree(s->mem);
     delete s;
 }

@@ -198,6 +198,9 @@ def _declare(L):
     L.hmec_footprint_bytes.restype = C.c_int64
     L.hmec_footprint_bytes.argtypes = [P(Params)]
     L.hmec_set_precision.argtypes = [vp, C.c_int]
+    L.hmec_set_refinement.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+    L.hmec_set_factor_cache.argtypes = [vp, C.c_int]
+    L.hmec_init_coseismic.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_uint64]
     L.hmec_set_markers.argtypes = [vp, P(P(C.c_double)), C.c_int64]
     L.hmec_get_markers.argtypes = [vp, P(P(C.c_double))]
     L.hmec_get_marker_cells.argtypes = [vp, P(C.c_int32)]
@@ -316,6 +319,17 @@ class Context:
 
     def set_precision(self, prec):
         check(lib().hmec_set_precision(self.h, int(prec)), "set_precision")
+
+    def set_refinement(self, max_stale=12, max_fresh=40, tol=1e-12):
+        check(lib().hmec_set_refinement(self.h, int(max_stale), int(max_fresh), float(tol)),
+              "set_refinement")
+
+    def set_factor_cache(self, slots=2):
+        check(lib().hmec_set_factor_cache(self.h, int(slots)), "set_factor_cache")
+
+    def init_coseismic(self, frac=0.98, noise=0.02, pf_ratio=0.7, seed=4):
+        check(lib().hmec_init_coseismic(self.h, float(frac), float(noise), float(pf_ratio),
+                                        C.c_uint64(seed)), "init_coseismic")
 
     def set_markers(self, soa):
         soa = np.ascontiguousarray(soa, dtype=np.float64)

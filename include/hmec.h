@@ -187,6 +187,23 @@ void hmec_destroy(hmec_ctx *ctx);
  * CSR, factor fill) before it is created; used for admission. */
 int64_t hmec_footprint_bytes(const hmec_params *p);
 int hmec_set_precision(hmec_ctx *ctx, int precision);
+/* Refinement policy of HMEC_PREC_FP32: GMRES iterations allowed with a reused
+ * (stale) FP32 factor before refactoring, iterations with a fresh FP32
+ * factor before falling back to an FP64 factor, and the target relative
+ * inf-norm residual (defaults 12, 40, 1e-12). */
+int hmec_set_refinement(hmec_ctx *ctx, int max_stale, int max_fresh, double tol);
+/* Number of FP32 factors HMEC_PREC_FP32 keeps, keyed by the dt they were
+ * computed at (1..8, default 2; each costs one sweep arena). */
+int hmec_set_factor_cache(hmec_ctx *ctx, int slots);
+/* Coseismic initial state (the pre-stressed fault workload): lithostatic
+ * pressure PT0 = PCONF + PTFDIFF with seeded relative noise of `noise`
+ * (normal deviate clipped at 3 sigma) per pressure node, pore pressure
+ * PF0 = pf_ratio * PT0, current PT/PF set to the same, and shear stress
+ * SXY0 = SXY = frac * mu * (PT0 - PF0) averaged over the four pressure nodes
+ * around each basic node (mu = the fault markers' friction 0.6).  The
+ * reference starts from uniform PT0/PF0 and zero stress
+ * (h_mec_rsf_v9.cpp:316-337); this replaces that start for the stress test. */
+int hmec_init_coseismic(hmec_ctx *ctx, double frac, double noise, double pf_ratio, uint64_t seed);
 
 /* Upload the 19 marker arrays (host, M doubles each); marker order kept.
  * Replaces the marker init of h_mec_rsf_v9.cpp:188-209. */
