@@ -206,6 +206,15 @@ def main():
                                                    "achieved": ach / 1e12, "peak": PEAK_FP32 / 1e12,
                                                    "unit": "TFLOP/s", "frac": ach / PEAK_FP32,
                                                    "traffic": None}))
+    # measured HBM bytes per launch of the sweep kernels (rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes, copy-calibrated), committed beside the kernel summary
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "sweep_801x201_traffic.json")
+    if args.cfg == 3 and os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        traffic = {"sweep_fwd": tj["dir0"]["traffic_bytes_per_launch"],
+                   "sweep_bwd": tj["dir1"]["traffic_bytes_per_launch"]}
     for nm, kn in (("sweep_fwd", "k_sweep<float,0> (forward, per level)"),
                    ("sweep_bwd", "k_sweep<float,1> (backward, per level)")):
         k = kt32.get(nm)
@@ -213,7 +222,9 @@ def main():
             ach = k["work"] / (k["avg_ms"] * 1e-3)
             cands.append((k["n"] * k["avg_ms"], {"kernel": kn, "bound": "hbm", "achieved": ach / 1e9,
                                                   "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                                                  "frac": ach / PEAK_HBM, "traffic": None}))
+                                                  "frac": ach / PEAK_HBM,
+                                                  "algorithmic_bytes": k["work"],
+                                                  "traffic": traffic.get(nm)}))
     sc = kt32.get("scatter")
     if sc:
         ach = sc["work"] / (sc["avg_ms"] * 1e-3)

@@ -671,22 +671,48 @@ __global__ void k_residual(const int32_t *rowp, const int32_t *coli, const doubl
     for (int p = rowp[t]; p < rowp[t + 1]; p++) s -= val[p] * x[coli[p]];
     r[t] = s;
 }
-// out[0] = max|a|, out[1] = max|b|
-__global__ void __launch_bounds__(1024) k_norms(const double *a, const double *b, int64_t N,
-                                                double *out) {
-    __shared__ double ra[1024], rb[1024];
+// out[0] = max|a|, out[1] = max|b| in two stages (block maxima, then one
+// block over them); a NaN anywhere propagates to the result
+constexpr int NRM_BLOCKS = 256;
+__device__ __forceinline__ double nan_max(double m, double v) { return (v > m || v != v) ? v : m; }
+__global__ void __launch_bounds__(256) k_norms_part(const double *a, const double *b, int64_t N,
+                                                    double *part) {
+    __shared__ double ra[256], rb[256];
     double ma = 0, mb = 0;
-    for (int64_t t = threadIdx.x; t < N; t += blockDim.x) {
-        ma = fmax(ma, fabs(a[t]));
-        mb = fmax(mb, fabs(b[t]));
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < N;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        ma = nan_max(ma, fabs(a[t]));
+        mb = nan_max(mb, fabs(b[t]));
     }
     ra[threadIdx.x] = ma;
     rb[threadIdx.x] = mb;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
-            ra[threadIdx.x] = fmax(ra[threadIdx.x], ra[threadIdx.x + s]);
-            rb[threadIdx.x] = fmax(rb[threadIdx.x], rb[threadIdx.x + s]);
+            ra[threadIdx.x] = nan_max(ra[threadIdx.x], ra[threadIdx.x + s]);
+            rb[threadIdx.x] = nan_max(rb[threadIdx.x], rb[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = ra[0];
+        part[2 * blockIdx.x + 1] = rb[0];
+    }
+}
+__global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb, double *out) {
+    __shared__ double ra[256], rb[256];
+    double ma = 0, mb = 0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        ma = nan_max(ma, part[2 * k]);
+        mb = nan_max(mb, part[2 * k + 1]);
+    }
+    ra[threadIdx.x] = ma;
+    rb[threadIdx.x] = mb;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            ra[threadIdx.x] = nan_max(ra[threadIdx.x], ra[threadIdx.x + s]);
+            rb[threadIdx.x] = nan_max(rb[threadIdx.x], rb[threadIdx.x + s]);
         }
         __syncthreads();
     }
@@ -802,6 +828,7 @@ struct SolverDev {
     int max_level_fronts = 0;
     std::vector<int> maxu;         // per level max update size
     std::vector<int> maxt_fwd, maxt_bwd;   // per level row-chunk counts
+    double *nrm_part = nullptr;    // block maxima of the residual norms
     void *mem = nullptr;           // member pointer table of the last factorization
     int64_t mem_cap = 0;           // bytes
     std::vector<unsigned char> mem_host;   // its host copy
@@ -854,6 +881,7 @@ int sv_free_device(hmec_ctx *c) {
         hipFree(s->voff);
         hipFree(s->ea_off);
         hipFree(s->norms);
+        hipFree(s->nrm_part);
         hipFree(s->dscr);
         hipFree(s->dinv64);
         hipFree(s->dinv32);
@@ -1024,6 +1052,7 @@ int sv_setup_device(hmec_ctx *c) {
     // arenas
     HMEC_HIP(hipMalloc(&c->d.vec, sizeof(double) * std::max<int64_t>(1, pl.varena)));
     HMEC_HIP(hipMalloc(&s->norms, sizeof(double) * 4));
+    HMEC_HIP(hipMalloc(&s->nrm_part, sizeof(double) * 2 * NRM_BLOCKS));
     HMEC_HIP(hipHostMalloc(&s->h_norms, sizeof(double) * 4, hipHostMallocDefault));
     return HMEC_OK;
 }
@@ -1295,6 +1324,16 @@ static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
 // x = A^-1 b by the factor + iterative refinement on the FP64 CSR.
 // tol: target ||b - A x||inf / ||b||inf; max_ir: refinement sweeps after the
 // first solve.  Returns HMEC_E_NOCONV when the target is not met.
+// out[0..1] = max|a|, max|b| (device), two launches on the context stream
+static int launch_norms(hmec_ctx *c, const double *a, const double *b, double *out) {
+    SolverDev *s = sdev(c);
+    hipLaunchKernelGGL(k_norms_part, dim3(NRM_BLOCKS), dim3(256), 0, c->stream, a, b, c->N, s->nrm_part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(256), 0, c->stream, s->nrm_part, NRM_BLOCKS, out);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
 // z <- (LU)^-1 z with the current factor: the sweep form when it has been
 // formed for this factor, else the level-by-level inverted-triangle solves
 static int apply_factor(hmec_ctx *c, double *z) {
@@ -1325,8 +1364,7 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
         hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, x,
                            c->d.rhs, r, N);
         LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, st, r, c->d.rhs, N, s->norms);
-        LAUNCH_CHECK();
+        HMEC_TRY(launch_norms(c, r, c->d.rhs, s->norms));
         HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
         HMEC_HIP(hipStreamSynchronize(st));
         bnorm = s->h_norms[1];
@@ -1455,10 +1493,8 @@ static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *r
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, c->d.work, c->d.rscale, g.rsd, N);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, st, c->d.work, c->d.rhs, N, s->norms);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, st, g.rsd, g.bs, N, s->norms + 2);
-    LAUNCH_CHECK();
+    HMEC_TRY(launch_norms(c, c->d.work, c->d.rhs, s->norms));
+    HMEC_TRY(launch_norms(c, g.rsd, g.bs, s->norms + 2));
     HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 4, hipMemcpyDeviceToHost, st));
     HMEC_HIP(hipStreamSynchronize(st));
     const double *h = s->h_norms;
