@@ -68,7 +68,7 @@ def run_path(hm, p, soa, prec, warmup, steps):
     total = time.perf_counter() - t0
     kt = {}
     for name in ("scatter", "assemble", "factor", "solve", "schur", "sweep_form", "sweep_fwd",
-                 "sweep_bwd", "fwd_upd", "bwd_upd"):
+                 "sweep_bwd"):
         n, ms, work = ctx.kernel_time(name)
         if n > 0:
             kt[name] = {"n": n, "avg_ms": ms, "work": work}

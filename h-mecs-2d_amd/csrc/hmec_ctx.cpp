@@ -46,12 +46,13 @@ void tm_begin(hmec_ctx *c, Timer &t) {
     t.open = true;
 }
 
-void tm_end(hmec_ctx *c, Timer &t, double work) {
+void tm_end(hmec_ctx *c, Timer &t, double work, int launches) {
     if (!c->timing || !t.open) return;
     hipEvent_t b = tm_event(t);
     hipEventRecord(b, c->stream);
     t.pending.emplace_back(t.open_ev, b);
     t.pending_work.push_back(work);
+    t.pending_count.push_back(launches);
     t.open = false;
     if (t.pending.size() >= 4096) tm_resolve(t);
 }
@@ -63,12 +64,13 @@ void tm_resolve(Timer &t) {
         hipEventElapsedTime(&ms, t.pending[k].first, t.pending[k].second);
         t.total_ms += ms;
         t.bytes += t.pending_work[k];
-        t.n++;
+        t.n += t.pending_count[k];
         t.pool.push_back(t.pending[k].first);
         t.pool.push_back(t.pending[k].second);
     }
     t.pending.clear();
     t.pending_work.clear();
+    t.pending_count.clear();
 }
 
 void tm_free(Timer &t) {

@@ -119,6 +119,7 @@ struct DevPlan {
 struct Timer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     std::vector<double> pending_work;
+    std::vector<int> pending_count;   // launches covered by each interval
     std::vector<hipEvent_t> pool;
     hipEvent_t open_ev = nullptr;
     double total_ms = 0;
@@ -245,14 +246,14 @@ void sv_sweep_free(hmec_ctx *c);
 
 // timing helpers
 void tm_begin(hmec_ctx *c, Timer &t);
-void tm_end(hmec_ctx *c, Timer &t, double bytes);
+void tm_end(hmec_ctx *c, Timer &t, double bytes, int launches = 1);
 void tm_resolve(Timer &t);
 void tm_free(Timer &t);
 inline void kt_begin(hmec_ctx *c, const char *name) {
     if (c->timing) tm_begin(c, c->kt[name]);
 }
-inline void kt_end(hmec_ctx *c, const char *name, double work) {
-    if (c->timing) tm_end(c, c->kt[name], work);
+inline void kt_end(hmec_ctx *c, const char *name, double work, int launches = 1) {
+    if (c->timing) tm_end(c, c->kt[name], work, launches);
 }
 
 inline double *F(hmec_ctx *c, int f) { return c->d.fields + (int64_t)f * c->ns; }

@@ -1260,11 +1260,12 @@ static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
     SolverDev *s = sdev(c);
     hipStream_t st = c->stream;
     double *wbuf = c->d.vec;
+    // one timing interval per direction (levels run back to back)
+    kt_begin(c, "fwd");
     for (int lv = 0; lv < pl.nlevels; lv++) {
         const int nfr = (int)dp.h_tasks[lv].size();
         const int *fr = s->fronts + s->lvl_fronts_off[lv];
         const int bs = dp.maxn[lv] <= 128 ? 64 : 256;
-        kt_begin(c, "fwd");
         hipLaunchKernelGGL(k_fwd_gather, dim3(nfr), dim3(bs), 0, st, dp.d_tasks[lv], fr,
                            dp.d_front_dofs, dp.d_front_dof_off, s->voff, s->child_ptr, s->child_list,
                            s->ea_off, dp.d_ea_map, s->all_tasks, z, wbuf);
@@ -1278,30 +1279,26 @@ static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
                            s->part);
         LAUNCH_CHECK();
         if (s->gemv_fwd_grid[lv] > 0) {
-            kt_begin(c, "fwd_upd");
             hipLaunchKernelGGL((k_gemv_part<T, 0>), dim3(s->gemv_fwd_grid[lv], nfr), dim3(GR), 0, st,
                                dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff,
                                arena, z, wbuf, s->part);
             LAUNCH_CHECK();
-            kt_end(c, "fwd_upd", s->fwdupd_entries[lv] * sizeof(T));
             hipLaunchKernelGGL(k_gemv_reduce<0>, dim3((s->maxu[lv] + 255) / 256, nfr), dim3(256), 0,
                                st, dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, z,
                                wbuf, s->part);
             LAUNCH_CHECK();
         }
-        kt_end(c, "fwd", 0);
     }
+    kt_end(c, "fwd", 0, pl.nlevels);
+    kt_begin(c, "bwd");
     for (int lv = pl.nlevels - 1; lv >= 0; lv--) {
         const int nfr = (int)dp.h_tasks[lv].size();
         const int *fr = s->fronts + s->lvl_fronts_off[lv];
-        kt_begin(c, "bwd");
         if (s->gemv_bwd_grid[lv] > 0) {
-            kt_begin(c, "bwd_upd");
             hipLaunchKernelGGL((k_gemv_part<T, 1>), dim3(s->gemv_bwd_grid[lv], nfr), dim3(GR), 0, st,
                                dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff,
                                arena, z, wbuf, s->part);
             LAUNCH_CHECK();
-            kt_end(c, "bwd_upd", s->bwdupd_entries[lv] * sizeof(T));
         }
         // t = y - U12 x_u (a plain copy of y where the front has no update set)
         hipLaunchKernelGGL(k_gemv_reduce<1>, dim3((dp.maxp[lv] + 255) / 256, nfr), dim3(256), 0, st,
@@ -1316,8 +1313,8 @@ static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
                            dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, z, wbuf,
                            s->part);
         LAUNCH_CHECK();
-        kt_end(c, "bwd", 0);
     }
+    kt_end(c, "bwd", 0, pl.nlevels);
     return HMEC_OK;
 }
 

@@ -240,38 +240,76 @@ __global__ void __launch_bounds__(SW_T) k_sweep(const SwTask *tasks, const SwFro
     __syncthreads();
     const int lane = tid % L, grp = tid / L;
     constexpr int NG = SW_T / L;
+    constexpr int RB = 4;                          // rows in flight per lane group
+    constexpr int STEP = V::W * L;
     const T *base = sa + (DIR == 0 ? F.soff : F.boff);
     const int64_t ld = DIR == 0 ? F.ldf : F.ldb;
-    for (int q = grp; q < t.nr; q += NG) {
-        const int r = t.r0 + q;
-        int cb, ce;
-        if (DIR == 0) {
-            cb = 0;
-            ce = r < F.p ? r : F.p;      // Fm rows < p: strictly lower
-        } else {
-            cb = r & ~3;                 // Bm: upper in the pivot columns
-            ce = F.n;
-        }
-        const typename V::type *row = reinterpret_cast<const typename V::type *>(base + (int64_t)r * ld);
-        double acc = 0.0;
-        int c = cb + V::W * lane;
-        constexpr int STEP = V::W * L;
-        for (; c + 3 * STEP < ce; c += 4 * STEP) {
-            typename V::type v[4];
+    typedef typename V::type VT;
+    for (int q0 = grp; q0 < t.nr; q0 += NG * RB) {
+        int cb[RB], ce[RB];
+        const VT *row[RB];
+        double acc[RB];
+        int cmin = 1 << 30, cmax = 0;
 #pragma unroll
-            for (int u = 0; u < 4; u++) v[u] = row[(c + u * STEP) / V::W];
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc += V::dot(v[u], xs + c + u * STEP);
-        }
-        for (; c < ce; c += STEP) acc += V::dot(row[c / V::W], xs + c);
-#pragma unroll
-        for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
-        if (lane == 0) {
-            if (DIR == 0) {
-                const double b = r < F.p ? xs[r] : bi[q];
-                wbuf[F.voff + r] = b + acc;
+        for (int u = 0; u < RB; u++) {
+            const int q = q0 + u * NG;
+            const int r = t.r0 + q;
+            if (q < t.nr) {
+                if (DIR == 0) {
+                    cb[u] = 0;
+                    ce[u] = r < F.p ? r : F.p;      // Fm rows < p: strictly lower
+                } else {
+                    cb[u] = r & ~3;                 // Bm: upper in the pivot columns
+                    ce[u] = F.n;
+                }
             } else {
-                z[dof[r]] = acc;
+                cb[u] = 0;
+                ce[u] = 0;
+            }
+            row[u] = reinterpret_cast<const VT *>(base + (int64_t)(q < t.nr ? r : t.r0) * ld);
+            acc[u] = 0.0;
+            if (ce[u] > cb[u]) {
+                cmin = min(cmin, cb[u]);
+                cmax = max(cmax, ce[u]);
+            }
+        }
+        // each lane walks the rows' common column range; a row only loads the
+        // vectors of its own [cb, ce) range (cb, ce, STEP multiples of 4)
+        int c = (cmin < cmax ? cmin : 0) + V::W * lane;
+        for (; c + STEP < cmax; c += 2 * STEP) {
+            VT v0[RB], v1[RB];
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                v0[u] = (c >= cb[u] && c < ce[u]) ? row[u][c / V::W] : VT{};
+                v1[u] = (c + STEP >= cb[u] && c + STEP < ce[u]) ? row[u][(c + STEP) / V::W] : VT{};
+            }
+#pragma unroll
+            for (int u = 0; u < RB; u++) acc[u] += V::dot(v0[u], xs + c) + V::dot(v1[u], xs + c + STEP);
+        }
+        if (c < cmax) {
+            VT v0[RB];
+#pragma unroll
+            for (int u = 0; u < RB; u++) v0[u] = (c >= cb[u] && c < ce[u]) ? row[u][c / V::W] : VT{};
+#pragma unroll
+            for (int u = 0; u < RB; u++) acc[u] += V::dot(v0[u], xs + c);
+        }
+#pragma unroll
+        for (int u = 0; u < RB; u++) {
+#pragma unroll
+            for (int o = L / 2; o > 0; o >>= 1) acc[u] += __shfl_xor(acc[u], o, L);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const int q = q0 + u * NG;
+                if (q >= t.nr) continue;
+                const int r = t.r0 + q;
+                if (DIR == 0) {
+                    const double b = r < F.p ? xs[r] : bi[q];
+                    wbuf[F.voff + r] = b + acc[u];
+                } else {
+                    z[dof[r]] = acc[u];
+                }
             }
         }
     }
@@ -651,18 +689,25 @@ int sv_sweep_solve(hmec_ctx *c, double *z) {
     const FrontPlan &pl = c->plan;
     const bool q = s->built == HMEC_PREC_FP64;
     const double es = q ? 8.0 : 4.0;
+    // one timing interval per direction (the level launches run back to
+    // back): per-launch averages = interval / launches, bytes summed
+    double bf = 0, bb = 0;
     for (int lv = 0; lv < pl.nlevels; lv++) {
-        kt_begin(c, "sweep_fwd");
+        bf += s->bytes_fwd[lv] * es;
+        bb += s->bytes_bwd[lv] * es;
+    }
+    kt_begin(c, "sweep_fwd");
+    for (int lv = 0; lv < pl.nlevels; lv++) {
         if (q) HMEC_TRY((launch_sweep<double, 0>(c, s, lv, (const double *)s->sa[1], z)));
         else HMEC_TRY((launch_sweep<float, 0>(c, s, lv, (const float *)s->slots[s->cur].sa, z)));
-        kt_end(c, "sweep_fwd", s->bytes_fwd[lv] * es);
     }
+    kt_end(c, "sweep_fwd", bf, pl.nlevels);
+    kt_begin(c, "sweep_bwd");
     for (int lv = pl.nlevels - 1; lv >= 0; lv--) {
-        kt_begin(c, "sweep_bwd");
         if (q) HMEC_TRY((launch_sweep<double, 1>(c, s, lv, (const double *)s->sa[1], z)));
         else HMEC_TRY((launch_sweep<float, 1>(c, s, lv, (const float *)s->slots[s->cur].sa, z)));
-        kt_end(c, "sweep_bwd", s->bytes_bwd[lv] * es);
     }
+    kt_end(c, "sweep_bwd", bb, pl.nlevels);
     return HMEC_OK;
 }
 
