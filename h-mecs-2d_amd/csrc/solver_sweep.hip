@@ -37,6 +37,8 @@
 #include "hmec_internal.h"
 #include "mfma_traits.h"
 
+int hmec_var_pos(int v);
+
 #define SW_LAUNCH_CHECK()                                                   \
     do {                                                                    \
         hipError_t e_ = hipGetLastError();                                  \
@@ -48,8 +50,10 @@
 
 namespace {
 
-constexpr int SW_T = 256;      // threads per sweep workgroup
-constexpr int SW_RPG = 4;      // rows per lane group and workgroup block
+constexpr int SW_T = 256;      // threads per sweep workgroup (64 on small-front levels)
+constexpr int SW_RPG = 4;      // rows in flight per lane group
+constexpr int SW_PASSES = 4;   // most row passes per full-size workgroup
+constexpr int SW_FILL = 2048;  // workgroups a level keeps at least (8 per CU)
 constexpr int SW_TILE = 64;    // sweep-form GEMM tile
 constexpr int SW_TK = 16;      // its K chunk
 
@@ -64,9 +68,21 @@ struct SwFront {
     int ldf, ldb;    // padded row lengths (multiples of 4 entries)
 };
 
+// one workgroup's rows of one front, with what the kernel needs of the
+// front (no second dependent load for a front descriptor)
 struct SwTask {
-    int f;           // front
+    int64_t moff;    // Fm (forward) / Bm (backward) offset in the sweep arena
+    int64_t voff;    // front vector offset in the per-front vector arena
+    int64_t goff;    // first row source record of the front
+    int n, p, ld;    // front order, pivots, padded row length of the block
     int r0, nr;      // row block
+    int pad;
+};
+
+// where a front row's input comes from: z[z] (the row's global dof) and the
+// contributions wbuf[w0], wbuf[w1] of at most two children (-1: none)
+struct SwSrc {
+    int32_t z, w0, w1, pad;
 };
 
 struct SwTile {
@@ -202,48 +218,50 @@ template <> struct Vec16<double> {
 // out[r] = Bm[r,:] x -> z at the front's pivot dofs.
 // Dynamic LDS: xs (K entries padded to a multiple of 4, zero padded) then,
 // forward only, the gathered b of the block's rows.
-template <typename T, int DIR, int L>
-__global__ void __launch_bounds__(SW_T) k_sweep(const SwTask *tasks, const SwFront *fronts,
-                                                const T *sa, const int32_t *dofs,
-                                                const int32_t *gptr, const int32_t *gidx,
-                                                double *z, double *wbuf) {
+template <typename T, int DIR, int L, int NT>
+__global__ void __launch_bounds__(NT) k_sweep(const SwTask *tasks, const T *sa, const SwSrc *srcs,
+                                              double *z, double *wbuf) {
     extern __shared__ double xs[];
     typedef Vec16<T> V;
     const SwTask t = tasks[blockIdx.x];
-    const SwFront F = fronts[t.f];
     const int tid = threadIdx.x;
-    const int32_t *dof = dofs + F.dofo;
-    const int K = DIR == 0 ? F.p : F.n;
+    const SwSrc *src = srcs + t.goff;
+    const int K = DIR == 0 ? t.p : t.n;
     const int Kp = (K + 3) & ~3;
-    for (int k = tid; k < Kp; k += SW_T) {
+    for (int k = tid; k < Kp; k += NT) {
         double v = 0.0;
         if (k < K) {
+            const SwSrc e = src[k];
             if (DIR == 0) {
-                v = z[dof[k]];
-                for (int e = gptr[F.goff + k]; e < gptr[F.goff + k + 1]; e++) v += wbuf[gidx[e]];
+                v = z[e.z];
+                if (e.w0 >= 0) v += wbuf[e.w0];
+                if (e.w1 >= 0) v += wbuf[e.w1];
             } else {
-                v = k < F.p ? wbuf[F.voff + k] : z[dof[k]];
+                v = k < t.p ? wbuf[t.voff + k] : z[e.z];
             }
         }
         xs[k] = v;
     }
     double *bi = xs + Kp;      // forward: b of rows >= p of this block
     if (DIR == 0) {
-        for (int q = tid; q < t.nr; q += SW_T) {
+        for (int q = tid; q < t.nr; q += NT) {
             const int r = t.r0 + q;
             double v = 0.0;
-            if (r >= F.p)
-                for (int e = gptr[F.goff + r]; e < gptr[F.goff + r + 1]; e++) v += wbuf[gidx[e]];
+            if (r >= t.p) {
+                const SwSrc e = src[r];
+                if (e.w0 >= 0) v += wbuf[e.w0];
+                if (e.w1 >= 0) v += wbuf[e.w1];
+            }
             bi[q] = v;
         }
     }
     __syncthreads();
     const int lane = tid % L, grp = tid / L;
-    constexpr int NG = SW_T / L;
+    constexpr int NG = NT / L;
     constexpr int RB = 4;                          // rows in flight per lane group
     constexpr int STEP = V::W * L;
-    const T *base = sa + (DIR == 0 ? F.soff : F.boff);
-    const int64_t ld = DIR == 0 ? F.ldf : F.ldb;
+    const T *base = sa + t.moff;
+    const int64_t ld = t.ld;
     typedef typename V::type VT;
     for (int q0 = grp; q0 < t.nr; q0 += NG * RB) {
         int cb[RB], ce[RB];
@@ -257,10 +275,10 @@ __global__ void __launch_bounds__(SW_T) k_sweep(const SwTask *tasks, const SwFro
             if (q < t.nr) {
                 if (DIR == 0) {
                     cb[u] = 0;
-                    ce[u] = r < F.p ? r : F.p;      // Fm rows < p: strictly lower
+                    ce[u] = r < t.p ? r : t.p;      // Fm rows < p: strictly lower
                 } else {
                     cb[u] = r & ~3;                 // Bm: upper in the pivot columns
-                    ce[u] = F.n;
+                    ce[u] = t.n;
                 }
             } else {
                 cb[u] = 0;
@@ -305,10 +323,10 @@ __global__ void __launch_bounds__(SW_T) k_sweep(const SwTask *tasks, const SwFro
                 if (q >= t.nr) continue;
                 const int r = t.r0 + q;
                 if (DIR == 0) {
-                    const double b = r < F.p ? xs[r] : bi[q];
-                    wbuf[F.voff + r] = b + acc[u];
+                    const double b = r < t.p ? xs[r] : bi[q];
+                    wbuf[t.voff + r] = b + acc[u];
                 } else {
-                    z[dof[r]] = acc[u];
+                    z[src[r].z] = acc[u];
                 }
             }
         }
@@ -320,12 +338,13 @@ struct SweepDev {
     bool ready = false;
     SwFront *fronts = nullptr;
     std::vector<SwFront> h_fronts;
-    int32_t *gptr = nullptr, *gidx = nullptr;
+    SwSrc *srcs = nullptr;
     SwTile *tiles = nullptr;
     int ntiles = 0;
     std::vector<SwTask *> d_fwd, d_bwd;
     std::vector<int> n_fwd, n_bwd;
     std::vector<int> l_fwd, l_bwd;               // lanes per row
+    std::vector<int> t_fwd, t_bwd;               // threads per workgroup
     std::vector<int> lds_fwd, lds_bwd;           // dynamic LDS bytes
     std::vector<double> bytes_fwd, bytes_bwd;    // stored entries read per level
     int64_t sa_entries = 0;
@@ -372,21 +391,36 @@ int upload(V **dst, const std::vector<V> &v) {
 // row-block tasks of one level and direction; lanes per row from the
 // level's largest row length
 void level_tasks(const std::vector<SwFront> &fr, const std::vector<int> &fronts, int dir,
-                 int vw, std::vector<SwTask> &tasks, int &lanes, int &lds, double &entries) {
+                 int vw, std::vector<SwTask> &tasks, int &lanes, int &nt, int &lds, double &entries) {
     tasks.clear();
     entries = 0;
-    int kmax = 0;
-    for (int f : fronts) kmax = std::max(kmax, dir == 0 ? fr[f].p : fr[f].n);
+    int kmax = 0, mmax = 0;
+    for (int f : fronts) {
+        kmax = std::max(kmax, dir == 0 ? fr[f].p : fr[f].n);
+        mmax = std::max(mmax, dir == 0 ? fr[f].n : fr[f].p);
+    }
     lanes = 8;
     while (lanes < 64 && lanes * vw * 2 < kmax) lanes *= 2;
-    const int rows_per_block = (SW_T / lanes) * SW_RPG;
+    // small fronts: one-wave workgroups, so a CU keeps 4x more fronts (and
+    // their dependent gather/load chains) in flight
+    nt = (mmax <= 128 && lanes <= 16) ? 64 : SW_T;
+    // a workgroup may take several passes of rows so that the gather
+    // of the front's input vector into LDS is shared by more rows, as long as
+    // the level keeps enough workgroups to fill the device
+    const int rows1 = (nt / lanes) * SW_RPG;
+    int64_t blocks1 = 0;
+    for (int f : fronts) blocks1 += ((dir == 0 ? fr[f].n : fr[f].p) + rows1 - 1) / rows1;
+    const int passes = (int)std::max<int64_t>(1, std::min<int64_t>(SW_PASSES, blocks1 / SW_FILL));
+    const int rows_per_block = rows1 * passes;
     int nrmax = 0;
     for (int f : fronts) {
         const int n = fr[f].n, p = fr[f].p;
         const int m = dir == 0 ? n : p;
         for (int r0 = 0; r0 < m; r0 += rows_per_block) {
             const int nr = std::min(rows_per_block, m - r0);
-            tasks.push_back(SwTask{f, r0, nr});
+            const SwFront &F = fr[f];
+            tasks.push_back(SwTask{dir == 0 ? F.soff : F.boff, F.voff, F.goff, F.n, F.p,
+                                   dir == 0 ? F.ldf : F.ldb, r0, nr, 0});
             nrmax = std::max(nrmax, nr);
         }
         // stored nonzero entries read: Fm = strict lower p x p + u x p,
@@ -420,25 +454,32 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
         hmec_set_last_error("sweep: vector arena exceeds 32-bit gather indices");
         return HMEC_E_ARG;
     }
-    // parent row -> child contribution entries, children in plan order
-    std::vector<int32_t> gptr(goff), gidx;
-    std::vector<std::vector<int32_t>> rows;
+    // row sources: the row's global dof and the child contribution entries
+    // that land on it (children in plan order; at most two in a binary
+    // dissection tree)
+    int inv[6];
+    for (int v = 0; v < 6; v++) inv[hmec_var_pos(v)] = v;
+    std::vector<SwSrc> srcs(goff);
     for (int f = 0; f < nf; f++) {
         const SwFront &F = s->h_fronts[f];
-        rows.assign(F.n, {});
+        for (int r = 0; r < F.n; r++) {
+            const int node = pl.nodes[f][r / 6];
+            srcs[F.goff + r] = SwSrc{(int32_t)(node * 6 + inv[r % 6]), -1, -1, 0};
+        }
         for (int ch : pl.children[f]) {
             const int cn = 6 * (pl.np[ch] + pl.nu[ch]), cp = 6 * pl.np[ch];
             const int32_t *map = pl.ea_map.data() + pl.ea_off[ch];
             for (int a = 0; a < cn - cp; a++) {
-                const int loc = map[a / 6] * 6 + a % 6;
-                rows[loc].push_back((int32_t)(pl.voff[ch] + cp + a));
+                SwSrc &e = srcs[F.goff + map[a / 6] * 6 + a % 6];
+                const int32_t w = (int32_t)(pl.voff[ch] + cp + a);
+                if (e.w0 < 0) e.w0 = w;
+                else if (e.w1 < 0) e.w1 = w;
+                else {
+                    hmec_set_last_error("sweep: a front row receives from more than two children");
+                    return HMEC_E_ARG;
+                }
             }
         }
-        for (int r = 0; r < F.n; r++) {
-            gptr[F.goff + r] = (int32_t)gidx.size();
-            gidx.insert(gidx.end(), rows[r].begin(), rows[r].end());
-        }
-        gptr[F.goff + F.n] = (int32_t)gidx.size();
     }
     // sweep-form tiles
     std::vector<SwTile> tiles;
@@ -457,8 +498,7 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
     s->form_flops = flops;
     s->ntiles = (int)tiles.size();
     HMEC_TRY(upload(&s->fronts, s->h_fronts));
-    HMEC_TRY(upload(&s->gptr, gptr));
-    HMEC_TRY(upload(&s->gidx, gidx));
+    HMEC_TRY(upload(&s->srcs, srcs));
     HMEC_TRY(upload(&s->tiles, tiles));
     // tasks per level (the LDS need is the same for both precisions)
     s->d_fwd.assign(pl.nlevels, nullptr);
@@ -467,6 +507,8 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
     s->n_bwd.assign(pl.nlevels, 0);
     s->l_fwd.assign(pl.nlevels, 0);
     s->l_bwd.assign(pl.nlevels, 0);
+    s->t_fwd.assign(pl.nlevels, 0);
+    s->t_bwd.assign(pl.nlevels, 0);
     s->lds_fwd.assign(pl.nlevels, 0);
     s->lds_bwd.assign(pl.nlevels, 0);
     s->bytes_fwd.assign(pl.nlevels, 0);
@@ -474,9 +516,9 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
     std::vector<SwTask> tk;
     for (int lv = 0; lv < pl.nlevels; lv++) {
         for (int dir = 0; dir < 2; dir++) {
-            int lanes, lds;
+            int lanes, nt, lds;
             double by;
-            level_tasks(s->h_fronts, pl.levels[lv], dir, 4, tk, lanes, lds, by);
+            level_tasks(s->h_fronts, pl.levels[lv], dir, 4, tk, lanes, nt, lds, by);
             if (lds > 64 * 1024) {
                 hmec_set_last_error("sweep: front too large for the LDS input vector");
                 return HMEC_E_ARG;
@@ -484,6 +526,7 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
             HMEC_TRY(upload(dir == 0 ? &s->d_fwd[lv] : &s->d_bwd[lv], tk));
             (dir == 0 ? s->n_fwd : s->n_bwd)[lv] = (int)tk.size();
             (dir == 0 ? s->l_fwd : s->l_bwd)[lv] = lanes;
+            (dir == 0 ? s->t_fwd : s->t_bwd)[lv] = nt;
             (dir == 0 ? s->lds_fwd : s->lds_bwd)[lv] = lds;
             (dir == 0 ? s->bytes_fwd : s->bytes_bwd)[lv] = by;
         }
@@ -662,20 +705,27 @@ void sv_sweep_invalidate(hmec_ctx *c, int precision) {
 template <typename T, int DIR>
 static int launch_sweep(hmec_ctx *c, SweepDev *s, int lv, const T *sa, double *z) {
     const int lanes = DIR == 0 ? s->l_fwd[lv] : s->l_bwd[lv];
+    const int nth = DIR == 0 ? s->t_fwd[lv] : s->t_bwd[lv];
     const int n = DIR == 0 ? s->n_fwd[lv] : s->n_bwd[lv];
     const int lds = DIR == 0 ? s->lds_fwd[lv] : s->lds_bwd[lv];
     const SwTask *tk = DIR == 0 ? s->d_fwd[lv] : s->d_bwd[lv];
     hipStream_t st = c->stream;
-    const int32_t *dofs = c->dp.d_front_dofs;
     double *wbuf = c->d.vec;
-#define SW_GO(LN)                                                                           \
-    hipLaunchKernelGGL((k_sweep<T, DIR, LN>), dim3(n), dim3(SW_T), lds, st, tk, s->fronts, sa, \
-                       dofs, s->gptr, s->gidx, z, wbuf)
-    switch (lanes) {
-        case 8: SW_GO(8); break;
-        case 16: SW_GO(16); break;
-        case 32: SW_GO(32); break;
-        default: SW_GO(64); break;
+#define SW_GO(LN, NTH)                                                                          \
+    hipLaunchKernelGGL((k_sweep<T, DIR, LN, NTH>), dim3(n), dim3(NTH), lds, st, tk, sa, s->srcs, z, \
+                       wbuf)
+    if (nth == 64) {
+        switch (lanes) {
+            case 8: SW_GO(8, 64); break;
+            default: SW_GO(16, 64); break;
+        }
+    } else {
+        switch (lanes) {
+            case 8: SW_GO(8, SW_T); break;
+            case 16: SW_GO(16, SW_T); break;
+            case 32: SW_GO(32, SW_T); break;
+            default: SW_GO(64, SW_T); break;
+        }
     }
 #undef SW_GO
     SW_LAUNCH_CHECK();
@@ -721,8 +771,7 @@ void sv_sweep_free(hmec_ctx *c) {
         g_sw.erase(it);
     }
     hipFree(s->fronts);
-    hipFree(s->gptr);
-    hipFree(s->gidx);
+    hipFree(s->srcs);
     hipFree(s->tiles);
     for (auto *p : s->d_fwd) hipFree(p);
     for (auto *p : s->d_bwd) hipFree(p);
