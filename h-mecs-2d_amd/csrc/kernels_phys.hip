@@ -740,25 +740,17 @@ __global__ void k_stress(double *f, int64_t ns, int Nx, int Ny, double dx, doubl
 // the rate-and-state bisection per fault node (:1165-1308), then ynpl / ddd
 // (summed sequentially in node order like the reference) / dtlapusta.
 // Nodal rows other than line_fault keep ETA5 = ETA0, DSY = YNY = 0 etc.
-__global__ void __launch_bounds__(1024) k_rsf(double *f, int64_t ns, int Nx, int Ny,
-                                              hmec_params P, double dt, int active,
-                                              double *scal, double *dsy2, int *flag) {
-    __shared__ double red[1024];
-    const int nb = Nx * Ny, Ny1 = Ny + 1;
-    const int tid = threadIdx.x;
-    double *VS = f + (int64_t)HMEC_F_VSLIPB * ns;
+// ETA5 = ETA0, DSY = YNY = SigmaY = SII_fault = 0, OM5 = OM on every basic
+// node (h_mec_rsf_v9.cpp:1152-1159) and block maxima of VSLIPB (the Vmax of
+// the previous iteration, taken before the fault solve overwrites it)
+constexpr int RSF_PREP_T = 256;
+__global__ void __launch_bounds__(RSF_PREP_T) k_rsf_prep(double *f, int64_t ns, int nb,
+                                                         double *vmax_part) {
+    __shared__ double red[RSF_PREP_T];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double mx = -INFINITY;
-    for (int t = tid; t < nb; t += blockDim.x) mx = fmax(mx, VS[t]);
-    red[tid] = mx;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
-        __syncthreads();
-    }
-    if (tid == 0) scal[SC_VMAX] = red[0];
-    __syncthreads();
-    // ETA5 = ETA0, DSY = YNY = SigmaY = SII_fault = 0, OM5 = OM
-    for (int t = tid; t < nb; t += blockDim.x) {
+    if (t < nb) {
+        mx = f[(int64_t)HMEC_F_VSLIPB * ns + t];
         f[(int64_t)HMEC_F_ETA5 * ns + t] = f[(int64_t)HMEC_F_ETA0 * ns + t];
         f[(int64_t)HMEC_F_DSY * ns + t] = 0;
         f[(int64_t)HMEC_F_YNY * ns + t] = 0;
@@ -766,6 +758,31 @@ __global__ void __launch_bounds__(1024) k_rsf(double *f, int64_t ns, int Nx, int
         f[(int64_t)HMEC_F_SII_FAULT * ns + t] = 0;
         f[(int64_t)HMEC_F_OM5 * ns + t] = f[(int64_t)HMEC_F_OM * ns + t];
     }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = RSF_PREP_T / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) vmax_part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(1024) k_rsf(double *f, int64_t ns, int Nx, int Ny,
+                                              hmec_params P, double dt, int active,
+                                              double *scal, double *dsy2, int *flag,
+                                              const double *vmax_part, int nvpart) {
+    __shared__ double red[1024];
+    const int Ny1 = Ny + 1;
+    const int tid = threadIdx.x;
+    double mx = -INFINITY;
+    for (int t = tid; t < nvpart; t += blockDim.x) mx = fmax(mx, vmax_part[t]);
+    red[tid] = mx;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) scal[SC_VMAX] = red[0];
     __syncthreads();
     double dtl = 1e7;
     int capped = 0;
@@ -893,17 +910,22 @@ __global__ void __launch_bounds__(1024) k_rsf(double *f, int64_t ns, int Nx, int
 }
 
 // velocity extrema, fault-band maxima and dtslip (h_mec_rsf_v9.cpp:1344-1410)
-__global__ void __launch_bounds__(1024) k_reduce(const double *f, int64_t ns, int Nx, int Ny,
-                                                 const double *yvx, const double *yvy,
-                                                 double upper, double lower, double dxstp,
-                                                 double *scal) {
-    __shared__ double red[7][1024];
+// velocity extrema, fault-band maxima and the slip-rate timestep
+// (h_mec_rsf_v9.cpp:1310-1436 inputs) in two stages: block partials, then one
+// block over them (min / max are exact, so the order does not matter)
+constexpr int RED_BLOCKS = 256;
+__global__ void __launch_bounds__(256) k_reduce_part(const double *f, int64_t ns, int Nx, int Ny,
+                                                     const double *yvx, const double *yvy,
+                                                     double upper, double lower, double dxstp,
+                                                     double *part) {
+    __shared__ double red[7][256];
     const int Ny1 = Ny + 1, tid = threadIdx.x;
     const double *vxs = f + (int64_t)HMEC_F_VXS * ns, *vys = f + (int64_t)HMEC_F_VYS * ns;
     const double *VS = f + (int64_t)HMEC_F_VSLIPB * ns;
     double vxmax = -INFINITY, vxmin = INFINITY, vymax = -INFINITY, vymin = INFINITY, bx = 0, by = 0,
            dts = 1e30;
-    for (int t = tid; t < (Nx + 1) * Ny1; t += blockDim.x) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int t = blockIdx.x * blockDim.x + tid; t < (Nx + 1) * Ny1; t += stride) {
         const int i = t % Ny1;
         const double a = vxs[t], b = vys[t];
         vxmax = fmax(vxmax, a);
@@ -913,12 +935,32 @@ __global__ void __launch_bounds__(1024) k_reduce(const double *f, int64_t ns, in
         if (yvx[i] >= upper && yvx[i] <= lower) bx = fmax(bx, fabs(a));
         if (yvy[i] >= upper && yvy[i] <= lower) by = fmax(by, fabs(b));
     }
-    for (int t = tid; t < Nx * Ny; t += blockDim.x)
+    for (int t = blockIdx.x * blockDim.x + tid; t < Nx * Ny; t += stride)
         if (VS[t] > 0) dts = fmin(dts, dxstp / VS[t]);
     red[0][tid] = vxmax; red[1][tid] = vxmin; red[2][tid] = vymax; red[3][tid] = vymin;
     red[4][tid] = bx; red[5][tid] = by; red[6][tid] = dts;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[0][tid] = fmax(red[0][tid], red[0][tid + s]);
+            red[1][tid] = fmin(red[1][tid], red[1][tid + s]);
+            red[2][tid] = fmax(red[2][tid], red[2][tid + s]);
+            red[3][tid] = fmin(red[3][tid], red[3][tid + s]);
+            red[4][tid] = fmax(red[4][tid], red[4][tid + s]);
+            red[5][tid] = fmax(red[5][tid], red[5][tid + s]);
+            red[6][tid] = fmin(red[6][tid], red[6][tid + s]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0)
+        for (int q = 0; q < 7; q++) part[q * RED_BLOCKS + blockIdx.x] = red[q][0];
+}
+__global__ void __launch_bounds__(RED_BLOCKS) k_reduce_final(const double *part, double *scal) {
+    __shared__ double red[7][RED_BLOCKS];
+    const int tid = threadIdx.x;
+    for (int q = 0; q < 7; q++) red[q][tid] = part[q * RED_BLOCKS + tid];
+    __syncthreads();
+    for (int s = RED_BLOCKS / 2; s > 0; s >>= 1) {
         if (tid < s) {
             red[0][tid] = fmax(red[0][tid], red[0][tid + s]);
             red[1][tid] = fmin(red[1][tid], red[1][tid + s]);
@@ -947,12 +989,22 @@ int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out) {
     const int active = (c->timestep > P.tyield) ? 1 : 0;
     double *dsy2 = c->d.work2;
     int *flag = c->d.counters + 16;
+    // block partials in c->d.work (free between solves)
+    const int nb = c->Nx * c->Ny;
+    const int nvp = (nb + RSF_PREP_T - 1) / RSF_PREP_T;
+    double *part = c->d.work;
+    hipLaunchKernelGGL(k_rsf_prep, dim3(nvp), dim3(RSF_PREP_T), 0, c->stream, c->d.fields, c->ns, nb,
+                       part);
+    LAUNCH_CHECK();
     hipLaunchKernelGGL(k_rsf, dim3(1), dim3(1024), 0, c->stream, c->d.fields, c->ns, c->Nx, c->Ny,
-                       P, dt, active, c->d.scal, dsy2, flag);
+                       P, dt, active, c->d.scal, dsy2, flag, (const double *)part, nvp);
     LAUNCH_CHECK();
     Coords k = coords(c);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, c->stream, c->d.fields, c->ns, c->Nx,
-                       c->Ny, k.yvx, k.yvy, P.upper_block, P.lower_block, P.dx * P.stpmax,
+    hipLaunchKernelGGL(k_reduce_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->d.fields, c->ns,
+                       c->Nx, c->Ny, k.yvx, k.yvy, P.upper_block, P.lower_block, P.dx * P.stpmax,
+                       part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCKS), 0, c->stream, (const double *)part,
                        c->d.scal);
     LAUNCH_CHECK();
     HMEC_HIP(hipMemcpyAsync(c->d.h_scal, c->d.scal, sizeof(double) * SC_COUNT,

@@ -783,19 +783,29 @@ __global__ void __launch_bounds__(256) k_dots_final(const double *partial, int n
 struct GmCoef {
     double v[32];
 };
-// w -= sum_i h_i V_i
-__global__ void k_gs_update(const double *V, int64_t ldv, int nv, GmCoef h, double *w,
+// w -= sum_i h_i V_i, coefficients from device memory (no host round trip)
+__global__ void k_gs_update(const double *V, int64_t ldv, int nv, const double *h, double *w,
                             int64_t N) {
+    __shared__ double hs[32];
+    if ((int)threadIdx.x < nv) hs[threadIdx.x] = h[threadIdx.x];
+    __syncthreads();
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     double s = w[t];
-    for (int i = 0; i < nv; i++) s -= h.v[i] * V[(int64_t)i * ldv + t];
+    for (int i = 0; i < nv; i++) s -= hs[i] * V[(int64_t)i * ldv + t];
     w[t] = s;
 }
 // y = a * x
 __global__ void k_vscale(const double *x, double a, double *y, int64_t N) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < N) y[t] = a * x[t];
+}
+// y = x / sqrt(ww[0]) (zeros on breakdown, which the host then detects)
+__global__ void k_vnormalize(const double *x, const double *ww, double *y, int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const double n = sqrt(ww[0]);
+    y[t] = n > 0 ? x[t] * (1.0 / n) : 0.0;
 }
 // x += sum_j c_j Z_j
 __global__ void k_combine(const double *Z, int64_t ldz, int nz, GmCoef c, double *x,
@@ -1441,10 +1451,21 @@ static int gm_buf(hmec_ctx *c, int m, GmresBuf **out) {
         HMEC_HIP(hipMalloc(&g.bs, sizeof(double) * c->N));
         HMEC_HIP(hipMalloc(&g.rsd, sizeof(double) * c->N));
         HMEC_HIP(hipMalloc(&g.part, sizeof(double) * RB * (m + 2)));
-        HMEC_HIP(hipMalloc(&g.hdev, sizeof(double) * (m + 2)));
-        HMEC_HIP(hipHostMalloc(&g.hhost, sizeof(double) * (m + 2), hipHostMallocDefault));
+        HMEC_HIP(hipMalloc(&g.hdev, sizeof(double) * 3 * (m + 2)));
+        HMEC_HIP(hipHostMalloc(&g.hhost, sizeof(double) * 3 * (m + 2), hipHostMallocDefault));
     }
     *out = &g;
+    return HMEC_OK;
+}
+
+// dot products of w with V_0..V_{nv-1} -> device out[0..nv)
+static int gm_dots_dev(hmec_ctx *c, GmresBuf &g, const double *V, int nv, const double *w,
+                       double *out) {
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_dots_part, dim3(RB), dim3(256), 0, st, V, c->N, nv, w, c->N, g.part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dots_final, dim3(nv), dim3(256), 0, st, g.part, RB, nv, out);
+    LAUNCH_CHECK();
     return HMEC_OK;
 }
 
@@ -1546,19 +1567,25 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
             hipLaunchKernelGGL(k_spmv_rs, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val,
                                c->d.rscale, Zj, g.w, N);
             LAUNCH_CHECK();
-            // classical Gram-Schmidt, twice
-            GmCoef co;
-            HMEC_TRY(gm_dots(c, g, g.V, j + 1, g.w, h.data()));
-            for (int i = 0; i <= j; i++) co.v[i] = h[i];
-            hipLaunchKernelGGL(k_gs_update, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, co, g.w, N);
+            // classical Gram-Schmidt, twice, coefficients kept on the device;
+            // one copy of the Hessenberg column per iteration
+            const int M2 = m + 2;
+            double *h1d = g.hdev, *h2d = g.hdev + M2, *wwd = g.hdev + 2 * M2;
+            HMEC_TRY(gm_dots_dev(c, g, g.V, j + 1, g.w, h1d));
+            hipLaunchKernelGGL(k_gs_update, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h1d, g.w, N);
             LAUNCH_CHECK();
-            HMEC_TRY(gm_dots(c, g, g.V, j + 1, g.w, h2.data()));
-            for (int i = 0; i <= j; i++) co.v[i] = h2[i];
-            hipLaunchKernelGGL(k_gs_update, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, co, g.w, N);
+            HMEC_TRY(gm_dots_dev(c, g, g.V, j + 1, g.w, h2d));
+            hipLaunchKernelGGL(k_gs_update, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h2d, g.w, N);
             LAUNCH_CHECK();
-            for (int i = 0; i <= j; i++) h[i] += h2[i];
-            double ww;
-            HMEC_TRY(gm_dots(c, g, g.w, 1, g.w, &ww));
+            HMEC_TRY(gm_dots_dev(c, g, g.w, 1, g.w, wwd));
+            if (j + 1 < m)
+                hipLaunchKernelGGL(k_vnormalize, dim3(gN), dim3(256), 0, st, g.w, wwd,
+                                   g.V + (int64_t)(j + 1) * N, N);
+            LAUNCH_CHECK();
+            HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * 3 * M2, hipMemcpyDeviceToHost, st));
+            HMEC_HIP(hipStreamSynchronize(st));
+            for (int i = 0; i <= j; i++) h[i] = g.hhost[i] + g.hhost[M2 + i];
+            const double ww = g.hhost[2 * M2];
             const double hn = std::sqrt(ww);
             h[j + 1] = hn;
             total_it++;
@@ -1580,9 +1607,6 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
                 j++;
                 break;
             }
-            hipLaunchKernelGGL(k_vscale, dim3(gN), dim3(256), 0, st, g.w, 1.0 / hn,
-                               g.V + (int64_t)(j + 1) * N, N);
-            LAUNCH_CHECK();
         }
         if (j == 0) break;
         // y = R^-1 g ; x += Z y

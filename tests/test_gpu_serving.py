@@ -72,7 +72,10 @@ def test_server_alone_vs_mix(gpu, prec):
     f1 = hm.member_footprint(p1, prec)
     f5 = hm.member_footprint(p5, prec)
     budget = int(max(2.5 * f1, f5 + 0.2 * f1))
-    s = hm.Server(precision=prec, max_active=3, hbm_budget=budget, batch_window_ms=5.0,
+    # a window long enough that the first requester of a factorization always
+    # waits for its running same-size partner: batching then does not depend
+    # on how the members' steps happen to line up
+    s = hm.Server(precision=prec, max_active=3, hbm_budget=budget, batch_window_ms=200.0,
                   keep_fields=True)
     ix = s.submit(p1, nsteps, cfg=1, seed=11, arrival_ms=0.0)
     s.submit(pb, nsteps, cfg=1, seed=12, arrival_ms=0.0)
