@@ -425,7 +425,8 @@ extern "C" int hmec_set_csr_values(hmec_ctx *c, const double *values, const doub
 extern "C" int hmec_analyze(hmec_ctx *c) {
     if (!c) return HMEC_E_ARG;
     if (c->analyzed) return HMEC_OK;
-    HMEC_TRY(hmec_plan_build(c->Nx1, c->Ny1, c->h_rowp, c->h_coli, c->plan));
+    HMEC_TRY(hmec_condense_build(c->ns, c->h_rowp, c->h_coli, c->cp));
+    HMEC_TRY(hmec_plan_build(c->Nx1, c->Ny1, c->cp.srowp, c->cp.scoli, c->plan, 4));
     HMEC_TRY(sv_setup_device(c));
     c->analyzed = true;
     return HMEC_OK;

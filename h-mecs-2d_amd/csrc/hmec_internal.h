@@ -49,6 +49,7 @@ enum MarkerProp {
 // Dense-front multifrontal plan (symbolic analysis, host side), see
 // solver_symbolic.cpp.
 struct FrontPlan {
+    int dpn = 4;                        // unknowns per node of the factored system
     // per front
     std::vector<int> np, nu;            // pivot / update node counts
     std::vector<int64_t> off;           // offset of the n x n front in the arena
@@ -73,8 +74,43 @@ struct FrontPlan {
     std::vector<int32_t> node_local;    // local node index inside that front
 };
 
+// dpn: unknowns per node of the system the plan factors (6: the coupled
+// system, 4: the Darcy-condensed system of CondPlan)
 int hmec_plan_build(int Nx1, int Ny1, const std::vector<int32_t> &rowp,
-                    const std::vector<int32_t> &coli, FrontPlan &plan);
+                    const std::vector<int32_t> &coli, FrontPlan &plan, int dpn = 4);
+// position of variable v inside a node block of a front (dpn 4 or 6)
+int hmec_var_pos(int dpn, int v);
+
+// Darcy condensation (solver_condense.cpp).  The Darcy unknowns (vxD, vyD)
+// of the coupled system are eliminated exactly: every Darcy row holds its
+// diagonal plus either up to three non-Darcy entries (its node's velocity and
+// two Pf) or one other Darcy unknown (boundary rows), so x_D = g + E x_R
+// with E = -A_DD^-1 A_DR of at most three entries per row, and the factored
+// system is S = A_RR + A_RD E on the 4 remaining unknowns per node (index
+// node*4 + v, v = 0 Pt, 1 vx, 2 vy, 3 Pf).  S keeps the node graph of the
+// coupled system (axis neighbours and one anti-diagonal pair).
+struct CondPlan {
+    int64_t N = 0, NS = 0, ND = 0;
+    std::vector<int32_t> srowp, scoli;     // S pattern (condensed numbering)
+    std::vector<int32_t> ssrc;             // per S entry: A entry of A_rc, -1: none
+    std::vector<int32_t> s_of_full;        // N: condensed index, -1 for Darcy unknowns
+    std::vector<int32_t> d_of_full;        // N: Darcy index, -1 otherwise
+    std::vector<int32_t> full_of_s;        // NS: full index
+    // per Darcy unknown d
+    std::vector<int32_t> dglob;            // full index
+    std::vector<int32_t> ddiag;            // A entry of its diagonal
+    std::vector<int32_t> dnb, dnbdiag, dnbe;   // neighbour Darcy unknown (full index), its
+                                               // diagonal entry, the coupling entry (-1: none)
+    std::vector<int32_t> des, dec;         // 3 per d: A entries of the interior row's
+                                           // non-Darcy part (own row or the neighbour's)
+                                           // and their condensed columns (-1: none)
+    // per condensed row: Darcy terms A_{r,d}
+    std::vector<int32_t> rptr, raslot, rd;
+    // per condensed row: S fill terms A_{r,d} E_{d,k} -> S entry
+    std::vector<int32_t> tptr, taslot, tdk, tsslot;   // tdk = 3 d + k
+};
+int hmec_condense_build(int64_t nnodes, const std::vector<int32_t> &rowp,
+                        const std::vector<int32_t> &coli, CondPlan &cp);
 
 // Device-side task descriptors for the batched dense kernels.
 struct FrontTask {
@@ -132,6 +168,22 @@ struct hmec_ctx_impl;
 struct hmec_batcher;
 
 // Device buffers owned by a context.
+// device copies of the condensation structures and the condensed system
+struct CondDev {
+    int32_t *srowp = nullptr, *scoli = nullptr, *ssrc = nullptr;
+    double *sval = nullptr;                  // S values (formed each factorization)
+    int32_t *s_of_full = nullptr, *d_of_full = nullptr, *full_of_s = nullptr;
+    int32_t *dglob = nullptr, *ddiag = nullptr, *dnb = nullptr, *dnbdiag = nullptr,
+            *dnbe = nullptr, *des = nullptr, *dec = nullptr;
+    int32_t *rptr = nullptr, *raslot = nullptr, *rd = nullptr;
+    int32_t *tptr = nullptr, *taslot = nullptr, *tdk = nullptr, *tsslot = nullptr;
+    double *e = nullptr;                     // 3 x ND: E of the current matrix
+    double *gv = nullptr;                    // ND: A_DD^-1 v_D of the current vector
+    double *vs = nullptr;                    // NS: condensed right-hand side / solution
+    double *rs = nullptr, *cs = nullptr;     // NS: equilibration of the active S factor
+    int64_t snnz = 0;
+};
+
 struct DeviceState {
     double *fields = nullptr;        // HMEC_NFIELDS x stag_size
     double *mk = nullptr;            // 19 x M markers
@@ -183,6 +235,8 @@ struct hmec_ctx {
     bool analyzed = false;
     FrontPlan plan;
     DevPlan dp;
+    CondPlan cp;                     // Darcy condensation (host structures)
+    CondDev dc;                      // ... and their device copies
     int factor_prec = -1;            // precision of the factor in the arena, -1: none
     bool factor_valid = false;
     int last_ir = 0, last_fallback = 0, last_perturbed = 0;

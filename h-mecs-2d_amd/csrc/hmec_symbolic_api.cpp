@@ -11,7 +11,6 @@
 int hmec_pattern_host(const hmec_params &P, std::vector<int32_t> &rowp,
                       std::vector<int32_t> &coli, std::vector<int32_t> *slot_out,
                       std::vector<int32_t> *node_trip_out);
-int hmec_var_pos(int v);
 void hmec_footprint_record(int Nx, int Ny, int64_t arena);
 
 extern "C" int hmec_pattern(const hmec_params *p, int64_t *n, int64_t *nnz, int32_t *rowp,
@@ -28,15 +27,38 @@ extern "C" int hmec_pattern(const hmec_params *p, int64_t *n, int64_t *nnz, int3
     return HMEC_OK;
 }
 
-extern "C" int hmec_plan_info(const hmec_params *p, hmec_solve_stats *st, int64_t *arena_entries,
-                              int32_t *elim_order) {
+// the factored (Darcy-condensed) system's pattern and front plan of a grid
+static int condensed_plan(const hmec_params &P, CondPlan &cp, FrontPlan &pl) {
+    std::vector<int32_t> rp, ci;
+    HMEC_TRY(hmec_pattern_host(P, rp, ci, nullptr, nullptr));
+    HMEC_TRY(hmec_condense_build((int64_t)(P.Nx + 1) * (P.Ny + 1), rp, ci, cp));
+    return hmec_plan_build(P.Nx + 1, P.Ny + 1, cp.srowp, cp.scoli, pl, 4);
+}
+
+extern "C" int hmec_condensed_pattern(const hmec_params *p, int64_t *n, int64_t *nnz, int32_t *rowp,
+                                      int32_t *coli) {
     if (!p) return HMEC_E_ARG;
     hmec_params P = *p;
     HMEC_TRY(hmec_params_finalize(&P));
     std::vector<int32_t> rp, ci;
     HMEC_TRY(hmec_pattern_host(P, rp, ci, nullptr, nullptr));
+    CondPlan cp;
+    HMEC_TRY(hmec_condense_build((int64_t)(P.Nx + 1) * (P.Ny + 1), rp, ci, cp));
+    if (n) *n = cp.NS;
+    if (nnz) *nnz = (int64_t)cp.scoli.size();
+    if (rowp) std::memcpy(rowp, cp.srowp.data(), sizeof(int32_t) * cp.srowp.size());
+    if (coli) std::memcpy(coli, cp.scoli.data(), sizeof(int32_t) * cp.scoli.size());
+    return HMEC_OK;
+}
+
+extern "C" int hmec_plan_info(const hmec_params *p, hmec_solve_stats *st, int64_t *arena_entries,
+                              int32_t *elim_order) {
+    if (!p) return HMEC_E_ARG;
+    hmec_params P = *p;
+    HMEC_TRY(hmec_params_finalize(&P));
+    CondPlan cp;
     FrontPlan pl;
-    HMEC_TRY(hmec_plan_build(P.Nx + 1, P.Ny + 1, rp, ci, pl));
+    HMEC_TRY(condensed_plan(P, cp, pl));
     hmec_footprint_record(P.Nx, P.Ny, pl.arena);
     if (st) {
         std::memset(st, 0, sizeof(*st));
@@ -48,14 +70,16 @@ extern "C" int hmec_plan_info(const hmec_params *p, hmec_solve_stats *st, int64_
     }
     if (arena_entries) *arena_entries = pl.arena;
     if (elim_order) {
-        // dofs in elimination order: fronts by level, pivots in front order
+        // condensed unknowns in elimination order: fronts by level, pivots in
+        // front order
+        const int dpn = pl.dpn;
         int inv[6];
-        for (int v = 0; v < 6; v++) inv[hmec_var_pos(v)] = v;
+        for (int v = 0; v < dpn; v++) inv[hmec_var_pos(dpn, v)] = v;
         int64_t k = 0;
         for (int lv = 0; lv < pl.nlevels; lv++)
             for (int f : pl.levels[lv])
                 for (int l = 0; l < pl.np[f]; l++)
-                    for (int q = 0; q < 6; q++) elim_order[k++] = pl.nodes[f][l] * 6 + inv[q];
+                    for (int q = 0; q < dpn; q++) elim_order[k++] = pl.nodes[f][l] * dpn + inv[q];
     }
     return HMEC_OK;
 }
@@ -68,16 +92,15 @@ extern "C" int hmec_plan_levels(const hmec_params *p, int *nlevels, int *nfronts
     if (!p || !nlevels) return HMEC_E_ARG;
     hmec_params P = *p;
     HMEC_TRY(hmec_params_finalize(&P));
-    std::vector<int32_t> rp, ci;
-    HMEC_TRY(hmec_pattern_host(P, rp, ci, nullptr, nullptr));
+    CondPlan cp;
     FrontPlan pl;
-    HMEC_TRY(hmec_plan_build(P.Nx + 1, P.Ny + 1, rp, ci, pl));
+    HMEC_TRY(condensed_plan(P, cp, pl));
     *nlevels = pl.nlevels;
     for (int lv = 0; lv < pl.nlevels; lv++) {
         int mn = 0, mp = 0;
         double fl = 0;
         for (int f : pl.levels[lv]) {
-            const int n = 6 * (pl.np[f] + pl.nu[f]), pp = 6 * pl.np[f];
+            const int n = pl.dpn * (pl.np[f] + pl.nu[f]), pp = pl.dpn * pl.np[f];
             mn = std::max(mn, n);
             mp = std::max(mp, pp);
             for (int k = 0; k < pp; k++) {
@@ -102,7 +125,7 @@ extern "C" int hmec_plan_levels(const hmec_params *p, int *nlevels, int *nfronts
 // allocates the FP64 arena on top).
 namespace {
 struct PlanSizes {
-    int64_t arena = 0, varena = 0, nnz = 0, ntrip = 0, sweep = 0, dinv = 0, nfronts = 0;
+    int64_t arena = 0, varena = 0, nnz = 0, snnz = 0, ntrip = 0, sweep = 0, dinv = 0, nfronts = 0;
 };
 std::mutex g_ps_mu;
 std::map<std::pair<int, int>, PlanSizes> g_ps;
@@ -125,15 +148,18 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     if (!have) {
         std::vector<int32_t> rp, ci, slot, nt;
         if (hmec_pattern_host(P, rp, ci, &slot, &nt) != HMEC_OK) return -1;
+        CondPlan cp;
+        if (hmec_condense_build((int64_t)(P.Nx + 1) * (P.Ny + 1), rp, ci, cp) != HMEC_OK) return -1;
         FrontPlan pl;
-        if (hmec_plan_build(P.Nx + 1, P.Ny + 1, rp, ci, pl) != HMEC_OK) return -1;
+        if (hmec_plan_build(P.Nx + 1, P.Ny + 1, cp.srowp, cp.scoli, pl, 4) != HMEC_OK) return -1;
         z.arena = pl.arena;
         z.varena = pl.varena;
         z.nnz = (int64_t)ci.size();
+        z.snnz = (int64_t)cp.scoli.size();
         z.ntrip = (int64_t)slot.size();
         z.nfronts = pl.nfronts;
         for (int f = 0; f < pl.nfronts; f++) {
-            const int64_t n = 6 * (pl.np[f] + pl.nu[f]), pv = 6 * pl.np[f];
+            const int64_t n = pl.dpn * (pl.np[f] + pl.nu[f]), pv = pl.dpn * pl.np[f];
             z.sweep += n * ((pv + 3) & ~3) + pv * ((n + 3) & ~3);
             z.dinv += ((pv + 31) / 32) * 2 * 32 * 32;
         }
@@ -144,13 +170,15 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     const int64_t M = hmec_marker_count(&P);
     int64_t b = M * 8 * (HMEC_NMARKER_FIELDS + MP_NPROP) + M * 4 * (4 + 4 + 2) + 16 * ns;
     b += ns * 8 * HMEC_NFIELDS;
-    b += z.nnz * (4 + 8 + 8 + 4 + 3 * 4) + (N + 1) * 8 + z.ntrip * 4;   // CSR, plan maps, CSC
-    b += 6 * N * 8 + z.varena * 8 + z.nfronts * 256;                    // vectors, plan arrays
+    const int64_t NS = 4 * ns;
+    b += z.nnz * (4 + 8) + (N + 1) * 4 + z.ntrip * 4;                     // coupled CSR
+    b += z.snnz * (4 + 8 + 8 + 4 + 3 * 4 + 4 + 4 * 4) + (NS + 1) * 16;  // condensed CSR, maps
+    b += 6 * N * 8 + 4 * NS * 8 + 3 * N * 4 + z.varena * 8 + z.nfronts * 256;   // vectors, plans
     if (precision == HMEC_PREC_FP64) {
         b += z.arena * 8 + z.dinv * 8;
     } else {
         // two FP32 factor slots (sweep form + scales each)
-        b += z.arena * 4 + z.dinv * 4 + 2 * (z.sweep * 4 + 2 * N * 8) + (2 * 20 + 3) * N * 8;
+        b += z.arena * 4 + z.dinv * 4 + 2 * (z.sweep * 4 + 2 * NS * 8) + (2 * 20 + 3) * N * 8;
     }
     return b;
 }

@@ -25,7 +25,6 @@
 #include "hmec_internal.h"
 #include "mfma_traits.h"
 
-int hmec_var_pos(int v);
 
 namespace {
 
@@ -105,7 +104,8 @@ __global__ void k_front_assemble(const FacPtrs<T> *mem, const int32_t *row_of, c
 // parent(map(a), map(b)) += child CB(a, b); one launch per child slot so no
 // two workgroups of a launch touch the same parent entry.
 template <typename T>
-__global__ void k_extend_add(const EaTask *tasks, const int32_t *ea_map, const FacPtrs<T> *mem) {
+__global__ void k_extend_add(const EaTask *tasks, const int32_t *ea_map, const FacPtrs<T> *mem,
+                             int dpn) {
     T *arena = mem[blockIdx.z].arena;
     const EaTask tk = tasks[blockIdx.y];
     const int u = tk.child_n - tk.child_p;
@@ -120,7 +120,7 @@ __global__ void k_extend_add(const EaTask *tasks, const int32_t *ea_map, const F
     for (int e = threadIdx.x; e < TM * TM; e += blockDim.x) {
         const int a = tr * TM + (e % TM), b = tc * TM + (e / TM);
         if (a >= u || b >= u) continue;
-        const int pa_r = map[a / 6] * 6 + a % 6, pa_c = map[b / 6] * 6 + b % 6;
+        const int pa_r = map[a / dpn] * dpn + a % dpn, pa_c = map[b / dpn] * dpn + b % dpn;
         const int ca = tk.child_p + a, cb = tk.child_p + b;
         pa[(int64_t)pa_c * pn + pa_r] += ch[(int64_t)cb * cn + ca];
     }
@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(256) k_fwd_gather(const FrontTask *tasks, cons
                                                     const int64_t *voff, const int *child_ptr,
                                                     const int *child_list, const int64_t *ea_off,
                                                     const int32_t *ea_map, const FrontTask *all_tasks,
-                                                    const double *z, double *wbuf) {
+                                                    const double *z, double *wbuf, int dpn) {
     const int f = fronts[blockIdx.x];
     const FrontTask tk = tasks[blockIdx.x];
     const int n = tk.n, p = tk.p;
@@ -562,7 +562,7 @@ __global__ void __launch_bounds__(256) k_fwd_gather(const FrontTask *tasks, cons
         const int u = ct.n - ct.p;
         const double *cvc = wbuf + voff[ch] + ct.p;
         const int32_t *map = ea_map + ea_off[ch];
-        for (int a = threadIdx.x; a < u; a += blockDim.x) w[map[a / 6] * 6 + a % 6] += cvc[a];
+        for (int a = threadIdx.x; a < u; a += blockDim.x) w[map[a / dpn] * dpn + a % dpn] += cvc[a];
         __syncthreads();
     }
 }
@@ -723,6 +723,91 @@ __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb,
 }
 
 
+// ------------------------------------------------------------ Darcy condensation
+// E = -A_DD^-1 A_DR of the current matrix (3 entries per Darcy unknown)
+__global__ void k_darcy_e(const double *val, int64_t ND, const int32_t *ddiag,
+                          const int32_t *dnbdiag, const int32_t *dnbe, const int32_t *des,
+                          double *e) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= ND) return;
+    const double add = val[ddiag[d]];
+    if (dnbe[d] < 0) {
+        for (int k = 0; k < 3; k++) {
+            const int32_t p = des[3 * d + k];
+            e[3 * d + k] = p >= 0 ? -val[p] / add : 0.0;
+        }
+    } else {
+        // boundary row d -> n: E[d,k] = -(a_dn / a_dd) E[n,k]
+        const double r = val[dnbe[d]] / add, ann = val[dnbdiag[d]];
+        for (int k = 0; k < 3; k++) {
+            const int32_t p = des[3 * d + k];
+            e[3 * d + k] = p >= 0 ? -r * (-val[p] / ann) : 0.0;
+        }
+    }
+}
+// S = A_RR + A_RD E, one thread per condensed row, terms in a fixed order
+__global__ void k_condense_matrix(const double *val, int64_t NS, const int32_t *srowp,
+                                  const int32_t *ssrc, const int32_t *tptr, const int32_t *taslot,
+                                  const int32_t *tdk, const int32_t *tsslot, const double *e,
+                                  double *sval) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= NS) return;
+    for (int j = srowp[r]; j < srowp[r + 1]; j++) sval[j] = ssrc[j] >= 0 ? val[ssrc[j]] : 0.0;
+    for (int t = tptr[r]; t < tptr[r + 1]; t++) sval[tsslot[t]] += val[taslot[t]] * e[tdk[t]];
+}
+// gv = A_DD^-1 (v_D / rs)   (rs: optional exact power-of-two row scaling)
+__global__ void k_darcy_gv(const double *v, const double *rs, const double *val, int64_t ND,
+                           const int32_t *dglob, const int32_t *ddiag, const int32_t *dnb,
+                           const int32_t *dnbdiag, const int32_t *dnbe, double *gv) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= ND) return;
+    const int64_t k = dglob[d];
+    const double vb = rs ? v[k] / rs[k] : v[k];
+    if (dnbe[d] < 0) {
+        gv[d] = vb / val[ddiag[d]];
+    } else {
+        const int64_t n = dnb[d];
+        const double vn = rs ? v[n] / rs[n] : v[n];
+        const double gn = vn / val[dnbdiag[d]];
+        gv[d] = (vb - val[dnbe[d]] * gn) / val[ddiag[d]];
+    }
+}
+// vs = rsS * (v_R / rs - A_RD gv)   (condensed right-hand side, equilibrated)
+__global__ void k_cond_rhs(const double *v, const double *rs, const double *val, int64_t NS,
+                           const int32_t *full_of_s, const int32_t *rptr, const int32_t *raslot,
+                           const int32_t *rd, const double *gv, const double *rsS, double *vs) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= NS) return;
+    const int64_t k = full_of_s[r];
+    double s = rs ? v[k] / rs[k] : v[k];
+    for (int t = rptr[r]; t < rptr[r + 1]; t++) s -= val[raslot[t]] * gv[rd[t]];
+    vs[r] = rsS[r] * s;
+}
+// z_R = csS * vs, z_D = gv + E z_R
+__global__ void k_expand(const double *vs, const double *csS, const double *gv, const double *e,
+                         int64_t N, const int32_t *s_of_full, const int32_t *d_of_full,
+                         const int32_t *dec, double *z) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int32_t si = s_of_full[k];
+    if (si >= 0) {
+        z[k] = csS[si] * vs[si];
+        return;
+    }
+    const int64_t d = d_of_full[k];
+    double s = gv[d];
+    for (int q = 0; q < 3; q++) {
+        const int32_t cidx = dec[3 * d + q];
+        if (cidx >= 0) s += e[3 * d + q] * (csS[cidx] * vs[cidx]);
+    }
+    z[k] = s;
+}
+// x = z (first) or x += z
+__global__ void k_add(double *x, const double *z, int64_t N, int first) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) x[t] = first ? z[t] : x[t] + z[t];
+}
+
 // ------------------------------------------------------------ GMRES helpers
 constexpr int RB = 512;      // blocks of the two-stage reductions
 // y = A x
@@ -874,6 +959,18 @@ int sv_free_device(hmec_ctx *c) {
     for (int k = 0; k < 2; k++)
         for (auto p : dp.d_ea[k]) hipFree(p);
     dp.d_tasks.clear();
+    {
+        CondDev &q = c->dc;
+        for (void *ptr : {(void *)q.srowp, (void *)q.scoli, (void *)q.ssrc, (void *)q.sval,
+                          (void *)q.s_of_full, (void *)q.d_of_full, (void *)q.full_of_s,
+                          (void *)q.dglob, (void *)q.ddiag, (void *)q.dnb, (void *)q.dnbdiag,
+                          (void *)q.dnbe, (void *)q.des, (void *)q.dec, (void *)q.rptr,
+                          (void *)q.raslot, (void *)q.rd, (void *)q.tptr, (void *)q.taslot,
+                          (void *)q.tdk, (void *)q.tsslot, (void *)q.e, (void *)q.gv, (void *)q.vs,
+                          (void *)q.rs, (void *)q.cs})
+            hipFree(ptr);
+        q = CondDev();
+    }
     hipFree(dp.d_nz_dst);
     hipFree(dp.d_ea_map);
     hipFree(dp.d_row_of);
@@ -921,14 +1018,18 @@ int sv_setup_device(hmec_ctx *c) {
     const FrontPlan &pl = c->plan;
     DevPlan &dp = c->dp;
     SolverDev *s = sdev(c);
-    const int64_t N = c->N, nnz = c->nnz;
+    // the factored system is the Darcy-condensed one
+    const CondPlan &cp = c->cp;
+    const std::vector<int32_t> &srp = cp.srowp, &sci = cp.scoli;
+    const int64_t N = cp.NS, nnz = (int64_t)sci.size();
+    const int dpn = pl.dpn;
     dp.nlevels = pl.nlevels;
     // all tasks (indexed by front id)
     std::vector<FrontTask> all(pl.nfronts);
     int64_t doff = 0;
     for (int f = 0; f < pl.nfronts; f++) {
-        const int p = 6 * pl.np[f];
-        all[f] = FrontTask{pl.off[f], 6 * (pl.np[f] + pl.nu[f]), p, doff, 0, 0};
+        const int p = dpn * pl.np[f];
+        all[f] = FrontTask{pl.off[f], dpn * (pl.np[f] + pl.nu[f]), p, doff, 0, 0};
         doff += (int64_t)((p + NB - 1) / NB) * 2 * NB * NB;
     }
     s->dinv_size = doff;
@@ -1031,15 +1132,15 @@ int sv_setup_device(hmec_ctx *c) {
     // row of each CSR entry and the transposed pattern
     std::vector<int32_t> row_of(nnz);
     for (int64_t r = 0; r < N; r++)
-        for (int p = c->h_rowp[r]; p < c->h_rowp[r + 1]; p++) row_of[p] = (int32_t)r;
+        for (int p = srp[r]; p < srp[r + 1]; p++) row_of[p] = (int32_t)r;
     HMEC_TRY(upload(&dp.d_row_of, row_of));
     std::vector<int32_t> cptr2(N + 1, 0), cidx(nnz), crow(nnz);
-    for (int64_t p = 0; p < nnz; p++) cptr2[c->h_coli[p] + 1]++;
+    for (int64_t p = 0; p < nnz; p++) cptr2[sci[p] + 1]++;
     for (int64_t k = 0; k < N; k++) cptr2[k + 1] += cptr2[k];
     std::vector<int32_t> fill(cptr2.begin(), cptr2.end() - 1);
     for (int64_t r = 0; r < N; r++)
-        for (int p = c->h_rowp[r]; p < c->h_rowp[r + 1]; p++) {
-            const int q = fill[c->h_coli[p]]++;
+        for (int p = srp[r]; p < srp[r + 1]; p++) {
+            const int q = fill[sci[p]]++;
             cidx[q] = p;
             crow[q] = (int32_t)r;
         }
@@ -1050,15 +1151,44 @@ int sv_setup_device(hmec_ctx *c) {
     std::vector<int32_t> dofs;
     dp.h_front_dof_off.assign(pl.nfronts + 1, 0);
     int inv[6];
-    for (int v = 0; v < 6; v++) inv[hmec_var_pos(v)] = v;
+    for (int v = 0; v < dpn; v++) inv[hmec_var_pos(dpn, v)] = v;
     for (int f = 0; f < pl.nfronts; f++) {
         dp.h_front_dof_off[f] = (int64_t)dofs.size();
         for (int node : pl.nodes[f])
-            for (int q = 0; q < 6; q++) dofs.push_back(node * 6 + inv[q]);
+            for (int q = 0; q < dpn; q++) dofs.push_back(node * dpn + inv[q]);
     }
     dp.h_front_dof_off[pl.nfronts] = (int64_t)dofs.size();
     HMEC_TRY(upload(&dp.d_front_dofs, dofs));
     HMEC_TRY(upload(&dp.d_front_dof_off, dp.h_front_dof_off));
+    // condensation structures and the condensed system's vectors
+    CondDev &q = c->dc;
+    q.snnz = nnz;
+    HMEC_TRY(upload(&q.srowp, cp.srowp));
+    HMEC_TRY(upload(&q.scoli, cp.scoli));
+    HMEC_TRY(upload(&q.ssrc, cp.ssrc));
+    HMEC_TRY(upload(&q.s_of_full, cp.s_of_full));
+    HMEC_TRY(upload(&q.d_of_full, cp.d_of_full));
+    HMEC_TRY(upload(&q.full_of_s, cp.full_of_s));
+    HMEC_TRY(upload(&q.dglob, cp.dglob));
+    HMEC_TRY(upload(&q.ddiag, cp.ddiag));
+    HMEC_TRY(upload(&q.dnb, cp.dnb));
+    HMEC_TRY(upload(&q.dnbdiag, cp.dnbdiag));
+    HMEC_TRY(upload(&q.dnbe, cp.dnbe));
+    HMEC_TRY(upload(&q.des, cp.des));
+    HMEC_TRY(upload(&q.dec, cp.dec));
+    HMEC_TRY(upload(&q.rptr, cp.rptr));
+    HMEC_TRY(upload(&q.raslot, cp.raslot));
+    HMEC_TRY(upload(&q.rd, cp.rd));
+    HMEC_TRY(upload(&q.tptr, cp.tptr));
+    HMEC_TRY(upload(&q.taslot, cp.taslot));
+    HMEC_TRY(upload(&q.tdk, cp.tdk));
+    HMEC_TRY(upload(&q.tsslot, cp.tsslot));
+    HMEC_HIP(hipMalloc(&q.sval, sizeof(double) * std::max<int64_t>(1, nnz)));
+    HMEC_HIP(hipMalloc(&q.e, sizeof(double) * 3 * std::max<int64_t>(1, cp.ND)));
+    HMEC_HIP(hipMalloc(&q.gv, sizeof(double) * std::max<int64_t>(1, cp.ND)));
+    HMEC_HIP(hipMalloc(&q.vs, sizeof(double) * std::max<int64_t>(1, N)));
+    HMEC_HIP(hipMalloc(&q.rs, sizeof(double) * std::max<int64_t>(1, N)));
+    HMEC_HIP(hipMalloc(&q.cs, sizeof(double) * std::max<int64_t>(1, N)));
     // arenas
     HMEC_HIP(hipMalloc(&c->d.vec, sizeof(double) * std::max<int64_t>(1, pl.varena)));
     HMEC_HIP(hipMalloc(&s->norms, sizeof(double) * 4));
@@ -1098,13 +1228,13 @@ static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipS
     const FrontPlan &pl = c->plan;
     DevPlan &dp = c->dp;
     SolverDev *s = sdev(c);
-    const int64_t nnz = c->nnz;
+    const int64_t nnz = c->dc.snnz;     // the condensed system
     const unsigned Bz = (unsigned)B;
     // zero + assemble
     hipLaunchKernelGGL(k_fill_zero<T>, dim3(4096, 1, Bz), dim3(256), 0, st, mem, pl.arena);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_front_assemble<T>, dim3((unsigned)((nnz + 255) / 256), 1, Bz), dim3(256), 0,
-                       st, mem, dp.d_row_of, c->d.coli, dp.d_nz_dst, nnz);
+                       st, mem, dp.d_row_of, c->dc.scoli, dp.d_nz_dst, nnz);
     LAUNCH_CHECK();
     for (int lv = 0; lv < pl.nlevels; lv++) {
         const int nfr = (int)dp.h_tasks[lv].size();
@@ -1120,7 +1250,7 @@ static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipS
             }
             if (maxt == 0) continue;
             hipLaunchKernelGGL(k_extend_add<T>, dim3(maxt, (unsigned)ea.size(), Bz), dim3(256), 0, st,
-                               dp.d_ea[q][lv], dp.d_ea_map, mem);
+                               dp.d_ea[q][lv], dp.d_ea_map, mem, pl.dpn);
             LAUNCH_CHECK();
         }
         // pivot panels
@@ -1174,6 +1304,8 @@ static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipS
     return HMEC_OK;
 }
 
+static int darcy_e(hmec_ctx *c, hipStream_t st);
+
 // Numeric factorization of n same-size contexts' current matrices with one
 // launch per kernel (n = 1: a single context).  The batch runs on the first
 // context's stream; the other members' streams are joined before and after
@@ -1198,17 +1330,25 @@ int sv_factorize_batch(hmec_ctx *const *cs, int n, int precision) {
         return HMEC_OK;
     };
     for (int k = 1; k < n; k++) HMEC_TRY(join(cs[k]->stream, st));
-    // equilibration per member (small kernels), counters cleared
+    // per member (small kernels): Darcy condensation S = A_RR + A_RD E of the
+    // current matrix, power-of-two equilibration of S, counters cleared
     for (int k = 0; k < n; k++) {
         hmec_ctx *c = cs[k];
-        const int64_t N = c->N;
+        const CondPlan &cp = c->cp;
+        CondDev &q = c->dc;
+        const int64_t NS = cp.NS;
         HMEC_HIP(hipMemsetAsync(c->d.counters, 0, sizeof(int) * 4, st));
-        hipLaunchKernelGGL(k_row_scale, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, c0->d.rowp,
-                           c->d.val, N, c->d.rscale);
+        HMEC_TRY(darcy_e(c, st));
+        hipLaunchKernelGGL(k_condense_matrix, dim3((unsigned)((NS + 255) / 256)), dim3(256), 0, st,
+                           c->d.val, NS, q.srowp, q.ssrc, q.tptr, q.taslot, q.tdk, q.tsslot, q.e,
+                           q.sval);
         LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_col_scale, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
-                           c0->dp.d_csc_ptr, c0->dp.d_csc_idx, c0->dp.d_csc_row, c->d.val, c->d.rscale,
-                           N, c->d.cscale);
+        hipLaunchKernelGGL(k_row_scale, dim3((unsigned)((NS + 255) / 256)), dim3(256), 0, st,
+                           c0->dc.srowp, q.sval, NS, q.rs);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_col_scale, dim3((unsigned)((NS + 255) / 256)), dim3(256), 0, st,
+                           c0->dp.d_csc_ptr, c0->dp.d_csc_idx, c0->dp.d_csc_row, q.sval, q.rs, NS,
+                           q.cs);
         LAUNCH_CHECK();
         sv_sweep_invalidate(c, precision);
     }
@@ -1221,13 +1361,13 @@ int sv_factorize_batch(hmec_ctx *const *cs, int n, int precision) {
         hmec_ctx *c = cs[k];
         SolverDev *s = sdev(c);
         if (fp64) {
-            FacPtrs<double> f{c->d.arena, s->dinv64, s->dscr, s->icopy, c->d.counters, c->d.val,
-                              c->d.rscale, c->d.cscale};
+            FacPtrs<double> f{c->d.arena, s->dinv64, s->dscr, s->icopy, c->d.counters, c->dc.sval,
+                              c->dc.rs, c->dc.cs};
             std::memcpy(host.data() + psz * k, &f, psz);
         } else {
             FacPtrs<float> f{c->d.arena32, s->dinv32, reinterpret_cast<float *>(s->dscr),
-                             reinterpret_cast<float *>(s->icopy), c->d.counters, c->d.val, c->d.rscale,
-                             c->d.cscale};
+                             reinterpret_cast<float *>(s->icopy), c->d.counters, c->dc.sval, c->dc.rs,
+                             c->dc.cs};
             std::memcpy(host.data() + psz * k, &f, psz);
         }
     }
@@ -1278,7 +1418,7 @@ static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
         const int bs = dp.maxn[lv] <= 128 ? 64 : 256;
         hipLaunchKernelGGL(k_fwd_gather, dim3(nfr), dim3(bs), 0, st, dp.d_tasks[lv], fr,
                            dp.d_front_dofs, dp.d_front_dof_off, s->voff, s->child_ptr, s->child_list,
-                           s->ea_off, dp.d_ea_map, s->all_tasks, z, wbuf);
+                           s->ea_off, dp.d_ea_map, s->all_tasks, z, wbuf, pl.dpn);
         LAUNCH_CHECK();
         hipLaunchKernelGGL((k_gemv_part<T, 2>), dim3(s->gemv_tri_grid[lv], nfr), dim3(GR), 0, st,
                            dp.d_tasks[lv], fr, dp.d_front_dofs, dp.d_front_dof_off, s->voff, arena, z,
@@ -1341,12 +1481,43 @@ static int launch_norms(hmec_ctx *c, const double *a, const double *b, double *o
     return HMEC_OK;
 }
 
-// z <- (LU)^-1 z with the current factor: the sweep form when it has been
-// formed for this factor, else the level-by-level inverted-triangle solves
-static int apply_factor(hmec_ctx *c, double *z) {
-    if (sv_sweep_ready(c, c->factor_prec)) return sv_sweep_solve(c, z);
-    if (c->factor_prec == HMEC_PREC_FP64) return tri_solve<double>(c, c->d.arena, z);
-    return tri_solve<float>(c, c->d.arena32, z);
+// vs <- S^-1 vs (equilibrated condensed vector) with the current factor: the
+// sweep form when it has been formed for this factor, else the level-by-level
+// inverted-triangle solves
+static int solve_condensed(hmec_ctx *c, double *vs) {
+    if (sv_sweep_ready(c, c->factor_prec)) return sv_sweep_solve(c, vs);
+    if (c->factor_prec == HMEC_PREC_FP64) return tri_solve<double>(c, c->d.arena, vs);
+    return tri_solve<float>(c, c->d.arena32, vs);
+}
+
+// E = -A_DD^-1 A_DR of the current matrix
+static int darcy_e(hmec_ctx *c, hipStream_t st) {
+    const CondPlan &cp = c->cp;
+    CondDev &q = c->dc;
+    hipLaunchKernelGGL(k_darcy_e, dim3((unsigned)((cp.ND + 255) / 256)), dim3(256), 0, st, c->d.val,
+                       cp.ND, q.ddiag, q.dnbdiag, q.dnbe, q.des, q.e);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// z = A^-1 (v / rs) for the full coupled system by block elimination of the
+// Darcy unknowns around the condensed factor (rs: optional row scaling of
+// the operator, exact powers of two); E must be the current matrix's
+static int precond_apply(hmec_ctx *c, const double *v, const double *rs, double *z) {
+    const CondPlan &cp = c->cp;
+    CondDev &q = c->dc;
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_darcy_gv, dim3((unsigned)((cp.ND + 255) / 256)), dim3(256), 0, st, v, rs,
+                       c->d.val, cp.ND, q.dglob, q.ddiag, q.dnb, q.dnbdiag, q.dnbe, q.gv);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_cond_rhs, dim3((unsigned)((cp.NS + 255) / 256)), dim3(256), 0, st, v, rs,
+                       c->d.val, cp.NS, q.full_of_s, q.rptr, q.raslot, q.rd, q.gv, q.rs, q.vs);
+    LAUNCH_CHECK();
+    HMEC_TRY(solve_condensed(c, q.vs));
+    hipLaunchKernelGGL(k_expand, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, st, q.vs, q.cs,
+                       q.gv, q.e, c->N, q.s_of_full, q.d_of_full, q.dec, z);
+    LAUNCH_CHECK();
+    return HMEC_OK;
 }
 
 int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
@@ -1358,15 +1529,13 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
     double *x = c->d.sol, *r = c->d.work, *z = c->d.work2;
     // r = b
     HMEC_HIP(hipMemcpyAsync(r, c->d.rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    HMEC_TRY(darcy_e(c, st));
     double res = 0, bnorm = 0, prev = INFINITY;
     int it = 0;
     int stalls = 0;
     for (;; it++) {
-        hipLaunchKernelGGL(k_scale_vec, dim3(g), dim3(256), 0, st, r, c->d.rscale, z, N);
-        LAUNCH_CHECK();
-        HMEC_TRY(apply_factor(c, z));
-        hipLaunchKernelGGL(k_axpy_scaled, dim3(g), dim3(256), 0, st, x, c->d.cscale, z, N,
-                           it == 0 ? 1 : 0);
+        HMEC_TRY(precond_apply(c, r, nullptr, z));
+        hipLaunchKernelGGL(k_add, dim3(g), dim3(256), 0, st, x, z, N, it == 0 ? 1 : 0);
         LAUNCH_CHECK();
         hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, x,
                            c->d.rhs, r, N);
@@ -1483,18 +1652,9 @@ static int gm_dots(hmec_ctx *c, GmresBuf &g, const double *V, int nv, const doub
     return HMEC_OK;
 }
 
-// z = Dc (LU)^-1 v: right preconditioner of the row-equilibrated operator
-// Dr A (the factor is of Dr A Dc)
+// z = A^-1 Dr^-1 v: right preconditioner of the row-equilibrated operator Dr A
 static int gm_precond(hmec_ctx *c, const double *v, double *zout) {
-    const int64_t N = c->N;
-    hipStream_t st = c->stream;
-    double *z = c->d.work2;
-    HMEC_HIP(hipMemcpyAsync(z, v, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-    HMEC_TRY(apply_factor(c, z));
-    const unsigned gN = (unsigned)((N + 255) / 256);
-    hipLaunchKernelGGL(k_axpy_scaled, dim3(gN), dim3(256), 0, st, zout, c->d.cscale, z, N, 1);
-    LAUNCH_CHECK();
-    return HMEC_OK;
+    return precond_apply(c, v, c->d.rscale, zout);
 }
 
 // Relative inf-norm residuals of x: unscaled ||b - A x|| / ||b|| (the
@@ -1543,6 +1703,10 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     int total_it = 0;
     double relinf = 0, last_rel_s = 0;
     std::vector<double> H((m + 1) * m), cs(m), sn(m), gv(m + 1), h(m + 2), h2(m + 2), y(m);
+    // the operator's row scaling (powers of two) and E of the current matrix
+    hipLaunchKernelGGL(k_row_scale, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.val, N, c->d.rscale);
+    LAUNCH_CHECK();
+    HMEC_TRY(darcy_e(c, st));
     hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, c->d.rhs, c->d.rscale, g.bs, N);
     LAUNCH_CHECK();
     for (;;) {

@@ -37,7 +37,6 @@
 #include "hmec_internal.h"
 #include "mfma_traits.h"
 
-int hmec_var_pos(int v);
 
 #define SW_LAUNCH_CHECK()                                                   \
     do {                                                                    \
@@ -351,7 +350,8 @@ struct SweepDev {
     void *sa[2] = {nullptr, nullptr};            // [1]: FP64 sweep arena (when formed)
     int built = -1;                              // precision of the current sweep form
     // FP32 factor cache: each slot keeps the sweep form and the equilibration
-    // scales of one FP32 factorization and the dt it was computed at
+    // scales of one FP32 factorization of the condensed system and the dt it
+    // was computed at
     struct Slot {
         void *sa = nullptr;
         double *rs = nullptr, *cs = nullptr;
@@ -438,10 +438,11 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
     const FrontPlan &pl = c->plan;
     const DevPlan &dp = c->dp;
     const int nf = pl.nfronts;
+    const int dpn = pl.dpn;
     s->h_fronts.resize(nf);
     int64_t soff = 0, goff = 0;
     for (int f = 0; f < nf; f++) {
-        const int n = 6 * (pl.np[f] + pl.nu[f]), p = 6 * pl.np[f];
+        const int n = dpn * (pl.np[f] + pl.nu[f]), p = dpn * pl.np[f];
         const int ldf = (p + 3) & ~3, ldb = (n + 3) & ~3;
         const int64_t fo = soff, bo = soff + (int64_t)n * ldf;
         s->h_fronts[f] = SwFront{pl.off[f], fo, bo, pl.voff[f], dp.h_front_dof_off[f], goff, n, p,
@@ -458,19 +459,19 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
     // that land on it (children in plan order; at most two in a binary
     // dissection tree)
     int inv[6];
-    for (int v = 0; v < 6; v++) inv[hmec_var_pos(v)] = v;
+    for (int v = 0; v < dpn; v++) inv[hmec_var_pos(dpn, v)] = v;
     std::vector<SwSrc> srcs(goff);
     for (int f = 0; f < nf; f++) {
         const SwFront &F = s->h_fronts[f];
         for (int r = 0; r < F.n; r++) {
-            const int node = pl.nodes[f][r / 6];
-            srcs[F.goff + r] = SwSrc{(int32_t)(node * 6 + inv[r % 6]), -1, -1, 0};
+            const int node = pl.nodes[f][r / dpn];
+            srcs[F.goff + r] = SwSrc{(int32_t)(node * dpn + inv[r % dpn]), -1, -1, 0};
         }
         for (int ch : pl.children[f]) {
-            const int cn = 6 * (pl.np[ch] + pl.nu[ch]), cp = 6 * pl.np[ch];
+            const int cn = dpn * (pl.np[ch] + pl.nu[ch]), cp = dpn * pl.np[ch];
             const int32_t *map = pl.ea_map.data() + pl.ea_off[ch];
             for (int a = 0; a < cn - cp; a++) {
-                SwSrc &e = srcs[F.goff + map[a / 6] * 6 + a % 6];
+                SwSrc &e = srcs[F.goff + map[a / dpn] * dpn + a % dpn];
                 const int32_t w = (int32_t)(pl.voff[ch] + cp + a);
                 if (e.w0 < 0) e.w0 = w;
                 else if (e.w1 < 0) e.w1 = w;
@@ -592,8 +593,8 @@ int sv_sweep_build_batch(hmec_ctx *const *cs, int n, int precision) {
         } else {
             SweepDev::Slot &sl = s->slots[s->target];
             HMEC_TRY(alloc_arena(&sl.sa));
-            if (!sl.rs) HMEC_HIP(hipMalloc(&sl.rs, sizeof(double) * cs[k]->N));
-            if (!sl.cs) HMEC_HIP(hipMalloc(&sl.cs, sizeof(double) * cs[k]->N));
+            if (!sl.rs) HMEC_HIP(hipMalloc(&sl.rs, sizeof(double) * cs[k]->cp.NS));
+            if (!sl.cs) HMEC_HIP(hipMalloc(&sl.cs, sizeof(double) * cs[k]->cp.NS));
             sl.valid = false;
         }
     }
@@ -609,9 +610,9 @@ int sv_sweep_build_batch(hmec_ctx *const *cs, int n, int precision) {
             // keep this factorization's scales with its slot (same stream as
             // the factorization that computed them)
             SweepDev::Slot &sl = s->slots[s->target];
-            const size_t nb = sizeof(double) * cs[k]->N;
-            HMEC_HIP(hipMemcpyAsync(sl.rs, cs[k]->d.rscale, nb, hipMemcpyDeviceToDevice, cs[0]->stream));
-            HMEC_HIP(hipMemcpyAsync(sl.cs, cs[k]->d.cscale, nb, hipMemcpyDeviceToDevice, cs[0]->stream));
+            const size_t nb = sizeof(double) * cs[k]->cp.NS;
+            HMEC_HIP(hipMemcpyAsync(sl.rs, cs[k]->dc.rs, nb, hipMemcpyDeviceToDevice, cs[0]->stream));
+            HMEC_HIP(hipMemcpyAsync(sl.cs, cs[k]->dc.cs, nb, hipMemcpyDeviceToDevice, cs[0]->stream));
             sl.valid = true;
             sl.dt = cs[k]->dt;
             sl.used = ++s->clock;
@@ -676,9 +677,9 @@ int sv_slot_activate(hmec_ctx *c, int k) {
     SweepDev *s = swdev(c);
     if (k < 0 || k >= (int)s->slots.size() || !s->slots[k].valid) return HMEC_E_ARG;
     SweepDev::Slot &sl = s->slots[k];
-    const size_t nb = sizeof(double) * c->N;
-    HMEC_HIP(hipMemcpyAsync(c->d.rscale, sl.rs, nb, hipMemcpyDeviceToDevice, c->stream));
-    HMEC_HIP(hipMemcpyAsync(c->d.cscale, sl.cs, nb, hipMemcpyDeviceToDevice, c->stream));
+    const size_t nb = sizeof(double) * c->cp.NS;
+    HMEC_HIP(hipMemcpyAsync(c->dc.rs, sl.rs, nb, hipMemcpyDeviceToDevice, c->stream));
+    HMEC_HIP(hipMemcpyAsync(c->dc.cs, sl.cs, nb, hipMemcpyDeviceToDevice, c->stream));
     s->cur = k;
     s->built = HMEC_PREC_FP32;
     sl.used = ++s->clock;

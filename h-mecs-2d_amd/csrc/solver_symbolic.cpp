@@ -7,9 +7,10 @@
 // one anti-diagonal pair, h_mec_rsf_v9.cpp:761-765, :833-837), so a full row
 // or column of nodes separates the grid.  Geometric nested dissection cuts the
 // node rectangle through its longer side recursively; each separator is one
-// dense front whose pivots are the separator's 6 unknowns per node, ordered
-// velocities first (vx, vy, vxD, vyD, Pt, Pf) so that every pressure pivot
-// comes after the velocities it couples to.  The front's update set is the
+// dense front whose pivots are the separator's unknowns per node (6 for the
+// coupled system, 4 after the Darcy condensation), ordered velocities first
+// (vx, vy[, vxD, vyD], Pt, Pf) so that every pressure pivot comes after the
+// velocities it couples to.  The front's update set is the
 // set of ancestor nodes adjacent to its region (exact for connected regions).
 #include <algorithm>
 #include <cstdio>
@@ -18,9 +19,11 @@
 #include "hmec_internal.h"
 
 namespace {
-// position of each reference variable (0 Pt, 1 vxs, 2 vys, 3 Pf, 4 vxD, 5 vyD)
-// inside a node block of a front
-const int kVarPos[6] = {4, 0, 1, 5, 2, 3};
+// position of each variable inside a node block of a front:
+// coupled system (0 Pt, 1 vxs, 2 vys, 3 Pf, 4 vxD, 5 vyD), condensed (0 Pt,
+// 1 vxs, 2 vys, 3 Pf)
+const int kVarPos6[6] = {4, 0, 1, 5, 2, 3};
+const int kVarPos4[4] = {2, 0, 1, 3};
 const int kLeafNodes = 4;
 
 struct Rect {
@@ -28,18 +31,20 @@ struct Rect {
 };
 }  // namespace
 
-int hmec_var_pos(int v) { return kVarPos[v]; }
+int hmec_var_pos(int dpn, int v) { return dpn == 4 ? kVarPos4[v] : kVarPos6[v]; }
 
 int hmec_plan_build(int Nx1, int Ny1, const std::vector<int32_t> &rowp,
-                    const std::vector<int32_t> &coli, FrontPlan &pl) {
+                    const std::vector<int32_t> &coli, FrontPlan &pl, int dpn) {
+    if (dpn != 4 && dpn != 6) return HMEC_E_ARG;
+    const int *kVarPos = dpn == 4 ? kVarPos4 : kVarPos6;
     const int nn = Nx1 * Ny1;
-    const int64_t N = (int64_t)nn * 6;
+    const int64_t N = (int64_t)nn * dpn;
     if ((int64_t)rowp.size() != N + 1) return HMEC_E_ARG;
     // symmetrized node graph
     std::vector<std::vector<int>> adj(nn);
     for (int64_t r = 0; r < N; r++)
         for (int p = rowp[r]; p < rowp[r + 1]; p++) {
-            const int a = (int)(r / 6), b = coli[p] / 6;
+            const int a = (int)(r / dpn), b = coli[p] / dpn;
             if (a == b) continue;
             adj[a].push_back(b);
             adj[b].push_back(a);
@@ -50,6 +55,7 @@ int hmec_plan_build(int Nx1, int Ny1, const std::vector<int32_t> &rowp,
     }
     // nested dissection
     pl = FrontPlan();
+    pl.dpn = dpn;
     std::vector<Rect> region;
     std::function<int(Rect)> build = [&](Rect R) -> int {
         const int h = R.i1 - R.i0, w = R.j1 - R.j0;
@@ -155,7 +161,7 @@ int hmec_plan_build(int Nx1, int Ny1, const std::vector<int32_t> &rowp,
     pl.factor_nnz = 0;
     for (int lv = 0; lv < pl.nlevels; lv++)
         for (int f : pl.levels[lv]) {
-            const int64_t n = 6LL * (pl.np[f] + pl.nu[f]), p = 6LL * pl.np[f], u = n - p;
+            const int64_t n = (int64_t)dpn * (pl.np[f] + pl.nu[f]), p = (int64_t)dpn * pl.np[f], u = n - p;
             pl.off[f] = a;
             a += n * n;
             pl.voff[f] = va;
@@ -176,7 +182,7 @@ int hmec_plan_build(int Nx1, int Ny1, const std::vector<int32_t> &rowp,
     std::vector<int64_t> bucket_cnt(nf + 1, 0);
     for (int64_t r = 0; r < N; r++)
         for (int p = rowp[r]; p < rowp[r + 1]; p++) {
-            const int nr = (int)(r / 6), nc = coli[p] / 6;
+            const int nr = (int)(r / dpn), nc = coli[p] / dpn;
             const int first = node_pos[nr] <= node_pos[nc] ? nr : nc;
             owner[p] = pl.node_front[first];
             bucket_cnt[owner[p] + 1]++;
@@ -194,16 +200,16 @@ int hmec_plan_build(int Nx1, int Ny1, const std::vector<int32_t> &rowp,
     for (int f = 0; f < nf; f++) {
         const auto &nl = pl.nodes[f];
         for (int l = 0; l < (int)nl.size(); l++) loc[nl[l]] = l;
-        const int64_t n = 6LL * (int64_t)nl.size();
+        const int64_t n = (int64_t)dpn * (int64_t)nl.size();
         for (int64_t q = bucket_cnt[f]; q < bucket_cnt[f + 1]; q++) {
             const int64_t p = bucket[q];
             const int r = row_of[p], cc = coli[p];
-            const int lr = loc[r / 6], lc = loc[cc / 6];
+            const int lr = loc[r / dpn], lc = loc[cc / dpn];
             if (lr < 0 || lc < 0) {
                 hmec_set_last_error("CSR entry outside its front");
                 return HMEC_E_ARG;
             }
-            const int64_t rr = 6LL * lr + kVarPos[r % 6], ccol = 6LL * lc + kVarPos[cc % 6];
+            const int64_t rr = (int64_t)dpn * lr + kVarPos[r % dpn], ccol = (int64_t)dpn * lc + kVarPos[cc % dpn];
             pl.nz_dst[p] = pl.off[f] + ccol * n + rr;
         }
         for (int l = 0; l < (int)nl.size(); l++) loc[nl[l]] = -1;

@@ -226,6 +226,8 @@ def _declare(L):
     L.hmec_step.argtypes = [vp, P(StepInfo)]
     L.hmec_batch_factorize.argtypes = [P(vp), C.c_int, C.c_int]
     L.hmec_batch_step.argtypes = [P(vp), C.c_int, P(StepInfo)]
+    L.hmec_condensed_pattern.argtypes = [P(Params), P(C.c_int64), P(C.c_int64), P(C.c_int32),
+                                         P(C.c_int32)]
     L.hmec_member_footprint.restype = C.c_int64
     L.hmec_member_footprint.argtypes = [P(Params), C.c_int]
     L.hmec_server_create.argtypes = [P(ServerConfig), P(vp)]

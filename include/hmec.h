@@ -292,11 +292,20 @@ int hmec_step(hmec_ctx *ctx, hmec_step_info *info);
  * The fixed CSR pattern of the coupled system (what setFromTriplets +
  * makeCompressed give, h_mec_rsf_v9.cpp:1046-1049) and the statistics of the
  * nested-dissection front plan used by hmec_analyze.  Call hmec_pattern with
- * NULL arrays first to get n and nnz.  elim_order (N ints or NULL) receives the
- * unknowns in elimination order. */
+ * NULL arrays first to get n and nnz.  elim_order (NS = 4 (Nx+1)(Ny+1) ints or
+ * NULL) receives the condensed unknowns in elimination order. */
 int hmec_pattern(const hmec_params *p, int64_t *n, int64_t *nnz, int32_t *row_ptr, int32_t *col_idx);
 int hmec_plan_info(const hmec_params *p, hmec_solve_stats *stats, int64_t *arena_entries,
                    int32_t *elim_order);
+/* The factored system is the Darcy-condensed one: vxD and vyD are eliminated
+ * exactly (every Darcy row holds its diagonal plus its node's velocity and two
+ * fluid pressures, or one other Darcy unknown on the boundary), leaving 4
+ * unknowns per node, index node*4 + v (v = 0 Pt, 1 vx, 2 vy, 3 Pf).  Its
+ * pattern (NULL arrays: sizes only); hmec_plan_info and hmec_plan_levels
+ * describe the plan of this system and elim_order holds n condensed
+ * unknowns. */
+int hmec_condensed_pattern(const hmec_params *p, int64_t *n, int64_t *nnz, int32_t *row_ptr,
+                           int32_t *col_idx);
 /* per level of the plan: fronts, largest front order, largest pivot count and
  * dense flops (arrays of *nlevels entries; NULL arrays return nlevels only) */
 int hmec_plan_levels(const hmec_params *p, int *nlevels, int *nfronts, int *max_front,
