@@ -86,6 +86,7 @@ static void free_ctx(hmec_ctx *c) {
     sv_free_device(c);
     DeviceState &d = c->d;
     hipFree(d.fields); hipFree(d.mk); hipFree(d.prop); hipFree(d.cell); hipFree(d.perm);
+    hipFree(d.sorted);
     hipFree(d.cell_start); hipFree(d.sort_tmp); hipFree(d.keys_tmp); hipFree(d.coords);
     hipFree(d.scal);
     if (d.h_scal) hipHostFree(d.h_scal);
@@ -227,6 +228,8 @@ extern "C" int hmec_set_markers(hmec_ctx *c, const double *const soa[HMEC_NMARKE
     DeviceState &d = c->d;
     if (M != c->M) {
         hipFree(d.mk); hipFree(d.prop); hipFree(d.cell); hipFree(d.perm); hipFree(d.cell_start);
+        hipFree(d.sorted);
+        d.sorted = nullptr;
         hipFree(d.keys_tmp); hipFree(d.sort_tmp);
         d.mk = d.prop = nullptr;
         d.cell = d.perm = d.cell_start = d.keys_tmp = nullptr;
@@ -238,6 +241,7 @@ extern "C" int hmec_set_markers(hmec_ctx *c, const double *const soa[HMEC_NMARKE
         HMEC_HIP(hipMalloc(&d.prop, sizeof(double) * MP_NPROP * M));
         HMEC_HIP(hipMalloc(&d.cell, sizeof(int32_t) * 4 * M));
         HMEC_HIP(hipMalloc(&d.perm, sizeof(int32_t) * 4 * M));
+        HMEC_HIP(hipMalloc(&d.sorted, sizeof(double) * 31 * M));
         HMEC_HIP(hipMalloc(&d.cell_start, sizeof(int32_t) * 4 * (ncell + 1)));
         HMEC_HIP(hipMalloc(&d.keys_tmp, sizeof(int32_t) * 2 * M));
         d.sort_tmp_bytes = hmec_sort_tmp_bytes(M, nbits);

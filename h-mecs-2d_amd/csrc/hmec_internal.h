@@ -191,6 +191,7 @@ struct DeviceState {
     int32_t *cell = nullptr;         // 4 x M anchor cells
     int32_t *perm = nullptr;         // 4 x M markers sorted by cell
     int32_t *cell_start = nullptr;   // 4 x (ncell + 1)
+    double *sorted = nullptr;        // 31 x M: per grid, its scatter inputs in cell order
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int32_t *keys_tmp = nullptr;     // 2 x M scratch for sorting

@@ -168,7 +168,7 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     }
     const int64_t ns = (int64_t)(P.Nx + 1) * (P.Ny + 1), N = 6 * ns;
     const int64_t M = hmec_marker_count(&P);
-    int64_t b = M * 8 * (HMEC_NMARKER_FIELDS + MP_NPROP) + M * 4 * (4 + 4 + 2) + 16 * ns;
+    int64_t b = M * 8 * (HMEC_NMARKER_FIELDS + MP_NPROP + 31) + M * 4 * (4 + 4 + 2) + 16 * ns;
     b += ns * 8 * HMEC_NFIELDS;
     const int64_t NS = 4 * ns;
     b += z.nnz * (4 + 8) + (N + 1) * 4 + z.ntrip * 4;                     // coupled CSR

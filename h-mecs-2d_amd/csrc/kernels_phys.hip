@@ -65,6 +65,40 @@ __device__ __forceinline__ double d_enforce(double v, double mn, double mx) {
     return v;
 }
 
+// ---------------------------------------------- cell-ordered marker streams
+// Per grid g, the marker values its scatter reads (x, y and the grid's
+// properties) copied once per marker set into the grid's cell order, so a
+// node reads each of its cells' markers as one contiguous run.  Layout: grid
+// blocks at SORTED_OFF[g] * M, (2 + nprop_g) streams of M doubles each.
+constexpr int SORTED_OFF[4] = {0, 12, 19, 25};
+constexpr int SORTED_STREAMS = 31;
+template <int G>
+__global__ void k_sort_copy(const double *__restrict__ mk, const double *__restrict__ prop,
+                            int64_t M, const int32_t *__restrict__ perm, double *__restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M) return;
+    const int m = perm[k];
+    double *o = out + (int64_t)SORTED_OFF[G] * M;
+    o[k] = mk[(int64_t)HMEC_M_XM * M + m];
+    o[M + k] = mk[(int64_t)HMEC_M_YM * M + m];
+#define PR(f) prop[(int64_t)(f) * M + m]
+    if (G == 0) {
+        const int pf[10] = {MP_RHOMM, MP_KKKMM, MP_TMARK, MP_INVG, MP_ETAMM0,
+                            MP_COHT, MP_FRIT, MP_COHC, MP_FRIC, MP_DILC};
+#pragma unroll
+        for (int f = 0; f < 10; f++) o[(int64_t)(2 + f) * M + k] = PR(pf[f]);
+    } else if (G == 1) {
+        const int pf[5] = {MP_ETAMM, MP_PORM, MP_INVG, MP_ETAMM0, MP_ETASMM0};
+#pragma unroll
+        for (int f = 0; f < 5; f++) o[(int64_t)(2 + f) * M + k] = PR(pf[f]);
+    } else {
+        const int pf[4] = {MP_RHOMM, MP_INVETAD, MP_RHOFM, MP_PORM};
+#pragma unroll
+        for (int f = 0; f < 4; f++) o[(int64_t)(2 + f) * M + k] = PR(pf[f]);
+    }
+#undef PR
+}
+
 // ------------------------------------------------------------ marker prep
 // Derived marker properties (h_mec_rsf_v9.cpp:389-423) and the anchor cell
 // of each of the four grids (:432-433, :468-469, :493-494, :515-516).
@@ -191,6 +225,19 @@ int k_marker_prep(hmec_ctx *c) {
                            reinterpret_cast<const int32_t *>(keys_out), M, ncell, c->d.cell_start + (int64_t)g * (ncell + 1));
         LAUNCH_CHECK();
     }
+    // cell-ordered streams of the scatter (one copy per marker set)
+    hipLaunchKernelGGL(k_sort_copy<0>, dim3(blocks), dim3(threads), 0, c->stream, c->d.mk, c->d.prop,
+                       M, c->d.perm, c->d.sorted);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sort_copy<1>, dim3(blocks), dim3(threads), 0, c->stream, c->d.mk, c->d.prop,
+                       M, c->d.perm + M, c->d.sorted);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sort_copy<2>, dim3(blocks), dim3(threads), 0, c->stream, c->d.mk, c->d.prop,
+                       M, c->d.perm + 2 * M, c->d.sorted);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sort_copy<3>, dim3(blocks), dim3(threads), 0, c->stream, c->d.mk, c->d.prop,
+                       M, c->d.perm + 3 * M, c->d.sorted);
+    LAUNCH_CHECK();
     c->markers_dirty = false;
     return HMEC_OK;
 }
@@ -204,28 +251,31 @@ size_t hmec_sort_tmp_bytes(int64_t M, int nbits) {
 }
 
 // ----------------------------------------------------------------- scatter
-// One thread per node of grid g.  The node gathers the markers of the (up to)
-// four anchor cells whose 2x2 block covers it, in fixed cell order and, inside
-// a cell, in marker order — deterministic, no atomics.  The sums then go
-// through the reference's normalization (h_mec_rsf_v9.cpp:531-583).
-template <int G>
-__global__ void k_scatter_grid(const double *__restrict__ mk, const double *__restrict__ prop,
-                               int64_t M, const int32_t *__restrict__ perm,
-                               const int32_t *__restrict__ start, const double *__restrict__ cx,
-                               const double *__restrict__ cyv, double *__restrict__ fields,
-                               int64_t ns, int Nx, int Ny, double dx, double dy) {
+// L lanes per node of grid g.  The node gathers the markers of the (up to)
+// four anchor cells whose 2x2 block covers it, in fixed cell order; inside a
+// cell, lane l takes the cell's markers l, l + L, ... of its cell-ordered run
+// (the L lanes read consecutive addresses), and a butterfly over the L lanes
+// sums the partials — deterministic, no atomics.  The sums then go through the
+// reference's normalization (h_mec_rsf_v9.cpp:531-583).
+constexpr int SCATTER_T = 256;
+template <int G, int L>
+__global__ void __launch_bounds__(SCATTER_T)
+k_scatter_grid(const double *__restrict__ sorted, int64_t M, const int32_t *__restrict__ start,
+               const double *__restrict__ cx, const double *__restrict__ cyv,
+               double *__restrict__ fields, int64_t ns, int Nx, int Ny, double dx, double dy) {
     const int rows = (G == 0) ? Ny : Ny + 1;
     const int cols = (G == 0) ? Nx : Nx + 1;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= rows * cols) return;
+    const int t = blockIdx.x * (SCATTER_T / L) + threadIdx.x / L;
+    const int l = threadIdx.x % L;
+    if (t >= rows * cols) return;   // whole lane groups leave together
     const int i = t % rows, j = t / rows;
     const int cy = Ny - 1, cxn = Nx - 1;
     constexpr int NF = (G == 0) ? 10 : (G == 1) ? 5 : 4;
     double acc[NF + 1];
 #pragma unroll
     for (int f = 0; f <= NF; f++) acc[f] = 0.;
-    const double *xm = mk + (int64_t)HMEC_M_XM * M;
-    const double *ym = mk + (int64_t)HMEC_M_YM * M;
+    const double *src = sorted + (int64_t)SORTED_OFF[G] * M;   // this grid's cell-ordered streams
+    const double *xm = src, *ym = src + M;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int di = q & 1, dj = q >> 1;          // corner of the node in the cell
@@ -233,39 +283,21 @@ __global__ void k_scatter_grid(const double *__restrict__ mk, const double *__re
         if (ci < 0 || cj < 0 || ci >= cy || cj >= cxn) continue;
         const int cid = ci + cj * cy;
         const double x0 = cx[cj], y0 = cyv[ci];
-        for (int k = start[cid]; k < start[cid + 1]; k++) {
-            const int m = perm[k];
-            const double dxm = (xm[m] - x0) / dx;
-            const double dym = (ym[m] - y0) / dy;
+        const int end = start[cid + 1];
+        for (int k = start[cid] + l; k < end; k += L) {
+            const double dxm = (xm[k] - x0) / dx;
+            const double dym = (ym[k] - y0) / dy;
             const double w = (di ? dym : (1 - dym)) * (dj ? dxm : (1 - dxm));
-#define PR(f) prop[(int64_t)(f) * M + m]
-            if (G == 0) {
-                acc[0] += PR(MP_RHOMM) * w;
-                acc[1] += PR(MP_KKKMM) * w;
-                acc[2] += PR(MP_TMARK) * w;
-                acc[3] += PR(MP_INVG) * w;
-                acc[4] += PR(MP_ETAMM0) * w;
-                acc[5] += PR(MP_COHT) * w;
-                acc[6] += PR(MP_FRIT) * w;
-                acc[7] += PR(MP_COHC) * w;
-                acc[8] += PR(MP_FRIC) * w;
-                acc[9] += PR(MP_DILC) * w;
-            } else if (G == 1) {
-                acc[0] += PR(MP_ETAMM) * w;
-                acc[1] += PR(MP_PORM) * w;
-                acc[2] += PR(MP_INVG) * w;
-                acc[3] += PR(MP_ETAMM0) * w;
-                acc[4] += PR(MP_ETASMM0) * w;
-            } else {
-                acc[0] += PR(MP_RHOMM) * w;
-                acc[1] += PR(MP_INVETAD) * w;
-                acc[2] += PR(MP_RHOFM) * w;
-                acc[3] += PR(MP_PORM) * w;
-            }
-#undef PR
+#pragma unroll
+            for (int f = 0; f < NF; f++) acc[f] += src[(int64_t)(2 + f) * M + k] * w;
             acc[NF] += w;
         }
     }
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int f = 0; f <= NF; f++) acc[f] += __shfl_xor(acc[f], o, L);
+    if (l != 0) return;
     const double wt = acc[NF];
     if (!(wt > 0)) return;
     const int64_t idx = t;   // i + j * rows
@@ -310,21 +342,22 @@ int k_scatter(hmec_ctx *c) {
     const int64_t M = c->M;
     const hmec_params &P = c->P;
     const int nbasic = c->Nx * c->Ny, nstag = c->Nx1 * c->Ny1;
-    hipLaunchKernelGGL(k_scatter_grid<0>, dim3((nbasic + 127) / 128), dim3(128), 0, c->stream,
-                       c->d.mk, c->d.prop, M, c->d.perm, c->d.cell_start, k.x, k.y, c->d.fields,
-                       c->ns, c->Nx, c->Ny, P.dx, P.dy);
+    constexpr int L = 16, NPB = SCATTER_T / L;
+    hipLaunchKernelGGL((k_scatter_grid<0, L>), dim3((nbasic + NPB - 1) / NPB), dim3(SCATTER_T), 0,
+                       c->stream, c->d.sorted, M, c->d.cell_start, k.x, k.y, c->d.fields, c->ns,
+                       c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scatter_grid<1>, dim3((nstag + 127) / 128), dim3(128), 0, c->stream,
-                       c->d.mk, c->d.prop, M, c->d.perm + M, c->d.cell_start + (ncell + 1), k.xp,
-                       k.yp, c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
+    hipLaunchKernelGGL((k_scatter_grid<1, L>), dim3((nstag + NPB - 1) / NPB), dim3(SCATTER_T), 0,
+                       c->stream, c->d.sorted, M, c->d.cell_start + (ncell + 1), k.xp, k.yp,
+                       c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scatter_grid<2>, dim3((nstag + 127) / 128), dim3(128), 0, c->stream,
-                       c->d.mk, c->d.prop, M, c->d.perm + 2 * M, c->d.cell_start + 2 * (ncell + 1),
-                       k.xvx, k.yvx, c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
+    hipLaunchKernelGGL((k_scatter_grid<2, L>), dim3((nstag + NPB - 1) / NPB), dim3(SCATTER_T), 0,
+                       c->stream, c->d.sorted, M, c->d.cell_start + 2 * (ncell + 1), k.xvx, k.yvx,
+                       c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scatter_grid<3>, dim3((nstag + 127) / 128), dim3(128), 0, c->stream,
-                       c->d.mk, c->d.prop, M, c->d.perm + 3 * M, c->d.cell_start + 3 * (ncell + 1),
-                       k.xvy, k.yvy, c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
+    hipLaunchKernelGGL((k_scatter_grid<3, L>), dim3((nstag + NPB - 1) / NPB), dim3(SCATTER_T), 0,
+                       c->stream, c->d.sorted, M, c->d.cell_start + 3 * (ncell + 1), k.xvy, k.yvy,
+                       c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
