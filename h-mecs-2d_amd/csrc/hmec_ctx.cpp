@@ -389,14 +389,14 @@ extern "C" int hmec_vep_pre(hmec_ctx *c, double dt) {
 
 extern "C" int hmec_assemble(hmec_ctx *c, double dt, double *ptscale_out) {
     if (!c) return HMEC_E_ARG;
-    double ps = 0;
-    HMEC_TRY(k_scale(c, dt, &ps));
-    c->ptscale = ps;
+    // ptscale stays on the device for the assembly and the update; a caller
+    // that asks for it waits for it
+    HMEC_TRY(k_scale(c, dt, ptscale_out));
+    if (ptscale_out) c->ptscale = *ptscale_out;
     tm_begin(c, c->t_assemble);
-    HMEC_TRY(k_assemble(c, dt, ps));
+    HMEC_TRY(k_assemble(c, dt));
     // CSR values + row pointers/columns + RHS written, ~26 nodal fields read
     tm_end(c, c->t_assemble, 8.0 * (c->nnz + c->N + 26.0 * c->ns) + 4.0 * (c->nnz + c->ntrip));
-    if (ptscale_out) *ptscale_out = ps;
     return HMEC_OK;
 }
 
@@ -466,9 +466,15 @@ extern "C" int hmec_update(hmec_ctx *c, double dt, double ptscale, hmec_iter_sca
     if (!c || !out) return HMEC_E_ARG;
     c->d.h_scal[0] = ptscale;
     HMEC_HIP(hipMemcpyAsync(c->d.scal, c->d.h_scal, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HMEC_TRY(k_reload(c, ptscale));
+    return hmec_update_dev(c, dt, out);
+}
+
+// the update with the ptscale the assembly left in the device scalars
+int hmec_update_dev(hmec_ctx *c, double dt, hmec_iter_scalars *out) {
+    HMEC_TRY(k_reload(c, 0.));
     HMEC_TRY(k_update(c, dt, out));
     c->Vmax = out->Vmax;
+    c->ptscale = c->d.h_scal[SC_PTSCALE_SLOT];
     return HMEC_OK;
 }
 

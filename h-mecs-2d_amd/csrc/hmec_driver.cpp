@@ -169,15 +169,14 @@ extern "C" int hmec_step(hmec_ctx *c, hmec_step_info *info) {
     for (c->iterstep = 0; c->iterstep < P.niterglobal; c->iterstep++) {
         const double a = now_ms();
         HMEC_TRY(hmec_vep_pre(c, c->dt));
-        double ptscale = 0;
-        HMEC_TRY(hmec_assemble(c, c->dt, &ptscale));
+        HMEC_TRY(hmec_assemble(c, c->dt, nullptr));
         sync();
         const double b = now_ms();
         HMEC_TRY(linear_solve(c, acc));
         sync();
         const double cc = now_ms();
         hmec_iter_scalars sc;
-        HMEC_TRY(hmec_update(c, c->dt, ptscale, &sc));
+        HMEC_TRY(hmec_update_dev(c, c->dt, &sc));
         c->dtlapusta = sc.dtlapusta;
         capped += sc.bisect_capped;
         const bool stop = controller(c, sc);

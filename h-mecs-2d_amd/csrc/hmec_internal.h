@@ -229,6 +229,7 @@ struct hmec_ctx {
     int timestep = 1, ynlast = 0, iterstep = 0, yndtdecrease = 1;
     double timesum = 0, dt = 0, dt00 = 0, dtx = 0, dty = 0, dtlapusta = 0, Vmax = 0, maxvxy = 0;
     double ptscale = 0;
+    int64_t asm_lds_entries = 0;     // largest CSR range of an assembly workgroup
     std::vector<double> DSYLSQ;
     bool markers_dirty = true;       // cell keys / sort need recomputing
     bool have_markers = false;
@@ -263,8 +264,10 @@ int k_scatter(hmec_ctx *c);
 int k_step_begin_fields(hmec_ctx *c);
 int k_vep_pre(hmec_ctx *c, double dt);
 int k_scale(hmec_ctx *c, double dt, double *ptscale);
-int k_assemble(hmec_ctx *c, double dt, double ptscale);
+int k_assemble(hmec_ctx *c, double dt);   // ptscale from the device scalars
 int k_reload(hmec_ctx *c, double ptscale);
+int hmec_update_dev(hmec_ctx *c, double dt, hmec_iter_scalars *out);
+constexpr int SC_PTSCALE_SLOT = 0;   // ptscale's index in the device scalars
 int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out);
 int k_iter_commit(hmec_ctx *c, int ynpl, int converged);
 int k_final_stress(hmec_ctx *c, double dt);

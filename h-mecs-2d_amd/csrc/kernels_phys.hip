@@ -24,6 +24,7 @@ enum {
     SC_BANDX, SC_BANDY, SC_DTSLIP, SC_CAPPED,
     SC_COUNT
 };
+static_assert(SC_PTSCALE == SC_PTSCALE_SLOT, "ptscale slot");
 static_assert(SC_COUNT <= 32, "scalar slots");
 
 // coordinates: x(Nx), y(Ny), xvx(Nx1), yvx(Ny1), xvy(Nx1), yvy(Ny1), xp(Nx1), yp(Ny1)
@@ -517,33 +518,51 @@ int k_vep_pre(hmec_ctx *c, double dt) {
 // ---------------------------------------------------------- scale (a8)
 // pfscale = ptscale = min(ETADX(1:Ny-1, 0:Nx-1)) * dx * (dt > 2e5 ? 1e17/dt^2 : 1)
 // (h_mec_rsf_v9.cpp:671-677); a min is order independent, so exact.
-__global__ void k_scale(const double *ETADX, int Nx, int Ny, double dx, double dt,
-                        double *scal) {
-    __shared__ double red[1024];
+// two stages: block minima, then one block over them
+constexpr int SCALE_BLOCKS = 256;
+__global__ void __launch_bounds__(256) k_scale_part(const double *ETADX, int Nx, int Ny,
+                                                    double *part) {
+    __shared__ double red[256];
     const int Ny1 = Ny + 1;
     double mn = INFINITY;
-    for (int t = threadIdx.x; t < (Ny - 1) * Nx; t += blockDim.x) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < (Ny - 1) * Nx;
+         t += gridDim.x * blockDim.x) {
         const int i = 1 + t % (Ny - 1), j = t / (Ny - 1);
         mn = fmin(mn, ETADX[i + (int64_t)j * Ny1]);
     }
     red[threadIdx.x] = mn;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void __launch_bounds__(SCALE_BLOCKS) k_scale_final(const double *part, double dx,
+                                                              double dt, double *scal) {
+    __shared__ double red[SCALE_BLOCKS];
+    red[threadIdx.x] = part[threadIdx.x];
+    __syncthreads();
+    for (int s = SCALE_BLOCKS / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double m = red[0], pf;
-        if (dt > 2e5) pf = m * dx * 1e17 / (dt * dt);
-        else pf = m * dx;
-        scal[SC_PTSCALE] = pf;
+        const double m = red[0];
+        scal[SC_PTSCALE] = dt > 2e5 ? m * dx * 1e17 / (dt * dt) : m * dx;
     }
 }
 
+// ptscale into the device scalars; *ptscale (when given) waits for it
 int k_scale(hmec_ctx *c, double dt, double *ptscale) {
-    hipLaunchKernelGGL(k_scale, dim3(1), dim3(1024), 0, c->stream, F(c, HMEC_F_ETADX), c->Nx,
-                       c->Ny, c->P.dx, dt, c->d.scal);
+    double *part = c->d.work;
+    hipLaunchKernelGGL(k_scale_part, dim3(SCALE_BLOCKS), dim3(256), 0, c->stream,
+                       F(c, HMEC_F_ETADX), c->Nx, c->Ny, part);
     LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scale_final, dim3(1), dim3(SCALE_BLOCKS), 0, c->stream,
+                       (const double *)part, c->P.dx, dt, c->d.scal);
+    LAUNCH_CHECK();
+    if (!ptscale) return HMEC_OK;
     HMEC_HIP(hipMemcpyAsync(c->d.h_scal + SC_PTSCALE, c->d.scal + SC_PTSCALE, sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
     HMEC_HIP(hipStreamSynchronize(c->stream));
@@ -676,35 +695,69 @@ int hmec_build_pattern(hmec_ctx *c) {
     return HMEC_OK;
 }
 
-struct ValueEmit {
-    double *val;
+// Value emitter into the workgroup's LDS copy of its CSR range (the six rows
+// of every node of the workgroup are consecutive in the CSR, and every triplet
+// of a node lands in the node's own rows)
+struct LdsEmit {
+    double *val;            // LDS: CSR entries [base, ...)
     const int32_t *slot;
-    int t;
-    double *R;
-    __device__ void add(int, int, double v) { val[slot[t++]] += v; }
-    __device__ void rhs(int r, double v) { R[r] = v; }
+    int t, base;
+    double *R;              // LDS: rows [rbase, rbase + 6 * nodes)
+    int rbase;
+    __device__ void add(int, int, double v) { val[slot[t++] - base] += v; }
+    __device__ void rhs(int r, double v) { R[r - rbase] = v; }
 };
 
-// One thread per staggered node writes its six rows in place.
-__global__ void k_assemble(AsmIn in, const int32_t *__restrict__ rowp,
-                           const int32_t *__restrict__ node_trip,
-                           const int32_t *__restrict__ slot, double *__restrict__ val,
-                           double *__restrict__ R, int Nx1, int Ny1) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= Nx1 * Ny1) return;
-    const int i = t % Ny1, j = t / Ny1;
-    const int kp = t * 6;
-    for (int p = rowp[kp]; p < rowp[kp + 6]; p++) val[p] = 0.;
-    for (int r = 0; r < 6; r++) R[kp + r] = 0.;
-    ValueEmit e{val, slot + node_trip[t], 0, R};
-    emit_node(i, j, in, e);
+// ASM_T nodes per workgroup: the triplets are summed in LDS in emission order
+// (duplicates in the reference's triplet order, as setFromTriplets sums them),
+// then the workgroup's contiguous CSR value range and its right-hand-side rows
+// are stored with coalesced writes.  ptscale comes from the device scalars.
+constexpr int ASM_T = 64;
+__global__ void __launch_bounds__(ASM_T)
+k_assemble(AsmIn in, const double *__restrict__ scal, const int32_t *__restrict__ rowp,
+           const int32_t *__restrict__ node_trip, const int32_t *__restrict__ slot,
+           double *__restrict__ val, double *__restrict__ R, int Nx1, int Ny1) {
+    extern __shared__ double lv[];
+    __shared__ double lr[6 * ASM_T];
+    const int tid = threadIdx.x;
+    const int n0 = blockIdx.x * ASM_T;
+    const int nn = min(ASM_T, Nx1 * Ny1 - n0);
+    const int base = rowp[6 * n0], ne = rowp[6 * (n0 + nn)] - base;
+    for (int p = tid; p < ne; p += ASM_T) lv[p] = 0.;
+    for (int r = tid; r < 6 * nn; r += ASM_T) lr[r] = 0.;
+    in.ptscale = scal[SC_PTSCALE];
+    __syncthreads();
+    if (tid < nn) {
+        const int t = n0 + tid;
+        LdsEmit e{lv, slot + node_trip[t], 0, base, lr, 6 * n0};
+        emit_node(t % Ny1, t / Ny1, in, e);
+    }
+    __syncthreads();
+    for (int p = tid; p < ne; p += ASM_T) val[base + p] = lv[p];
+    for (int r = tid; r < 6 * nn; r += ASM_T) R[6 * n0 + r] = lr[r];
 }
 
-int k_assemble(hmec_ctx *c, double dt, double ptscale) {
-    AsmIn in = asm_in(c, c->d.fields, c->ns, dt, ptscale);
+int k_assemble(hmec_ctx *c, double dt) {
+    AsmIn in = asm_in(c, c->d.fields, c->ns, dt, 0.);
     const int n = c->Nx1 * c->Ny1;
-    hipLaunchKernelGGL(k_assemble, dim3((n + 127) / 128), dim3(128), 0, c->stream, in, c->d.rowp,
-                       c->d.node_trip, c->d.trip_slot, c->d.val, c->d.rhs, c->Nx1, c->Ny1);
+    const int nblk = (n + ASM_T - 1) / ASM_T;
+    if (c->asm_lds_entries == 0) {
+        // largest CSR range of a workgroup (fixed with the pattern)
+        int64_t mx = 0;
+        for (int b = 0; b < nblk; b++) {
+            const int n0 = b * ASM_T, n1 = std::min(n, n0 + ASM_T);
+            mx = std::max<int64_t>(mx, c->h_rowp[6 * (int64_t)n1] - c->h_rowp[6 * (int64_t)n0]);
+        }
+        c->asm_lds_entries = mx;
+    }
+    const size_t lds = sizeof(double) * (size_t)c->asm_lds_entries;
+    if (lds > 60 * 1024) {
+        hmec_set_last_error("assembly: CSR range of a workgroup exceeds the LDS budget");
+        return HMEC_E_ARG;
+    }
+    hipLaunchKernelGGL(k_assemble, dim3(nblk), dim3(ASM_T), lds, c->stream, in, c->d.scal,
+                       c->d.rowp, c->d.node_trip, c->d.trip_slot, c->d.val, c->d.rhs, c->Nx1,
+                       c->Ny1);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
@@ -930,13 +983,31 @@ __global__ void __launch_bounds__(1024) k_rsf(double *f, int64_t ns, int Nx, int
         if (tid < s) red[tid] += red[tid + s];
         __syncthreads();
     }
+    if (tid == 0) scal[SC_CAPPED] = red[0];
+    // ynpl and ddd in node order (one thread, from LDS chunks so the ordered
+    // sum does not wait on a global load per node).  An uncounted node has
+    // dsy2 = +0, and adding +0 to the non-negative sum is exact.
+    int ynpl = 0;
+    double ddd = 0;
+    int *cnt = reinterpret_cast<int *>(red + 512);
+    if (active) {
+        for (int j0 = 0; j0 < Nx; j0 += 512) {
+            __syncthreads();
+            if (tid < 512 && j0 + tid < Nx) {
+                red[tid] = dsy2[j0 + tid];
+                cnt[tid] = flag[j0 + tid];
+            }
+            __syncthreads();
+            if (tid == 0) {
+                const int n = min(512, Nx - j0);
+                for (int q = 0; q < n; q++) {
+                    ddd += red[q];
+                    ynpl += cnt[q];
+                }
+            }
+        }
+    }
     if (tid == 0) {
-        scal[SC_CAPPED] = red[0];
-        int ynpl = 0;
-        double ddd = 0;
-        if (active)
-            for (int j = 0; j < Nx; j++)
-                if (flag[j]) { ddd += dsy2[j]; ynpl++; }
         scal[SC_YNPL] = ynpl;
         scal[SC_DDD] = ddd;
     }

@@ -663,13 +663,60 @@ __global__ void k_axpy_scaled(double *x, const double *cs, const double *z, int6
     x[t] = first ? d : x[t] + d;
 }
 // r = b - A x
-__global__ void k_residual(const int32_t *rowp, const int32_t *coli, const double *val,
-                           const double *x, const double *b, double *r, int64_t N) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N) return;
-    double s = b[t];
-    for (int p = rowp[t]; p < rowp[t + 1]; p++) s -= val[p] * x[coli[p]];
-    r[t] = s;
+// CSR products over a block of SPMV_ROWS rows per workgroup: the threads
+// form the products of the block's contiguous entry range (coalesced val /
+// coli loads, LDS chunks of SPMV_CHUNK entries), then each thread sums its
+// row's products in entry order.  MODE 0: y = rs * (A x); MODE 1: y = b - A x
+// and, when y2 is given, y2 = rs * y.
+constexpr int SPMV_ROWS = 256;
+constexpr int SPMV_CHUNK = 2048;
+template <int MODE>
+__global__ void __launch_bounds__(SPMV_ROWS)
+k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
+             const double *__restrict__ val, const double *__restrict__ x,
+             const double *__restrict__ rs, const double *__restrict__ b, double *__restrict__ y,
+             double *__restrict__ y2, int64_t N) {
+    __shared__ double prod[SPMV_CHUNK];
+    __shared__ int32_t rp[SPMV_ROWS + 1];
+    const int64_t r0 = (int64_t)blockIdx.x * SPMV_ROWS;
+    const int nr = (int)min((int64_t)SPMV_ROWS, N - r0);
+    const int t = threadIdx.x;
+    for (int k = t; k <= nr; k += SPMV_ROWS) rp[k] = rowp[r0 + k];
+    __syncthreads();
+    const int e0 = rp[0], e1 = rp[nr];
+    const int mb = t < nr ? rp[t] : 0, me = t < nr ? rp[t + 1] : 0;
+    double s = 0;
+    for (int c0 = e0; c0 < e1; c0 += SPMV_CHUNK) {
+        const int c1 = min(e1, c0 + SPMV_CHUNK);
+        for (int p = c0 + t; p < c1; p += SPMV_ROWS) prod[p - c0] = val[p] * x[coli[p]];
+        __syncthreads();
+        const int pe = min(me, c1);
+        for (int p = max(mb, c0); p < pe; p++) s += prod[p - c0];
+        __syncthreads();
+    }
+    if (t >= nr) return;
+    const int64_t r = r0 + t;
+    if (MODE == 0) {
+        y[r] = rs[r] * s;
+    } else {
+        const double v = b[r] - s;
+        y[r] = v;
+        if (y2) y2[r] = rs[r] * v;
+    }
+}
+// y = b - A x (y2 = rs * y when given)
+static void launch_residual(hmec_ctx *c, const double *x, double *y, double *y2, const double *rs) {
+    const int64_t N = c->N;
+    hipLaunchKernelGGL(k_csr_stream<1>, dim3((unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS)),
+                       dim3(SPMV_ROWS), 0, c->stream, c->d.rowp, c->d.coli, c->d.val, x, rs,
+                       c->d.rhs, y, y2, N);
+}
+// y = rs * (A x)
+static void launch_spmv_rs(hmec_ctx *c, const double *x, double *y) {
+    const int64_t N = c->N;
+    hipLaunchKernelGGL(k_csr_stream<0>, dim3((unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS)),
+                       dim3(SPMV_ROWS), 0, c->stream, c->d.rowp, c->d.coli, c->d.val, x,
+                       c->d.rscale, nullptr, y, nullptr, N);
 }
 // out[0] = max|a|, out[1] = max|b| in two stages (block maxima, then one
 // block over them); a NaN anywhere propagates to the result
@@ -820,14 +867,6 @@ __global__ void k_spmv(const int32_t *rowp, const int32_t *coli, const double *v
     y[t] = s;
 }
 // y = Dr (A x)   (row-equilibrated operator)
-__global__ void k_spmv_rs(const int32_t *rowp, const int32_t *coli, const double *val,
-                          const double *rs, const double *x, double *y, int64_t N) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N) return;
-    double s = 0;
-    for (int p = rowp[t]; p < rowp[t + 1]; p++) s += val[p] * x[coli[p]];
-    y[t] = rs[t] * s;
-}
 // partial[b * nv + i] = sum over block b's slice of V_i . w   (nv vectors)
 __global__ void __launch_bounds__(256) k_dots_part(const double *V, int64_t ldv, int nv,
                                                    const double *w, int64_t N, double *partial) {
@@ -1537,8 +1576,7 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
         HMEC_TRY(precond_apply(c, r, nullptr, z));
         hipLaunchKernelGGL(k_add, dim3(g), dim3(256), 0, st, x, z, N, it == 0 ? 1 : 0);
         LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, x,
-                           c->d.rhs, r, N);
+        launch_residual(c, x, r, nullptr, nullptr);
         LAUNCH_CHECK();
         HMEC_TRY(launch_norms(c, r, c->d.rhs, s->norms));
         HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
@@ -1666,10 +1704,7 @@ static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *r
     const int64_t N = c->N;
     const unsigned gN = (unsigned)((N + 255) / 256);
     hipStream_t st = c->stream;
-    hipLaunchKernelGGL(k_residual, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val, x,
-                       c->d.rhs, c->d.work, N);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, c->d.work, c->d.rscale, g.rsd, N);
+    launch_residual(c, x, c->d.work, g.rsd, c->d.rscale);
     LAUNCH_CHECK();
     HMEC_TRY(launch_norms(c, c->d.work, c->d.rhs, s->norms));
     HMEC_TRY(launch_norms(c, g.rsd, g.bs, s->norms + 2));
@@ -1728,8 +1763,7 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
         for (; j < m && total_it < max_it; j++) {
             double *Zj = g.Z + (int64_t)j * N;
             HMEC_TRY(gm_precond(c, g.V + (int64_t)j * N, Zj));
-            hipLaunchKernelGGL(k_spmv_rs, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.coli, c->d.val,
-                               c->d.rscale, Zj, g.w, N);
+            launch_spmv_rs(c, Zj, g.w);
             LAUNCH_CHECK();
             // classical Gram-Schmidt, twice, coefficients kept on the device;
             // one copy of the Hessenberg column per iteration
