@@ -503,6 +503,7 @@ extern "C" int hmec_step_end(hmec_ctx *c, double dt) {
 extern "C" int hmec_timing_enable(hmec_ctx *c, int on) {
     if (!c) return HMEC_E_ARG;
     c->timing = on != 0;
+    c->phase_sync = on >= 2;
     return HMEC_OK;
 }
 

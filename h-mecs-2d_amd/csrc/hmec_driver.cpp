@@ -155,7 +155,7 @@ extern "C" int hmec_step(hmec_ctx *c, hmec_step_info *info) {
     const hmec_params &P = c->P;
     hmec_step_info &acc = c->acc;
     std::memset(&acc, 0, sizeof(acc));
-    const bool sync_timing = c->timing;
+    const bool sync_timing = c->phase_sync;
     auto sync = [&]() {
         if (sync_timing) hipStreamSynchronize(c->stream);
     };

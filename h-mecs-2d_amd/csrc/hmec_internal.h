@@ -254,6 +254,7 @@ struct hmec_ctx {
     hmec_step_info acc;
     // timing
     bool timing = false;
+    bool phase_sync = false;         // wait for the device at step phase boundaries
     Timer t_scatter, t_assemble, t_factor, t_solve;
     std::map<std::string, Timer> kt;    // per kernel class (schur, fwd, bwd, ...)
 };

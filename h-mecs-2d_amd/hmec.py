@@ -443,8 +443,11 @@ class Context:
         check(lib().hmec_step(self.h, C.byref(s)), "step")
         return s
 
-    def timing(self, on=True):
-        check(lib().hmec_timing_enable(self.h, int(bool(on))), "timing_enable")
+    def timing(self, on=True, phase_sync=False):
+        """Kernel event timers; phase_sync also waits for the device at every
+        phase boundary of a step (per-phase wall-clock split in StepInfo)."""
+        check(lib().hmec_timing_enable(self.h, 2 if (on and phase_sync) else int(bool(on))),
+              "timing_enable")
 
     def timing_reset(self):
         check(lib().hmec_timing_reset(self.h), "timing_reset")

@@ -393,7 +393,11 @@ int hmec_server_get_field(hmec_server *srv, int id, int field, double *buf, int 
 /* ------------------------------------------------------------------ timing
  * Average device duration (ms, HIP events on the context's stream) of the
  * dominant kernels since the last reset; name = "scatter", "assemble",
- * "factor", "solve".  Returns the number of samples. */
+ * "factor", "solve", "schur", "sweep_form", "sweep_fwd", "sweep_bwd".
+ * Returns the number of samples.  hmec_timing_enable: on = 1 records the
+ * events only; on = 2 also waits for the device at every phase boundary of a
+ * step, so that hmec_step_info's wall-clock split is per phase (without it
+ * only ms_total is the step's wall-clock time). */
 int hmec_kernel_time(hmec_ctx *ctx, const char *name, double *avg_ms, double *bytes);
 int hmec_timing_enable(hmec_ctx *ctx, int on);
 int hmec_timing_reset(hmec_ctx *ctx);
