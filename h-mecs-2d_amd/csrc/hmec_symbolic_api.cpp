@@ -126,6 +126,7 @@ extern "C" int hmec_plan_levels(const hmec_params *p, int *nlevels, int *nfronts
 namespace {
 struct PlanSizes {
     int64_t arena = 0, varena = 0, nnz = 0, snnz = 0, ntrip = 0, sweep = 0, dinv = 0, nfronts = 0;
+    int64_t swrows = 0;   // front rows (+1 per front) of the sweep's row source maps
 };
 std::mutex g_ps_mu;
 std::map<std::pair<int, int>, PlanSizes> g_ps;
@@ -162,6 +163,7 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
             const int64_t n = pl.dpn * (pl.np[f] + pl.nu[f]), pv = pl.dpn * pl.np[f];
             z.sweep += n * ((pv + 3) & ~3) + pv * ((n + 3) & ~3);
             z.dinv += ((pv + 31) / 32) * 2 * 32 * 32;
+            z.swrows += n + 1;
         }
         std::lock_guard<std::mutex> lk(g_ps_mu);
         g_ps[{P.Nx, P.Ny}] = z;
@@ -174,6 +176,7 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     b += z.nnz * (4 + 8) + (N + 1) * 4 + z.ntrip * 4;                     // coupled CSR
     b += z.snnz * (4 + 8 + 8 + 4 + 3 * 4 + 4 + 4 * 4) + (NS + 1) * 16;  // condensed CSR, maps
     b += 6 * N * 8 + 4 * NS * 8 + 3 * N * 4 + z.varena * 8 + z.nfronts * 256;   // vectors, plans
+    b += z.swrows * (4 + 8);                                            // sweep row sources
     if (precision == HMEC_PREC_FP64) {
         b += z.arena * 8 + z.dinv * 8;
     } else {

@@ -75,14 +75,13 @@ struct SwTask {
     int64_t goff;    // first row source record of the front
     int n, p, ld;    // front order, pivots, padded row length of the block
     int r0, nr;      // row block
-    int pad;
+    int kids;        // the front has children (forward: contributions to gather)
 };
 
-// where a front row's input comes from: z[z] (the row's global dof) and the
-// contributions wbuf[w0], wbuf[w1] of at most two children (-1: none)
-struct SwSrc {
-    int32_t z, w0, w1, pad;
-};
+// where a front row's input comes from: z[zidx] (the row's global dof) and,
+// for fronts with children, the contributions wbuf[w.x], wbuf[w.y] of at most
+// two children (-1: none), in separate arrays so that rows without children
+// read 4 bytes
 
 struct SwTile {
     int f;
@@ -217,56 +216,16 @@ template <> struct Vec16<double> {
 // out[r] = Bm[r,:] x -> z at the front's pivot dofs.
 // Dynamic LDS: xs (K entries padded to a multiple of 4, zero padded) then,
 // forward only, the gathered b of the block's rows.
-template <typename T, int DIR, int L, int NT>
-__global__ void __launch_bounds__(NT) k_sweep(const SwTask *tasks, const T *sa, const SwSrc *srcs,
-                                              double *z, double *wbuf) {
-    extern __shared__ double xs[];
-    typedef Vec16<T> V;
-    const SwTask t = tasks[blockIdx.x];
-    const int tid = threadIdx.x;
-    const SwSrc *src = srcs + t.goff;
-    const int K = DIR == 0 ? t.p : t.n;
-    const int Kp = (K + 3) & ~3;
-    for (int k = tid; k < Kp; k += NT) {
-        double v = 0.0;
-        if (k < K) {
-            const SwSrc e = src[k];
-            if (DIR == 0) {
-                v = z[e.z];
-                if (e.w0 >= 0) v += wbuf[e.w0];
-                if (e.w1 >= 0) v += wbuf[e.w1];
-            } else {
-                v = k < t.p ? wbuf[t.voff + k] : z[e.z];
-            }
-        }
-        xs[k] = v;
-    }
-    double *bi = xs + Kp;      // forward: b of rows >= p of this block
-    if (DIR == 0) {
-        for (int q = tid; q < t.nr; q += NT) {
-            const int r = t.r0 + q;
-            double v = 0.0;
-            if (r >= t.p) {
-                const SwSrc e = src[r];
-                if (e.w0 >= 0) v += wbuf[e.w0];
-                if (e.w1 >= 0) v += wbuf[e.w1];
-            }
-            bi[q] = v;
-        }
-    }
-    __syncthreads();
-    const int lane = tid % L, grp = tid / L;
-    constexpr int NG = NT / L;
-    constexpr int RB = 4;                          // rows in flight per lane group
-    constexpr int STEP = V::W * L;
-    const T *base = sa + t.moff;
-    const int64_t ld = t.ld;
-    typedef typename V::type VT;
-    for (int q0 = grp; q0 < t.nr; q0 += NG * RB) {
-        int cb[RB], ce[RB];
-        const VT *row[RB];
-        double acc[RB];
-        int cmin = 1 << 30, cmax = 0;
+// Row descriptors of one pass of a lane group: RB rows q0, q0 + NG, ...
+template <typename T, int DIR, int RB, int NG>
+struct SwPass {
+    typedef typename Vec16<T>::type VT;
+    int cb[RB], ce[RB];
+    const VT *row[RB];
+    int cmin, cmax;
+    __device__ __forceinline__ void setup(const SwTask &t, const T *base, int q0) {
+        cmin = 1 << 30;
+        cmax = 0;
 #pragma unroll
         for (int u = 0; u < RB; u++) {
             const int q = q0 + u * NG;
@@ -283,30 +242,115 @@ __global__ void __launch_bounds__(NT) k_sweep(const SwTask *tasks, const T *sa, 
                 cb[u] = 0;
                 ce[u] = 0;
             }
-            row[u] = reinterpret_cast<const VT *>(base + (int64_t)(q < t.nr ? r : t.r0) * ld);
-            acc[u] = 0.0;
+            row[u] = reinterpret_cast<const VT *>(base + (int64_t)(q < t.nr ? r : t.r0) * t.ld);
             if (ce[u] > cb[u]) {
                 cmin = min(cmin, cb[u]);
                 cmax = max(cmax, ce[u]);
             }
         }
+    }
+    // the rows' vectors at column c (zero outside a row's own [cb, ce))
+    __device__ __forceinline__ void load(int c, VT *v) const {
+#pragma unroll
+        for (int u = 0; u < RB; u++)
+            v[u] = (c >= cb[u] && c < ce[u]) ? row[u][c / Vec16<T>::W] : VT{};
+    }
+};
+
+// DIR 0 (forward, leaves -> root): x = b1 gathered from z (scaled right-hand
+// side at the front's pivot dofs) and the children's contributions;
+// out[r] = b[r] + Fm[r,:] x -> per-front vector (y1 rows < p, contribution c
+// rows >= p).
+// DIR 1 (backward, root -> leaves): x = [y1; z(update dofs)],
+// out[r] = Bm[r,:] x -> z at the front's pivot dofs.
+// Dynamic LDS: xs (K entries padded to a multiple of 4, zero padded) then,
+// forward only, the gathered b of the block's rows.  The first pass's first
+// column chunk is loaded before the gather, so the matrix loads overlap the
+// task -> source record -> vector chain.
+template <typename T, int DIR, int L, int NT>
+__global__ void __launch_bounds__(NT) k_sweep(const SwTask *tasks, const T *sa,
+                                              const int32_t *zidx, const int2 *wmap, double *z,
+                                              double *wbuf) {
+    extern __shared__ double xs[];
+    typedef Vec16<T> V;
+    typedef typename V::type VT;
+    constexpr int NG = NT / L;
+    constexpr int RB = SW_RPG;                     // rows in flight per lane group
+    constexpr int STEP = V::W * L;
+    const SwTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int lane = tid % L, grp = tid / L;
+    const T *base = sa + t.moff;
+    SwPass<T, DIR, RB, NG> ps;
+    ps.setup(t, base, grp);
+    int c = (ps.cmin < ps.cmax ? ps.cmin : 0) + V::W * lane;
+    VT v0[RB], v1[RB];
+    ps.load(c, v0);
+    ps.load(c + STEP, v1);
+
+    const int32_t *zi = zidx + t.goff;
+    const int2 *wm = wmap + t.goff;
+    const int K = DIR == 0 ? t.p : t.n;
+    const int Kp = (K + 3) & ~3;
+    for (int k = tid; k < Kp; k += NT) {
+        double v = 0.0;
+        if (k < K) {
+            if (DIR == 0) {
+                v = z[zi[k]];
+                if (t.kids) {
+                    const int2 w = wm[k];
+                    if (w.x >= 0) v += wbuf[w.x];
+                    if (w.y >= 0) v += wbuf[w.y];
+                }
+            } else {
+                v = k < t.p ? wbuf[t.voff + k] : z[zi[k]];
+            }
+        }
+        xs[k] = v;
+    }
+    double *bi = xs + Kp;      // forward: b of rows >= p of this block
+    if (DIR == 0) {
+        for (int q = tid; q < t.nr; q += NT) {
+            const int r = t.r0 + q;
+            double v = 0.0;
+            if (r >= t.p && t.kids) {
+                const int2 w = wm[r];
+                if (w.x >= 0) v += wbuf[w.x];
+                if (w.y >= 0) v += wbuf[w.y];
+            }
+            bi[q] = v;
+        }
+    }
+    __syncthreads();
+    for (int q0 = grp; q0 < t.nr; q0 += NG * RB) {
+        if (q0 != grp) {
+            ps.setup(t, base, q0);
+            c = (ps.cmin < ps.cmax ? ps.cmin : 0) + V::W * lane;
+            ps.load(c, v0);
+            ps.load(c + STEP, v1);
+        }
+        double acc[RB];
+#pragma unroll
+        for (int u = 0; u < RB; u++) acc[u] = 0.0;
         // each lane walks the rows' common column range; a row only loads the
         // vectors of its own [cb, ce) range (cb, ce, STEP multiples of 4)
-        int c = (cmin < cmax ? cmin : 0) + V::W * lane;
-        for (; c + STEP < cmax; c += 2 * STEP) {
-            VT v0[RB], v1[RB];
-#pragma unroll
-            for (int u = 0; u < RB; u++) {
-                v0[u] = (c >= cb[u] && c < ce[u]) ? row[u][c / V::W] : VT{};
-                v1[u] = (c + STEP >= cb[u] && c + STEP < ce[u]) ? row[u][(c + STEP) / V::W] : VT{};
-            }
+        if (c + STEP < ps.cmax) {
 #pragma unroll
             for (int u = 0; u < RB; u++) acc[u] += V::dot(v0[u], xs + c) + V::dot(v1[u], xs + c + STEP);
-        }
-        if (c < cmax) {
-            VT v0[RB];
+            c += 2 * STEP;
+            for (; c + STEP < ps.cmax; c += 2 * STEP) {
+                ps.load(c, v0);
+                ps.load(c + STEP, v1);
 #pragma unroll
-            for (int u = 0; u < RB; u++) v0[u] = (c >= cb[u] && c < ce[u]) ? row[u][c / V::W] : VT{};
+                for (int u = 0; u < RB; u++)
+                    acc[u] += V::dot(v0[u], xs + c) + V::dot(v1[u], xs + c + STEP);
+            }
+            if (c < ps.cmax) {
+                ps.load(c, v0);
+#pragma unroll
+                for (int u = 0; u < RB; u++) acc[u] += V::dot(v0[u], xs + c);
+            }
+        } else if (c < ps.cmax) {
 #pragma unroll
             for (int u = 0; u < RB; u++) acc[u] += V::dot(v0[u], xs + c);
         }
@@ -325,7 +369,7 @@ __global__ void __launch_bounds__(NT) k_sweep(const SwTask *tasks, const T *sa, 
                     const double b = r < t.p ? xs[r] : bi[q];
                     wbuf[t.voff + r] = b + acc[u];
                 } else {
-                    z[src[r].z] = acc[u];
+                    z[zi[r]] = acc[u];
                 }
             }
         }
@@ -337,7 +381,8 @@ struct SweepDev {
     bool ready = false;
     SwFront *fronts = nullptr;
     std::vector<SwFront> h_fronts;
-    SwSrc *srcs = nullptr;
+    int32_t *zidx = nullptr;
+    int2 *wmap = nullptr;
     SwTile *tiles = nullptr;
     int ntiles = 0;
     std::vector<SwTask *> d_fwd, d_bwd;
@@ -390,8 +435,9 @@ int upload(V **dst, const std::vector<V> &v) {
 
 // row-block tasks of one level and direction; lanes per row from the
 // level's largest row length
-void level_tasks(const std::vector<SwFront> &fr, const std::vector<int> &fronts, int dir,
-                 int vw, std::vector<SwTask> &tasks, int &lanes, int &nt, int &lds, double &entries) {
+void level_tasks(const std::vector<SwFront> &fr, const std::vector<int> &fronts,
+                 const std::vector<char> &kids, int dir, int vw, std::vector<SwTask> &tasks,
+                 int &lanes, int &nt, int &lds, double &entries) {
     tasks.clear();
     entries = 0;
     int kmax = 0, mmax = 0;
@@ -420,7 +466,7 @@ void level_tasks(const std::vector<SwFront> &fr, const std::vector<int> &fronts,
             const int nr = std::min(rows_per_block, m - r0);
             const SwFront &F = fr[f];
             tasks.push_back(SwTask{dir == 0 ? F.soff : F.boff, F.voff, F.goff, F.n, F.p,
-                                   dir == 0 ? F.ldf : F.ldb, r0, nr, 0});
+                                   dir == 0 ? F.ldf : F.ldb, r0, nr, kids[f] ? 1 : 0});
             nrmax = std::max(nrmax, nr);
         }
         // stored nonzero entries read: Fm = strict lower p x p + u x p,
@@ -460,21 +506,22 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
     // dissection tree)
     int inv[6];
     for (int v = 0; v < dpn; v++) inv[hmec_var_pos(dpn, v)] = v;
-    std::vector<SwSrc> srcs(goff);
+    std::vector<int32_t> zidx(goff);
+    std::vector<int2> wmap(goff, make_int2(-1, -1));
     for (int f = 0; f < nf; f++) {
         const SwFront &F = s->h_fronts[f];
         for (int r = 0; r < F.n; r++) {
             const int node = pl.nodes[f][r / dpn];
-            srcs[F.goff + r] = SwSrc{(int32_t)(node * dpn + inv[r % dpn]), -1, -1, 0};
+            zidx[F.goff + r] = (int32_t)(node * dpn + inv[r % dpn]);
         }
         for (int ch : pl.children[f]) {
             const int cn = dpn * (pl.np[ch] + pl.nu[ch]), cp = dpn * pl.np[ch];
             const int32_t *map = pl.ea_map.data() + pl.ea_off[ch];
             for (int a = 0; a < cn - cp; a++) {
-                SwSrc &e = srcs[F.goff + map[a / dpn] * dpn + a % dpn];
+                int2 &e = wmap[F.goff + map[a / dpn] * dpn + a % dpn];
                 const int32_t w = (int32_t)(pl.voff[ch] + cp + a);
-                if (e.w0 < 0) e.w0 = w;
-                else if (e.w1 < 0) e.w1 = w;
+                if (e.x < 0) e.x = w;
+                else if (e.y < 0) e.y = w;
                 else {
                     hmec_set_last_error("sweep: a front row receives from more than two children");
                     return HMEC_E_ARG;
@@ -499,7 +546,8 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
     s->form_flops = flops;
     s->ntiles = (int)tiles.size();
     HMEC_TRY(upload(&s->fronts, s->h_fronts));
-    HMEC_TRY(upload(&s->srcs, srcs));
+    HMEC_TRY(upload(&s->zidx, zidx));
+    HMEC_TRY(upload(&s->wmap, wmap));
     HMEC_TRY(upload(&s->tiles, tiles));
     // tasks per level (the LDS need is the same for both precisions)
     s->d_fwd.assign(pl.nlevels, nullptr);
@@ -515,11 +563,13 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
     s->bytes_fwd.assign(pl.nlevels, 0);
     s->bytes_bwd.assign(pl.nlevels, 0);
     std::vector<SwTask> tk;
+    std::vector<char> kids(nf, 0);
+    for (int f = 0; f < nf; f++) kids[f] = pl.children[f].empty() ? 0 : 1;
     for (int lv = 0; lv < pl.nlevels; lv++) {
         for (int dir = 0; dir < 2; dir++) {
             int lanes, nt, lds;
             double by;
-            level_tasks(s->h_fronts, pl.levels[lv], dir, 4, tk, lanes, nt, lds, by);
+            level_tasks(s->h_fronts, pl.levels[lv], kids, dir, 4, tk, lanes, nt, lds, by);
             if (lds > 64 * 1024) {
                 hmec_set_last_error("sweep: front too large for the LDS input vector");
                 return HMEC_E_ARG;
@@ -713,7 +763,8 @@ static int launch_sweep(hmec_ctx *c, SweepDev *s, int lv, const T *sa, double *z
     hipStream_t st = c->stream;
     double *wbuf = c->d.vec;
 #define SW_GO(LN, NTH)                                                                          \
-    hipLaunchKernelGGL((k_sweep<T, DIR, LN, NTH>), dim3(n), dim3(NTH), lds, st, tk, sa, s->srcs, z, \
+    hipLaunchKernelGGL((k_sweep<T, DIR, LN, NTH>), dim3(n), dim3(NTH), lds, st, tk, sa, s->zidx, \
+                       s->wmap, z, \
                        wbuf)
     if (nth == 64) {
         switch (lanes) {
@@ -747,16 +798,18 @@ int sv_sweep_solve(hmec_ctx *c, double *z) {
         bf += s->bytes_fwd[lv] * es;
         bb += s->bytes_bwd[lv] * es;
     }
+    const double *sa64 = (const double *)s->sa[1];
+    const float *sa32 = q ? nullptr : (const float *)s->slots[s->cur].sa;
     kt_begin(c, "sweep_fwd");
     for (int lv = 0; lv < pl.nlevels; lv++) {
-        if (q) HMEC_TRY((launch_sweep<double, 0>(c, s, lv, (const double *)s->sa[1], z)));
-        else HMEC_TRY((launch_sweep<float, 0>(c, s, lv, (const float *)s->slots[s->cur].sa, z)));
+        if (q) HMEC_TRY((launch_sweep<double, 0>(c, s, lv, sa64, z)));
+        else HMEC_TRY((launch_sweep<float, 0>(c, s, lv, sa32, z)));
     }
     kt_end(c, "sweep_fwd", bf, pl.nlevels);
     kt_begin(c, "sweep_bwd");
     for (int lv = pl.nlevels - 1; lv >= 0; lv--) {
-        if (q) HMEC_TRY((launch_sweep<double, 1>(c, s, lv, (const double *)s->sa[1], z)));
-        else HMEC_TRY((launch_sweep<float, 1>(c, s, lv, (const float *)s->slots[s->cur].sa, z)));
+        if (q) HMEC_TRY((launch_sweep<double, 1>(c, s, lv, sa64, z)));
+        else HMEC_TRY((launch_sweep<float, 1>(c, s, lv, sa32, z)));
     }
     kt_end(c, "sweep_bwd", bb, pl.nlevels);
     return HMEC_OK;
@@ -772,7 +825,8 @@ void sv_sweep_free(hmec_ctx *c) {
         g_sw.erase(it);
     }
     hipFree(s->fronts);
-    hipFree(s->srcs);
+    hipFree(s->zidx);
+    hipFree(s->wmap);
     hipFree(s->tiles);
     for (auto *p : s->d_fwd) hipFree(p);
     for (auto *p : s->d_bwd) hipFree(p);
@@ -791,3 +845,4 @@ int64_t sv_sweep_bytes(hmec_ctx *c, int precision) {
     SweepDev *s = swdev(c);
     return s->sa_entries * (precision == HMEC_PREC_FP64 ? 8 : 4);
 }
+
