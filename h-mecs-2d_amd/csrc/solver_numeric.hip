@@ -138,35 +138,91 @@ __device__ __forceinline__ float bcast(float v, int k) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-// Inverse of the upper (U, with diagonal) or unit-lower (L) triangle of the
-// factored NB x NB block D, one column per thread of the first NB threads.
+// Value of v in lane l of the wavefront (l uniform)
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// One wavefront: in-place LU (no row exchanges, static-pivot perturbation
+// below tau on the first kb pivots) of the NB x NB block D in LDS
+// (identity-padded beyond kb, which the steps leave as it is).  Lane r keeps
+// row r in registers; step k broadcasts the pivot row from lane k, and every
+// lane below it forms its multiplier and updates its row.
 template <typename T>
-__device__ __forceinline__ void invert_tri(T (*D)[NB + 1], T (*Inv)[NB + 1], bool upper) {
-    if (threadIdx.x >= NB) return;
-    const int c = threadIdx.x;
-    if (upper) {
+__device__ __forceinline__ void factor_diag_wave(T (*D)[NB + 1], int kb, double tau, int *npert,
+                                                 bool count) {
+    const int lane = threadIdx.x & 63, r = lane & (NB - 1);
+    T a[NB];
+#pragma unroll
+    for (int c = 0; c < NB; c++) a[c] = D[r][c];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        if (r == k && k < kb && !(fabs((double)a[k]) >= tau)) {
+            a[k] = (a[k] < (T)0) ? (T)(-tau) : (T)tau;
+            if (count && lane == k) atomicAdd(npert, 1);
+        }
+        const T d = lane_value(a[k], k);
+        const T l = a[k] / d;
+#pragma unroll
+        for (int c = k + 1; c < NB; c++) {
+            const T u = lane_value(a[c], k);
+            if (r > k) a[c] = a[c] - l * u;
+        }
+        if (r > k) a[k] = l;
+    }
+    if (lane < NB) {
+#pragma unroll
+        for (int c = 0; c < NB; c++) D[r][c] = a[c];
+    }
+}
+
+// One wavefront: inverse of the upper (U, with diagonal) or unit-lower (L)
+// triangle of the factored block D.  Lane c keeps row c of D (broadcast by
+// row) and column c of the inverse in registers.
+template <typename T, bool UPPER>
+__device__ __forceinline__ void invert_tri_wave(T (*D)[NB + 1], T (*Inv)[NB + 1]) {
+    const int lane = threadIdx.x & 63, c = lane & (NB - 1);
+    T a[NB], x[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) a[q] = D[c][q];
+    if (UPPER) {
+#pragma unroll
         for (int r = NB - 1; r >= 0; r--) {
             T s = (r == c) ? (T)1 : (T)0;
-            for (int q = r + 1; q <= c; q++) s = s - D[r][q] * Inv[q][c];
-            Inv[r][c] = (r <= c) ? s / D[r][r] : (T)0;
+#pragma unroll
+            for (int q = r + 1; q < NB; q++) s = s - lane_value(a[q], r) * x[q];
+            x[r] = (r <= c) ? s / lane_value(a[r], r) : (T)0;
         }
     } else {
+#pragma unroll
         for (int r = 0; r < NB; r++) {
             T s = (r == c) ? (T)1 : (T)0;
-            for (int q = c; q < r; q++) s = s - D[r][q] * Inv[q][c];
-            Inv[r][c] = (r >= c) ? s : (T)0;
+#pragma unroll
+            for (int q = 0; q < r; q++) s = s - lane_value(a[q], r) * x[q];
+            x[r] = (r >= c) ? s : (T)0;
         }
+    }
+    if (lane < NB) {
+#pragma unroll
+        for (int r = 0; r < NB; r++) Inv[r][c] = x[r];
     }
 }
 
 // Panel k0 of every front of a level (no row exchanges):
-//   every workgroup loads the kb x kb diagonal block D and factors it in LDS
-//   (static-pivot perturbation), then inverts its triangles; workgroup 0
-//   stores the factored D in dscr (copied into the front by the following
-//   update launch, so no workgroup of this launch reads a half-written D);
-//   row workgroups form X = A U_D^-1 for rows [k0+kb, n), column workgroups
-//   Y = L_D^-1 A for columns [k0+kb, n) (products with the inverted
-//   triangles: no serial chain per row).
+//   every workgroup loads the kb x kb diagonal block D and factors it with
+//   one wavefront, then inverts both triangles (one wavefront each);
+//   workgroup 0 stores the factored D in dscr (copied into the front by the
+//   following update launch, so no workgroup of this launch reads a
+//   half-written D) and the inverses in dinv; row workgroups form
+//   X = A U_D^-1 for rows [k0+kb, n), column workgroups Y = L_D^-1 A for
+//   columns [k0+kb, n) (products with the inverted triangles: no serial chain
+//   per row).
 template <typename T>
 __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, const FacPtrs<T> *mem,
                                               double tau) {
@@ -175,7 +231,8 @@ __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, co
     T *const dinv = mem[blockIdx.z].dinv;
     int *const npert = mem[blockIdx.z].npert;
     __shared__ T D[NB][NB + 1];
-    __shared__ T Inv[NB][NB + 1];     // U_D^-1 (upper) for row blocks, L_D^-1 (unit lower) for columns
+    __shared__ T Iu[NB][NB + 1];      // U_D^-1 (upper)
+    __shared__ T Il[NB][NB + 1];      // L_D^-1 (unit lower)
     const FrontTask tk = tasks[blockIdx.y];
     if (tk.p <= k0) return;
     const int kb = min(NB, tk.p - k0);
@@ -187,104 +244,61 @@ __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, co
     const bool solo = nch <= 1;
     if ((int)blockIdx.x >= (solo ? 1 : 1 + 2 * nch)) return;
     T *F = arena + tk.off;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6;
     for (int e = tid; e < NB * NB; e += blockDim.x) {
         const int r = e % NB, c = e / NB;
         D[r][c] = (r < kb && c < kb) ? F[(int64_t)(k0 + c) * n + k0 + r] : (r == c ? (T)1 : (T)0);
     }
     __syncthreads();
-    for (int k = 0; k < kb; k++) {
-        if (tid == 0) {
-            T d = D[k][k];
-            if (!(fabs((double)d) >= tau)) {
-                D[k][k] = (d < (T)0) ? (T)(-tau) : (T)tau;
-                if (blockIdx.x == 0) atomicAdd(npert, 1);
-            }
-        }
-        __syncthreads();
-        const T piv = D[k][k];
-        if (tid > k && tid < kb) D[tid][k] = D[tid][k] / piv;
-        __syncthreads();
-        const int m = kb - k - 1;
-        for (int e = tid; e < m * m; e += blockDim.x) {
-            const int r = k + 1 + e % m, c = k + 1 + e / m;
-            D[r][c] = D[r][c] - D[r][k] * D[k][c];
-        }
-        __syncthreads();
-    }
+    if (wave == 0) factor_diag_wave<T>(D, kb, tau, npert, blockIdx.x == 0);
+    __syncthreads();
+    if (wave == 0) invert_tri_wave<T, true>(D, Iu);
+    else if (wave == 1) invert_tri_wave<T, false>(D, Il);
+    __syncthreads();
     if (blockIdx.x == 0) {
         // factored D for the update launch, and both inverted triangles of
         // this diagonal block for the solves (block k0/NB of the front:
         // [0] = L^-1 unit lower, [1] = U^-1 upper)
         T *ds = dscr + (int64_t)blockIdx.y * NB * NB;
-        for (int e = tid; e < NB * NB; e += blockDim.x) ds[e] = D[e % NB][e / NB];
         T *dst = dinv + tk.doff + (int64_t)(k0 / NB) * 2 * NB * NB;
-        invert_tri<T>(D, Inv, true);
-        __syncthreads();
-        for (int e = tid; e < NB * NB; e += blockDim.x) dst[NB * NB + e] = Inv[e % NB][e / NB];
-        __syncthreads();
-        invert_tri<T>(D, Inv, false);
-        __syncthreads();
-        for (int e = tid; e < NB * NB; e += blockDim.x) dst[e] = Inv[e % NB][e / NB];
+        for (int e = tid; e < NB * NB; e += blockDim.x) {
+            const int r = e % NB, c = e / NB;
+            ds[e] = D[r][c];
+            dst[e] = Il[r][c];
+            dst[NB * NB + e] = Iu[r][c];
+        }
         if (!solo) return;
-        // solo: columns with L^-1 (in Inv now), then rows with U^-1
-        __syncthreads();
-        const int col = k0 + kb + tid;
+    }
+    const bool cols = solo || (int)blockIdx.x > nch;
+    const bool rows = solo || !cols;
+    T a[NB];
+    if (cols) {
+        // Y = L_D^-1 A on columns [k0+kb, n)
+        const int col = k0 + kb + (solo ? 0 : (blockIdx.x - 1 - nch) * PR) + tid;
         if (col < tk.n) {
-            T a[NB];
             T *Fc = F + (int64_t)col * n + k0;
 #pragma unroll
             for (int q = 0; q < NB; q++) a[q] = (q < kb) ? Fc[q] : (T)0;
             for (int r = 0; r < kb; r++) {
                 T s = (T)0;
 #pragma unroll
-                for (int q = 0; q < NB; q++) s = fma(Inv[r][q], a[q], s);
+                for (int q = 0; q < NB; q++) s = fma(Il[r][q], a[q], s);
                 Fc[r] = s;
             }
         }
-        __syncthreads();
-        invert_tri<T>(D, Inv, true);
-        __syncthreads();
-        const int r = k0 + kb + tid;
+    }
+    if (rows) {
+        // X = A U_D^-1 on rows [k0+kb, n)
+        const int r = k0 + kb + (solo ? 0 : (blockIdx.x - 1) * PR) + tid;
         if (r < tk.n) {
-            T a[NB];
 #pragma unroll
             for (int q = 0; q < NB; q++) a[q] = (q < kb) ? F[(int64_t)(k0 + q) * n + r] : (T)0;
             for (int c = 0; c < kb; c++) {
                 T s = (T)0;
 #pragma unroll
-                for (int q = 0; q < NB; q++) s = fma(a[q], Inv[q][c], s);
+                for (int q = 0; q < NB; q++) s = fma(a[q], Iu[q][c], s);
                 F[(int64_t)(k0 + c) * n + r] = s;
             }
-        }
-        return;
-    }
-    const bool rows = (int)blockIdx.x <= nch;
-    invert_tri<T>(D, Inv, rows);
-    __syncthreads();
-    T a[NB];
-    if (rows) {
-        const int r = k0 + kb + (blockIdx.x - 1) * PR + tid;
-        if (r >= tk.n) return;
-#pragma unroll
-        for (int q = 0; q < NB; q++) a[q] = (q < kb) ? F[(int64_t)(k0 + q) * n + r] : (T)0;
-        for (int c = 0; c < kb; c++) {
-            T s = (T)0;
-#pragma unroll
-            for (int q = 0; q < NB; q++) s = fma(a[q], Inv[q][c], s);
-            F[(int64_t)(k0 + c) * n + r] = s;
-        }
-    } else {
-        const int col = k0 + kb + (blockIdx.x - 1 - nch) * PR + tid;
-        if (col >= tk.n) return;
-        T *Fc = F + (int64_t)col * n + k0;
-#pragma unroll
-        for (int q = 0; q < NB; q++) a[q] = (q < kb) ? Fc[q] : (T)0;
-        for (int r = 0; r < kb; r++) {
-            T s = (T)0;
-#pragma unroll
-            for (int q = 0; q < NB; q++) s = fma(Inv[r][q], a[q], s);
-            Fc[r] = s;
         }
     }
 }
