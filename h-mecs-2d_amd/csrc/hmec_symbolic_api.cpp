@@ -127,6 +127,8 @@ namespace {
 struct PlanSizes {
     int64_t arena = 0, varena = 0, nnz = 0, snnz = 0, ntrip = 0, sweep = 0, dinv = 0, nfronts = 0;
     int64_t swrows = 0;   // front rows (+1 per front) of the sweep's row source maps
+    int64_t icopy = 0;    // pivot-block copies of the largest level (triangle inversion)
+    int64_t iscr = 0;     // inversion products of the largest level and doubling stage
 };
 std::mutex g_ps_mu;
 std::map<std::pair<int, int>, PlanSizes> g_ps;
@@ -165,6 +167,24 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
             z.dinv += ((pv + 31) / 32) * 2 * 32 * 32;
             z.swrows += n + 1;
         }
+        for (int lv = 0; lv < pl.nlevels; lv++) {
+            int64_t co = 0;
+            int maxp = 0;
+            for (int f : pl.levels[lv]) {
+                const int64_t pv = pl.dpn * pl.np[f];
+                co += pv * pv;
+                maxp = std::max<int>(maxp, (int)pv);
+            }
+            z.icopy = std::max(z.icopy, co);
+            for (int b = 32; b < maxp; b *= 2) {
+                int64_t to = 0;
+                for (int f : pl.levels[lv]) {
+                    const int pf = pl.dpn * pl.np[f];
+                    for (int a0 = 0; a0 + b < pf; a0 += 2 * b) to += 2LL * b * std::min(b, pf - a0 - b);
+                }
+                z.iscr = std::max(z.iscr, to);
+            }
+        }
         std::lock_guard<std::mutex> lk(g_ps_mu);
         g_ps[{P.Nx, P.Ny}] = z;
     }
@@ -177,6 +197,7 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     b += z.snnz * (4 + 8 + 8 + 4 + 3 * 4 + 4 + 4 * 4) + (NS + 1) * 16;  // condensed CSR, maps
     b += 6 * N * 8 + 4 * NS * 8 + 3 * N * 4 + z.varena * 8 + z.nfronts * 256;   // vectors, plans
     b += z.swrows * (4 + 8);                                            // sweep row sources
+    b += (z.icopy + z.iscr) * 8;                                        // inversion scratch
     if (precision == HMEC_PREC_FP64) {
         b += z.arena * 8 + z.dinv * 8;
     } else {

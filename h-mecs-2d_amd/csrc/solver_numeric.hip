@@ -52,6 +52,7 @@ struct FacPtrs {
     T *dinv;               // inverted NB x NB diagonal blocks
     T *dscr;               // factored diagonal blocks of the current panel
     T *icopy;              // pivot-block copies of the current level (inversion)
+    T *iscr;               // inversion products of the current level and stage
     int *npert;            // perturbed-pivot counter
     const double *val;     // CSR values
     const double *rs, *cs; // equilibration scales
@@ -439,109 +440,141 @@ __global__ void k_pivot_copy(const FrontTask *tasks, const FacPtrs<T> *mem) {
     }
 }
 
+// ------------------------------------------------------------ triangle inversion
+// The inverses of the factored pivot block's triangles, in place (strict
+// lower part: L11^-1 without its unit diagonal; upper part with the
+// diagonal: U11^-1), by recursive doubling over the NB-wide diagonal blocks.
+// With the inverses over two adjacent pivot ranges A and B (B after A) known,
+//   (L^-1)_BA = -L_BB^-1 (L_BA L_AA^-1),   (U^-1)_AB = -U_AA^-1 (U_AB U_BB^-1).
+// Stage t pairs ranges of NB 2^t pivots, so a front of p pivots needs
+// ceil(log2(p / NB)) stages; a stage is two batched MFMA tile launches (the
+// parenthesized products into scratch, then the outer products in place).
+// L_BA and U_AB come from the pivot-block copy.
+struct InvTile {
+    int f;           // front (index in the level's task list)
+    int kind;        // 0: lower (output |B| x |A|), 1: upper (output |A| x |B|)
+    int a0, na, nb;  // A = [a0, a0 + na), B = [a0 + na, a0 + na + nb)
+    int ti, tj;      // TM x TM output tile
+    int pad;
+    int64_t toff;    // the pair's product block in the level scratch
+};
+
+// NB x NB diagonal-block inverses (from the panels) into the pivot block
 template <typename T>
-__global__ void __launch_bounds__(256) k_tri_inv(const FrontTask *tasks, int j,
-                                                 const FacPtrs<T> *mem) {
+__global__ void __launch_bounds__(256) k_inv_diag(const FrontTask *tasks, const FacPtrs<T> *mem) {
     T *const arena = mem[blockIdx.z].arena;
-    const T *const copy = mem[blockIdx.z].icopy;
     const T *const dinv = mem[blockIdx.z].dinv;
-    __shared__ T As[64][NB + 1];
-    __shared__ T Bs[NB][64 + 1];
-    __shared__ T Ts[64][NB + 1];
-    __shared__ T Iv[NB][NB + 1];
     const FrontTask tk = tasks[blockIdx.y];
-    const int p = tk.p;
-    const int c0 = j * NB;
-    if (c0 >= p) return;
-    const int jw = min(NB, p - c0), r0 = c0 + jw;
-    const bool lower = (blockIdx.x & 1) == 0;
-    const int tile = blockIdx.x >> 1;
-    const int64_t n = tk.n;
+    const int c0 = blockIdx.x * NB;
+    if (c0 >= tk.p) return;
+    const int jw = min(NB, tk.p - c0);
+    const T *Dl = dinv + tk.doff + (int64_t)blockIdx.x * 2 * NB * NB, *Du = Dl + NB * NB;
     T *F = arena + tk.off;
-    const T *Cp = copy + tk.coff;
-    const T *Dv = dinv + tk.doff + (int64_t)j * 2 * NB * NB + (lower ? 0 : NB * NB);
-    const int tid = threadIdx.x;
-    if (tile == 0) {
-        // diagonal block: strict lower part of inv(L_blk) / upper part of inv(U_blk)
-        for (int e = tid; e < jw * jw; e += blockDim.x) {
-            const int r = e % jw, c = e / jw;
-            if (lower ? (r > c) : (r <= c)) F[(int64_t)(c0 + c) * n + c0 + r] = Dv[c * NB + r];
-        }
+    for (int e = threadIdx.x; e < jw * jw; e += blockDim.x) {
+        const int r = e % jw, c = e / jw;
+        F[(int64_t)(c0 + c) * tk.n + c0 + r] = r > c ? Dl[c * NB + r] : Du[c * NB + r];
     }
-    const int S0 = r0 + tile * 64;
-    if (S0 >= p) return;
-    const int w = min(64, p - S0);
-    for (int e = tid; e < NB * NB; e += blockDim.x) Iv[e % NB][e / NB] = Dv[(e / NB) * NB + e % NB];
-    // thread layout: 64 "long" indices (rows for L, columns for U) x 4 groups of 8 block indices
-    const int li = tid & 63, g = tid >> 6;
-    T acc[8];
+}
+
+// PASS 1: S = L_BA L_AA^-1 (lower) / U_AB U_BB^-1 (upper)
+// PASS 2: (L^-1)_BA = -L_BB^-1 S (lower) / (U^-1)_AB = -U_AA^-1 S (upper)
+// One TM x TM output tile per workgroup of 4 wavefronts (2 x 2 of 32 x 32,
+// each a 2 x 2 grid of 16 x 16 MFMA blocks); K in chunks of TK through LDS,
+// restricted to where the triangular factor is nonzero.
+template <typename T, int PASS>
+__global__ void __launch_bounds__(256) k_inv_gemm(const InvTile *tiles, const FrontTask *tasks,
+                                                  const FacPtrs<T> *mem) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    __shared__ T As[TK][TM + 4];
+    __shared__ T Bs[TK][TM + 4];
+    const InvTile t = tiles[blockIdx.x];
+    const FrontTask tk = tasks[t.f];
+    T *const F = mem[blockIdx.z].arena + tk.off;
+    const T *const Cp = mem[blockIdx.z].icopy + tk.coff;
+    T *const S = mem[blockIdx.z].iscr + t.toff;
+    const int64_t n = tk.n, p = tk.p;
+    const int a0 = t.a0, b0 = t.a0 + t.na, na = t.na, nb = t.nb;
+    const bool lower = t.kind == 0;
+    const int m = lower ? nb : na, nn = lower ? na : nb;
+    const int i0 = t.ti * TM, j0 = t.tj * TM;
+    int kmin, kmax;
+    if (PASS == 1) {
+        kmin = lower ? j0 : 0;                       // L_AA^-1[k, j]: k >= j
+        kmax = lower ? na : min(nb, j0 + TM);        // U_BB^-1[k, j]: k <= j
+    } else {
+        kmin = lower ? 0 : i0;                       // U_AA^-1[i, k]: k >= i
+        kmax = lower ? min(nb, i0 + TM) : na;        // L_BB^-1[i, k]: k <= i
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    acc_t acc[2][2];
 #pragma unroll
-    for (int q = 0; q < 8; q++) acc[q] = (T)0;
-    const int kend = min(p, S0 + w);
-    for (int kk = r0; kk < kend; kk += NB) {
-        // index order per operand so that consecutive threads read
-        // consecutive addresses (column-major storage)
-        for (int e = tid; e < 64 * NB; e += blockDim.x) {
-            const int a = lower ? e % 64 : e / NB, kl = lower ? e / 64 : e % NB;
-            const int k = kk + kl, sidx = S0 + a;
-            T v = (T)0;
-            if (k < kend && sidx < p && k <= sidx) {
-                if (lower) v = (k == sidx) ? (T)1 : F[(int64_t)k * n + sidx];      // X[sidx, k]
-                else v = F[(int64_t)sidx * n + k];                                 // Y[k, sidx]
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) acc[x][y] = acc_t{0, 0, 0, 0};
+    for (int kk = kmin; kk < kmax; kk += TK) {
+        for (int e = threadIdx.x; e < TK * TM; e += blockDim.x) {
+            // left operand (i, k), consecutive threads along i
+            {
+                const int r = e % TM, kl = e / TM;
+                const int i = i0 + r, k = kk + kl;
+                T v = (T)0;
+                if (i < m && k < kmax) {
+                    if (PASS == 1) {
+                        v = lower ? Cp[(int64_t)(a0 + k) * p + b0 + i]      // L_BA
+                                  : Cp[(int64_t)(b0 + k) * p + a0 + i];     // U_AB
+                    } else if (lower) {                                   // L_BB^-1 (unit)
+                        v = k < i ? F[(int64_t)(b0 + k) * n + b0 + i] : (k == i ? (T)1 : (T)0);
+                    } else {                                              // U_AA^-1
+                        v = k >= i ? F[(int64_t)(a0 + k) * n + a0 + i] : (T)0;
+                    }
+                }
+                As[kl][r] = v;
             }
-            if (lower) As[a][kl] = v;
-            else Bs[kl][a] = v;
-        }
-        for (int e = tid; e < NB * NB; e += blockDim.x) {
-            const int b = lower ? e / NB : e % NB, kl = lower ? e % NB : e / NB, k = kk + kl;
-            T v = (T)0;
-            if (k < kend && b < jw) {
-                if (lower) v = Cp[(int64_t)(c0 + b) * p + k];     // L[k, c0 + b]
-                else v = Cp[(int64_t)k * p + c0 + b];             // U[c0 + b, k]
+            // right operand (k, j), consecutive threads along k
+            {
+                const int c = e / TK, kl = e % TK;
+                const int j = j0 + c, k = kk + kl;
+                T v = (T)0;
+                if (j < nn && k < kmax) {
+                    if (PASS == 1) {
+                        if (lower)                                        // L_AA^-1 (unit)
+                            v = k > j ? F[(int64_t)(a0 + j) * n + a0 + k] : (k == j ? (T)1 : (T)0);
+                        else                                              // U_BB^-1
+                            v = k <= j ? F[(int64_t)(b0 + j) * n + b0 + k] : (T)0;
+                    } else {
+                        v = S[(int64_t)j * (lower ? nb : na) + k];        // the PASS 1 product
+                    }
+                }
+                Bs[kl][c] = v;
             }
-            if (lower) Bs[kl][b] = v;
-            else As[b][kl] = v;
         }
         __syncthreads();
-        if (lower) {
-            // T[li, b] += sum_k X[li, k] L[k, b]
-#pragma unroll 8
-            for (int kl = 0; kl < NB; kl++) {
-                const T x = As[li][kl];
 #pragma unroll
-                for (int q = 0; q < 8; q++) acc[q] += x * Bs[kl][g * 8 + q];
-            }
-        } else {
-            // T[b, li] += sum_k U[b, k] Y[k, li]
-#pragma unroll 8
-            for (int kl = 0; kl < NB; kl++) {
-                const T y = Bs[kl][li];
-#pragma unroll
-                for (int q = 0; q < 8; q++) acc[q] += As[g * 8 + q][kl] * y;
-            }
+        for (int k4 = 0; k4 < TK; k4 += 4) {
+            const T x0 = As[k4 + lk][wr + li], x1 = As[k4 + lk][wr + 16 + li];
+            const T y0 = Bs[k4 + lk][wc + li], y1 = Bs[k4 + lk][wc + 16 + li];
+            acc[0][0] = Mfma<T>::mma(x0, y0, acc[0][0]);
+            acc[0][1] = Mfma<T>::mma(x0, y1, acc[0][1]);
+            acc[1][0] = Mfma<T>::mma(x1, y0, acc[1][0]);
+            acc[1][1] = Mfma<T>::mma(x1, y1, acc[1][1]);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < 8; q++) Ts[li][g * 8 + q] = acc[q];
-    __syncthreads();
-    if (li < w) {
+    for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int b = g * 8 + q;
-            if (b >= jw) continue;
-            T r = (T)0;
-            if (lower) {
-                // (T inv(L_blk))[li, b]
-                for (int c = 0; c < jw; c++) r += Ts[li][c] * Iv[c][b];
-                F[(int64_t)(c0 + b) * n + S0 + li] = -r;
-            } else {
-                // (inv(U_blk) T)[b, li]
-                for (int c = 0; c < jw; c++) r += Iv[b][c] * Ts[li][c];
-                F[(int64_t)(S0 + li) * n + c0 + b] = -r;
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int i = i0 + wr + x * 16 + Mfma<T>::row(lane, q), j = j0 + wc + y * 16 + li;
+                if (i >= m || j >= nn) continue;
+                const T v = acc[x][y][q];
+                if (PASS == 1) S[(int64_t)j * m + i] = v;
+                else if (lower) F[(int64_t)(a0 + j) * n + b0 + i] = -v;
+                else F[(int64_t)(b0 + j) * n + a0 + i] = -v;
             }
-        }
-    }
 }
 
 // ------------------------------------------------------------ solves
@@ -970,6 +1003,9 @@ struct SolverDev {
     float *dinv32 = nullptr;       // inverted diagonal blocks (FP32 factor)
     double *part = nullptr;        // GEMV partial sums of one level
     double *icopy = nullptr;       // pivot-block copies of one level (inversion)
+    double *iscr = nullptr;        // inversion products of one level and stage
+    std::vector<std::vector<InvTile *>> inv_tiles;   // per level, per doubling stage
+    std::vector<std::vector<int>> inv_ntiles;
     int64_t dinv_size = 0;
     std::vector<int> gemv_fwd_grid, gemv_bwd_grid, gemv_tri_grid;   // per level grid.x
     std::vector<double> schur_flops, fwdupd_entries, bwdupd_entries;   // per level work
@@ -1047,6 +1083,11 @@ int sv_free_device(hmec_ctx *c) {
         hipFree(s->dinv32);
         hipFree(s->part);
         hipFree(s->icopy);
+        hipFree(s->iscr);
+        for (auto &v : s->inv_tiles)
+            for (InvTile *q : v) hipFree(q);
+        s->inv_tiles.clear();
+        s->inv_ntiles.clear();
         hipFree(s->mem);
         if (s->h_norms) hipHostFree(s->h_norms);
         delete s;
@@ -1110,6 +1151,45 @@ int sv_setup_device(hmec_ctx *c) {
         copy_max = std::max(copy_max, co);
     }
     HMEC_HIP(hipMalloc(&s->icopy, sizeof(double) * std::max<int64_t>(1, copy_max)));
+    // triangle-inversion tiles per level and doubling stage: pairs of
+    // adjacent ranges of NB 2^t pivots (the last range of a front may be
+    // shorter), products placed one after another in the level scratch
+    {
+        int64_t scr_max = 0;
+        s->inv_tiles.assign(pl.nlevels, {});
+        s->inv_ntiles.assign(pl.nlevels, {});
+        for (int lv = 0; lv < pl.nlevels; lv++) {
+            int maxp = 0;
+            for (int f : pl.levels[lv]) maxp = std::max(maxp, all[f].p);
+            for (int b = NB; b < maxp; b *= 2) {
+                std::vector<InvTile> tl;
+                int64_t to = 0;
+                int fi = 0;
+                for (int f : pl.levels[lv]) {
+                    const int pf = all[f].p;
+                    for (int a0 = 0; a0 + b < pf; a0 += 2 * b) {
+                        const int na = b, nb = std::min(b, pf - a0 - b);
+                        for (int kind = 0; kind < 2; kind++) {
+                            const int m = kind == 0 ? nb : na, nn = kind == 0 ? na : nb;
+                            for (int ti = 0; ti < (m + TM - 1) / TM; ti++)
+                                for (int tj = 0; tj < (nn + TM - 1) / TM; tj++)
+                                    tl.push_back(InvTile{fi, kind, a0, na, nb, ti, tj, 0, to});
+                            to += (int64_t)na * nb;
+                        }
+                    }
+                    fi++;
+                }
+                scr_max = std::max(scr_max, to);
+                InvTile *d = nullptr;
+                HMEC_HIP(hipMalloc(&d, sizeof(InvTile) * std::max<size_t>(1, tl.size())));
+                if (!tl.empty())
+                    HMEC_HIP(hipMemcpy(d, tl.data(), sizeof(InvTile) * tl.size(), hipMemcpyHostToDevice));
+                s->inv_tiles[lv].push_back(d);
+                s->inv_ntiles[lv].push_back((int)tl.size());
+            }
+        }
+        HMEC_HIP(hipMalloc(&s->iscr, sizeof(double) * std::max<int64_t>(1, scr_max)));
+    }
     HMEC_HIP(hipMalloc(&s->part, sizeof(double) * std::max<int64_t>(1, part_max)));
     HMEC_TRY(upload(&s->all_tasks, all));
     std::vector<int> flist;
@@ -1347,10 +1427,17 @@ static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipS
                            dim3(256), 0, st, dp.d_tasks[lv], mem);
         LAUNCH_CHECK();
         const int nblk = (maxp + NB - 1) / NB;
-        const int ntile = std::max(1, (maxp + 63) / 64);
-        for (int j = nblk - 1; j >= 0; j--) {
-            hipLaunchKernelGGL(k_tri_inv<T>, dim3(2 * ntile, nfr, Bz), dim3(256), 0, st, dp.d_tasks[lv], j,
-                               mem);
+        hipLaunchKernelGGL(k_inv_diag<T>, dim3(std::max(1, nblk), nfr, Bz), dim3(256), 0, st,
+                           dp.d_tasks[lv], mem);
+        LAUNCH_CHECK();
+        for (size_t t = 0; t < s->inv_tiles[lv].size(); t++) {
+            const int nt = s->inv_ntiles[lv][t];
+            if (nt == 0) continue;
+            hipLaunchKernelGGL((k_inv_gemm<T, 1>), dim3(nt, 1, Bz), dim3(256), 0, st,
+                               s->inv_tiles[lv][t], dp.d_tasks[lv], mem);
+            LAUNCH_CHECK();
+            hipLaunchKernelGGL((k_inv_gemm<T, 2>), dim3(nt, 1, Bz), dim3(256), 0, st,
+                               s->inv_tiles[lv][t], dp.d_tasks[lv], mem);
             LAUNCH_CHECK();
         }
     }
@@ -1414,13 +1501,13 @@ int sv_factorize_batch(hmec_ctx *const *cs, int n, int precision) {
         hmec_ctx *c = cs[k];
         SolverDev *s = sdev(c);
         if (fp64) {
-            FacPtrs<double> f{c->d.arena, s->dinv64, s->dscr, s->icopy, c->d.counters, c->dc.sval,
-                              c->dc.rs, c->dc.cs};
+            FacPtrs<double> f{c->d.arena, s->dinv64, s->dscr, s->icopy, s->iscr, c->d.counters,
+                              c->dc.sval, c->dc.rs, c->dc.cs};
             std::memcpy(host.data() + psz * k, &f, psz);
         } else {
             FacPtrs<float> f{c->d.arena32, s->dinv32, reinterpret_cast<float *>(s->dscr),
-                             reinterpret_cast<float *>(s->icopy), c->d.counters, c->dc.sval, c->dc.rs,
-                             c->dc.cs};
+                             reinterpret_cast<float *>(s->icopy), reinterpret_cast<float *>(s->iscr),
+                             c->d.counters, c->dc.sval, c->dc.rs, c->dc.cs};
             std::memcpy(host.data() + psz * k, &f, psz);
         }
     }
