@@ -717,12 +717,16 @@ __global__ void k_axpy_scaled(double *x, const double *cs, const double *z, int6
 // and, when y2 is given, y2 = rs * y.
 constexpr int SPMV_ROWS = 256;
 constexpr int SPMV_CHUNK = 2048;
+// MODE 2: y = rs * (A x) and, per workgroup, the partial dot products of its
+// rows of y with V_0..V_{nv-1} (columns ldv apart) -> part[block * nv + i].
+constexpr int GM_MAXV = 32;
 template <int MODE>
 __global__ void __launch_bounds__(SPMV_ROWS)
 k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
              const double *__restrict__ val, const double *__restrict__ x,
              const double *__restrict__ rs, const double *__restrict__ b, double *__restrict__ y,
-             double *__restrict__ y2, int64_t N) {
+             double *__restrict__ y2, int64_t N, const double *__restrict__ V = nullptr,
+             int64_t ldv = 0, int nv = 0, double *__restrict__ part = nullptr) {
     __shared__ double prod[SPMV_CHUNK];
     __shared__ int32_t rp[SPMV_ROWS + 1];
     const int64_t r0 = (int64_t)blockIdx.x * SPMV_ROWS;
@@ -741,6 +745,29 @@ k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
         for (int p = max(mb, c0); p < pe; p++) s += prod[p - c0];
         __syncthreads();
     }
+    if (MODE == 2) {
+        __shared__ double red[SPMV_ROWS / 64][GM_MAXV];
+        double yv = 0.0;
+        if (t < nr) {
+            yv = rs[r0 + t] * s;
+            y[r0 + t] = yv;
+        }
+        const int lane = t & 63, wv = t >> 6;
+        for (int i = 0; i < nv; i++) {
+            double d = t < nr ? yv * V[(int64_t)i * ldv + r0 + t] : 0.0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
+            if (lane == 0) red[wv][i] = d;
+        }
+        __syncthreads();
+        if (t < nv) {
+            double d = 0.0;
+#pragma unroll
+            for (int q = 0; q < SPMV_ROWS / 64; q++) d += red[q][t];
+            part[(int64_t)blockIdx.x * nv + t] = d;
+        }
+        return;
+    }
     if (t >= nr) return;
     const int64_t r = r0 + t;
     if (MODE == 0) {
@@ -757,6 +784,14 @@ static void launch_residual(hmec_ctx *c, const double *x, double *y, double *y2,
     hipLaunchKernelGGL(k_csr_stream<1>, dim3((unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS)),
                        dim3(SPMV_ROWS), 0, c->stream, c->d.rowp, c->d.coli, c->d.val, x, rs,
                        c->d.rhs, y, y2, N);
+}
+// y = rs * (A x) with the partial dot products of y with V_0..V_{nv-1}
+static void launch_spmv_rs_dots(hmec_ctx *c, const double *x, double *y, const double *V,
+                                int nv, double *part) {
+    const int64_t N = c->N;
+    hipLaunchKernelGGL(k_csr_stream<2>, dim3((unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS)),
+                       dim3(SPMV_ROWS), 0, c->stream, c->d.rowp, c->d.coli, c->d.val, x,
+                       c->d.rscale, nullptr, y, nullptr, N, V, N, nv, part);
 }
 // y = rs * (A x)
 static void launch_spmv_rs(hmec_ctx *c, const double *x, double *y) {
@@ -915,13 +950,16 @@ __global__ void k_spmv(const int32_t *rowp, const int32_t *coli, const double *v
 }
 // y = Dr (A x)   (row-equilibrated operator)
 // partial[b * nv + i] = sum over block b's slice of V_i . w   (nv vectors)
+// (self: one more output, partial[.. + nv] = w . w)
 __global__ void __launch_bounds__(256) k_dots_part(const double *V, int64_t ldv, int nv,
-                                                   const double *w, int64_t N, double *partial) {
+                                                   const double *w, int64_t N, double *partial,
+                                                   int self = 0) {
     __shared__ double red[256];
     const int64_t chunk = (N + gridDim.x - 1) / gridDim.x;
     const int64_t b0 = (int64_t)blockIdx.x * chunk, b1 = min(N, b0 + chunk);
-    for (int i = 0; i < nv; i++) {
-        const double *v = V + (int64_t)i * ldv;
+    const int no = nv + self;
+    for (int i = 0; i < no; i++) {
+        const double *v = i < nv ? V + (int64_t)i * ldv : w;
         double s = 0;
         for (int64_t t = b0 + threadIdx.x; t < b1; t += blockDim.x) s += v[t] * w[t];
         red[threadIdx.x] = s;
@@ -930,7 +968,7 @@ __global__ void __launch_bounds__(256) k_dots_part(const double *V, int64_t ldv,
             if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
             __syncthreads();
         }
-        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nv + i] = red[0];
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * no + i] = red[0];
         __syncthreads();
     }
 }
@@ -955,6 +993,26 @@ struct GmCoef {
     double v[32];
 };
 // w -= sum_i h_i V_i, coefficients from device memory (no host round trip)
+// Second Gram-Schmidt pass and normalization: w'' = w' - V h (h = h[0..nv),
+// ||w'||^2 = h[nv]); ||w''||^2 = ||w'||^2 - ||h||^2 (V orthonormal, h the
+// small second-pass coefficients) -> *ww; y = w'' / ||w''|| when y is given.
+__global__ void k_gs_update_norm(const double *V, int64_t ldv, int nv, const double *h,
+                                 const double *w, double *y, double *ww, int64_t N) {
+    __shared__ double hs[33];
+    if ((int)threadIdx.x <= nv) hs[threadIdx.x] = h[threadIdx.x];
+    __syncthreads();
+    double n2 = hs[nv];
+    for (int i = 0; i < nv; i++) n2 -= hs[i] * hs[i];
+    n2 = n2 > 0 ? n2 : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ww[0] = n2;
+    if (!y) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    double s = w[t];
+    for (int i = 0; i < nv; i++) s -= hs[i] * V[(int64_t)i * ldv + t];
+    const double n = sqrt(n2);
+    y[t] = n > 0 ? s * (1.0 / n) : 0.0;
+}
 __global__ void k_gs_update(const double *V, int64_t ldv, int nv, const double *h, double *w,
                             int64_t N) {
     __shared__ double hs[32];
@@ -972,12 +1030,6 @@ __global__ void k_vscale(const double *x, double a, double *y, int64_t N) {
     if (t < N) y[t] = a * x[t];
 }
 // y = x / sqrt(ww[0]) (zeros on breakdown, which the host then detects)
-__global__ void k_vnormalize(const double *x, const double *ww, double *y, int64_t N) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N) return;
-    const double n = sqrt(ww[0]);
-    y[t] = n > 0 ? x[t] * (1.0 / n) : 0.0;
-}
 // x += sum_j c_j Z_j
 __global__ void k_combine(const double *Z, int64_t ldz, int nz, GmCoef c, double *x,
                           int64_t N) {
@@ -1758,7 +1810,8 @@ static int gm_buf(hmec_ctx *c, int m, GmresBuf **out) {
         HMEC_HIP(hipMalloc(&g.w, sizeof(double) * c->N));
         HMEC_HIP(hipMalloc(&g.bs, sizeof(double) * c->N));
         HMEC_HIP(hipMalloc(&g.rsd, sizeof(double) * c->N));
-        HMEC_HIP(hipMalloc(&g.part, sizeof(double) * RB * (m + 2)));
+        const int64_t nbs = std::max<int64_t>(RB, (c->N + SPMV_ROWS - 1) / SPMV_ROWS);
+        HMEC_HIP(hipMalloc(&g.part, sizeof(double) * nbs * (m + 2)));
         HMEC_HIP(hipMalloc(&g.hdev, sizeof(double) * 3 * (m + 2)));
         HMEC_HIP(hipHostMalloc(&g.hhost, sizeof(double) * 3 * (m + 2), hipHostMallocDefault));
     }
@@ -1827,7 +1880,8 @@ static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *r
 // a margin of 2).
 int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     if (!c->factor_valid) return HMEC_E_ARG;
-    const int m = 20;               // restart length (<= 31: GmCoef holds the coefficients)
+    const int m = 20;               // restart length (<= 31: GmCoef and the fused dots hold the coefficients)
+    static_assert(20 + 1 <= GM_MAXV, "restart length above the fused dot-product width");
     GmresBuf *gp;
     HMEC_TRY(gm_buf(c, m, &gp));
     GmresBuf &g = *gp;
@@ -1864,22 +1918,25 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
         for (; j < m && total_it < max_it; j++) {
             double *Zj = g.Z + (int64_t)j * N;
             HMEC_TRY(gm_precond(c, g.V + (int64_t)j * N, Zj));
-            launch_spmv_rs(c, Zj, g.w);
-            LAUNCH_CHECK();
             // classical Gram-Schmidt, twice, coefficients kept on the device;
-            // one copy of the Hessenberg column per iteration
+            // one copy of the Hessenberg column per iteration.  The first
+            // pass's dot products come with the operator product, the second
+            // pass's with ||w'||^2, and its update writes the next basis vector.
             const int M2 = m + 2;
             double *h1d = g.hdev, *h2d = g.hdev + M2, *wwd = g.hdev + 2 * M2;
-            HMEC_TRY(gm_dots_dev(c, g, g.V, j + 1, g.w, h1d));
+            const unsigned nbs = (unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS);
+            launch_spmv_rs_dots(c, Zj, g.w, g.V, j + 1, g.part);
+            LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_dots_final, dim3(j + 1), dim3(256), 0, st, g.part, (int)nbs, j + 1, h1d);
+            LAUNCH_CHECK();
             hipLaunchKernelGGL(k_gs_update, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h1d, g.w, N);
             LAUNCH_CHECK();
-            HMEC_TRY(gm_dots_dev(c, g, g.V, j + 1, g.w, h2d));
-            hipLaunchKernelGGL(k_gs_update, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h2d, g.w, N);
+            hipLaunchKernelGGL(k_dots_part, dim3(RB), dim3(256), 0, st, g.V, N, j + 1, g.w, N, g.part, 1);
             LAUNCH_CHECK();
-            HMEC_TRY(gm_dots_dev(c, g, g.w, 1, g.w, wwd));
-            if (j + 1 < m)
-                hipLaunchKernelGGL(k_vnormalize, dim3(gN), dim3(256), 0, st, g.w, wwd,
-                                   g.V + (int64_t)(j + 1) * N, N);
+            hipLaunchKernelGGL(k_dots_final, dim3(j + 2), dim3(256), 0, st, g.part, RB, j + 2, h2d);
+            LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_gs_update_norm, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h2d, g.w,
+                               j + 1 < m ? g.V + (int64_t)(j + 1) * N : nullptr, wwd, N);
             LAUNCH_CHECK();
             HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * 3 * M2, hipMemcpyDeviceToHost, st));
             HMEC_HIP(hipStreamSynchronize(st));
