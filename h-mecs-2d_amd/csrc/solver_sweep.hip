@@ -50,9 +50,10 @@
 namespace {
 
 constexpr int SW_T = 256;      // threads per sweep workgroup (64 on small-front levels)
-constexpr int SW_RPG = 4;      // rows in flight per lane group
+constexpr int SW_RPG = 2;      // rows in flight per lane group (64 VGPRs: 8 waves per SIMD)
 constexpr int SW_PASSES = 4;   // most row passes per full-size workgroup
-constexpr int SW_FILL = 2048;  // workgroups a level keeps at least (8 per CU)
+constexpr int SW_FILL = 1024;  // workgroups a level keeps at least (4 per CU)
+constexpr int SW_SMALL = 256;  // levels whose fronts have at most this many rows: one-wave workgroups
 constexpr int SW_TILE = 64;    // sweep-form GEMM tile
 constexpr int SW_TK = 16;      // its K chunk
 
@@ -449,7 +450,7 @@ void level_tasks(const std::vector<SwFront> &fr, const std::vector<int> &fronts,
     while (lanes < 64 && lanes * vw * 2 < kmax) lanes *= 2;
     // small fronts: one-wave workgroups, so a CU keeps 4x more fronts (and
     // their dependent gather/load chains) in flight
-    nt = (mmax <= 128 && lanes <= 16) ? 64 : SW_T;
+    nt = (mmax <= SW_SMALL && lanes <= 16) ? 64 : SW_T;
     // a workgroup may take several passes of rows so that the gather
     // of the front's input vector into LDS is shared by more rows, as long as
     // the level keeps enough workgroups to fill the device
