@@ -446,7 +446,7 @@ void level_tasks(const std::vector<SwFront> &fr, const std::vector<int> &fronts,
         kmax = std::max(kmax, dir == 0 ? fr[f].p : fr[f].n);
         mmax = std::max(mmax, dir == 0 ? fr[f].n : fr[f].p);
     }
-    lanes = 8;
+    lanes = 4;
     while (lanes < 64 && lanes * vw * 2 < kmax) lanes *= 2;
     // small fronts: one-wave workgroups, so a CU keeps 4x more fronts (and
     // their dependent gather/load chains) in flight
@@ -769,11 +769,13 @@ static int launch_sweep(hmec_ctx *c, SweepDev *s, int lv, const T *sa, double *z
                        wbuf)
     if (nth == 64) {
         switch (lanes) {
+            case 4: SW_GO(4, 64); break;
             case 8: SW_GO(8, 64); break;
             default: SW_GO(16, 64); break;
         }
     } else {
         switch (lanes) {
+            case 4: SW_GO(4, SW_T); break;
             case 8: SW_GO(8, SW_T); break;
             case 16: SW_GO(16, SW_T); break;
             case 32: SW_GO(32, SW_T); break;
