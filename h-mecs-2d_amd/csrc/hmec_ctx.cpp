@@ -142,7 +142,9 @@ extern "C" int hmec_create(const hmec_params *p, hmec_ctx **out) {
     if (hipMalloc(&d.scal, sizeof(double) * 32) != hipSuccess) return fail(HMEC_E_OOM);
     hipMemset(d.scal, 0, sizeof(double) * 32);
     if (hipHostMalloc(&d.h_scal, sizeof(double) * 32, hipHostMallocDefault) != hipSuccess) return fail(HMEC_E_OOM);
-    const size_t ncount = 16 + c->Nx1 + 64;
+    // [0, 16): solver counters; then per fault node the yield flag and the
+    // bisection-cap flag of the RSF solve; spare
+    const size_t ncount = 16 + 2 * (size_t)c->Nx1 + 64;
     if (hipMalloc(&d.counters, sizeof(int) * ncount) != hipSuccess) return fail(HMEC_E_OOM);
     hipMemset(d.counters, 0, sizeof(int) * ncount);
     for (double **v : {&d.rhs, &d.sol, &d.work, &d.work2, &d.rscale, &d.cscale}) {
@@ -369,10 +371,11 @@ extern "C" int hmec_step_begin(hmec_ctx *c) {
     if (!c) return HMEC_E_ARG;
     const hmec_params &P = c->P;
     const int src = (c->timestep == 1) ? HMEC_F_ETA0 : HMEC_F_ETA00;   // :586-594
-    HMEC_TRY(k_copy_field(c, HMEC_F_ETA1, src));
-    HMEC_TRY(k_copy_field(c, HMEC_F_ETA, src));
-    HMEC_TRY(k_copy_field(c, HMEC_F_ETA50, src));
-    HMEC_TRY(k_copy_field(c, HMEC_F_OM, HMEC_F_OM0));                 // :596
+    {
+        const int dst[4] = {HMEC_F_ETA1, HMEC_F_ETA, HMEC_F_ETA50, HMEC_F_OM};
+        const int from[4] = {src, src, src, HMEC_F_OM0};                 // :596
+        HMEC_TRY(k_copy_fields(c, dst, from, 4));
+    }
     if (!c->yndtdecrease) c->dt = std::max(std::min(c->dt * P.dtkoefup, P.dtelastic0), P.dtmin);
     else c->dt = std::max(std::min(c->dt, P.dtelastic0), P.dtmin);     // :599-603
     c->yndtdecrease = 0;
@@ -492,8 +495,11 @@ extern "C" int hmec_advect(hmec_ctx *c, double dt) {
 // the final dt (:1480-1521), advection (:1561-1656), history (:1658-1726).
 extern "C" int hmec_step_end(hmec_ctx *c, double dt) {
     if (!c) return HMEC_E_ARG;
-    HMEC_TRY(k_copy_field(c, HMEC_F_ETA00, HMEC_F_ETA50));
-    HMEC_TRY(k_copy_field(c, HMEC_F_OM0, HMEC_F_OM));
+    {
+        const int dst[2] = {HMEC_F_ETA00, HMEC_F_OM0};
+        const int from[2] = {HMEC_F_ETA50, HMEC_F_OM};
+        HMEC_TRY(k_copy_fields(c, dst, from, 2));
+    }
     HMEC_TRY(k_final_stress(c, dt));
     if (c->P.advect_markers) HMEC_TRY(k_advect(c, dt));
     HMEC_TRY(k_history(c));

@@ -275,6 +275,7 @@ int k_final_stress(hmec_ctx *c, double dt);
 int k_history(hmec_ctx *c);
 int k_advect(hmec_ctx *c, double dt);
 int k_init_state(hmec_ctx *c);
+int k_copy_fields(hmec_ctx *c, const int *dst, const int *src, int count);
 int k_copy_field(hmec_ctx *c, int dst, int src);
 // symbolic CSR pattern of the reference's triplet emission (host)
 int hmec_build_pattern(hmec_ctx *c);

@@ -410,12 +410,39 @@ int k_init_state(hmec_ctx *c) {
     return HMEC_OK;
 }
 
-int k_copy_field(hmec_ctx *c, int dst, int src) {
-    const int64_t n = (dst < HMEC_NBASIC_FIELDS) ? c->nb : c->ns;
-    HMEC_HIP(hipMemcpyAsync(F(c, dst), F(c, src), n * sizeof(double), hipMemcpyDeviceToDevice,
-                            c->stream));
+// up to four field copies in one launch (dst[q] = src[q], n[q] entries)
+struct FieldCopies {
+    double *dst[4];
+    const double *src[4];
+    int64_t n[4];
+    int count;
+};
+__global__ void k_copy_fields(FieldCopies fc, int64_t nmax) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nmax) return;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (q < fc.count && t < fc.n[q]) fc.dst[q][t] = fc.src[q][t];
+}
+
+// fields dst[q] = src[q] (q < count <= 4) in one launch
+int k_copy_fields(hmec_ctx *c, const int *dst, const int *src, int count) {
+    FieldCopies fc{};
+    int64_t nmax = 0;
+    for (int q = 0; q < count; q++) {
+        fc.dst[q] = F(c, dst[q]);
+        fc.src[q] = F(c, src[q]);
+        fc.n[q] = (dst[q] < HMEC_NBASIC_FIELDS) ? c->nb : c->ns;
+        nmax = std::max(nmax, fc.n[q]);
+    }
+    fc.count = count;
+    hipLaunchKernelGGL(k_copy_fields, dim3((unsigned)((nmax + 255) / 256)), dim3(256), 0, c->stream,
+                       fc, nmax);
+    LAUNCH_CHECK();
     return HMEC_OK;
 }
+
+int k_copy_field(hmec_ctx *c, int dst, int src) { return k_copy_fields(c, &dst, &src, 1); }
 
 // ------------------------------------------------------------ VEP (a7)
 // copy_bounds(pt), copy_bounds(pf) (h_mec_rsf_v9.cpp:616-617, :154-159):
@@ -853,153 +880,170 @@ __global__ void __launch_bounds__(RSF_PREP_T) k_rsf_prep(double *f, int64_t ns, 
     if (threadIdx.x == 0) vmax_part[blockIdx.x] = red[0];
 }
 
-__global__ void __launch_bounds__(1024) k_rsf(double *f, int64_t ns, int Nx, int Ny,
-                                              hmec_params P, double dt, int active,
-                                              double *scal, double *dsy2, int *flag,
-                                              const double *vmax_part, int nvpart) {
-    __shared__ double red[1024];
+// One thread per fault node: the rate-and-state bisection (:1165-1308) and
+// the node's dtlapusta candidate, bisection-cap flag, DSY^2 and yield flag.
+constexpr int RSF_T = 64;
+__global__ void __launch_bounds__(RSF_T) k_rsf_nodes(double *f, int64_t ns, int Nx, int Ny,
+                                                     hmec_params P, double dt, double *dsy2,
+                                                     int *flag, double *dtln, int *capn) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= Nx) return;
     const int Ny1 = Ny + 1;
+    const int i = P.line_fault;
+    const double V0 = P.V0, BETASOLID = P.BETASOLID, dx = P.dx;
+    int capped = 0;
+    {
+        const int64_t b = i + (int64_t)j * Ny;
+        const int64_t s = i + (int64_t)j * Ny1;
+#define BF(n) f[(int64_t)(n) * ns + b]
+#define SB(n) (((f[(int64_t)(n) * ns + s] + f[(int64_t)(n) * ns + s + 1]) + f[(int64_t)(n) * ns + s + Ny1]) + f[(int64_t)(n) * ns + s + Ny1 + 1])
+        const double arsf = BF(HMEC_F_ARSF), brsf = BF(HMEC_F_BRSF), lrsf = BF(HMEC_F_LRSF);
+        const double fric = BF(HMEC_F_FRIC), eta0 = BF(HMEC_F_ETA0);
+        const double sxx = SB(HMEC_F_SXX), syy = SB(HMEC_F_SYY);
+        const double G = BF(HMEC_F_GGG), ETA = BF(HMEC_F_ETA), OMij = BF(HMEC_F_OM);
+        const double sxy = BF(HMEC_F_SXY);
+        const double SIIBij = sqrt(sxy * sxy + (sxx * sxx + syy * syy + (sxx + syy) * (sxx + syy)) / 32.);
+        BF(HMEC_F_SIIB) = SIIBij;
+        const double siiel = SIIBij / (ETA / (G * dt + ETA));
+        const double prB_temp = (SB(HMEC_F_PT) - SB(HMEC_F_PF)) / 4.;
+        double prB = prB_temp;
+        if (prB < 1e3) prB = 1e3;
+        const double expf_ = exp(-(brsf * OMij + fric) / arsf);
+        double SIIB1 = SIIBij;
+        double EIISLIP = V0 * sinh(fmax(SIIB1, 0.) / arsf / prB) * expf_ / dx;
+        double ETAPL = SIIB1 / 2. / EIISLIP;
+        double ETAVP = 1. / (1. / eta0 + 1. / ETAPL);
+        double SIIB2 = siiel * ETAVP / (G * dt + ETAVP);
+        const double DSIIB1 = SIIB2 - SIIB1;
+        double V = 2 * V0 * sinh(fmax(SIIB2, 0.) / arsf / prB) * expf_;
+        EIISLIP = V / dx / 2.;
+        ETAPL = SIIB2 / 2. / EIISLIP;
+        ETAVP = 1. / (1. / eta0 + 1. / ETAPL);
+        const double DSIIB2 = siiel * ETAVP / (G * dt + ETAVP) - SIIB2;
+        double SIIB4 = 0.;
+        if ((DSIIB1 >= 0 && DSIIB2 <= 0) || (DSIIB1 <= 0 && DSIIB2 >= 0)) {
+            double DSIIB = 1e9;
+            int it = 0;
+            while (fabs(DSIIB) > 1e-3) {
+                if (it++ >= P.bisect_cap) { capped = 1; break; }
+                SIIB4 = (SIIB1 + SIIB2) / 2.;
+                V = 2 * V0 * sinh(fmax((SIIB4), 0.) / arsf / prB) * expf_;
+                EIISLIP = V / dx / 2.;
+                ETAPL = SIIB4 / 2. / EIISLIP;
+                ETAVP = 1. / (1. / eta0 + 1. / ETAPL);
+                DSIIB = siiel * ETAVP / (G * dt + ETAVP) - SIIB4;
+                if ((DSIIB >= 0 && DSIIB1 >= 0) || (DSIIB <= 0 && DSIIB1 <= 0)) SIIB1 = SIIB4;
+                else SIIB2 = SIIB4;
+            }
+        }
+        double om5;
+        if (V * dt / lrsf > 1e-6) om5 = log(V0 / V + (exp(BF(HMEC_F_OM0)) - V0 / V) * exp(-V * dt / lrsf));
+        else om5 = log(exp(BF(HMEC_F_OM0)) * (1 - V * dt / lrsf) + V0 * dt / lrsf);
+        BF(HMEC_F_OM5) = om5;
+        const double syield = fmax(P.syieldmin, prB_temp * arsf * asinh(V / 2. / V0 * exp((brsf * om5 + fric) / arsf)));
+        double etapl = eta0 * syield / (eta0 * V + syield);
+        BF(HMEC_F_SIGMAY) = syield;
+        BF(HMEC_F_VSLIPB) = V;
+        BF(HMEC_F_SII_FAULT) = SIIB4;
+        const double g_temp = 2 * G;
+        const double k = g_temp / (M_PI * (1 - ((3. / BETASOLID - g_temp) / (6. / BETASOLID + g_temp))) * dx);
+        double dTETAmax;
+        const double q = (k * lrsf / prB - brsf) / arsf - 1;
+        if ((q * q / 4. - k * lrsf / arsf / prB) < 0)
+            dTETAmax = fmin(1. - (brsf - arsf) * prB / (k * lrsf), .2);
+        else
+            dTETAmax = fmin(arsf * prB / (k * lrsf - (brsf - arsf) * prB), .2);
+        const double dtl = dTETAmax * lrsf / V;
+        int counted = 0;
+        double d2 = 0;
+        if (BF(HMEC_F_YNY0) > 0) {
+            counted = 1;
+            const double dsy = SIIBij - syield;
+            BF(HMEC_F_DSY) = dsy;
+            d2 = dsy * dsy;
+        }
+        const double A = syield / siiel;
+        if (A < 1) {
+            etapl = dt * G * A / (1 - A);
+            if (etapl < eta0) {
+                BF(HMEC_F_ETA5) = pow(etapl, 1 - P.etawt) * pow(ETA, P.etawt);
+                BF(HMEC_F_YNY) = 1;
+                if (!counted) {
+                    counted = 1;
+                    const double dsy = SIIBij - syield;
+                    BF(HMEC_F_DSY) = dsy;
+                    d2 = dsy * dsy;
+                }
+            } else {
+                BF(HMEC_F_ETA5) = eta0;
+            }
+        } else {
+            BF(HMEC_F_ETA5) = eta0;
+        }
+        dsy2[j] = d2;
+        flag[j] = counted;
+        dtln[j] = dtl;
+        capn[j] = capped;
+#undef BF
+#undef SB
+    }
+}
+
+// Vmax (before the solve's RSF, h_mec_rsf_v9.cpp:1077) from the block
+// maxima of k_rsf_prep, min of the nodes' dtlapusta candidates, bisection
+// caps, and ynpl / ddd summed sequentially in node order like the reference
+// (from LDS chunks; an uncounted node has dsy2 = +0, and adding +0 to the
+// non-negative sum is exact).
+constexpr int RSF_FT = 512;
+__global__ void __launch_bounds__(RSF_FT) k_rsf_final(int Nx, int active, double *scal,
+                                                      const double *dsy2, const int *flag,
+                                                      const double *dtln, const int *capn,
+                                                      const double *vmax_part, int nvpart) {
+    __shared__ double red[RSF_FT];
+    __shared__ int cnt[RSF_FT];
     const int tid = threadIdx.x;
-    double mx = -INFINITY;
-    for (int t = tid; t < nvpart; t += blockDim.x) mx = fmax(mx, vmax_part[t]);
+    double mx = -INFINITY, dtl = 1e7;
+    int cap = 0;
+    for (int t = tid; t < nvpart; t += RSF_FT) mx = fmax(mx, vmax_part[t]);
+    if (active)
+        for (int j = tid; j < Nx; j += RSF_FT) {
+            dtl = fmin(dtl, dtln[j]);
+            cap += capn[j];
+        }
     red[tid] = mx;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    for (int s = RSF_FT / 2; s > 0; s >>= 1) {
         if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
         __syncthreads();
     }
     if (tid == 0) scal[SC_VMAX] = red[0];
     __syncthreads();
-    double dtl = 1e7;
-    int capped = 0;
-    if (active) {
-        const int i = P.line_fault;
-        const double V0 = P.V0, BETASOLID = P.BETASOLID, dx = P.dx;
-        for (int j = tid; j < Nx; j += blockDim.x) {
-            const int64_t b = i + (int64_t)j * Ny;
-            const int64_t s = i + (int64_t)j * Ny1;
-#define BF(n) f[(int64_t)(n) * ns + b]
-#define SB(n) (((f[(int64_t)(n) * ns + s] + f[(int64_t)(n) * ns + s + 1]) + f[(int64_t)(n) * ns + s + Ny1]) + f[(int64_t)(n) * ns + s + Ny1 + 1])
-            const double arsf = BF(HMEC_F_ARSF), brsf = BF(HMEC_F_BRSF), lrsf = BF(HMEC_F_LRSF);
-            const double fric = BF(HMEC_F_FRIC), eta0 = BF(HMEC_F_ETA0);
-            const double sxx = SB(HMEC_F_SXX), syy = SB(HMEC_F_SYY);
-            const double G = BF(HMEC_F_GGG), ETA = BF(HMEC_F_ETA), OMij = BF(HMEC_F_OM);
-            const double sxy = BF(HMEC_F_SXY);
-            const double SIIBij = sqrt(sxy * sxy + (sxx * sxx + syy * syy + (sxx + syy) * (sxx + syy)) / 32.);
-            BF(HMEC_F_SIIB) = SIIBij;
-            const double siiel = SIIBij / (ETA / (G * dt + ETA));
-            const double prB_temp = (SB(HMEC_F_PT) - SB(HMEC_F_PF)) / 4.;
-            double prB = prB_temp;
-            if (prB < 1e3) prB = 1e3;
-            const double expf_ = exp(-(brsf * OMij + fric) / arsf);
-            double SIIB1 = SIIBij;
-            double EIISLIP = V0 * sinh(fmax(SIIB1, 0.) / arsf / prB) * expf_ / dx;
-            double ETAPL = SIIB1 / 2. / EIISLIP;
-            double ETAVP = 1. / (1. / eta0 + 1. / ETAPL);
-            double SIIB2 = siiel * ETAVP / (G * dt + ETAVP);
-            const double DSIIB1 = SIIB2 - SIIB1;
-            double V = 2 * V0 * sinh(fmax(SIIB2, 0.) / arsf / prB) * expf_;
-            EIISLIP = V / dx / 2.;
-            ETAPL = SIIB2 / 2. / EIISLIP;
-            ETAVP = 1. / (1. / eta0 + 1. / ETAPL);
-            const double DSIIB2 = siiel * ETAVP / (G * dt + ETAVP) - SIIB2;
-            double SIIB4 = 0.;
-            if ((DSIIB1 >= 0 && DSIIB2 <= 0) || (DSIIB1 <= 0 && DSIIB2 >= 0)) {
-                double DSIIB = 1e9;
-                int it = 0;
-                while (fabs(DSIIB) > 1e-3) {
-                    if (it++ >= P.bisect_cap) { capped++; break; }
-                    SIIB4 = (SIIB1 + SIIB2) / 2.;
-                    V = 2 * V0 * sinh(fmax((SIIB4), 0.) / arsf / prB) * expf_;
-                    EIISLIP = V / dx / 2.;
-                    ETAPL = SIIB4 / 2. / EIISLIP;
-                    ETAVP = 1. / (1. / eta0 + 1. / ETAPL);
-                    DSIIB = siiel * ETAVP / (G * dt + ETAVP) - SIIB4;
-                    if ((DSIIB >= 0 && DSIIB1 >= 0) || (DSIIB <= 0 && DSIIB1 <= 0)) SIIB1 = SIIB4;
-                    else SIIB2 = SIIB4;
-                }
-            }
-            double om5;
-            if (V * dt / lrsf > 1e-6) om5 = log(V0 / V + (exp(BF(HMEC_F_OM0)) - V0 / V) * exp(-V * dt / lrsf));
-            else om5 = log(exp(BF(HMEC_F_OM0)) * (1 - V * dt / lrsf) + V0 * dt / lrsf);
-            BF(HMEC_F_OM5) = om5;
-            const double syield = fmax(P.syieldmin, prB_temp * arsf * asinh(V / 2. / V0 * exp((brsf * om5 + fric) / arsf)));
-            double etapl = eta0 * syield / (eta0 * V + syield);
-            BF(HMEC_F_SIGMAY) = syield;
-            BF(HMEC_F_VSLIPB) = V;
-            BF(HMEC_F_SII_FAULT) = SIIB4;
-            const double g_temp = 2 * G;
-            const double k = g_temp / (M_PI * (1 - ((3. / BETASOLID - g_temp) / (6. / BETASOLID + g_temp))) * dx);
-            double dTETAmax;
-            const double q = (k * lrsf / prB - brsf) / arsf - 1;
-            if ((q * q / 4. - k * lrsf / arsf / prB) < 0)
-                dTETAmax = fmin(1. - (brsf - arsf) * prB / (k * lrsf), .2);
-            else
-                dTETAmax = fmin(arsf * prB / (k * lrsf - (brsf - arsf) * prB), .2);
-            dtl = fmin(dtl, dTETAmax * lrsf / V);
-            int counted = 0;
-            double d2 = 0;
-            if (BF(HMEC_F_YNY0) > 0) {
-                counted = 1;
-                const double dsy = SIIBij - syield;
-                BF(HMEC_F_DSY) = dsy;
-                d2 = dsy * dsy;
-            }
-            const double A = syield / siiel;
-            if (A < 1) {
-                etapl = dt * G * A / (1 - A);
-                if (etapl < eta0) {
-                    BF(HMEC_F_ETA5) = pow(etapl, 1 - P.etawt) * pow(ETA, P.etawt);
-                    BF(HMEC_F_YNY) = 1;
-                    if (!counted) {
-                        counted = 1;
-                        const double dsy = SIIBij - syield;
-                        BF(HMEC_F_DSY) = dsy;
-                        d2 = dsy * dsy;
-                    }
-                } else {
-                    BF(HMEC_F_ETA5) = eta0;
-                }
-            } else {
-                BF(HMEC_F_ETA5) = eta0;
-            }
-            dsy2[j] = d2;
-            flag[j] = counted;
-#undef BF
-#undef SB
-        }
-    }
     red[tid] = dtl;
+    cnt[tid] = cap;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] = fmin(red[tid], red[tid + s]);
+    for (int s = RSF_FT / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[tid] = fmin(red[tid], red[tid + s]);
+            cnt[tid] += cnt[tid + s];
+        }
         __syncthreads();
     }
-    if (tid == 0) scal[SC_DTLAP] = red[0];
-    __syncthreads();
-    red[tid] = (double)capped;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
-        __syncthreads();
+    if (tid == 0) {
+        scal[SC_DTLAP] = red[0];
+        scal[SC_CAPPED] = cnt[0];
     }
-    if (tid == 0) scal[SC_CAPPED] = red[0];
-    // ynpl and ddd in node order (one thread, from LDS chunks so the ordered
-    // sum does not wait on a global load per node).  An uncounted node has
-    // dsy2 = +0, and adding +0 to the non-negative sum is exact.
     int ynpl = 0;
     double ddd = 0;
-    int *cnt = reinterpret_cast<int *>(red + 512);
     if (active) {
-        for (int j0 = 0; j0 < Nx; j0 += 512) {
+        for (int j0 = 0; j0 < Nx; j0 += RSF_FT) {
             __syncthreads();
-            if (tid < 512 && j0 + tid < Nx) {
+            if (j0 + tid < Nx) {
                 red[tid] = dsy2[j0 + tid];
                 cnt[tid] = flag[j0 + tid];
             }
             __syncthreads();
             if (tid == 0) {
-                const int n = min(512, Nx - j0);
+                const int n = min(RSF_FT, Nx - j0);
                 for (int q = 0; q < n; q++) {
                     ddd += red[q];
                     ynpl += cnt[q];
@@ -1100,8 +1144,16 @@ int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out) {
     hipLaunchKernelGGL(k_rsf_prep, dim3(nvp), dim3(RSF_PREP_T), 0, c->stream, c->d.fields, c->ns, nb,
                        part);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_rsf, dim3(1), dim3(1024), 0, c->stream, c->d.fields, c->ns, c->Nx, c->Ny,
-                       P, dt, active, c->d.scal, dsy2, flag, (const double *)part, nvp);
+    double *dtln = c->d.work2 + c->Nx;
+    int *capn = flag + c->Nx;
+    if (active) {
+        hipLaunchKernelGGL(k_rsf_nodes, dim3((c->Nx + RSF_T - 1) / RSF_T), dim3(RSF_T), 0, c->stream,
+                           c->d.fields, c->ns, c->Nx, c->Ny, P, dt, dsy2, flag, dtln, capn);
+        LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_rsf_final, dim3(1), dim3(RSF_FT), 0, c->stream, c->Nx, active, c->d.scal,
+                       (const double *)dsy2, (const int *)flag, (const double *)dtln,
+                       (const int *)capn, (const double *)part, nvp);
     LAUNCH_CHECK();
     Coords k = coords(c);
     hipLaunchKernelGGL(k_reduce_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->d.fields, c->ns,
