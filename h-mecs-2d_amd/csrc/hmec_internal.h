@@ -301,6 +301,8 @@ int sv_slot_nearest(hmec_ctx *c, double dt, double *dist);
 int sv_slot_victim(hmec_ctx *c, int keep);
 void sv_slot_target(hmec_ctx *c, int k);
 int sv_slot_activate(hmec_ctx *c, int k);
+void sv_sweep_rhs_target(hmec_ctx *c, int precision, double **zf, const int32_t **fpos);
+void sv_sweep_rhs_given(hmec_ctx *c);
 int sv_sweep_solve(hmec_ctx *c, double *z);
 void sv_sweep_free(hmec_ctx *c);
 

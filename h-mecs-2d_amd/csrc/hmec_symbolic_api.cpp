@@ -196,7 +196,7 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     b += z.nnz * (4 + 8) + (N + 1) * 4 + z.ntrip * 4;                     // coupled CSR
     b += z.snnz * (4 + 8 + 8 + 4 + 3 * 4 + 4 + 4 * 4) + (NS + 1) * 16;  // condensed CSR, maps
     b += 6 * N * 8 + 4 * NS * 8 + 3 * N * 4 + z.varena * 8 + z.nfronts * 256;   // vectors, plans
-    b += z.swrows * (4 + 8);                                            // sweep row sources
+    b += z.swrows * (4 + 4) + 3 * z.varena * 8 + NS * 4;                // sweep row maps, slots, zf
     b += (z.icopy + z.iscr) * 8;                                        // inversion scratch
     if (precision == HMEC_PREC_FP64) {
         b += z.arena * 8 + z.dinv * 8;

@@ -902,15 +902,19 @@ __global__ void k_darcy_gv(const double *v, const double *rs, const double *val,
     }
 }
 // vs = rsS * (v_R / rs - A_RD gv)   (condensed right-hand side, equilibrated)
+// (zf, when given: the same values in the sweep's front order, zf[fpos[r]])
 __global__ void k_cond_rhs(const double *v, const double *rs, const double *val, int64_t NS,
                            const int32_t *full_of_s, const int32_t *rptr, const int32_t *raslot,
-                           const int32_t *rd, const double *gv, const double *rsS, double *vs) {
+                           const int32_t *rd, const double *gv, const double *rsS, double *vs,
+                           double *zf, const int32_t *fpos) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= NS) return;
     const int64_t k = full_of_s[r];
     double s = rs ? v[k] / rs[k] : v[k];
     for (int t = rptr[r]; t < rptr[r + 1]; t++) s -= val[raslot[t]] * gv[rd[t]];
-    vs[r] = rsS[r] * s;
+    const double o = rsS[r] * s;
+    vs[r] = o;
+    if (zf) zf[fpos[r]] = o;
 }
 // z_R = csS * vs, z_D = gv + E z_R
 __global__ void k_expand(const double *vs, const double *csS, const double *gv, const double *e,
@@ -1702,9 +1706,14 @@ static int precond_apply(hmec_ctx *c, const double *v, const double *rs, double 
     hipLaunchKernelGGL(k_darcy_gv, dim3((unsigned)((cp.ND + 255) / 256)), dim3(256), 0, st, v, rs,
                        c->d.val, cp.ND, q.dglob, q.ddiag, q.dnb, q.dnbdiag, q.dnbe, q.gv);
     LAUNCH_CHECK();
+    double *zf = nullptr;
+    const int32_t *fpos = nullptr;
+    sv_sweep_rhs_target(c, c->factor_prec, &zf, &fpos);
     hipLaunchKernelGGL(k_cond_rhs, dim3((unsigned)((cp.NS + 255) / 256)), dim3(256), 0, st, v, rs,
-                       c->d.val, cp.NS, q.full_of_s, q.rptr, q.raslot, q.rd, q.gv, q.rs, q.vs);
+                       c->d.val, cp.NS, q.full_of_s, q.rptr, q.raslot, q.rd, q.gv, q.rs, q.vs, zf,
+                       fpos);
     LAUNCH_CHECK();
+    if (zf) sv_sweep_rhs_given(c);
     HMEC_TRY(solve_condensed(c, q.vs));
     hipLaunchKernelGGL(k_expand, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, st, q.vs, q.cs,
                        q.gv, q.e, c->N, q.s_of_full, q.d_of_full, q.dec, z);

@@ -20,11 +20,18 @@
 // nonzero entry once with 16-byte vector loads.
 //
 // A sweep is one launch per level and direction; a workgroup owns a block of
-// rows of one front, gathers the front's input vector into LDS once, and a
+// rows of one front, loads the front's input vector into LDS once, and a
 // group of L lanes (L chosen per level from the row length) forms each row's
-// dot product, reduced by a fixed butterfly.  The forward gathers the
-// children's contributions through a precomputed parent-row -> child-entry
-// map (children in a fixed order).
+// dot product, reduced by a fixed butterfly.  Forward, every input is
+// contiguous and in front order, so a level's loads depend only on its task:
+// the right-hand side at the fronts' pivots arrives in front order (zf,
+// written by the operator that forms it), and a child stores each
+// contribution row into one of two per-row slots of its parent's input (slot
+// 0: the first child, in plan order, that reaches the row; a slot no child
+// fills stays 0), which the parent adds.  Backward, a front reads its update
+// rows' values from z through their dofs (pushing them into the children's
+// inputs instead measured slower: its extra stores cost more than the index
+// load they save).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -79,10 +86,9 @@ struct SwTask {
     int kids;        // the front has children (forward: contributions to gather)
 };
 
-// where a front row's input comes from: z[zidx] (the row's global dof) and,
-// for fronts with children, the contributions wbuf[w.x], wbuf[w.y] of at most
-// two children (-1: none), in separate arrays so that rows without children
-// read 4 bytes
+// Per front row (indexed from the front's goff): zidx, the row's dof in the
+// condensed system; fdest, where a contribution row (r >= p) stores its value
+// in the parent's slots (-1: no parent).
 
 struct SwTile {
     int f;
@@ -217,27 +223,36 @@ template <> struct Vec16<double> {
 // out[r] = Bm[r,:] x -> z at the front's pivot dofs.
 // Dynamic LDS: xs (K entries padded to a multiple of 4, zero padded) then,
 // forward only, the gathered b of the block's rows.
-// Row descriptors of one pass of a lane group: RB rows q0, q0 + NG, ...
+// Row descriptors of one pass of a lane group: RB rows q0, q0 + NG, ...,
+// with the store targets of the rows (loaded with the descriptors, so the
+// stores do not wait on an index load)
 template <typename T, int DIR, int RB, int NG>
 struct SwPass {
     typedef typename Vec16<T>::type VT;
     int cb[RB], ce[RB];
     const VT *row[RB];
     int cmin, cmax;
-    __device__ __forceinline__ void setup(const SwTask &t, const T *base, int q0) {
+    int zdst[RB];        // backward: dof of the row
+    int fdst[RB];        // forward: parent slot entry of a contribution row
+    __device__ __forceinline__ void setup(const SwTask &t, const T *base, int q0,
+                                          const int32_t *zi, const int32_t *fd) {
         cmin = 1 << 30;
         cmax = 0;
 #pragma unroll
         for (int u = 0; u < RB; u++) {
             const int q = q0 + u * NG;
             const int r = t.r0 + q;
+            fdst[u] = -1;
+            zdst[u] = 0;
             if (q < t.nr) {
                 if (DIR == 0) {
                     cb[u] = 0;
                     ce[u] = r < t.p ? r : t.p;      // Fm rows < p: strictly lower
+                    if (r >= t.p) fdst[u] = fd[r];
                 } else {
                     cb[u] = r & ~3;                 // Bm: upper in the pivot columns
                     ce[u] = t.n;
+                    zdst[u] = zi[r];
                 }
             } else {
                 cb[u] = 0;
@@ -258,20 +273,28 @@ struct SwPass {
     }
 };
 
-// DIR 0 (forward, leaves -> root): x = b1 gathered from z (scaled right-hand
-// side at the front's pivot dofs) and the children's contributions;
-// out[r] = b[r] + Fm[r,:] x -> per-front vector (y1 rows < p, contribution c
-// rows >= p).
-// DIR 1 (backward, root -> leaves): x = [y1; z(update dofs)],
-// out[r] = Bm[r,:] x -> z at the front's pivot dofs.
+// Sweep state shared by the level launches of a context
+struct SwVec {
+    const int32_t *zidx;
+    const int32_t *fdest;
+    const double *zf;    // right-hand side at the fronts' pivots, front order
+    double *z;           // backward: the ancestors' values in, the solution out (condensed dofs)
+    double *wbuf;        // forward y1 (pivot rows) of every front
+    double *xin;         // forward contribution slots: [0, varena) and [varena, 2 varena);
+                         // slots no child fills stay 0
+    int64_t varena;
+};
+
+// DIR 0 (forward, leaves -> root): x = b1 (right-hand side at the front's
+// pivots plus the children's contribution slots); out[r] = b[r] + Fm[r,:] x:
+// rows < p (y1) -> wbuf, rows >= p (the contribution c) -> the parent's slot.
+// DIR 1 (backward, root -> leaves): x = [y1; x2] (x2: the ancestors' values
+// at the update rows' dofs); out[r] = Bm[r,:] x -> z at the row's dof.
 // Dynamic LDS: xs (K entries padded to a multiple of 4, zero padded) then,
-// forward only, the gathered b of the block's rows.  The first pass's first
-// column chunk is loaded before the gather, so the matrix loads overlap the
-// task -> source record -> vector chain.
+// forward only, the b of the block's rows.  The first pass's first column
+// chunk is loaded before the input, so the matrix loads overlap it.
 template <typename T, int DIR, int L, int NT>
-__global__ void __launch_bounds__(NT) k_sweep(const SwTask *tasks, const T *sa,
-                                              const int32_t *zidx, const int2 *wmap, double *z,
-                                              double *wbuf) {
+__global__ void __launch_bounds__(NT) k_sweep(const SwTask *tasks, const T *sa, SwVec sv) {
     extern __shared__ double xs[];
     typedef Vec16<T> V;
     typedef typename V::type VT;
@@ -282,29 +305,27 @@ __global__ void __launch_bounds__(NT) k_sweep(const SwTask *tasks, const T *sa,
     const int tid = threadIdx.x;
     const int lane = tid % L, grp = tid / L;
     const T *base = sa + t.moff;
+    const int32_t *zi = sv.zidx + t.goff;
+    const int32_t *fd = sv.fdest + t.goff;
     SwPass<T, DIR, RB, NG> ps;
-    ps.setup(t, base, grp);
+    ps.setup(t, base, grp, zi, fd);
     int c = (ps.cmin < ps.cmax ? ps.cmin : 0) + V::W * lane;
     VT v0[RB], v1[RB];
     ps.load(c, v0);
     ps.load(c + STEP, v1);
 
-    const int32_t *zi = zidx + t.goff;
-    const int2 *wm = wmap + t.goff;
+    const double *x0 = sv.xin + t.voff, *x1 = sv.xin + sv.varena + t.voff;
     const int K = DIR == 0 ? t.p : t.n;
     const int Kp = (K + 3) & ~3;
     for (int k = tid; k < Kp; k += NT) {
         double v = 0.0;
         if (k < K) {
             if (DIR == 0) {
-                v = z[zi[k]];
-                if (t.kids) {
-                    const int2 w = wm[k];
-                    if (w.x >= 0) v += wbuf[w.x];
-                    if (w.y >= 0) v += wbuf[w.y];
-                }
+                // a slot no child fills holds 0 (set once with the plan)
+                v = sv.zf[t.voff + k];
+                if (t.kids) v = (v + x0[k]) + x1[k];
             } else {
-                v = k < t.p ? wbuf[t.voff + k] : z[zi[k]];
+                v = k < t.p ? sv.wbuf[t.voff + k] : sv.z[zi[k]];
             }
         }
         xs[k] = v;
@@ -314,18 +335,14 @@ __global__ void __launch_bounds__(NT) k_sweep(const SwTask *tasks, const T *sa,
         for (int q = tid; q < t.nr; q += NT) {
             const int r = t.r0 + q;
             double v = 0.0;
-            if (r >= t.p && t.kids) {
-                const int2 w = wm[r];
-                if (w.x >= 0) v += wbuf[w.x];
-                if (w.y >= 0) v += wbuf[w.y];
-            }
+            if (r >= t.p && t.kids) v = (v + x0[r]) + x1[r];
             bi[q] = v;
         }
     }
     __syncthreads();
     for (int q0 = grp; q0 < t.nr; q0 += NG * RB) {
         if (q0 != grp) {
-            ps.setup(t, base, q0);
+            ps.setup(t, base, q0, zi, fd);
             c = (ps.cmin < ps.cmax ? ps.cmin : 0) + V::W * lane;
             ps.load(c, v0);
             ps.load(c + STEP, v1);
@@ -368,13 +385,22 @@ __global__ void __launch_bounds__(NT) k_sweep(const SwTask *tasks, const T *sa,
                 const int r = t.r0 + q;
                 if (DIR == 0) {
                     const double b = r < t.p ? xs[r] : bi[q];
-                    wbuf[t.voff + r] = b + acc[u];
+                    const double out = b + acc[u];
+                    if (r < t.p) sv.wbuf[t.voff + r] = out;
+                    else if (ps.fdst[u] >= 0) sv.xin[ps.fdst[u]] = out;
                 } else {
-                    z[zi[r]] = acc[u];
+                    sv.z[ps.zdst[u]] = acc[u];
                 }
             }
         }
     }
+}
+
+// zf[fpos[r]] = z[r]: the right-hand side in front order (when the operator
+// that formed z did not write it)
+__global__ void k_front_rhs(const double *z, const int32_t *fpos, int64_t NS, double *zf) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < NS) zf[fpos[r]] = z[r];
 }
 
 // ------------------------------------------------------------ host state
@@ -383,7 +409,11 @@ struct SweepDev {
     SwFront *fronts = nullptr;
     std::vector<SwFront> h_fronts;
     int32_t *zidx = nullptr;
-    int2 *wmap = nullptr;
+    int32_t *fdest = nullptr;
+    int32_t *fpos = nullptr;                     // condensed dof -> its pivot row in front order
+    double *zf = nullptr;                        // right-hand side in front order
+    double *xin = nullptr;                       // forward contribution slots (2 x varena)
+    bool zf_given = false;                       // the caller wrote zf for the next solve
     SwTile *tiles = nullptr;
     int ntiles = 0;
     std::vector<SwTask *> d_fwd, d_bwd;
@@ -502,33 +532,56 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
         hmec_set_last_error("sweep: vector arena exceeds 32-bit gather indices");
         return HMEC_E_ARG;
     }
-    // row sources: the row's global dof and the child contribution entries
-    // that land on it (children in plan order; at most two in a binary
-    // dissection tree)
+    // row maps: the row's dof; the forward contribution slots (a parent row
+    // takes at most two children's rows, slot 0 for the first child in plan
+    // order) and where each child row stores into them; the children's
+    // backward input entries each row feeds
     int inv[6];
     for (int v = 0; v < dpn; v++) inv[hmec_var_pos(dpn, v)] = v;
-    std::vector<int32_t> zidx(goff);
-    std::vector<int2> wmap(goff, make_int2(-1, -1));
+    const int64_t va = pl.varena;
+    std::vector<int32_t> zidx(goff), fdest(goff, -1), fpos(c->cp.NS, -1);
+    std::vector<uint8_t> fmask(goff, 0);
     for (int f = 0; f < nf; f++) {
         const SwFront &F = s->h_fronts[f];
         for (int r = 0; r < F.n; r++) {
             const int node = pl.nodes[f][r / dpn];
             zidx[F.goff + r] = (int32_t)(node * dpn + inv[r % dpn]);
         }
+        for (int r = 0; r < F.p; r++) {
+            const int32_t d = zidx[F.goff + r];
+            if (d < 0 || d >= c->cp.NS || fpos[d] >= 0) {
+                hmec_set_last_error("sweep: a condensed dof is not the pivot of exactly one front");
+                return HMEC_E_ARG;
+            }
+            fpos[d] = (int32_t)(F.voff + r);
+        }
         for (int ch : pl.children[f]) {
+            const SwFront &Ch = s->h_fronts[ch];
             const int cn = dpn * (pl.np[ch] + pl.nu[ch]), cp = dpn * pl.np[ch];
             const int32_t *map = pl.ea_map.data() + pl.ea_off[ch];
             for (int a = 0; a < cn - cp; a++) {
-                int2 &e = wmap[F.goff + map[a / dpn] * dpn + a % dpn];
-                const int32_t w = (int32_t)(pl.voff[ch] + cp + a);
-                if (e.x < 0) e.x = w;
-                else if (e.y < 0) e.y = w;
+                const int R = map[a / dpn] * dpn + a % dpn;   // parent row of child row cp + a
+                uint8_t &m = fmask[F.goff + R];
+                int slot;
+                if (!(m & 1)) slot = 0;
+                else if (!(m & 2)) slot = 1;
                 else {
                     hmec_set_last_error("sweep: a front row receives from more than two children");
                     return HMEC_E_ARG;
                 }
+                m |= (uint8_t)(1 << slot);
+                fdest[Ch.goff + cp + a] = (int32_t)(slot * va + F.voff + R);
             }
         }
+    }
+    for (int64_t d = 0; d < c->cp.NS; d++)
+        if (fpos[d] < 0) {
+            hmec_set_last_error("sweep: a condensed dof is the pivot of no front");
+            return HMEC_E_ARG;
+        }
+    if (2 * va > INT32_MAX) {
+        hmec_set_last_error("sweep: contribution slots exceed 32-bit indices");
+        return HMEC_E_ARG;
     }
     // sweep-form tiles
     std::vector<SwTile> tiles;
@@ -548,7 +601,11 @@ static int sweep_setup(hmec_ctx *c, SweepDev *s) {
     s->ntiles = (int)tiles.size();
     HMEC_TRY(upload(&s->fronts, s->h_fronts));
     HMEC_TRY(upload(&s->zidx, zidx));
-    HMEC_TRY(upload(&s->wmap, wmap));
+    HMEC_TRY(upload(&s->fdest, fdest));
+    HMEC_TRY(upload(&s->fpos, fpos));
+    HMEC_HIP(hipMalloc(&s->zf, sizeof(double) * std::max<int64_t>(1, va)));
+    HMEC_HIP(hipMalloc(&s->xin, sizeof(double) * std::max<int64_t>(1, 2 * va)));
+    HMEC_HIP(hipMemset(s->xin, 0, sizeof(double) * std::max<int64_t>(1, 2 * va)));
     HMEC_TRY(upload(&s->tiles, tiles));
     // tasks per level (the LDS need is the same for both precisions)
     s->d_fwd.assign(pl.nlevels, nullptr);
@@ -762,11 +819,9 @@ static int launch_sweep(hmec_ctx *c, SweepDev *s, int lv, const T *sa, double *z
     const int lds = DIR == 0 ? s->lds_fwd[lv] : s->lds_bwd[lv];
     const SwTask *tk = DIR == 0 ? s->d_fwd[lv] : s->d_bwd[lv];
     hipStream_t st = c->stream;
-    double *wbuf = c->d.vec;
+    const SwVec sv{s->zidx, s->fdest, s->zf, z, c->d.vec, s->xin, c->plan.varena};
 #define SW_GO(LN, NTH)                                                                          \
-    hipLaunchKernelGGL((k_sweep<T, DIR, LN, NTH>), dim3(n), dim3(NTH), lds, st, tk, sa, s->zidx, \
-                       s->wmap, z, \
-                       wbuf)
+    hipLaunchKernelGGL((k_sweep<T, DIR, LN, NTH>), dim3(n), dim3(NTH), lds, st, tk, sa, sv)
     if (nth == 64) {
         switch (lanes) {
             case 4: SW_GO(4, 64); break;
@@ -787,6 +842,20 @@ static int launch_sweep(hmec_ctx *c, SweepDev *s, int lv, const T *sa, double *z
     return HMEC_OK;
 }
 
+// Where the operator forming the next sweep's right-hand side may also store
+// it in front order (zf[fpos[r]] = z[r]); null when the sweep form of that
+// precision is not the active solver.  The next sv_sweep_solve then skips its
+// own reordering.
+void sv_sweep_rhs_target(hmec_ctx *c, int precision, double **zf, const int32_t **fpos) {
+    *zf = nullptr;
+    *fpos = nullptr;
+    if (!sv_sweep_ready(c, precision)) return;
+    SweepDev *s = swdev(c);
+    *zf = s->zf;
+    *fpos = s->fpos;
+}
+void sv_sweep_rhs_given(hmec_ctx *c) { swdev(c)->zf_given = true; }
+
 // z (scaled right-hand side) -> solution of the scaled system
 int sv_sweep_solve(hmec_ctx *c, double *z) {
     SweepDev *s = swdev(c);
@@ -803,6 +872,13 @@ int sv_sweep_solve(hmec_ctx *c, double *z) {
     }
     const double *sa64 = (const double *)s->sa[1];
     const float *sa32 = q ? nullptr : (const float *)s->slots[s->cur].sa;
+    if (!s->zf_given) {
+        const int64_t NS = c->cp.NS;
+        hipLaunchKernelGGL(k_front_rhs, dim3((unsigned)((NS + 255) / 256)), dim3(256), 0, c->stream,
+                           (const double *)z, (const int32_t *)s->fpos, NS, s->zf);
+        SW_LAUNCH_CHECK();
+    }
+    s->zf_given = false;
     kt_begin(c, "sweep_fwd");
     for (int lv = 0; lv < pl.nlevels; lv++) {
         if (q) HMEC_TRY((launch_sweep<double, 0>(c, s, lv, sa64, z)));
@@ -829,7 +905,10 @@ void sv_sweep_free(hmec_ctx *c) {
     }
     hipFree(s->fronts);
     hipFree(s->zidx);
-    hipFree(s->wmap);
+    hipFree(s->fdest);
+    hipFree(s->fpos);
+    hipFree(s->zf);
+    hipFree(s->xin);
     hipFree(s->tiles);
     for (auto *p : s->d_fwd) hipFree(p);
     for (auto *p : s->d_bwd) hipFree(p);
