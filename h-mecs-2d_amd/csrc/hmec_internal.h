@@ -178,6 +178,8 @@ struct CondDev {
     int32_t *rptr = nullptr, *raslot = nullptr, *rd = nullptr;
     int32_t *tptr = nullptr, *taslot = nullptr, *tdk = nullptr, *tsslot = nullptr;
     double *e = nullptr;                     // 3 x ND: E of the current matrix
+    double *dcoef = nullptr;                 // 3 x ND: a_dd, a_nn, a_dn of the current matrix
+    double *rcoef = nullptr;                 // A_RD entries of the right-hand-side terms
     double *gv = nullptr;                    // ND: A_DD^-1 v_D of the current vector
     double *vs = nullptr;                    // NS: condensed right-hand side / solution
     double *rs = nullptr, *cs = nullptr;     // NS: equilibration of the active S factor

@@ -854,12 +854,17 @@ __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb,
 
 // ------------------------------------------------------------ Darcy condensation
 // E = -A_DD^-1 A_DR of the current matrix (3 entries per Darcy unknown)
+// (also the Darcy coefficients the preconditioner applications read,
+// dcoef[3d..3d+3) = a_dd, a_nn, a_dn, gathered once per matrix)
 __global__ void k_darcy_e(const double *val, int64_t ND, const int32_t *ddiag,
                           const int32_t *dnbdiag, const int32_t *dnbe, const int32_t *des,
-                          double *e) {
+                          double *e, double *dcoef) {
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ND) return;
     const double add = val[ddiag[d]];
+    dcoef[3 * d] = add;
+    dcoef[3 * d + 1] = dnbe[d] < 0 ? 0.0 : val[dnbdiag[d]];
+    dcoef[3 * d + 2] = dnbe[d] < 0 ? 0.0 : val[dnbe[d]];
     if (dnbe[d] < 0) {
         for (int k = 0; k < 3; k++) {
             const int32_t p = des[3 * d + k];
@@ -885,33 +890,38 @@ __global__ void k_condense_matrix(const double *val, int64_t NS, const int32_t *
     for (int t = tptr[r]; t < tptr[r + 1]; t++) sval[tsslot[t]] += val[taslot[t]] * e[tdk[t]];
 }
 // gv = A_DD^-1 (v_D / rs)   (rs: optional exact power-of-two row scaling)
-__global__ void k_darcy_gv(const double *v, const double *rs, const double *val, int64_t ND,
-                           const int32_t *dglob, const int32_t *ddiag, const int32_t *dnb,
-                           const int32_t *dnbdiag, const int32_t *dnbe, double *gv) {
+__global__ void k_darcy_gv(const double *v, const double *rs, int64_t ND, const int32_t *dglob,
+                           const int32_t *dnb, const int32_t *dnbe, const double *dcoef,
+                           double *gv) {
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ND) return;
     const int64_t k = dglob[d];
     const double vb = rs ? v[k] / rs[k] : v[k];
     if (dnbe[d] < 0) {
-        gv[d] = vb / val[ddiag[d]];
+        gv[d] = vb / dcoef[3 * d];
     } else {
         const int64_t n = dnb[d];
         const double vn = rs ? v[n] / rs[n] : v[n];
-        const double gn = vn / val[dnbdiag[d]];
-        gv[d] = (vb - val[dnbe[d]] * gn) / val[ddiag[d]];
+        const double gn = vn / dcoef[3 * d + 1];
+        gv[d] = (vb - dcoef[3 * d + 2] * gn) / dcoef[3 * d];
     }
+}
+// rcoef[t] = the A_RD entry of right-hand-side term t (once per matrix)
+__global__ void k_cond_coef(const double *val, const int32_t *raslot, int64_t nt, double *rcoef) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nt) rcoef[t] = val[raslot[t]];
 }
 // vs = rsS * (v_R / rs - A_RD gv)   (condensed right-hand side, equilibrated)
 // (zf, when given: the same values in the sweep's front order, zf[fpos[r]])
-__global__ void k_cond_rhs(const double *v, const double *rs, const double *val, int64_t NS,
-                           const int32_t *full_of_s, const int32_t *rptr, const int32_t *raslot,
-                           const int32_t *rd, const double *gv, const double *rsS, double *vs,
-                           double *zf, const int32_t *fpos) {
+__global__ void k_cond_rhs(const double *v, const double *rs, const double *rcoef, int64_t NS,
+                           const int32_t *full_of_s, const int32_t *rptr, const int32_t *rd,
+                           const double *gv, const double *rsS, double *vs, double *zf,
+                           const int32_t *fpos) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= NS) return;
     const int64_t k = full_of_s[r];
     double s = rs ? v[k] / rs[k] : v[k];
-    for (int t = rptr[r]; t < rptr[r + 1]; t++) s -= val[raslot[t]] * gv[rd[t]];
+    for (int t = rptr[r]; t < rptr[r + 1]; t++) s -= rcoef[t] * gv[rd[t]];
     const double o = rsS[r] * s;
     vs[r] = o;
     if (zf) zf[fpos[r]] = o;
@@ -997,6 +1007,39 @@ struct GmCoef {
     double v[32];
 };
 // w -= sum_i h_i V_i, coefficients from device memory (no host round trip)
+// First Gram-Schmidt update with the second pass's dot products:
+// w' = w - V h (h = h[0..nv)), and per workgroup the partial sums of
+// V_i . w' (i < nv) and w' . w' -> part[block * (nv + 1) + i]
+__global__ void __launch_bounds__(256) k_gs_update_dots(const double *V, int64_t ldv, int nv,
+                                                        const double *h, double *w, int64_t N,
+                                                        double *part) {
+    __shared__ double hs[GM_MAXV];
+    __shared__ double red[4][GM_MAXV + 1];
+    if ((int)threadIdx.x < nv) hs[threadIdx.x] = h[threadIdx.x];
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (t < N) {
+        s = w[t];
+        for (int i = 0; i < nv; i++) s -= hs[i] * V[(int64_t)i * ldv + t];
+        w[t] = s;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = 0; i <= nv; i++) {
+        double d = t < N ? (i < nv ? V[(int64_t)i * ldv + t] : s) * s : 0.0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
+        if (lane == 0) red[wv][i] = d;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x <= nv) {
+        double d = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) d += red[q][threadIdx.x];
+        part[(int64_t)blockIdx.x * (nv + 1) + threadIdx.x] = d;
+    }
+}
+
 // Second Gram-Schmidt pass and normalization: w'' = w' - V h (h = h[0..nv),
 // ||w'||^2 = h[nv]); ||w''||^2 = ||w'||^2 - ||h||^2 (V orthonormal, h the
 // small second-pass coefficients) -> *ww; y = w'' / ||w''|| when y is given.
@@ -1112,7 +1155,7 @@ int sv_free_device(hmec_ctx *c) {
                           (void *)q.dnbe, (void *)q.des, (void *)q.dec, (void *)q.rptr,
                           (void *)q.raslot, (void *)q.rd, (void *)q.tptr, (void *)q.taslot,
                           (void *)q.tdk, (void *)q.tsslot, (void *)q.e, (void *)q.gv, (void *)q.vs,
-                          (void *)q.rs, (void *)q.cs})
+                          (void *)q.rs, (void *)q.cs, (void *)q.dcoef, (void *)q.rcoef})
             hipFree(ptr);
         q = CondDev();
     }
@@ -1374,6 +1417,8 @@ int sv_setup_device(hmec_ctx *c) {
     HMEC_TRY(upload(&q.tsslot, cp.tsslot));
     HMEC_HIP(hipMalloc(&q.sval, sizeof(double) * std::max<int64_t>(1, nnz)));
     HMEC_HIP(hipMalloc(&q.e, sizeof(double) * 3 * std::max<int64_t>(1, cp.ND)));
+    HMEC_HIP(hipMalloc(&q.dcoef, sizeof(double) * 3 * std::max<int64_t>(1, cp.ND)));
+    HMEC_HIP(hipMalloc(&q.rcoef, sizeof(double) * std::max<int64_t>(1, cp.rptr.empty() ? 1 : (int64_t)cp.rptr.back())));
     HMEC_HIP(hipMalloc(&q.gv, sizeof(double) * std::max<int64_t>(1, cp.ND)));
     HMEC_HIP(hipMalloc(&q.vs, sizeof(double) * std::max<int64_t>(1, N)));
     HMEC_HIP(hipMalloc(&q.rs, sizeof(double) * std::max<int64_t>(1, N)));
@@ -1691,8 +1736,14 @@ static int darcy_e(hmec_ctx *c, hipStream_t st) {
     const CondPlan &cp = c->cp;
     CondDev &q = c->dc;
     hipLaunchKernelGGL(k_darcy_e, dim3((unsigned)((cp.ND + 255) / 256)), dim3(256), 0, st, c->d.val,
-                       cp.ND, q.ddiag, q.dnbdiag, q.dnbe, q.des, q.e);
+                       cp.ND, q.ddiag, q.dnbdiag, q.dnbe, q.des, q.e, q.dcoef);
     LAUNCH_CHECK();
+    const int64_t nt = cp.rptr.empty() ? 0 : cp.rptr.back();
+    if (nt > 0) {
+        hipLaunchKernelGGL(k_cond_coef, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st,
+                           c->d.val, q.raslot, nt, q.rcoef);
+        LAUNCH_CHECK();
+    }
     return HMEC_OK;
 }
 
@@ -1704,14 +1755,13 @@ static int precond_apply(hmec_ctx *c, const double *v, const double *rs, double 
     CondDev &q = c->dc;
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(k_darcy_gv, dim3((unsigned)((cp.ND + 255) / 256)), dim3(256), 0, st, v, rs,
-                       c->d.val, cp.ND, q.dglob, q.ddiag, q.dnb, q.dnbdiag, q.dnbe, q.gv);
+                       cp.ND, q.dglob, q.dnb, q.dnbe, q.dcoef, q.gv);
     LAUNCH_CHECK();
     double *zf = nullptr;
     const int32_t *fpos = nullptr;
     sv_sweep_rhs_target(c, c->factor_prec, &zf, &fpos);
     hipLaunchKernelGGL(k_cond_rhs, dim3((unsigned)((cp.NS + 255) / 256)), dim3(256), 0, st, v, rs,
-                       c->d.val, cp.NS, q.full_of_s, q.rptr, q.raslot, q.rd, q.gv, q.rs, q.vs, zf,
-                       fpos);
+                       q.rcoef, cp.NS, q.full_of_s, q.rptr, q.rd, q.gv, q.rs, q.vs, zf, fpos);
     LAUNCH_CHECK();
     if (zf) sv_sweep_rhs_given(c);
     HMEC_TRY(solve_condensed(c, q.vs));
@@ -1938,11 +1988,10 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
             LAUNCH_CHECK();
             hipLaunchKernelGGL(k_dots_final, dim3(j + 1), dim3(256), 0, st, g.part, (int)nbs, j + 1, h1d);
             LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_gs_update, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h1d, g.w, N);
+            hipLaunchKernelGGL(k_gs_update_dots, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h1d, g.w,
+                               N, g.part);
             LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_dots_part, dim3(RB), dim3(256), 0, st, g.V, N, j + 1, g.w, N, g.part, 1);
-            LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_dots_final, dim3(j + 2), dim3(256), 0, st, g.part, RB, j + 2, h2d);
+            hipLaunchKernelGGL(k_dots_final, dim3(j + 2), dim3(256), 0, st, g.part, (int)gN, j + 2, h2d);
             LAUNCH_CHECK();
             hipLaunchKernelGGL(k_gs_update_norm, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h2d, g.w,
                                j + 1 < m ? g.V + (int64_t)(j + 1) * N : nullptr, wwd, N);
