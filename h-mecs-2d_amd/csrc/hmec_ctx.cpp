@@ -426,6 +426,7 @@ extern "C" int hmec_set_csr_values(hmec_ctx *c, const double *values, const doub
     if (values) HMEC_HIP(hipMemcpy(c->d.val, values, sizeof(double) * c->nnz, hipMemcpyHostToDevice));
     if (rhs) HMEC_HIP(hipMemcpy(c->d.rhs, rhs, sizeof(double) * c->N, hipMemcpyHostToDevice));
     c->factor_valid = false;
+    if (values) c->rscale_ready = false;
     return HMEC_OK;
 }
 

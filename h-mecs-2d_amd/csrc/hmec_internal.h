@@ -232,6 +232,7 @@ struct hmec_ctx {
     double timesum = 0, dt = 0, dt00 = 0, dtx = 0, dty = 0, dtlapusta = 0, Vmax = 0, maxvxy = 0;
     double ptscale = 0;
     int64_t asm_lds_entries = 0;     // largest CSR range of an assembly workgroup
+    bool rscale_ready = false;       // d.rscale holds the current values' row scales
     std::vector<double> DSYLSQ;
     bool markers_dirty = true;       // cell keys / sort need recomputing
     bool have_markers = false;

@@ -15,6 +15,7 @@
 
 #define HMEC_HD __host__ __device__
 #include "assemble_emit.h"
+#include "pow2_scale.h"
 #include "hmec_internal.h"
 
 // device scalar slots (ctx->d.scal)
@@ -743,7 +744,8 @@ constexpr int ASM_T = 64;
 __global__ void __launch_bounds__(ASM_T)
 k_assemble(AsmIn in, const double *__restrict__ scal, const int32_t *__restrict__ rowp,
            const int32_t *__restrict__ node_trip, const int32_t *__restrict__ slot,
-           double *__restrict__ val, double *__restrict__ R, int Nx1, int Ny1) {
+           double *__restrict__ val, double *__restrict__ R, double *__restrict__ rscale, int Nx1,
+           int Ny1) {
     extern __shared__ double lv[];
     __shared__ double lr[6 * ASM_T];
     const int tid = threadIdx.x;
@@ -762,6 +764,16 @@ k_assemble(AsmIn in, const double *__restrict__ scal, const int32_t *__restrict_
     __syncthreads();
     for (int p = tid; p < ne; p += ASM_T) val[base + p] = lv[p];
     for (int r = tid; r < 6 * nn; r += ASM_T) R[6 * n0 + r] = lr[r];
+    // power-of-two row scales of the node's six rows (the solver's equilibration)
+    if (tid < nn) {
+        const int64_t k0 = 6 * (int64_t)(n0 + tid);
+#pragma unroll
+        for (int v = 0; v < 6; v++) {
+            double m = 0;
+            for (int p = rowp[k0 + v]; p < rowp[k0 + v + 1]; p++) m = fmax(m, fabs(lv[p - base]));
+            rscale[k0 + v] = pow2_recip(m);
+        }
+    }
 }
 
 int k_assemble(hmec_ctx *c, double dt) {
@@ -783,9 +795,10 @@ int k_assemble(hmec_ctx *c, double dt) {
         return HMEC_E_ARG;
     }
     hipLaunchKernelGGL(k_assemble, dim3(nblk), dim3(ASM_T), lds, c->stream, in, c->d.scal,
-                       c->d.rowp, c->d.node_trip, c->d.trip_slot, c->d.val, c->d.rhs, c->Nx1,
-                       c->Ny1);
+                       c->d.rowp, c->d.node_trip, c->d.trip_slot, c->d.val, c->d.rhs, c->d.rscale,
+                       c->Nx1, c->Ny1);
     LAUNCH_CHECK();
+    c->rscale_ready = true;
     return HMEC_OK;
 }
 

@@ -24,6 +24,7 @@
 
 #include "hmec_internal.h"
 #include "mfma_traits.h"
+#include "pow2_scale.h"
 
 
 namespace {
@@ -59,13 +60,6 @@ struct FacPtrs {
 };
 
 // ------------------------------------------------------------ equilibration
-__device__ __forceinline__ double pow2_recip(double m) {
-    // 2^-e with m in [2^(e-1), 2^e): a power of two near 1/m (exact scaling)
-    if (!(m > 0) || !isfinite(m)) return 1.0;
-    int e;
-    frexp(m, &e);
-    return ldexp(1.0, -e + 1);
-}
 
 __global__ void k_row_scale(const int32_t *rowp, const double *val, int64_t N, double *rs) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -804,50 +798,61 @@ static void launch_spmv_rs(hmec_ctx *c, const double *x, double *y) {
 // block over them); a NaN anywhere propagates to the result
 constexpr int NRM_BLOCKS = 256;
 __device__ __forceinline__ double nan_max(double m, double v) { return (v > m || v != v) ? v : m; }
-__global__ void __launch_bounds__(256) k_norms_part(const double *a, const double *b, int64_t N,
-                                                    double *part) {
-    __shared__ double ra[256], rb[256];
-    double ma = 0, mb = 0;
+// Inf-norms of NV vectors (x[0..NV)) in two stages: block maxima, then one
+// block over them; a NaN anywhere propagates to the result
+template <int NV>
+struct NormArgs {
+    const double *x[NV];
+};
+template <int NV>
+__global__ void __launch_bounds__(256) k_norms_part(NormArgs<NV> a, int64_t N, double *part) {
+    __shared__ double r[NV][256];
+    double m[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) m[q] = 0;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < N;
          t += (int64_t)gridDim.x * blockDim.x) {
-        ma = nan_max(ma, fabs(a[t]));
-        mb = nan_max(mb, fabs(b[t]));
+#pragma unroll
+        for (int q = 0; q < NV; q++) m[q] = nan_max(m[q], fabs(a.x[q][t]));
     }
-    ra[threadIdx.x] = ma;
-    rb[threadIdx.x] = mb;
+#pragma unroll
+    for (int q = 0; q < NV; q++) r[q][threadIdx.x] = m[q];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
-            ra[threadIdx.x] = nan_max(ra[threadIdx.x], ra[threadIdx.x + s]);
-            rb[threadIdx.x] = nan_max(rb[threadIdx.x], rb[threadIdx.x + s]);
+#pragma unroll
+            for (int q = 0; q < NV; q++) r[q][threadIdx.x] = nan_max(r[q][threadIdx.x], r[q][threadIdx.x + s]);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = ra[0];
-        part[2 * blockIdx.x + 1] = rb[0];
+#pragma unroll
+        for (int q = 0; q < NV; q++) part[NV * blockIdx.x + q] = r[q][0];
     }
 }
+template <int NV>
 __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb, double *out) {
-    __shared__ double ra[256], rb[256];
-    double ma = 0, mb = 0;
+    __shared__ double r[NV][256];
+    double m[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) m[q] = 0;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) {
-        ma = nan_max(ma, part[2 * k]);
-        mb = nan_max(mb, part[2 * k + 1]);
+#pragma unroll
+        for (int q = 0; q < NV; q++) m[q] = nan_max(m[q], part[NV * k + q]);
     }
-    ra[threadIdx.x] = ma;
-    rb[threadIdx.x] = mb;
+#pragma unroll
+    for (int q = 0; q < NV; q++) r[q][threadIdx.x] = m[q];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
-            ra[threadIdx.x] = nan_max(ra[threadIdx.x], ra[threadIdx.x + s]);
-            rb[threadIdx.x] = nan_max(rb[threadIdx.x], rb[threadIdx.x + s]);
+#pragma unroll
+            for (int q = 0; q < NV; q++) r[q][threadIdx.x] = nan_max(r[q][threadIdx.x], r[q][threadIdx.x + s]);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[0] = ra[0];
-        out[1] = rb[0];
+#pragma unroll
+        for (int q = 0; q < NV; q++) out[q] = r[q][0];
     }
 }
 
@@ -1426,7 +1431,7 @@ int sv_setup_device(hmec_ctx *c) {
     // arenas
     HMEC_HIP(hipMalloc(&c->d.vec, sizeof(double) * std::max<int64_t>(1, pl.varena)));
     HMEC_HIP(hipMalloc(&s->norms, sizeof(double) * 4));
-    HMEC_HIP(hipMalloc(&s->nrm_part, sizeof(double) * 2 * NRM_BLOCKS));
+    HMEC_HIP(hipMalloc(&s->nrm_part, sizeof(double) * 4 * NRM_BLOCKS));
     HMEC_HIP(hipHostMalloc(&s->h_norms, sizeof(double) * 4, hipHostMallocDefault));
     return HMEC_OK;
 }
@@ -1715,9 +1720,21 @@ static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
 // out[0..1] = max|a|, max|b| (device), two launches on the context stream
 static int launch_norms(hmec_ctx *c, const double *a, const double *b, double *out) {
     SolverDev *s = sdev(c);
-    hipLaunchKernelGGL(k_norms_part, dim3(NRM_BLOCKS), dim3(256), 0, c->stream, a, b, c->N, s->nrm_part);
+    NormArgs<2> na{{a, b}};
+    hipLaunchKernelGGL(k_norms_part<2>, dim3(NRM_BLOCKS), dim3(256), 0, c->stream, na, c->N, s->nrm_part);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(256), 0, c->stream, s->nrm_part, NRM_BLOCKS, out);
+    hipLaunchKernelGGL(k_norms_final<2>, dim3(1), dim3(256), 0, c->stream, s->nrm_part, NRM_BLOCKS, out);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+// out[0..4) = inf-norms of a, b, c2, d in one pass
+static int launch_norms4(hmec_ctx *c, const double *a, const double *b, const double *c2,
+                         const double *d, double *out) {
+    SolverDev *s = sdev(c);
+    NormArgs<4> na{{a, b, c2, d}};
+    hipLaunchKernelGGL(k_norms_part<4>, dim3(NRM_BLOCKS), dim3(256), 0, c->stream, na, c->N, s->nrm_part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_norms_final<4>, dim3(1), dim3(256), 0, c->stream, s->nrm_part, NRM_BLOCKS, out);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
@@ -1919,8 +1936,7 @@ static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *r
     hipStream_t st = c->stream;
     launch_residual(c, x, c->d.work, g.rsd, c->d.rscale);
     LAUNCH_CHECK();
-    HMEC_TRY(launch_norms(c, c->d.work, c->d.rhs, s->norms));
-    HMEC_TRY(launch_norms(c, g.rsd, g.bs, s->norms + 2));
+    HMEC_TRY(launch_norms4(c, c->d.work, c->d.rhs, g.rsd, g.bs, s->norms));
     HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 4, hipMemcpyDeviceToHost, st));
     HMEC_HIP(hipStreamSynchronize(st));
     const double *h = s->h_norms;
@@ -1953,8 +1969,11 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     double relinf = 0, last_rel_s = 0;
     std::vector<double> H((m + 1) * m), cs(m), sn(m), gv(m + 1), h(m + 2), h2(m + 2), y(m);
     // the operator's row scaling (powers of two) and E of the current matrix
-    hipLaunchKernelGGL(k_row_scale, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.val, N, c->d.rscale);
-    LAUNCH_CHECK();
+    if (!c->rscale_ready) {   // the assembly forms it with the values
+        hipLaunchKernelGGL(k_row_scale, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.val, N, c->d.rscale);
+        LAUNCH_CHECK();
+        c->rscale_ready = true;
+    }
     HMEC_TRY(darcy_e(c, st));
     hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, c->d.rhs, c->d.rscale, g.bs, N);
     LAUNCH_CHECK();
