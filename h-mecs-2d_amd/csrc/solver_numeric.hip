@@ -350,48 +350,74 @@ __device__ __forceinline__ void gemm_tile_sub(T *C, const T *A, const T *B, int6
             }
 }
 
-// trailing update of panel k0 restricted to rows < p or columns < p
-template <typename T>
+// Trailing updates of a pair of panels P0 = [k0, k0 + kb0) and
+// P1 = [k0 + kb0, k0 + kb0 + kb1) (kb1 = 0 when the front has no second
+// panel there), restricted to rows < p or columns < p (the contribution block
+// is updated once per level by k_schur):
+//   MODE 0 (after P0's panel): P0's factored diagonal block back into the
+//     front, and P0's rank-kb0 update of P1's column and row strips only, so
+//     P1 can be factored;
+//   MODE 1 (after P1's panel): P1's factored diagonal block back, and the
+//     rank-(kb0 + kb1) update of everything after P1 in one pass.
+// A front without P1 gets the plain rank-kb0 update from MODE 1.
+template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_update(const FrontTask *tasks, int k0,
                                                 const FacPtrs<T> *mem) {
     T *const arena = mem[blockIdx.z].arena;
     const T *const dscr = mem[blockIdx.z].dscr;
     const FrontTask tk = tasks[blockIdx.y];
     if (tk.p <= k0) return;
-    const int kb = min(NB, tk.p - k0);
-    if (blockIdx.x == 0) {
-        // factored diagonal block of this panel back into the front
-        const T *ds = dscr + (int64_t)blockIdx.y * NB * NB;
-        T *F0 = arena + tk.off;
-        for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
-            const int r = e % kb, c = e / kb;
-            F0[(int64_t)(k0 + c) * tk.n + k0 + r] = ds[c * NB + r];
-        }
-    }
-    const int s = k0 + kb;                    // first trailing index
-    const int tp = (tk.p - s + TM - 1) / TM;  // pivot-part tiles
-    const int u = tk.n - tk.p;
-    const int tu = (u + TM - 1) / TM;
-    const int t = blockIdx.x;
-    int rt, ct;
-    if (t < tp * (tp + tu)) {
-        rt = t / (tp + tu);
-        ct = t % (tp + tu);
-    } else {
-        const int t2 = t - tp * (tp + tu);
-        if (t2 >= tu * tp) return;
-        rt = tp + t2 / tp;
-        ct = t2 % tp;
-    }
-    int r0, m, c0, nn;
-    if (rt < tp) { r0 = s + rt * TM; m = min(TM, tk.p - r0); }
-    else { r0 = tk.p + (rt - tp) * TM; m = min(TM, tk.n - r0); }
-    if (ct < tp) { c0 = s + ct * TM; nn = min(TM, tk.p - c0); }
-    else { c0 = tk.p + (ct - tp) * TM; nn = min(TM, tk.n - c0); }
+    const int kb0 = min(NB, tk.p - k0);
+    const int kb1 = max(0, min(NB, tk.p - k0 - NB));
     T *F = arena + tk.off;
     const int64_t n = tk.n;
+    if (blockIdx.x == 0 && (MODE == 0 || kb1 > 0)) {
+        // the factored diagonal block of the panel just factored
+        const T *ds = dscr + (int64_t)blockIdx.y * NB * NB;
+        const int d0 = MODE == 0 ? k0 : k0 + kb0, kb = MODE == 0 ? kb0 : kb1;
+        for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+            const int r = e % kb, c = e / kb;
+            F[(int64_t)(d0 + c) * n + d0 + r] = ds[c * NB + r];
+        }
+    }
+    const int t = blockIdx.x;
+    int r0, m, c0, nn, K;
+    if (MODE == 0) {
+        if (kb1 == 0) return;
+        const int s1 = k0 + kb0, e1 = s1 + kb1;
+        const int tr = (tk.n - s1 + TM - 1) / TM;   // P1's column strip, rows [s1, n)
+        const int tc = (tk.n - e1 + TM - 1) / TM;   // P1's row strip, columns [e1, n)
+        if (t < tr) {
+            r0 = s1 + t * TM; m = min(TM, tk.n - r0); c0 = s1; nn = kb1;
+        } else if (t < tr + tc) {
+            r0 = s1; m = kb1; c0 = e1 + (t - tr) * TM; nn = min(TM, tk.n - c0);
+        } else {
+            return;
+        }
+        K = kb0;
+    } else {
+        const int s = k0 + kb0 + kb1;             // first trailing index
+        const int tp = (tk.p - s + TM - 1) / TM;  // pivot-part tiles
+        const int u = tk.n - tk.p;
+        const int tu = (u + TM - 1) / TM;
+        int rt, ct;
+        if (t < tp * (tp + tu)) {
+            rt = t / (tp + tu);
+            ct = t % (tp + tu);
+        } else {
+            const int t2 = t - tp * (tp + tu);
+            if (t2 >= tu * tp) return;
+            rt = tp + t2 / tp;
+            ct = t2 % tp;
+        }
+        if (rt < tp) { r0 = s + rt * TM; m = min(TM, tk.p - r0); }
+        else { r0 = tk.p + (rt - tp) * TM; m = min(TM, tk.n - r0); }
+        if (ct < tp) { c0 = s + ct * TM; nn = min(TM, tk.p - c0); }
+        else { c0 = tk.p + (ct - tp) * TM; nn = min(TM, tk.n - c0); }
+        K = kb0 + kb1;
+    }
     gemm_tile_sub<T>(F + (int64_t)c0 * n + r0, F + (int64_t)k0 * n + r0, F + (int64_t)c0 * n + k0,
-                     n, m, nn, kb);
+                     n, m, nn, K);
 }
 
 // Schur complement CB -= L21 U12 (K = p), 64 x 64 MFMA tiles
@@ -1492,9 +1518,10 @@ static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipS
                                dp.d_ea[q][lv], dp.d_ea_map, mem, pl.dpn);
             LAUNCH_CHECK();
         }
-        // pivot panels
+        // pivot panels, in pairs: panel, narrow update of the next panel's
+        // strips, next panel, one update of the rest for both
         const int maxp = dp.maxp[lv];
-        for (int k0 = 0; k0 < maxp; k0 += NB) {
+        auto panel = [&](int k0) -> int {
             int gx = 1;
             for (const auto &t : dp.h_tasks[lv]) {
                 if (t.p <= k0) continue;
@@ -1505,16 +1532,32 @@ static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipS
             hipLaunchKernelGGL(k_panel<T>, dim3(gx, nfr, Bz), dim3(PR), 0, st, dp.d_tasks[lv], k0, mem,
                                tau);
             LAUNCH_CHECK();
-            int maxt = 0;
+            return HMEC_OK;
+        };
+        for (int k0 = 0; k0 < maxp; k0 += 2 * NB) {
+            HMEC_TRY(panel(k0));
+            int maxn = 1, maxw = 1;
+            bool second = false;
             for (const auto &t : dp.h_tasks[lv]) {
                 if (t.p <= k0) continue;
-                const int kb = std::min(NB, t.p - k0);
-                const int tp = (t.p - k0 - kb + TM - 1) / TM;
+                const int kb0 = std::min(NB, t.p - k0);
+                const int kb1 = std::max(0, std::min(NB, t.p - k0 - NB));
+                if (kb1 > 0) {
+                    second = true;
+                    const int s1 = k0 + kb0, e1 = s1 + kb1;
+                    maxn = std::max(maxn, (t.n - s1 + TM - 1) / TM + (t.n - e1 + TM - 1) / TM);
+                }
+                const int sw = k0 + kb0 + kb1;
+                const int tp = (t.p - sw + TM - 1) / TM;
                 const int tu = (t.n - t.p + TM - 1) / TM;
-                maxt = std::max(maxt, tp * (tp + tu) + tu * tp);
+                maxw = std::max(maxw, tp * (tp + tu) + tu * tp);
             }
-            hipLaunchKernelGGL(k_update<T>, dim3(std::max(1, maxt), nfr, Bz), dim3(256), 0, st,
-                               dp.d_tasks[lv], k0, mem);
+            hipLaunchKernelGGL((k_update<T, 0>), dim3(maxn, nfr, Bz), dim3(256), 0, st, dp.d_tasks[lv], k0,
+                               mem);
+            LAUNCH_CHECK();
+            if (second) HMEC_TRY(panel(k0 + NB));
+            hipLaunchKernelGGL((k_update<T, 1>), dim3(maxw, nfr, Bz), dim3(256), 0, st, dp.d_tasks[lv], k0,
+                               mem);
             LAUNCH_CHECK();
         }
         int maxt = 0;
