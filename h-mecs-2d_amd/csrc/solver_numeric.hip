@@ -1039,34 +1039,50 @@ struct GmCoef {
 };
 // w -= sum_i h_i V_i, coefficients from device memory (no host round trip)
 // First Gram-Schmidt update with the second pass's dot products:
-// w' = w - V h (h = h[0..nv)), and per workgroup the partial sums of
-// V_i . w' (i < nv) and w' . w' -> part[block * (nv + 1) + i]
+// w' = w - V h (h = h[0..nv), nv <= NVMAX), and per workgroup the partial
+// sums of V_i . w' (i < nv) and w' . w' -> part[block * (nv + 1) + i].
+// Grid-stride (RB workgroups): an element's basis values are loaded once.
+template <int NVMAX>
 __global__ void __launch_bounds__(256) k_gs_update_dots(const double *V, int64_t ldv, int nv,
                                                         const double *h, double *w, int64_t N,
                                                         double *part) {
-    __shared__ double hs[GM_MAXV];
-    __shared__ double red[4][GM_MAXV + 1];
+    __shared__ double hs[NVMAX];
+    __shared__ double red[4][NVMAX + 1];
     if ((int)threadIdx.x < nv) hs[threadIdx.x] = h[threadIdx.x];
     __syncthreads();
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double s = 0.0;
-    if (t < N) {
-        s = w[t];
-        for (int i = 0; i < nv; i++) s -= hs[i] * V[(int64_t)i * ldv + t];
+    double acc[NVMAX + 1];
+#pragma unroll
+    for (int i = 0; i <= NVMAX; i++) acc[i] = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < N;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        double vv[NVMAX];
+        double s = w[t];
+#pragma unroll
+        for (int i = 0; i < NVMAX; i++) {
+            vv[i] = i < nv ? V[(int64_t)i * ldv + t] : 0.0;
+            if (i < nv) s -= hs[i] * vv[i];
+        }
         w[t] = s;
+#pragma unroll
+        for (int i = 0; i < NVMAX; i++) acc[i] += vv[i] * s;
+        acc[NVMAX] += s * s;
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int i = 0; i <= nv; i++) {
-        double d = t < N ? (i < nv ? V[(int64_t)i * ldv + t] : s) * s : 0.0;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
-        if (lane == 0) red[wv][i] = d;
+    for (int i = 0; i <= NVMAX; i++) {
+        if (i < nv || i == NVMAX) {
+            double d = acc[i];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
+            if (lane == 0) red[wv][i] = d;
+        }
     }
     __syncthreads();
     if ((int)threadIdx.x <= nv) {
+        const int i = (int)threadIdx.x < nv ? (int)threadIdx.x : NVMAX;   // w'.w' last
         double d = 0.0;
 #pragma unroll
-        for (int q = 0; q < 4; q++) d += red[q][threadIdx.x];
+        for (int q = 0; q < 4; q++) d += red[q][i];
         part[(int64_t)blockIdx.x * (nv + 1) + threadIdx.x] = d;
     }
 }
@@ -2050,10 +2066,16 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
             LAUNCH_CHECK();
             hipLaunchKernelGGL(k_dots_final, dim3(j + 1), dim3(256), 0, st, g.part, (int)nbs, j + 1, h1d);
             LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_gs_update_dots, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h1d, g.w,
-                               N, g.part);
+#define GS_GO(NVM)                                                                          \
+    hipLaunchKernelGGL(k_gs_update_dots<NVM>, dim3(RB), dim3(256), 0, st, g.V, N, j + 1, h1d, g.w, \
+                       N, g.part)
+            if (j + 1 <= 4) GS_GO(4);
+            else if (j + 1 <= 8) GS_GO(8);
+            else if (j + 1 <= 16) GS_GO(16);
+            else GS_GO(GM_MAXV);
+#undef GS_GO
             LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_dots_final, dim3(j + 2), dim3(256), 0, st, g.part, (int)gN, j + 2, h2d);
+            hipLaunchKernelGGL(k_dots_final, dim3(j + 2), dim3(256), 0, st, g.part, RB, j + 2, h2d);
             LAUNCH_CHECK();
             hipLaunchKernelGGL(k_gs_update_norm, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h2d, g.w,
                                j + 1 < m ? g.V + (int64_t)(j + 1) * N : nullptr, wwd, N);
