@@ -314,20 +314,39 @@ __device__ __forceinline__ void gemm_tile_sub(T *C, const T *A, const T *B, int6
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
+    constexpr int PER = TK * TM / 256;       // operand elements per thread and chunk
     acc_t acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = acc_t{0, 0, 0, 0};
-    for (int kk = 0; kk < K; kk += TK) {
+    // the next K chunk is loaded into registers while the current one is
+    // multiplied from LDS (the global loads overlap the MFMAs)
+    T ra[PER], rb[PER];
+    auto fetch = [&](int kk) {
         const int kc = min(TK, K - kk);
-        for (int e = threadIdx.x; e < TK * TM; e += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = threadIdx.x + q * 256;
             const int r = e % TM, k = e / TM;
-            As[k][r] = (k < kc && r < m) ? A[(int64_t)(kk + k) * ld + r] : (T)0;
+            ra[q] = (k < kc && r < m) ? A[(int64_t)(kk + k) * ld + r] : (T)0;
             const int cc = e / TK, k2 = e % TK;
-            Bs[k2][cc] = (k2 < kc && cc < nn) ? B[(int64_t)cc * ld + kk + k2] : (T)0;
+            rb[q] = (k2 < kc && cc < nn) ? B[(int64_t)cc * ld + kk + k2] : (T)0;
         }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = threadIdx.x + q * 256;
+            As[e / TM][e % TM] = ra[q];
+            Bs[e % TK][e / TK] = rb[q];
+        }
+    };
+    if (K > 0) fetch(0);
+    for (int kk = 0; kk < K; kk += TK) {
+        stage();
         __syncthreads();
+        if (kk + TK < K) fetch(kk + TK);
 #pragma unroll
         for (int k4 = 0; k4 < TK; k4 += 4) {
             const T a0 = As[k4 + lk][wr + li], a1 = As[k4 + lk][wr + 16 + li];
