@@ -552,8 +552,14 @@ __global__ void __launch_bounds__(256) k_inv_gemm(const InvTile *tiles, const Fr
     for (int x = 0; x < 2; x++)
 #pragma unroll
         for (int y = 0; y < 2; y++) acc[x][y] = acc_t{0, 0, 0, 0};
-    for (int kk = kmin; kk < kmax; kk += TK) {
-        for (int e = threadIdx.x; e < TK * TM; e += blockDim.x) {
+    // the next K chunk is loaded into registers while the current one is
+    // multiplied from LDS
+    constexpr int PER = TK * TM / 256;
+    T ra[PER], rb[PER];
+    auto fetch = [&](int kk) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = threadIdx.x + q * 256;
             // left operand (i, k), consecutive threads along i
             {
                 const int r = e % TM, kl = e / TM;
@@ -569,7 +575,7 @@ __global__ void __launch_bounds__(256) k_inv_gemm(const InvTile *tiles, const Fr
                         v = k >= i ? F[(int64_t)(a0 + k) * n + a0 + i] : (T)0;
                     }
                 }
-                As[kl][r] = v;
+                ra[q] = v;
             }
             // right operand (k, j), consecutive threads along k
             {
@@ -586,10 +592,20 @@ __global__ void __launch_bounds__(256) k_inv_gemm(const InvTile *tiles, const Fr
                         v = S[(int64_t)j * (lower ? nb : na) + k];        // the PASS 1 product
                     }
                 }
-                Bs[kl][c] = v;
+                rb[q] = v;
             }
         }
+    };
+    if (kmin < kmax) fetch(kmin);
+    for (int kk = kmin; kk < kmax; kk += TK) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = threadIdx.x + q * 256;
+            As[e / TM][e % TM] = ra[q];
+            Bs[e % TK][e / TK] = rb[q];
+        }
         __syncthreads();
+        if (kk + TK < kmax) fetch(kk + TK);
 #pragma unroll
         for (int k4 = 0; k4 < TK; k4 += 4) {
             const T x0 = As[k4 + lk][wr + li], x1 = As[k4 + lk][wr + 16 + li];

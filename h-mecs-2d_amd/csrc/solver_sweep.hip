@@ -146,8 +146,14 @@ __global__ void __launch_bounds__(256) k_sweep_form(const SwTile *tiles, const S
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = acc_t{0, 0, 0, 0};
-    for (int kk = kb; kk < p; kk += SW_TK) {
-        for (int e = tid; e < SW_TK * SW_TILE; e += blockDim.x) {
+    // the next K chunk is loaded into registers while the current one is
+    // multiplied from LDS
+    constexpr int PER = SW_TK * SW_TILE / 256;
+    T ra[PER], rb[PER];
+    auto fetch = [&](int kk) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = tid + q * 256;
             // A operand, row index fastest (column-major source)
             const int r = e % SW_TILE, k = e / SW_TILE, kg = kk + k;
             T va = (T)0;
@@ -159,7 +165,7 @@ __global__ void __launch_bounds__(256) k_sweep_form(const SwTile *tiles, const S
                     va = kg >= i ? A[(int64_t)kg * n + i] : (T)0;                // U11^-1[i, kg]
                 }
             }
-            As[k][r] = va;
+            ra[q] = va;
             // B operand, K index fastest
             const int cc = e / SW_TK, k2 = e % SW_TK, kg2 = kk + k2;
             T vb = (T)0;
@@ -171,9 +177,19 @@ __global__ void __launch_bounds__(256) k_sweep_form(const SwTile *tiles, const S
                     vb = A[(int64_t)(p + cbase + cc) * n + kg2];                 // U12[kg2, cc]
                 }
             }
-            Bs[k2][cc] = vb;
+            rb[q] = vb;
+        }
+    };
+    if (kb < p) fetch(kb);
+    for (int kk = kb; kk < p; kk += SW_TK) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = tid + q * 256;
+            As[e / SW_TILE][e % SW_TILE] = ra[q];
+            Bs[e % SW_TK][e / SW_TK] = rb[q];
         }
         __syncthreads();
+        if (kk + SW_TK < p) fetch(kk + SW_TK);
 #pragma unroll
         for (int k4 = 0; k4 < SW_TK; k4 += 4) {
             const T a0 = As[k4 + lk][wr + li], a1 = As[k4 + lk][wr + 16 + li];
