@@ -865,55 +865,62 @@ template <int NV>
 struct NormArgs {
     const double *x[NV];
 };
-template <int NV>
+// (SQ: one more output, out[NV] = x[2] . x[2], summed in a fixed order)
+template <int NV, bool SQ = false>
 __global__ void __launch_bounds__(256) k_norms_part(NormArgs<NV> a, int64_t N, double *part) {
-    __shared__ double r[NV][256];
-    double m[NV];
+    constexpr int NO = NV + (SQ ? 1 : 0);
+    __shared__ double r[NO][256];
+    double m[NO];
 #pragma unroll
-    for (int q = 0; q < NV; q++) m[q] = 0;
+    for (int q = 0; q < NO; q++) m[q] = 0;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < N;
          t += (int64_t)gridDim.x * blockDim.x) {
 #pragma unroll
         for (int q = 0; q < NV; q++) m[q] = nan_max(m[q], fabs(a.x[q][t]));
+        if (SQ) m[NV] += a.x[2][t] * a.x[2][t];
     }
 #pragma unroll
-    for (int q = 0; q < NV; q++) r[q][threadIdx.x] = m[q];
+    for (int q = 0; q < NO; q++) r[q][threadIdx.x] = m[q];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
 #pragma unroll
             for (int q = 0; q < NV; q++) r[q][threadIdx.x] = nan_max(r[q][threadIdx.x], r[q][threadIdx.x + s]);
+            if (SQ) r[NV][threadIdx.x] += r[NV][threadIdx.x + s];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NV; q++) part[NV * blockIdx.x + q] = r[q][0];
+        for (int q = 0; q < NO; q++) part[NO * blockIdx.x + q] = r[q][0];
     }
 }
-template <int NV>
+template <int NV, bool SQ = false>
 __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb, double *out) {
-    __shared__ double r[NV][256];
-    double m[NV];
+    constexpr int NO = NV + (SQ ? 1 : 0);
+    __shared__ double r[NO][256];
+    double m[NO];
 #pragma unroll
-    for (int q = 0; q < NV; q++) m[q] = 0;
+    for (int q = 0; q < NO; q++) m[q] = 0;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) {
 #pragma unroll
-        for (int q = 0; q < NV; q++) m[q] = nan_max(m[q], part[NV * k + q]);
+        for (int q = 0; q < NV; q++) m[q] = nan_max(m[q], part[NO * k + q]);
+        if (SQ) m[NV] += part[NO * k + NV];
     }
 #pragma unroll
-    for (int q = 0; q < NV; q++) r[q][threadIdx.x] = m[q];
+    for (int q = 0; q < NO; q++) r[q][threadIdx.x] = m[q];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
 #pragma unroll
             for (int q = 0; q < NV; q++) r[q][threadIdx.x] = nan_max(r[q][threadIdx.x], r[q][threadIdx.x + s]);
+            if (SQ) r[NV][threadIdx.x] += r[NV][threadIdx.x + s];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NV; q++) out[q] = r[q][0];
+        for (int q = 0; q < NO; q++) out[q] = r[q][0];
     }
 }
 
@@ -1507,9 +1514,9 @@ int sv_setup_device(hmec_ctx *c) {
     HMEC_HIP(hipMalloc(&q.cs, sizeof(double) * std::max<int64_t>(1, N)));
     // arenas
     HMEC_HIP(hipMalloc(&c->d.vec, sizeof(double) * std::max<int64_t>(1, pl.varena)));
-    HMEC_HIP(hipMalloc(&s->norms, sizeof(double) * 4));
-    HMEC_HIP(hipMalloc(&s->nrm_part, sizeof(double) * 4 * NRM_BLOCKS));
-    HMEC_HIP(hipHostMalloc(&s->h_norms, sizeof(double) * 4, hipHostMallocDefault));
+    HMEC_HIP(hipMalloc(&s->norms, sizeof(double) * 5));
+    HMEC_HIP(hipMalloc(&s->nrm_part, sizeof(double) * 5 * NRM_BLOCKS));
+    HMEC_HIP(hipHostMalloc(&s->h_norms, sizeof(double) * 8, hipHostMallocDefault));
     return HMEC_OK;
 }
 
@@ -1821,14 +1828,16 @@ static int launch_norms(hmec_ctx *c, const double *a, const double *b, double *o
     LAUNCH_CHECK();
     return HMEC_OK;
 }
-// out[0..4) = inf-norms of a, b, c2, d in one pass
+// out[0..4) = inf-norms of a, b, c2, d and out[4] = c2 . c2, in one pass
 static int launch_norms4(hmec_ctx *c, const double *a, const double *b, const double *c2,
                          const double *d, double *out) {
     SolverDev *s = sdev(c);
     NormArgs<4> na{{a, b, c2, d}};
-    hipLaunchKernelGGL(k_norms_part<4>, dim3(NRM_BLOCKS), dim3(256), 0, c->stream, na, c->N, s->nrm_part);
+    hipLaunchKernelGGL((k_norms_part<4, true>), dim3(NRM_BLOCKS), dim3(256), 0, c->stream, na, c->N,
+                       s->nrm_part);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_norms_final<4>, dim3(1), dim3(256), 0, c->stream, s->nrm_part, NRM_BLOCKS, out);
+    hipLaunchKernelGGL((k_norms_final<4, true>), dim3(1), dim3(256), 0, c->stream, s->nrm_part,
+                       NRM_BLOCKS, out);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
@@ -2023,7 +2032,7 @@ static int gm_precond(hmec_ctx *c, const double *v, double *zout) {
 // reported one) and row-equilibrated ||Dr (b - A x)|| / ||Dr b||; leaves
 // Dr (b - A x) in g.rsd (the GMRES residual)
 static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *rel_u,
-                            double *rel_s) {
+                            double *rel_s, double *rr) {
     SolverDev *s = sdev(c);
     const int64_t N = c->N;
     const unsigned gN = (unsigned)((N + 255) / 256);
@@ -2031,11 +2040,12 @@ static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *r
     launch_residual(c, x, c->d.work, g.rsd, c->d.rscale);
     LAUNCH_CHECK();
     HMEC_TRY(launch_norms4(c, c->d.work, c->d.rhs, g.rsd, g.bs, s->norms));
-    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 4, hipMemcpyDeviceToHost, st));
+    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 5, hipMemcpyDeviceToHost, st));
     HMEC_HIP(hipStreamSynchronize(st));
     const double *h = s->h_norms;
     *rel_u = h[1] > 0 ? h[0] / h[1] : h[0];
     *rel_s = h[3] > 0 ? h[2] / h[3] : h[2];
+    *rr = h[4];
     return HMEC_OK;
 }
 
@@ -2072,13 +2082,11 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, c->d.rhs, c->d.rscale, g.bs, N);
     LAUNCH_CHECK();
     for (;;) {
-        double rel_s = 0;
-        HMEC_TRY(gm_true_residual(c, g, x, &relinf, &rel_s));
+        double rel_s = 0, rr = 0;   // rr = ||Dr (b - A x)||_2^2 from the same pass
+        HMEC_TRY(gm_true_residual(c, g, x, &relinf, &rel_s, &rr));
         last_rel_s = rel_s;
         const double worst = std::max(relinf, rel_s);
         if (!(worst > tol) || total_it >= max_it) break;
-        double rr;
-        HMEC_TRY(gm_dots(c, g, r, 1, r, &rr));
         const double beta = std::sqrt(rr);
         if (!(beta > 0)) break;
         const double target = beta * (tol / worst) * 0.5;
