@@ -304,6 +304,7 @@ int sv_slot_nearest(hmec_ctx *c, double dt, double *dist);
 int sv_slot_victim(hmec_ctx *c, int keep);
 void sv_slot_target(hmec_ctx *c, int k);
 int sv_slot_activate(hmec_ctx *c, int k);
+void sv_active_scales(hmec_ctx *c, const double **rs, const double **cs);
 void sv_sweep_rhs_target(hmec_ctx *c, int precision, double **zf, const int32_t **fpos);
 void sv_sweep_rhs_given(hmec_ctx *c);
 int sv_sweep_solve(hmec_ctx *c, double *z);

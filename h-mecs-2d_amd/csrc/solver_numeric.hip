@@ -895,8 +895,11 @@ __global__ void __launch_bounds__(256) k_norms_part(NormArgs<NV> a, int64_t N, d
         for (int q = 0; q < NO; q++) part[NO * blockIdx.x + q] = r[q][0];
     }
 }
+// (cnt: optional, out[NO] = cnt[0], so the count of perturbed pivots comes back
+// with the norms)
 template <int NV, bool SQ = false>
-__global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb, double *out) {
+__global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb, double *out,
+                                                     const int *cnt) {
     constexpr int NO = NV + (SQ ? 1 : 0);
     __shared__ double r[NO][256];
     double m[NO];
@@ -921,6 +924,7 @@ __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb,
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NO; q++) out[q] = r[q][0];
+        if (cnt) out[NO] = (double)cnt[0];
     }
 }
 
@@ -1514,7 +1518,7 @@ int sv_setup_device(hmec_ctx *c) {
     HMEC_HIP(hipMalloc(&q.cs, sizeof(double) * std::max<int64_t>(1, N)));
     // arenas
     HMEC_HIP(hipMalloc(&c->d.vec, sizeof(double) * std::max<int64_t>(1, pl.varena)));
-    HMEC_HIP(hipMalloc(&s->norms, sizeof(double) * 5));
+    HMEC_HIP(hipMalloc(&s->norms, sizeof(double) * 8));
     HMEC_HIP(hipMalloc(&s->nrm_part, sizeof(double) * 5 * NRM_BLOCKS));
     HMEC_HIP(hipHostMalloc(&s->h_norms, sizeof(double) * 8, hipHostMallocDefault));
     return HMEC_OK;
@@ -1818,17 +1822,20 @@ static int tri_solve(hmec_ctx *c, const T *arena, double *z) {
 // x = A^-1 b by the factor + iterative refinement on the FP64 CSR.
 // tol: target ||b - A x||inf / ||b||inf; max_ir: refinement sweeps after the
 // first solve.  Returns HMEC_E_NOCONV when the target is not met.
-// out[0..1] = max|a|, max|b| (device), two launches on the context stream
+// out[0..1] = max|a|, max|b|, out[2] = perturbed pivots of the current factor
+// (device), two launches on the context stream
 static int launch_norms(hmec_ctx *c, const double *a, const double *b, double *out) {
     SolverDev *s = sdev(c);
     NormArgs<2> na{{a, b}};
     hipLaunchKernelGGL(k_norms_part<2>, dim3(NRM_BLOCKS), dim3(256), 0, c->stream, na, c->N, s->nrm_part);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_norms_final<2>, dim3(1), dim3(256), 0, c->stream, s->nrm_part, NRM_BLOCKS, out);
+    hipLaunchKernelGGL(k_norms_final<2>, dim3(1), dim3(256), 0, c->stream, s->nrm_part, NRM_BLOCKS, out,
+                       (const int *)c->d.counters);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
-// out[0..4) = inf-norms of a, b, c2, d and out[4] = c2 . c2, in one pass
+// out[0..4) = inf-norms of a, b, c2, d, out[4] = c2 . c2 in one pass, and
+// out[5] = perturbed pivots of the current factor
 static int launch_norms4(hmec_ctx *c, const double *a, const double *b, const double *c2,
                          const double *d, double *out) {
     SolverDev *s = sdev(c);
@@ -1837,7 +1844,7 @@ static int launch_norms4(hmec_ctx *c, const double *a, const double *b, const do
                        s->nrm_part);
     LAUNCH_CHECK();
     hipLaunchKernelGGL((k_norms_final<4, true>), dim3(1), dim3(256), 0, c->stream, s->nrm_part,
-                       NRM_BLOCKS, out);
+                       NRM_BLOCKS, out, (const int *)c->d.counters);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
@@ -1880,12 +1887,14 @@ static int precond_apply(hmec_ctx *c, const double *v, const double *rs, double 
     double *zf = nullptr;
     const int32_t *fpos = nullptr;
     sv_sweep_rhs_target(c, c->factor_prec, &zf, &fpos);
+    const double *srs, *scs;
+    sv_active_scales(c, &srs, &scs);
     hipLaunchKernelGGL(k_cond_rhs, dim3((unsigned)((cp.NS + 255) / 256)), dim3(256), 0, st, v, rs,
-                       q.rcoef, cp.NS, q.full_of_s, q.rptr, q.rd, q.gv, q.rs, q.vs, zf, fpos);
+                       q.rcoef, cp.NS, q.full_of_s, q.rptr, q.rd, q.gv, srs, q.vs, zf, fpos);
     LAUNCH_CHECK();
     if (zf) sv_sweep_rhs_given(c);
     HMEC_TRY(solve_condensed(c, q.vs));
-    hipLaunchKernelGGL(k_expand, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, st, q.vs, q.cs,
+    hipLaunchKernelGGL(k_expand, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, st, q.vs, scs,
                        q.gv, q.e, c->N, q.s_of_full, q.d_of_full, q.dec, z);
     LAUNCH_CHECK();
     return HMEC_OK;
@@ -1902,7 +1911,7 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
     HMEC_HIP(hipMemcpyAsync(r, c->d.rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
     HMEC_TRY(darcy_e(c, st));
     double res = 0, bnorm = 0, prev = INFINITY;
-    int it = 0;
+    int it = 0, npert = 0;
     int stalls = 0;
     for (;; it++) {
         HMEC_TRY(precond_apply(c, r, nullptr, z));
@@ -1911,10 +1920,11 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
         launch_residual(c, x, r, nullptr, nullptr);
         LAUNCH_CHECK();
         HMEC_TRY(launch_norms(c, r, c->d.rhs, s->norms));
-        HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+        HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
         HMEC_HIP(hipStreamSynchronize(st));
         bnorm = s->h_norms[1];
         res = bnorm > 0 ? s->h_norms[0] / bnorm : s->h_norms[0];
+        npert = (int)s->h_norms[2];
         if (res <= tol || it >= max_ir) break;
         if (res > 0.5 * prev) {
             if (++stalls >= 2) break;
@@ -1923,8 +1933,6 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
         }
         prev = res;
     }
-    int npert = 0;
-    HMEC_HIP(hipMemcpy(&npert, c->d.counters, sizeof(int), hipMemcpyDeviceToHost));
     c->last_ir = it;
     c->last_res = res;
     c->last_perturbed = npert;
@@ -2032,7 +2040,7 @@ static int gm_precond(hmec_ctx *c, const double *v, double *zout) {
 // reported one) and row-equilibrated ||Dr (b - A x)|| / ||Dr b||; leaves
 // Dr (b - A x) in g.rsd (the GMRES residual)
 static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *rel_u,
-                            double *rel_s, double *rr) {
+                            double *rel_s, double *rr, int *npert) {
     SolverDev *s = sdev(c);
     const int64_t N = c->N;
     const unsigned gN = (unsigned)((N + 255) / 256);
@@ -2040,12 +2048,13 @@ static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *r
     launch_residual(c, x, c->d.work, g.rsd, c->d.rscale);
     LAUNCH_CHECK();
     HMEC_TRY(launch_norms4(c, c->d.work, c->d.rhs, g.rsd, g.bs, s->norms));
-    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 5, hipMemcpyDeviceToHost, st));
+    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
     HMEC_HIP(hipStreamSynchronize(st));
     const double *h = s->h_norms;
     *rel_u = h[1] > 0 ? h[0] / h[1] : h[0];
     *rel_s = h[3] > 0 ? h[2] / h[3] : h[2];
     *rr = h[4];
+    *npert = (int)h[5];
     return HMEC_OK;
 }
 
@@ -2069,7 +2078,7 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     hipStream_t st = c->stream;
     double *x = c->d.sol;
     double *r = g.rsd;
-    int total_it = 0;
+    int total_it = 0, npert = 0;
     double relinf = 0, last_rel_s = 0;
     std::vector<double> H((m + 1) * m), cs(m), sn(m), gv(m + 1), h(m + 2), h2(m + 2), y(m);
     // the operator's row scaling (powers of two) and E of the current matrix
@@ -2083,7 +2092,7 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     LAUNCH_CHECK();
     for (;;) {
         double rel_s = 0, rr = 0;   // rr = ||Dr (b - A x)||_2^2 from the same pass
-        HMEC_TRY(gm_true_residual(c, g, x, &relinf, &rel_s, &rr));
+        HMEC_TRY(gm_true_residual(c, g, x, &relinf, &rel_s, &rr, &npert));
         last_rel_s = rel_s;
         const double worst = std::max(relinf, rel_s);
         if (!(worst > tol) || total_it >= max_it) break;
@@ -2161,8 +2170,6 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
         hipLaunchKernelGGL(k_combine, dim3(gN), dim3(256), 0, st, g.Z, N, j, cy, x, N);
         LAUNCH_CHECK();
     }
-    int npert = 0;
-    HMEC_HIP(hipMemcpy(&npert, c->d.counters, sizeof(int), hipMemcpyDeviceToHost));
     c->last_ir = total_it;
     c->last_res = relinf;
     if (out) {

@@ -796,14 +796,12 @@ int sv_slot_victim(hmec_ctx *c, int keep) {
 
 void sv_slot_target(hmec_ctx *c, int k) { swdev(c)->target = k; }
 
-// make slot k the active FP32 factor: its scales become the context's
+// make slot k the active FP32 factor (its scales are read in place, see
+// sv_active_scales)
 int sv_slot_activate(hmec_ctx *c, int k) {
     SweepDev *s = swdev(c);
     if (k < 0 || k >= (int)s->slots.size() || !s->slots[k].valid) return HMEC_E_ARG;
     SweepDev::Slot &sl = s->slots[k];
-    const size_t nb = sizeof(double) * c->cp.NS;
-    HMEC_HIP(hipMemcpyAsync(c->dc.rs, sl.rs, nb, hipMemcpyDeviceToDevice, c->stream));
-    HMEC_HIP(hipMemcpyAsync(c->dc.cs, sl.cs, nb, hipMemcpyDeviceToDevice, c->stream));
     s->cur = k;
     s->built = HMEC_PREC_FP32;
     sl.used = ++s->clock;
@@ -817,6 +815,20 @@ bool sv_sweep_ready(hmec_ctx *c, int precision) {
     SweepDev *s = swdev(c);
     if (s->built != precision) return false;
     return precision != HMEC_PREC_FP32 || s->slots[s->cur].valid;
+}
+
+// The condensed row / column equilibration of the active factor: the FP32
+// slot's own copy while that slot's sweep form solves, else the scales the
+// last factorization computed
+void sv_active_scales(hmec_ctx *c, const double **rs, const double **cs) {
+    SweepDev *s = swdev(c);
+    if (c->factor_prec == HMEC_PREC_FP32 && sv_sweep_ready(c, HMEC_PREC_FP32)) {
+        *rs = s->slots[s->cur].rs;
+        *cs = s->slots[s->cur].cs;
+    } else {
+        *rs = c->dc.rs;
+        *cs = c->dc.cs;
+    }
 }
 
 // an FP32 factorization is about to overwrite the target slot; an FP64 one
