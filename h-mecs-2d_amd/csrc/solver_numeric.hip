@@ -1169,7 +1169,6 @@ __global__ void k_vscale(const double *x, double a, double *y, int64_t N) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < N) y[t] = a * x[t];
 }
-// y = x / sqrt(ww[0]) (zeros on breakdown, which the host then detects)
 // x += sum_j c_j Z_j
 __global__ void k_combine(const double *Z, int64_t ldz, int nz, GmCoef c, double *x,
                           int64_t N) {
