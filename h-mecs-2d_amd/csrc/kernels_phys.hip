@@ -1222,11 +1222,78 @@ int k_iter_commit(hmec_ctx *c, int ynpl, int converged) {
     return HMEC_OK;
 }
 
+// ------------------------------------------------ diagnostics (outputs)
+// Pressure cells i in [1, Ny), j in [1, Nx) of the final stress state,
+// h_mec_rsf_v9.cpp:1522-1558: EII, EIIVP, SII (the xy terms as sums of
+// squares over the four surrounding basic nodes, in the reference's block
+// order (i-1,j-1), (i-1,j), (i,j-1), (i,j)), DIS (its xy term summed in the
+// order (i,j), (i-1,j), (i,j-1), (i-1,j-1)), pt_ave / pf_ave (mean with the
+// cell below, the last row its own value), VIS_COMP and EL_DECOM.  The
+// fields outside those cells stay 0.
+__global__ void k_diagnostics(double *f, int64_t ns, int Nx, int Ny, double dt, double betasolid) {
+    const int Ny1 = Ny + 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = 1 + t % (Ny - 1), j = 1 + t / (Ny - 1);
+    if (t >= (Nx - 1) * (Ny - 1)) return;
+    const int64_t s = i + (int64_t)j * Ny1;
+    const int64_t b00 = (i - 1) + (int64_t)(j - 1) * Ny, b01 = (i - 1) + (int64_t)j * Ny;
+    const int64_t b10 = i + (int64_t)(j - 1) * Ny, b11 = i + (int64_t)j * Ny;
+#define FB(n, k) f[(int64_t)(n) * ns + (k)]
+#define FS(n) f[(int64_t)(n) * ns + s]
+    const double sxx = FS(HMEC_F_SXX), syy = FS(HMEC_F_SYY), etap = FS(HMEC_F_ETAP);
+    const double exx = FS(HMEC_F_EXX);
+    const double e00 = FB(HMEC_F_EXY, b00), e01 = FB(HMEC_F_EXY, b01);
+    const double e10 = FB(HMEC_F_EXY, b10), e11 = FB(HMEC_F_EXY, b11);
+    const double s00 = FB(HMEC_F_SXY, b00), s01 = FB(HMEC_F_SXY, b01);
+    const double s10 = FB(HMEC_F_SXY, b10), s11 = FB(HMEC_F_SXY, b11);
+    const double n00 = FB(HMEC_F_ETA, b00), n01 = FB(HMEC_F_ETA, b01);
+    const double n10 = FB(HMEC_F_ETA, b10), n11 = FB(HMEC_F_ETA, b11);
+    const double q00 = s00 / n00, q01 = s01 / n01, q10 = s10 / n10, q11 = s11 / n11;
+    const double exy2 = ((e00 * e00 + e01 * e01) + e10 * e10) + e11 * e11;
+    const double sxy2 = ((s00 * s00 + s01 * s01) + s10 * s10) + s11 * s11;
+    const double q2 = ((q00 * q00 + q01 * q01) + q10 * q10) + q11 * q11;
+    FS(HMEC_F_EII) = sqrt(exx * exx + exy2 / 4.);
+    const double rx = sxx / (2 * etap), ry = syy / (2 * etap);
+    FS(HMEC_F_EIIVP) = sqrt(.5 * (rx * rx + ry * ry) + q2 / 16.);
+    FS(HMEC_F_SII) = sqrt(.5 * (sxx * sxx + syy * syy) + sxy2 / 4.);
+    const double disxy = (((s11 * s11 / n11 + s01 * s01 / n01) + s10 * s10 / n10) + s00 * s00 / n00) / 2.;
+    FS(HMEC_F_DIS) = (sxx * sxx / (2 * etap) + syy * syy / (2 * etap)) + disxy;
+    const double *pt = f + (int64_t)HMEC_F_PT * ns, *pf = f + (int64_t)HMEC_F_PF * ns;
+    const double *pt0 = f + (int64_t)HMEC_F_PT0 * ns, *pf0 = f + (int64_t)HMEC_F_PF0 * ns;
+    double pta, pfa, pt0a, pf0a;
+    if (i < Ny - 1) {
+        pta = (pt[s] + pt[s + 1]) / 2.;
+        pfa = (pf[s] + pf[s + 1]) / 2.;
+        pt0a = (pt0[s] + pt0[s + 1]) / 2.;
+        pf0a = (pf0[s] + pf0[s + 1]) / 2.;
+    } else {
+        pta = pt[s];
+        pfa = pf[s];
+        pt0a = pt0[s];
+        pf0a = pf0[s];
+    }
+    FS(HMEC_F_PT_AVE) = pta;
+    FS(HMEC_F_PF_AVE) = pfa;
+    const double por = FS(HMEC_F_POR);
+    FS(HMEC_F_VIS_COMP) = (pta - pfa) / (FS(HMEC_F_ETAB) * (1 - por));
+    const double betadrained = (1 / FS(HMEC_F_GGGB) + betasolid) / (1 - por);
+    const double kbw = 1 - betasolid / betadrained;
+    FS(HMEC_F_EL_DECOM) = betadrained * (((pta - pt0a) - kbw * pfa) + kbw * pf0a) / dt;
+#undef FB
+#undef FS
+}
+
 int k_final_stress(hmec_ctx *c, double dt) {
     const int n = c->Nx1 * c->Ny1;
     hipLaunchKernelGGL(k_stress, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
                        c->ns, c->Nx, c->Ny, c->P.dx, c->P.dy, dt, 1);
     LAUNCH_CHECK();
+    const int nc = (c->Nx - 1) * (c->Ny - 1);
+    if (nc > 0) {
+        hipLaunchKernelGGL(k_diagnostics, dim3((nc + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
+                           c->ns, c->Nx, c->Ny, dt, c->P.BETASOLID);
+        LAUNCH_CHECK();
+    }
     return HMEC_OK;
 }
 

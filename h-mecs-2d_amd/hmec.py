@@ -25,7 +25,8 @@ BASIC_FIELDS = ["OM0", "OM", "OM5", "ARSF", "BRSF", "LRSF", "RHO", "ETA", "ETA0"
 STAG_FIELDS = ["pt", "vxs", "vys", "pf", "vxD", "vyD", "ETAB", "ETAB0", "ETAP", "ETAP0",
                "POR", "GGGP", "GGGB", "PTF0", "PT0", "PF0", "SXX", "SXX0", "SYY", "SYY0",
                "DILP", "RHOX", "RHOFX", "ETADX", "PORX", "VX0", "VXF0", "RHOY", "RHOFY",
-               "ETADY", "PORY", "VY0", "VYF0", "EXX", "EYY"]
+               "ETADY", "PORY", "VY0", "VYF0", "EXX", "EYY",
+               "EII", "EIIVP", "SII", "DSII", "DIS", "pt_ave", "pf_ave", "VIS_COMP", "EL_DECOM"]
 FIELDS = BASIC_FIELDS + STAG_FIELDS
 FIELD_ID = {n: k for k, n in enumerate(FIELDS)}
 NBASIC = len(BASIC_FIELDS)

@@ -139,6 +139,12 @@ enum hmec_field {
     HMEC_F_DILP, HMEC_F_RHOX, HMEC_F_RHOFX, HMEC_F_ETADX, HMEC_F_PORX,
     HMEC_F_VX0, HMEC_F_VXF0, HMEC_F_RHOY, HMEC_F_RHOFY, HMEC_F_ETADY,
     HMEC_F_PORY, HMEC_F_VY0, HMEC_F_VYF0, HMEC_F_EXX, HMEC_F_EYY,
+    /* diagnostics of the final stress state, outputs only
+     * (h_mec_rsf_v9.cpp:1522-1558): strain-rate and stress invariants,
+     * dissipation, vertically averaged pressures, viscous and elastic
+     * compaction */
+    HMEC_F_EII, HMEC_F_EIIVP, HMEC_F_SII, HMEC_F_DSII, HMEC_F_DIS,
+    HMEC_F_PT_AVE, HMEC_F_PF_AVE, HMEC_F_VIS_COMP, HMEC_F_EL_DECOM,
     HMEC_NFIELDS
 };
 #define HMEC_COLMAJOR 0   /* Eigen MatrixXd layout                        */

@@ -1127,9 +1127,14 @@ void orc_advect(orc *o) {
     }
 }
 
-/* final stresses with the final dt, h_mec_rsf_v9.cpp:1480-1521 (the
- * invariants / dissipation / compaction diagnostics of :1522-1558 only feed
- * outputs and are not computed) */
+/* square_block of h_mec_rsf_v9.cpp:134-136: the sum of squares of a 2x2
+ * block in the order (0,0), (0,1), (1,0), (1,1) */
+static double sqblock(double a00, double a01, double a10, double a11) {
+    return ((a00 * a00 + a01 * a01) + a10 * a10) + a11 * a11;
+}
+
+/* final stresses with the final dt, h_mec_rsf_v9.cpp:1480-1521, and the
+ * output diagnostics of the same loop, :1522-1558 */
 static void final_stress(orc *o) {
     const hmec_params *P = &o->P;
     const int Ny = o->Ny, Nx = o->Nx;
@@ -1152,6 +1157,39 @@ static void final_stress(orc *o) {
             const double KXX = orc_divplus(dt * S_(o, HMEC_F_GGGP, i, j), S_(o, HMEC_F_ETAP, i, j));
             S_(o, HMEC_F_SXX, i, j) = 2 * S_(o, HMEC_F_ETAP, i, j) * S_(o, HMEC_F_EXX, i, j) * KXX + S_(o, HMEC_F_SXX0, i, j) * (1 - KXX);
             S_(o, HMEC_F_SYY, i, j) = 2 * S_(o, HMEC_F_ETAP, i, j) * S_(o, HMEC_F_EYY, i, j) * KXX + S_(o, HMEC_F_SYY0, i, j) * (1 - KXX);
+            /* :1522-1535 invariants and dissipation */
+            const double sxx = S_(o, HMEC_F_SXX, i, j), syy = S_(o, HMEC_F_SYY, i, j);
+            const double etap = S_(o, HMEC_F_ETAP, i, j), exx = S_(o, HMEC_F_EXX, i, j);
+#define SXYq(a, b) (B(o, HMEC_F_SXY, a, b) / B(o, HMEC_F_ETA, a, b))
+            S_(o, HMEC_F_EII, i, j) = sqrt(exx * exx + sqblock(B(o, HMEC_F_EXY, i - 1, j - 1), B(o, HMEC_F_EXY, i - 1, j),
+                                                               B(o, HMEC_F_EXY, i, j - 1), B(o, HMEC_F_EXY, i, j)) / 4.);
+            S_(o, HMEC_F_EIIVP, i, j) = sqrt(.5 * (SQ(sxx / (2 * etap)) + SQ(syy / (2 * etap))) +
+                                             sqblock(SXYq(i - 1, j - 1), SXYq(i - 1, j), SXYq(i, j - 1), SXYq(i, j)) / 16.);
+#undef SXYq
+            S_(o, HMEC_F_SII, i, j) = sqrt(.5 * (SQ(sxx) + SQ(syy)) + sqblock(B(o, HMEC_F_SXY, i - 1, j - 1), B(o, HMEC_F_SXY, i - 1, j),
+                                                                         B(o, HMEC_F_SXY, i, j - 1), B(o, HMEC_F_SXY, i, j)) / 4.);
+            const double disxy = (SQ(B(o, HMEC_F_SXY, i, j)) / B(o, HMEC_F_ETA, i, j) + SQ(B(o, HMEC_F_SXY, i - 1, j)) / B(o, HMEC_F_ETA, i - 1, j) +
+                                  SQ(B(o, HMEC_F_SXY, i, j - 1)) / B(o, HMEC_F_ETA, i, j - 1) + SQ(B(o, HMEC_F_SXY, i - 1, j - 1)) / B(o, HMEC_F_ETA, i - 1, j - 1)) / 2.;
+            S_(o, HMEC_F_DIS, i, j) = SQ(sxx) / (2 * etap) + SQ(syy) / (2 * etap) + disxy;
+            /* :1538-1557 averaged pressures and compaction */
+            double pt0a, pf0a;
+            if (i < Ny - 1) {
+                S_(o, HMEC_F_PT_AVE, i, j) = (S_(o, HMEC_F_PT, i, j) + S_(o, HMEC_F_PT, i + 1, j)) / 2.;
+                S_(o, HMEC_F_PF_AVE, i, j) = (S_(o, HMEC_F_PF, i, j) + S_(o, HMEC_F_PF, i + 1, j)) / 2.;
+                pt0a = (S_(o, HMEC_F_PT0, i, j) + S_(o, HMEC_F_PT0, i + 1, j)) / 2.;
+                pf0a = (S_(o, HMEC_F_PF0, i, j) + S_(o, HMEC_F_PF0, i + 1, j)) / 2.;
+            } else {
+                S_(o, HMEC_F_PT_AVE, i, j) = S_(o, HMEC_F_PT, i, j);
+                S_(o, HMEC_F_PF_AVE, i, j) = S_(o, HMEC_F_PF, i, j);
+                pt0a = S_(o, HMEC_F_PT0, i, j);
+                pf0a = S_(o, HMEC_F_PF0, i, j);
+            }
+            const double pta = S_(o, HMEC_F_PT_AVE, i, j), pfa = S_(o, HMEC_F_PF_AVE, i, j);
+            const double por = S_(o, HMEC_F_POR, i, j);
+            S_(o, HMEC_F_VIS_COMP, i, j) = (pta - pfa) / (S_(o, HMEC_F_ETAB, i, j) * (1 - por));
+            const double betadrained = (1 / S_(o, HMEC_F_GGGB, i, j) + P->BETASOLID) / (1 - por);
+            const double kbw = 1 - P->BETASOLID / betadrained;
+            S_(o, HMEC_F_EL_DECOM, i, j) = betadrained * (pta - pt0a - kbw * pfa + kbw * pf0a) / dt;
         }
 }
 

@@ -142,6 +142,14 @@ def group_errors(ctx, o):
     return out
 
 
+# output diagnostics of the final stress state (h_mec_rsf_v9.cpp:1522-1558);
+# EL_DECOM divides the one-step pressure change (pt - PT0, a small difference
+# of ~1e8 Pa values) by dt, so it inherits the pressures' error relative to
+# that difference
+DIAG = {"EII": 1e-10, "EIIVP": 1e-10, "SII": 1e-10, "DIS": 1e-10, "pt_ave": 1e-10,
+        "pf_ave": 1e-10, "VIS_COMP": 1e-10, "EL_DECOM": 1e-6}
+
+
 @pytest.mark.parametrize("nsteps", [12])
 def test_steps_cfg1(hmec, orc, gpu, nsteps):
     p, soa, ctx, o = make(hmec, orc, 1)
@@ -151,6 +159,11 @@ def test_steps_cfg1(hmec, orc, gpu, nsteps):
         assert a.iterations == b.iterations, s
         assert abs(a.dt - b.dt) <= 1e-12 * b.dt, s
         for name, (e, tol) in group_errors(ctx, o).items():
+            assert e < tol, (s, name, e)
+        for name, tol in DIAG.items():
+            gb = o.get(name)
+            assert np.abs(gb).max() > 0, name
+            e = rel(ctx.get(name), gb)
             assert e < tol, (s, name, e)
 
 

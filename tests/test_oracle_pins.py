@@ -443,3 +443,75 @@ def test_advection_rotation_invariants(hmec, orc):
     assert np.allclose(j2a, j2b, rtol=1e-12)
     # the rotation angle is the spin times dt
     assert not np.allclose(nxy, sxy)
+
+
+def test_diagnostics_closed_forms(hmec, orc):
+    """Output diagnostics of the final stress state (h_mec_rsf_v9.cpp:1522-1558)
+    on a constructed state: simple shear vxs = g y (EXY = g/2, EXX = 0) in the
+    elastic limit (dt -> 0: SXX, SYY, SXY keep their old values), SXY0 linear
+    in (i, j), pressures linear in i with a uniform step change.  For a
+    linear L over a 2x2 block the sum of squares is 4 Lc^2 + b^2 + c^2 (Lc at
+    the block centre), which gives EII, EIIVP, SII and DIS in closed form;
+    the averaged pressures are the mid-row values (last row: its own), and
+    the compaction terms follow from them."""
+    p = small(hmec)
+    o, _ = mk_oracle(hmec, orc, p)
+    o.step()                                   # controller state (dtx, dty, dtlapusta)
+    Ny, Nx, Ny1, Nx1 = p.Ny, p.Nx, p.Ny + 1, p.Nx + 1
+    g = 3e-13
+    yvx = np.linspace(-p.dy / 2, p.ysize + p.dy / 2, Ny1)
+    o.set("vxs", np.repeat((g * yvx)[:, None], Nx1, axis=1))
+    o.set("vys", np.zeros((Ny1, Nx1)))
+    eta, etap, etab, por, gggb = 2e21, 3e21, 5e21, 0.01, 2e10
+    o.set("ETA", np.full((Ny, Nx), eta))
+    o.set("ETAP", np.full((Ny1, Nx1), etap))
+    o.set("ETAB", np.full((Ny1, Nx1), etab))
+    o.set("POR", np.full((Ny1, Nx1), por))
+    o.set("GGGB", np.full((Ny1, Nx1), gggb))
+    a, b, c = 2e6, 3e4, -7e3
+    ii, jj = np.meshgrid(np.arange(Ny), np.arange(Nx), indexing="ij")
+    o.set("SXY0", a + b * ii + c * jj)
+    sxx0 = np.full((Ny1, Nx1), 4e5)
+    syy0 = np.full((Ny1, Nx1), -4e5)
+    o.set("SXX0", sxx0)
+    o.set("SYY0", syy0)
+    I = np.arange(Ny1)[:, None] * np.ones((1, Nx1))
+    pt = 1e8 + 2e5 * I
+    pf = 6e7 + 1e5 * I
+    dpt, dpf = 125.0, 50.0
+    o.set("pt", pt)
+    o.set("pf", pf)
+    o.set("PT0", pt - dpt)
+    o.set("PF0", pf - dpf)
+    o.dt = 1e-30
+    o.step_end()
+    dt = o.dt
+    assert dt == 1e-30
+    ci, cj = np.meshgrid(np.arange(1, Ny), np.arange(1, Nx), indexing="ij")
+    inner = (slice(1, Ny), slice(1, Nx))
+    lc = a + b * (ci - 0.5) + c * (cj - 0.5)
+    s2 = 4 * lc ** 2 + b ** 2 + c ** 2           # sum of SXY^2 over the block
+    sxx, syy = 4e5, -4e5
+    want = {
+        "EII": np.full(ci.shape, g / 2),
+        "SII": np.sqrt(0.5 * (sxx ** 2 + syy ** 2) + s2 / 4),
+        "EIIVP": np.sqrt(0.5 * ((sxx / (2 * etap)) ** 2 + (syy / (2 * etap)) ** 2) + s2 / eta ** 2 / 16),
+        "DIS": (sxx ** 2 + syy ** 2) / (2 * etap) + s2 / eta / 2,
+    }
+    for name, w in want.items():
+        got = o.get(name)[inner]
+        assert np.allclose(got, w, rtol=1e-12, atol=0), (name, np.abs(got / w - 1).max())
+    pta = np.where(ci < Ny - 1, 1e8 + 2e5 * (ci + 0.5), 1e8 + 2e5 * ci)
+    pfa = np.where(ci < Ny - 1, 6e7 + 1e5 * (ci + 0.5), 6e7 + 1e5 * ci)
+    assert np.allclose(o.get("pt_ave")[inner], pta, rtol=1e-15, atol=0)
+    assert np.allclose(o.get("pf_ave")[inner], pfa, rtol=1e-15, atol=0)
+    assert np.allclose(o.get("VIS_COMP")[inner], (pta - pfa) / (etab * (1 - por)), rtol=1e-14, atol=0)
+    bd = (1 / gggb + p.BETASOLID) / (1 - por)
+    kbw = 1 - p.BETASOLID / bd
+    el = bd * (dpt - kbw * dpf) / dt
+    assert np.allclose(o.get("EL_DECOM")[inner], el, rtol=1e-9, atol=0)
+    # cells outside the pressure-cell range stay 0
+    for name in ("EII", "SII", "DIS", "VIS_COMP", "EL_DECOM", "DSII"):
+        f = o.get(name)
+        assert not f[0, :].any() and not f[:, 0].any() and not f[Ny, :].any() and not f[:, Nx].any(), name
+    assert not o.get("DSII").any()
