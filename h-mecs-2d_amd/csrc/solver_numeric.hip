@@ -989,7 +989,8 @@ __global__ void k_cond_coef(const double *val, const int32_t *raslot, int64_t nt
     if (t < nt) rcoef[t] = val[raslot[t]];
 }
 // vs = rsS * (v_R / rs - A_RD gv)   (condensed right-hand side, equilibrated)
-// (zf, when given: the same values in the sweep's front order, zf[fpos[r]])
+// (zf, when given: the values go to the sweep's front order, zf[fpos[r]],
+// instead; the sweep then writes its solution into vs)
 __global__ void k_cond_rhs(const double *v, const double *rs, const double *rcoef, int64_t NS,
                            const int32_t *full_of_s, const int32_t *rptr, const int32_t *rd,
                            const double *gv, const double *rsS, double *vs, double *zf,
@@ -1000,8 +1001,8 @@ __global__ void k_cond_rhs(const double *v, const double *rs, const double *rcoe
     double s = rs ? v[k] / rs[k] : v[k];
     for (int t = rptr[r]; t < rptr[r + 1]; t++) s -= rcoef[t] * gv[rd[t]];
     const double o = rsS[r] * s;
-    vs[r] = o;
     if (zf) zf[fpos[r]] = o;
+    else vs[r] = o;
 }
 // z_R = csS * vs, z_D = gv + E z_R
 __global__ void k_expand(const double *vs, const double *csS, const double *gv, const double *e,
