@@ -16,6 +16,7 @@
 #define HMEC_HD __host__ __device__
 #include "assemble_emit.h"
 #include "pow2_scale.h"
+#include "xcd_map.h"
 #include "hmec_internal.h"
 
 // device scalar slots (ctx->d.scal)
@@ -267,7 +268,11 @@ k_scatter_grid(const double *__restrict__ sorted, int64_t M, const int32_t *__re
                double *__restrict__ fields, int64_t ns, int Nx, int Ny, double dx, double dy) {
     const int rows = (G == 0) ? Ny : Ny + 1;
     const int cols = (G == 0) ? Nx : Nx + 1;
-    const int t = blockIdx.x * (SCATTER_T / L) + threadIdx.x / L;
+    // 16 consecutive nodes (column-major) per workgroup; each XCD takes a
+    // contiguous strip of them, so the four nodes around a cell mostly
+    // read its markers through one L2
+    const int nwg = (rows * cols + SCATTER_T / L - 1) / (SCATTER_T / L);
+    const int t = (int)xcd_block(nwg) * (SCATTER_T / L) + threadIdx.x / L;
     const int l = threadIdx.x % L;
     if (t >= rows * cols) return;   // whole lane groups leave together
     const int i = t % rows, j = t / rows;
@@ -345,19 +350,19 @@ int k_scatter(hmec_ctx *c) {
     const hmec_params &P = c->P;
     const int nbasic = c->Nx * c->Ny, nstag = c->Nx1 * c->Ny1;
     constexpr int L = 16, NPB = SCATTER_T / L;
-    hipLaunchKernelGGL((k_scatter_grid<0, L>), dim3((nbasic + NPB - 1) / NPB), dim3(SCATTER_T), 0,
+    hipLaunchKernelGGL((k_scatter_grid<0, L>), dim3((unsigned)xcd_grid((nbasic + NPB - 1) / NPB)), dim3(SCATTER_T), 0,
                        c->stream, c->d.sorted, M, c->d.cell_start, k.x, k.y, c->d.fields, c->ns,
                        c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_scatter_grid<1, L>), dim3((nstag + NPB - 1) / NPB), dim3(SCATTER_T), 0,
+    hipLaunchKernelGGL((k_scatter_grid<1, L>), dim3((unsigned)xcd_grid((nstag + NPB - 1) / NPB)), dim3(SCATTER_T), 0,
                        c->stream, c->d.sorted, M, c->d.cell_start + (ncell + 1), k.xp, k.yp,
                        c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_scatter_grid<2, L>), dim3((nstag + NPB - 1) / NPB), dim3(SCATTER_T), 0,
+    hipLaunchKernelGGL((k_scatter_grid<2, L>), dim3((unsigned)xcd_grid((nstag + NPB - 1) / NPB)), dim3(SCATTER_T), 0,
                        c->stream, c->d.sorted, M, c->d.cell_start + 2 * (ncell + 1), k.xvx, k.yvx,
                        c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_scatter_grid<3, L>), dim3((nstag + NPB - 1) / NPB), dim3(SCATTER_T), 0,
+    hipLaunchKernelGGL((k_scatter_grid<3, L>), dim3((unsigned)xcd_grid((nstag + NPB - 1) / NPB)), dim3(SCATTER_T), 0,
                        c->stream, c->d.sorted, M, c->d.cell_start + 3 * (ncell + 1), k.xvy, k.yvy,
                        c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();

@@ -25,6 +25,7 @@
 #include "hmec_internal.h"
 #include "mfma_traits.h"
 #include "pow2_scale.h"
+#include "xcd_map.h"
 
 
 namespace {
@@ -784,7 +785,12 @@ k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
              int64_t ldv = 0, int nv = 0, double *__restrict__ part = nullptr) {
     __shared__ double prod[SPMV_CHUNK];
     __shared__ int32_t rp[SPMV_ROWS + 1];
-    const int64_t r0 = (int64_t)blockIdx.x * SPMV_ROWS;
+    // row blocks in contiguous ranges per XCD (the x entries a block reads
+    // overlap its neighbours')
+    const int64_t nblk = (N + SPMV_ROWS - 1) / SPMV_ROWS;
+    const int64_t blk = xcd_block(nblk);
+    if (blk >= nblk) return;
+    const int64_t r0 = blk * SPMV_ROWS;
     const int nr = (int)min((int64_t)SPMV_ROWS, N - r0);
     const int t = threadIdx.x;
     for (int k = t; k <= nr; k += SPMV_ROWS) rp[k] = rowp[r0 + k];
@@ -819,7 +825,7 @@ k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
             double d = 0.0;
 #pragma unroll
             for (int q = 0; q < SPMV_ROWS / 64; q++) d += red[q][t];
-            part[(int64_t)blockIdx.x * nv + t] = d;
+            part[blk * nv + t] = d;
         }
         return;
     }
@@ -836,7 +842,7 @@ k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
 // y = b - A x (y2 = rs * y when given)
 static void launch_residual(hmec_ctx *c, const double *x, double *y, double *y2, const double *rs) {
     const int64_t N = c->N;
-    hipLaunchKernelGGL(k_csr_stream<1>, dim3((unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS)),
+    hipLaunchKernelGGL(k_csr_stream<1>, dim3((unsigned)xcd_grid((N + SPMV_ROWS - 1) / SPMV_ROWS)),
                        dim3(SPMV_ROWS), 0, c->stream, c->d.rowp, c->d.coli, c->d.val, x, rs,
                        c->d.rhs, y, y2, N);
 }
@@ -844,14 +850,14 @@ static void launch_residual(hmec_ctx *c, const double *x, double *y, double *y2,
 static void launch_spmv_rs_dots(hmec_ctx *c, const double *x, double *y, const double *V,
                                 int nv, double *part) {
     const int64_t N = c->N;
-    hipLaunchKernelGGL(k_csr_stream<2>, dim3((unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS)),
+    hipLaunchKernelGGL(k_csr_stream<2>, dim3((unsigned)xcd_grid((N + SPMV_ROWS - 1) / SPMV_ROWS)),
                        dim3(SPMV_ROWS), 0, c->stream, c->d.rowp, c->d.coli, c->d.val, x,
                        c->d.rscale, nullptr, y, nullptr, N, V, N, nv, part);
 }
 // y = rs * (A x)
 static void launch_spmv_rs(hmec_ctx *c, const double *x, double *y) {
     const int64_t N = c->N;
-    hipLaunchKernelGGL(k_csr_stream<0>, dim3((unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS)),
+    hipLaunchKernelGGL(k_csr_stream<0>, dim3((unsigned)xcd_grid((N + SPMV_ROWS - 1) / SPMV_ROWS)),
                        dim3(SPMV_ROWS), 0, c->stream, c->d.rowp, c->d.coli, c->d.val, x,
                        c->d.rscale, nullptr, y, nullptr, N);
 }
