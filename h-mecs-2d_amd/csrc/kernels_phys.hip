@@ -253,61 +253,12 @@ size_t hmec_sort_tmp_bytes(int64_t M, int nbits) {
     return tmp;
 }
 
-// ----------------------------------------------------------------- scatter
-// L lanes per node of grid g.  The node gathers the markers of the (up to)
-// four anchor cells whose 2x2 block covers it, in fixed cell order; inside a
-// cell, lane l takes the cell's markers l, l + L, ... of its cell-ordered run
-// (the L lanes read consecutive addresses), and a butterfly over the L lanes
-// sums the partials — deterministic, no atomics.  The sums then go through the
-// reference's normalization (h_mec_rsf_v9.cpp:531-583).
-constexpr int SCATTER_T = 256;
-template <int G, int L>
-__global__ void __launch_bounds__(SCATTER_T)
-k_scatter_grid(const double *__restrict__ sorted, int64_t M, const int32_t *__restrict__ start,
-               const double *__restrict__ cx, const double *__restrict__ cyv,
-               double *__restrict__ fields, int64_t ns, int Nx, int Ny, double dx, double dy) {
-    const int rows = (G == 0) ? Ny : Ny + 1;
-    const int cols = (G == 0) ? Nx : Nx + 1;
-    // 16 consecutive nodes (column-major) per workgroup; each XCD takes a
-    // contiguous strip of them, so the four nodes around a cell mostly
-    // read its markers through one L2
-    const int nwg = (rows * cols + SCATTER_T / L - 1) / (SCATTER_T / L);
-    const int t = (int)xcd_block(nwg) * (SCATTER_T / L) + threadIdx.x / L;
-    const int l = threadIdx.x % L;
-    if (t >= rows * cols) return;   // whole lane groups leave together
-    const int i = t % rows, j = t / rows;
-    const int cy = Ny - 1, cxn = Nx - 1;
-    constexpr int NF = (G == 0) ? 10 : (G == 1) ? 5 : 4;
-    double acc[NF + 1];
-#pragma unroll
-    for (int f = 0; f <= NF; f++) acc[f] = 0.;
-    const double *src = sorted + (int64_t)SORTED_OFF[G] * M;   // this grid's cell-ordered streams
-    const double *xm = src, *ym = src + M;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int di = q & 1, dj = q >> 1;          // corner of the node in the cell
-        const int ci = i - di, cj = j - dj;         // cell (ci, cj)
-        if (ci < 0 || cj < 0 || ci >= cy || cj >= cxn) continue;
-        const int cid = ci + cj * cy;
-        const double x0 = cx[cj], y0 = cyv[ci];
-        const int end = start[cid + 1];
-        for (int k = start[cid] + l; k < end; k += L) {
-            const double dxm = (xm[k] - x0) / dx;
-            const double dym = (ym[k] - y0) / dy;
-            const double w = (di ? dym : (1 - dym)) * (dj ? dxm : (1 - dxm));
-#pragma unroll
-            for (int f = 0; f < NF; f++) acc[f] += src[(int64_t)(2 + f) * M + k] * w;
-            acc[NF] += w;
-        }
-    }
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1)
-#pragma unroll
-        for (int f = 0; f <= NF; f++) acc[f] += __shfl_xor(acc[f], o, L);
-    if (l != 0) return;
+// normalized sums of one node -> its fields (skipped where no marker weight)
+template <int G, int NF>
+__device__ __forceinline__ void scatter_store(const double *acc, double *__restrict__ fields,
+                                              int64_t ns, int64_t idx) {
     const double wt = acc[NF];
     if (!(wt > 0)) return;
-    const int64_t idx = t;   // i + j * rows
 #define FLD(f) fields[(int64_t)(f) * ns + idx]
     if (G == 0) {
         FLD(HMEC_F_RHO) = acc[0] / wt;
@@ -341,6 +292,94 @@ k_scatter_grid(const double *__restrict__ sorted, int64_t M, const int32_t *__re
 #undef FLD
 }
 
+// ----------------------------------------------------------------- scatter
+// L lanes per run of R vertically adjacent nodes (i0 .. i0 + R - 1, column j)
+// of grid g.  The run reads each marker of the cells around it once: cells
+// (i0 - 1 .. i0 + R - 1) x (j, j - 1), in that order; a marker of cell
+// (ci, cj) adds to node ci with weight (1 - dym) wx and to node ci + 1 with
+// weight dym wx (wx = 1 - dxm in column j, dxm in column j - 1) where those
+// nodes belong to the run.  Inside a cell, lane l takes the cell's markers
+// l, l + L, ... of its cell-ordered run (the L lanes read consecutive
+// addresses), and a butterfly over the L lanes sums each node's partials —
+// deterministic, no atomics.  The sums then go through the reference's
+// normalization (h_mec_rsf_v9.cpp:531-583).
+constexpr int SCATTER_T = 256;
+constexpr int SCATTER_R = 2;   // nodes per run
+template <int G, int L, int R>
+__global__ void __launch_bounds__(SCATTER_T)
+k_scatter_grid(const double *__restrict__ sorted, int64_t M, const int32_t *__restrict__ start,
+               const double *__restrict__ cx, const double *__restrict__ cyv,
+               double *__restrict__ fields, int64_t ns, int Nx, int Ny, double dx, double dy) {
+    const int rows = (G == 0) ? Ny : Ny + 1;
+    const int cols = (G == 0) ? Nx : Nx + 1;
+    // runs in column-major order, 256 / L runs per workgroup; each XCD takes
+    // a contiguous strip of them, so the runs around a cell mostly read its
+    // markers through one L2
+    const int rpc = (rows + R - 1) / R;                 // runs per column
+    const int nrun = rpc * cols;
+    const int nwg = (nrun + SCATTER_T / L - 1) / (SCATTER_T / L);
+    const int g = (int)xcd_block(nwg) * (SCATTER_T / L) + threadIdx.x / L;
+    const int l = threadIdx.x % L;
+    if (g >= nrun) return;   // whole lane groups leave together
+    const int j = g / rpc, i0 = (g % rpc) * R;
+    const int cy = Ny - 1, cxn = Nx - 1;
+    constexpr int NF = (G == 0) ? 10 : (G == 1) ? 5 : 4;
+    double acc[R][NF + 1];
+#pragma unroll
+    for (int u = 0; u < R; u++)
+#pragma unroll
+        for (int f = 0; f <= NF; f++) acc[u][f] = 0.;
+    const double *src = sorted + (int64_t)SORTED_OFF[G] * M;   // this grid's cell-ordered streams
+    const double *xm = src, *ym = src + M;
+#pragma unroll
+    for (int dj = 0; dj < 2; dj++) {
+        const int cj = j - dj;
+        if (cj < 0 || cj >= cxn) continue;
+        const double x0 = cx[cj];
+#pragma unroll
+        for (int u = -1; u < R; u++) {                  // cell row ci = i0 + u: nodes u, u + 1
+            const int ci = i0 + u;
+            if (ci < 0 || ci >= cy) continue;
+            const int cid = ci + cj * cy;
+            const double y0 = cyv[ci];
+            const int end = start[cid + 1];
+            for (int k = start[cid] + l; k < end; k += L) {
+                const double dxm = (xm[k] - x0) / dx;
+                const double dym = (ym[k] - y0) / dy;
+                const double wx = dj ? dxm : (1 - dxm);
+                double pv[NF];
+#pragma unroll
+                for (int f = 0; f < NF; f++) pv[f] = src[(int64_t)(2 + f) * M + k];
+                if (u >= 0) {
+                    const double w = (1 - dym) * wx;
+#pragma unroll
+                    for (int f = 0; f < NF; f++) acc[u < 0 ? 0 : u][f] += pv[f] * w;
+                    acc[u < 0 ? 0 : u][NF] += w;
+                }
+                if (u + 1 < R) {
+                    const double w = dym * wx;
+#pragma unroll
+                    for (int f = 0; f < NF; f++) acc[u + 1 < R ? u + 1 : 0][f] += pv[f] * w;
+                    acc[u + 1 < R ? u + 1 : 0][NF] += w;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int u = 0; u < R; u++)
+#pragma unroll
+            for (int f = 0; f <= NF; f++) acc[u][f] += __shfl_xor(acc[u][f], o, L);
+    if (l != 0) return;
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+        const int i = i0 + u;
+        if (i >= rows) break;
+        scatter_store<G, NF>(acc[u], fields, ns, i + (int64_t)j * rows);
+    }
+}
+
 int k_scatter(hmec_ctx *c) {
     if (!c->have_markers) return HMEC_E_ARG;
     if (c->markers_dirty) HMEC_TRY(k_marker_prep(c));
@@ -348,21 +387,25 @@ int k_scatter(hmec_ctx *c) {
     const int ncell = (c->Nx - 1) * (c->Ny - 1);
     const int64_t M = c->M;
     const hmec_params &P = c->P;
-    const int nbasic = c->Nx * c->Ny, nstag = c->Nx1 * c->Ny1;
     constexpr int L = 16, NPB = SCATTER_T / L;
-    hipLaunchKernelGGL((k_scatter_grid<0, L>), dim3((unsigned)xcd_grid((nbasic + NPB - 1) / NPB)), dim3(SCATTER_T), 0,
+    // workgroups of a grid with rows x cols nodes in runs of SCATTER_R
+    auto blocks = [&](int rows, int cols) -> unsigned {
+        const int64_t runs = (int64_t)(rows + SCATTER_R - 1) / SCATTER_R * cols;
+        return (unsigned)xcd_grid((runs + NPB - 1) / NPB);
+    };
+    hipLaunchKernelGGL((k_scatter_grid<0, L, SCATTER_R>), dim3(blocks(c->Ny, c->Nx)), dim3(SCATTER_T), 0,
                        c->stream, c->d.sorted, M, c->d.cell_start, k.x, k.y, c->d.fields, c->ns,
                        c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_scatter_grid<1, L>), dim3((unsigned)xcd_grid((nstag + NPB - 1) / NPB)), dim3(SCATTER_T), 0,
+    hipLaunchKernelGGL((k_scatter_grid<1, L, SCATTER_R>), dim3(blocks(c->Ny1, c->Nx1)), dim3(SCATTER_T), 0,
                        c->stream, c->d.sorted, M, c->d.cell_start + (ncell + 1), k.xp, k.yp,
                        c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_scatter_grid<2, L>), dim3((unsigned)xcd_grid((nstag + NPB - 1) / NPB)), dim3(SCATTER_T), 0,
+    hipLaunchKernelGGL((k_scatter_grid<2, L, SCATTER_R>), dim3(blocks(c->Ny1, c->Nx1)), dim3(SCATTER_T), 0,
                        c->stream, c->d.sorted, M, c->d.cell_start + 2 * (ncell + 1), k.xvx, k.yvx,
                        c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_scatter_grid<3, L>), dim3((unsigned)xcd_grid((nstag + NPB - 1) / NPB)), dim3(SCATTER_T), 0,
+    hipLaunchKernelGGL((k_scatter_grid<3, L, SCATTER_R>), dim3(blocks(c->Ny1, c->Nx1)), dim3(SCATTER_T), 0,
                        c->stream, c->d.sorted, M, c->d.cell_start + 3 * (ncell + 1), k.xvy, k.yvy,
                        c->d.fields, c->ns, c->Nx, c->Ny, P.dx, P.dy);
     LAUNCH_CHECK();
