@@ -1281,8 +1281,8 @@ int k_iter_commit(hmec_ctx *c, int ynpl, int converged) {
 __global__ void k_diagnostics(double *f, int64_t ns, int Nx, int Ny, double dt, double betasolid) {
     const int Ny1 = Ny + 1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = 1 + t % (Ny - 1), j = 1 + t / (Ny - 1);
     if (t >= (Nx - 1) * (Ny - 1)) return;
+    const int i = 1 + t % (Ny - 1), j = 1 + t / (Ny - 1);
     const int64_t s = i + (int64_t)j * Ny1;
     const int64_t b00 = (i - 1) + (int64_t)(j - 1) * Ny, b01 = (i - 1) + (int64_t)j * Ny;
     const int64_t b10 = i + (int64_t)(j - 1) * Ny, b11 = i + (int64_t)j * Ny;
