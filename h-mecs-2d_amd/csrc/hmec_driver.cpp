@@ -75,6 +75,49 @@ int factor(hmec_ctx *c, int precision) {
     return c->batcher ? batcher_factorize(c->batcher, c, precision) : sv_factorize(c, precision);
 }
 
+// FP32 GMRES starts from the context's last solution.  When dt has changed
+// since that solve, the last solution computed at (nearly) the current dt is
+// usually the closer start: the controller alternates dt between a few
+// values (an increased one on a step's first Picard iteration, the reduced
+// one after it), and the state moves little from one step to the next.
+double dt_dist(double a, double b) { return (a > 0 && b > 0) ? std::fabs(std::log(a / b)) : INFINITY; }
+
+int guess_start(hmec_ctx *c) {
+    if (c->last_solve_dt == c->dt) return HMEC_OK;
+    const double cur = dt_dist(c->dt, c->last_solve_dt);
+    int best = -1;
+    double bd = cur;
+    for (int k = 0; k < 2; k++) {
+        const double d = c->xguess[k].x ? dt_dist(c->dt, c->xguess[k].dt) : INFINITY;
+        if (d < bd) {
+            bd = d;
+            best = k;
+        }
+    }
+    if (best < 0) return HMEC_OK;
+    c->xguess[best].used = ++c->xguess_clock;
+    HMEC_HIP(hipMemcpyAsync(c->d.sol, c->xguess[best].x, sizeof(double) * c->N, hipMemcpyDeviceToDevice,
+                            c->stream));
+    return HMEC_OK;
+}
+
+// keep the solution as the guess for its dt (the entry of that dt, else an
+// empty or the least recently used one)
+int guess_keep(hmec_ctx *c) {
+    int k = -1;
+    for (int q = 0; q < 2; q++)
+        if (c->xguess[q].x && c->xguess[q].dt == c->dt) k = q;
+    if (k < 0) {
+        k = !c->xguess[0].x ? 0 : !c->xguess[1].x ? 1 : (c->xguess[0].used <= c->xguess[1].used ? 0 : 1);
+        if (!c->xguess[k].x) HMEC_HIP(hipMalloc(&c->xguess[k].x, sizeof(double) * c->N));
+    }
+    c->xguess[k].dt = c->dt;
+    c->xguess[k].used = ++c->xguess_clock;
+    HMEC_HIP(hipMemcpyAsync(c->xguess[k].x, c->d.sol, sizeof(double) * c->N, hipMemcpyDeviceToDevice,
+                            c->stream));
+    return HMEC_OK;
+}
+
 // Linear solve of the freshly assembled system under the context's precision
 // policy.  FP64: numeric refactorization every Picard iteration, like
 // PardisoLU (h_mec_rsf_v9.cpp:1050-1052).  FP32: the equilibrated FP32 factor
@@ -117,6 +160,7 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         };
         if (k < 0) HMEC_TRY(refactor(sv_slot_victim(c, -1)));
         else HMEC_TRY(sv_slot_activate(c, k));
+        HMEC_TRY(guess_start(c));
         tm_begin(c, c->t_solve);
         int rc = sv_gmres(c, fresh ? c->ir_max_fresh : c->ir_max_stale, c->ir_tol, &st);
         tm_end(c, c->t_solve, 0);
@@ -137,6 +181,8 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
             if (rc == HMEC_E_NOCONV) rc = HMEC_OK;
         }
         if (rc != HMEC_OK) return rc;
+        HMEC_TRY(guess_keep(c));
+        c->last_solve_dt = c->dt;
     }
     acc.solves++;
     if (c->precision == HMEC_PREC_FP64) acc.ir_iterations += st.ir_iterations;

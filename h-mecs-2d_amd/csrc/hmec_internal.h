@@ -248,6 +248,17 @@ struct hmec_ctx {
     double last_res = 0;
     int stale_solves = 0;            // FP32: solves since the last factorization
     double factor_dt = -1;           // dt of the system the factor was computed for
+    // FP32 GMRES initial guesses: the last solution computed at each of the
+    // recent dt values (the controller alternates between a few), and the dt
+    // of the last solve
+    struct XGuess {
+        double *x = nullptr;
+        double dt = 0;
+        int64_t used = 0;
+    };
+    XGuess xguess[2];
+    int64_t xguess_clock = 0;
+    double last_solve_dt = 0;
     int ir_max_stale = 12;           // FP32 policy: GMRES budget with a reused factor
     int ir_max_fresh = 40;           //              ... with a fresh factor
     double ir_tol = 1e-12;           //              target relative inf-norm residual
