@@ -101,12 +101,12 @@ int guess_start(hmec_ctx *c) {
     return HMEC_OK;
 }
 
-// keep the solution as the guess for its dt (the entry of that dt, else an
-// empty or the least recently used one)
+// keep the solution as the guess for its dt (the entry of that dt, within
+// 1 %, else an empty or the least recently used one)
 int guess_keep(hmec_ctx *c) {
     int k = -1;
     for (int q = 0; q < 2; q++)
-        if (c->xguess[q].x && c->xguess[q].dt == c->dt) k = q;
+        if (c->xguess[q].x && dt_dist(c->xguess[q].dt, c->dt) < 0.01) k = q;
     if (k < 0) {
         k = !c->xguess[0].x ? 0 : !c->xguess[1].x ? 1 : (c->xguess[0].used <= c->xguess[1].used ? 0 : 1);
         if (!c->xguess[k].x) HMEC_HIP(hipMalloc(&c->xguess[k].x, sizeof(double) * c->N));
