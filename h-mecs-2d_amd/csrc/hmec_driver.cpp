@@ -75,46 +75,76 @@ int factor(hmec_ctx *c, int precision) {
     return c->batcher ? batcher_factorize(c->batcher, c, precision) : sv_factorize(c, precision);
 }
 
-// FP32 GMRES starts from the context's last solution.  When dt has changed
-// since that solve, the last solution computed at (nearly) the current dt is
-// usually the closer start: the controller alternates dt between a few
-// values (an increased one on a step's first Picard iteration, the reduced
-// one after it), and the state moves little from one step to the next.
+// FP32 GMRES starts from the context's last solution, or from a kept one
+// that is usually closer:
+// - from the third Picard iteration of a step on, the solution of the same
+//   iteration at the same dt in an earlier step (the iterations of a step
+//   repeat their pattern from one step to the next, and the state moves
+//   little);
+// - when dt has changed since the previous solve, the last solution
+//   computed at (nearly) the current dt: the controller alternates dt
+//   between a few values (an increased one on a step's first Picard
+//   iteration, the reduced one after it).
 double dt_dist(double a, double b) { return (a > 0 && b > 0) ? std::fabs(std::log(a / b)) : INFINITY; }
 
+// the entry kept for (dt within 1 %, it), -1 if none
+int guess_find(hmec_ctx *c, int it) {
+    for (int k = 0; k < hmec_ctx::NGUESS; k++)
+        if (c->xguess[k].x && c->xguess[k].it == it && dt_dist(c->xguess[k].dt, c->dt) < 0.01) return k;
+    return -1;
+}
+
+int guess_load(hmec_ctx *c, int k) {
+    c->xguess[k].used = ++c->xguess_clock;
+    HMEC_HIP(hipMemcpyAsync(c->d.sol, c->xguess[k].x, sizeof(double) * c->N, hipMemcpyDeviceToDevice,
+                            c->stream));
+    return HMEC_OK;
+}
+
 int guess_start(hmec_ctx *c) {
+    if (c->iterstep >= 2) {
+        const int k = guess_find(c, c->iterstep);
+        if (k >= 0) return guess_load(c, k);
+    }
     if (c->last_solve_dt == c->dt) return HMEC_OK;
-    const double cur = dt_dist(c->dt, c->last_solve_dt);
     int best = -1;
-    double bd = cur;
-    for (int k = 0; k < 2; k++) {
-        const double d = c->xguess[k].x ? dt_dist(c->dt, c->xguess[k].dt) : INFINITY;
+    double bd = dt_dist(c->dt, c->last_solve_dt);
+    for (int k = 0; k < hmec_ctx::NGUESS; k++) {
+        const double d = (c->xguess[k].x && c->xguess[k].it < 0) ? dt_dist(c->dt, c->xguess[k].dt) : INFINITY;
         if (d < bd) {
             bd = d;
             best = k;
         }
     }
-    if (best < 0) return HMEC_OK;
-    c->xguess[best].used = ++c->xguess_clock;
-    HMEC_HIP(hipMemcpyAsync(c->d.sol, c->xguess[best].x, sizeof(double) * c->N, hipMemcpyDeviceToDevice,
+    return best < 0 ? HMEC_OK : guess_load(c, best);
+}
+
+// keep the solution for (dt, it): its entry, else an empty or the least
+// recently used one
+int guess_store(hmec_ctx *c, int it) {
+    int k = guess_find(c, it);
+    if (k < 0) {
+        k = 0;
+        for (int q = 0; q < hmec_ctx::NGUESS; q++) {
+            if (!c->xguess[q].x) {
+                k = q;
+                break;
+            }
+            if (c->xguess[q].used < c->xguess[k].used) k = q;
+        }
+        if (!c->xguess[k].x) HMEC_HIP(hipMalloc(&c->xguess[k].x, sizeof(double) * c->N));
+    }
+    c->xguess[k].dt = c->dt;
+    c->xguess[k].it = it;
+    c->xguess[k].used = ++c->xguess_clock;
+    HMEC_HIP(hipMemcpyAsync(c->xguess[k].x, c->d.sol, sizeof(double) * c->N, hipMemcpyDeviceToDevice,
                             c->stream));
     return HMEC_OK;
 }
 
-// keep the solution as the guess for its dt (the entry of that dt, within
-// 1 %, else an empty or the least recently used one)
 int guess_keep(hmec_ctx *c) {
-    int k = -1;
-    for (int q = 0; q < 2; q++)
-        if (c->xguess[q].x && dt_dist(c->xguess[q].dt, c->dt) < 0.01) k = q;
-    if (k < 0) {
-        k = !c->xguess[0].x ? 0 : !c->xguess[1].x ? 1 : (c->xguess[0].used <= c->xguess[1].used ? 0 : 1);
-        if (!c->xguess[k].x) HMEC_HIP(hipMalloc(&c->xguess[k].x, sizeof(double) * c->N));
-    }
-    c->xguess[k].dt = c->dt;
-    c->xguess[k].used = ++c->xguess_clock;
-    HMEC_HIP(hipMemcpyAsync(c->xguess[k].x, c->d.sol, sizeof(double) * c->N, hipMemcpyDeviceToDevice,
-                            c->stream));
+    HMEC_TRY(guess_store(c, -1));
+    if (c->iterstep >= 2) HMEC_TRY(guess_store(c, c->iterstep));
     return HMEC_OK;
 }
 

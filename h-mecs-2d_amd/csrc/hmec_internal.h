@@ -249,14 +249,17 @@ struct hmec_ctx {
     int stale_solves = 0;            // FP32: solves since the last factorization
     double factor_dt = -1;           // dt of the system the factor was computed for
     // FP32 GMRES initial guesses: the last solution computed at each of the
-    // recent dt values (the controller alternates between a few), and the dt
-    // of the last solve
+    // recent dt values (the controller alternates between a few; it = -1),
+    // and at each (dt, Picard iteration >= 2) (it = the iteration), and the
+    // dt of the last solve
     struct XGuess {
         double *x = nullptr;
         double dt = 0;
+        int it = -1;
         int64_t used = 0;
     };
-    XGuess xguess[2];
+    static constexpr int NGUESS = 6;
+    XGuess xguess[NGUESS];
     int64_t xguess_clock = 0;
     double last_solve_dt = 0;
     int ir_max_stale = 12;           // FP32 policy: GMRES budget with a reused factor
