@@ -188,7 +188,16 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
             fresh = true;
             return HMEC_OK;
         };
+        // a dt more than 10 % from every kept factor, with a free slot (or
+        // one whose factor no solve has used lately): factor for it right
+        // away rather than spend the GMRES budget on the factor of another dt
+        int idle = -1;
+        if (k >= 0 && dist > 0.1) {
+            const int v = sv_slot_victim(c, k);
+            if (v != k && sv_slot_idle(c, v, 16)) idle = v;
+        }
         if (k < 0) HMEC_TRY(refactor(sv_slot_victim(c, -1)));
+        else if (idle >= 0) HMEC_TRY(refactor(idle));
         else HMEC_TRY(sv_slot_activate(c, k));
         HMEC_TRY(guess_start(c));
         tm_begin(c, c->t_solve);

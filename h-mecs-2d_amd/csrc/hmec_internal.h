@@ -286,6 +286,7 @@ int k_assemble(hmec_ctx *c, double dt);   // ptscale from the device scalars
 int k_reload(hmec_ctx *c, double ptscale);
 int hmec_update_dev(hmec_ctx *c, double dt, hmec_iter_scalars *out);
 constexpr int SC_PTSCALE_SLOT = 0;   // ptscale's index in the device scalars
+constexpr int HMEC_FP32_SLOTS = 3;   // FP32 factors kept by default (keyed by dt)
 int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out);
 int k_iter_commit(hmec_ctx *c, int ynpl, int converged);
 int k_final_stress(hmec_ctx *c, double dt);
@@ -316,6 +317,7 @@ int sv_slots_set(hmec_ctx *c, int n);
 int sv_slots_count(hmec_ctx *c);
 int sv_slot_nearest(hmec_ctx *c, double dt, double *dist);
 int sv_slot_victim(hmec_ctx *c, int keep);
+bool sv_slot_idle(hmec_ctx *c, int k, int64_t n);
 void sv_slot_target(hmec_ctx *c, int k);
 int sv_slot_activate(hmec_ctx *c, int k);
 void sv_active_scales(hmec_ctx *c, const double **rs, const double **cs);

@@ -201,8 +201,8 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     if (precision == HMEC_PREC_FP64) {
         b += z.arena * 8 + z.dinv * 8;
     } else {
-        // two FP32 factor slots (sweep form + scales each)
-        b += z.arena * 4 + z.dinv * 4 + 2 * (z.sweep * 4 + 2 * NS * 8) + (2 * 20 + 3) * N * 8;
+        // the FP32 factor slots (sweep form + scales each)
+        b += z.arena * 4 + z.dinv * 4 + HMEC_FP32_SLOTS * (z.sweep * 4 + 2 * NS * 8) + (2 * 20 + 3) * N * 8;
         b += hmec_ctx::NGUESS * N * 8;                                  // GMRES initial guesses
     }
     return b;

@@ -451,7 +451,7 @@ struct SweepDev {
         int64_t used = 0;
         bool valid = false;
     };
-    std::vector<Slot> slots = std::vector<Slot>(2);
+    std::vector<Slot> slots = std::vector<Slot>(HMEC_FP32_SLOTS);
     int cur = 0;                                 // slot of the active factor
     int target = 0;                              // slot the next FP32 factorization fills
     int64_t clock = 0;
@@ -795,6 +795,13 @@ int sv_slot_victim(hmec_ctx *c, int keep) {
 }
 
 void sv_slot_target(hmec_ctx *c, int k) { swdev(c)->target = k; }
+
+// slot k holds no factor, or one no solve has used in the last n activations
+bool sv_slot_idle(hmec_ctx *c, int k, int64_t n) {
+    SweepDev *s = swdev(c);
+    if (k < 0 || k >= (int)s->slots.size()) return false;
+    return !s->slots[k].valid || s->clock - s->slots[k].used >= n;
+}
 
 // make slot k the active FP32 factor (its scales are read in place, see
 // sv_active_scales)

@@ -327,7 +327,7 @@ class Context:
         check(lib().hmec_set_refinement(self.h, int(max_stale), int(max_fresh), float(tol)),
               "set_refinement")
 
-    def set_factor_cache(self, slots=2):
+    def set_factor_cache(self, slots=3):
         check(lib().hmec_set_factor_cache(self.h, int(slots)), "set_factor_cache")
 
     def init_coseismic(self, frac=0.98, noise=0.02, pf_ratio=0.7, seed=4):

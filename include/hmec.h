@@ -199,7 +199,7 @@ int hmec_set_precision(hmec_ctx *ctx, int precision);
  * inf-norm residual (defaults 12, 40, 1e-12). */
 int hmec_set_refinement(hmec_ctx *ctx, int max_stale, int max_fresh, double tol);
 /* Number of FP32 factors HMEC_PREC_FP32 keeps, keyed by the dt they were
- * computed at (1..8, default 2; each costs one sweep arena). */
+ * computed at (1..8, default 3; each costs one sweep arena). */
 int hmec_set_factor_cache(hmec_ctx *ctx, int slots);
 /* Coseismic initial state (the pre-stressed fault workload): lithostatic
  * pressure PT0 = PCONF + PTFDIFF with seeded relative noise of `noise`
