@@ -93,7 +93,10 @@ static void free_ctx(hmec_ctx *c) {
     hipFree(d.rowp); hipFree(d.coli); hipFree(d.val); hipFree(d.rhs); hipFree(d.sol);
     hipFree(d.trip_slot); hipFree(d.node_trip); hipFree(d.rscale); hipFree(d.cscale);
     hipFree(d.work); hipFree(d.work2); hipFree(d.counters);
-    for (auto &g : c->xguess) hipFree(g.x);
+    for (auto &g : c->xguess) {
+        hipFree(g.x);
+        hipFree(g.x2);
+    }
     for (Timer *t : {&c->t_scatter, &c->t_assemble, &c->t_factor, &c->t_solve}) tm_free(*t);
     for (auto &kv : c->kt) tm_free(kv.second);
     if (c->stream) hipStreamDestroy(c->stream);

@@ -76,15 +76,18 @@ int factor(hmec_ctx *c, int precision) {
 }
 
 // FP32 GMRES starts from the context's last solution, or from a kept one
-// that is usually closer:
-// - from the third Picard iteration of a step on, the solution of the same
-//   iteration at the same dt in an earlier step (the iterations of a step
-//   repeat their pattern from one step to the next, and the state moves
-//   little);
-// - when dt has changed since the previous solve, the last solution
-//   computed at (nearly) the current dt: the controller alternates dt
-//   between a few values (an increased one on a step's first Picard
-//   iteration, the reduced one after it).
+// that is usually closer.  Kept per key (dt within 1 %, Picard iteration):
+// the newest solution and the one of an earlier step, so the start can be
+// extrapolated linearly in model time (the interseismic state moves
+// steadily from step to step):
+// - the first solve of a step, or the first at a changed dt: the key (dt,
+//   any iteration), i.e. the last solutions computed at that dt, whose
+//   newest is the previous step's final one at the reduced dt (the
+//   controller alternates dt between an increased value on a step's first
+//   Picard iteration and the reduced one after it);
+// - from the third Picard iteration on: the key (dt, that iteration), the
+//   same iteration of earlier steps (the iterations of a step repeat their
+//   pattern from one step to the next).
 double dt_dist(double a, double b) { return (a > 0 && b > 0) ? std::fabs(std::log(a / b)) : INFINITY; }
 
 // the entry kept for (dt within 1 %, it), -1 if none
@@ -94,10 +97,16 @@ int guess_find(hmec_ctx *c, int it) {
     return -1;
 }
 
+// x0 from entry k: its newest solution, extrapolated to the current step's
+// time through the earlier one when there is one
 int guess_load(hmec_ctx *c, int k) {
-    c->xguess[k].used = ++c->xguess_clock;
-    HMEC_HIP(hipMemcpyAsync(c->d.sol, c->xguess[k].x, sizeof(double) * c->N, hipMemcpyDeviceToDevice,
-                            c->stream));
+    hmec_ctx::XGuess &g = c->xguess[k];
+    g.used = ++c->xguess_clock;
+    if (g.has2 && g.t > g.t2 && c->timesum > g.t) {
+        const double r = (c->timesum - g.t) / (g.t - g.t2);
+        return sv_lincomb(c, c->d.sol, 1.0 + r, g.x, -r, g.x2);
+    }
+    HMEC_HIP(hipMemcpyAsync(c->d.sol, g.x, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
     return HMEC_OK;
 }
 
@@ -106,7 +115,13 @@ int guess_start(hmec_ctx *c) {
         const int k = guess_find(c, c->iterstep);
         if (k >= 0) return guess_load(c, k);
     }
+    if (c->iterstep == 0 || c->last_solve_dt != c->dt) {
+        const int k = guess_find(c, -1);
+        if (k >= 0) return guess_load(c, k);
+    }
     if (c->last_solve_dt == c->dt) return HMEC_OK;
+    // no entry at this dt: the kept solution of the nearest dt, when that
+    // is nearer than the previous solve's
     int best = -1;
     double bd = dt_dist(c->dt, c->last_solve_dt);
     for (int k = 0; k < hmec_ctx::NGUESS; k++) {
@@ -116,11 +131,16 @@ int guess_start(hmec_ctx *c) {
             best = k;
         }
     }
-    return best < 0 ? HMEC_OK : guess_load(c, best);
+    if (best < 0) return HMEC_OK;
+    c->xguess[best].used = ++c->xguess_clock;
+    HMEC_HIP(hipMemcpyAsync(c->d.sol, c->xguess[best].x, sizeof(double) * c->N, hipMemcpyDeviceToDevice,
+                            c->stream));
+    return HMEC_OK;
 }
 
-// keep the solution for (dt, it): its entry, else an empty or the least
-// recently used one
+// keep the solution for (dt, it): its entry (the newest solution of an
+// earlier step becomes the entry's earlier one; within a step the newest is
+// replaced), else an empty or the least recently used entry
 int guess_store(hmec_ctx *c, int it) {
     int k = guess_find(c, it);
     if (k < 0) {
@@ -132,13 +152,23 @@ int guess_store(hmec_ctx *c, int it) {
             }
             if (c->xguess[q].used < c->xguess[k].used) k = q;
         }
-        if (!c->xguess[k].x) HMEC_HIP(hipMalloc(&c->xguess[k].x, sizeof(double) * c->N));
+        hmec_ctx::XGuess &g = c->xguess[k];
+        if (!g.x) HMEC_HIP(hipMalloc(&g.x, sizeof(double) * c->N));
+        g.has2 = false;
+        g.t = c->timesum;
     }
-    c->xguess[k].dt = c->dt;
-    c->xguess[k].it = it;
-    c->xguess[k].used = ++c->xguess_clock;
-    HMEC_HIP(hipMemcpyAsync(c->xguess[k].x, c->d.sol, sizeof(double) * c->N, hipMemcpyDeviceToDevice,
-                            c->stream));
+    hmec_ctx::XGuess &g = c->xguess[k];
+    if (g.t != c->timesum) {
+        if (!g.x2) HMEC_HIP(hipMalloc(&g.x2, sizeof(double) * c->N));
+        std::swap(g.x, g.x2);
+        g.t2 = g.t;
+        g.has2 = true;
+    }
+    g.dt = c->dt;
+    g.t = c->timesum;
+    g.it = it;
+    g.used = ++c->xguess_clock;
+    HMEC_HIP(hipMemcpyAsync(g.x, c->d.sol, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
     return HMEC_OK;
 }
 

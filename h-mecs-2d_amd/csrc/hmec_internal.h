@@ -248,13 +248,14 @@ struct hmec_ctx {
     double last_res = 0;
     int stale_solves = 0;            // FP32: solves since the last factorization
     double factor_dt = -1;           // dt of the system the factor was computed for
-    // FP32 GMRES initial guesses: the last solution computed at each of the
-    // recent dt values (the controller alternates between a few; it = -1),
-    // and at each (dt, Picard iteration >= 2) (it = the iteration), and the
-    // dt of the last solve
+    // FP32 GMRES initial guesses (hmec_driver.cpp): per key (dt within 1 %,
+    // Picard iteration it, -1 for any iteration) the last solution and the
+    // one kept at the key in an earlier step, with the model times of their
+    // steps; the dt of the last solve
     struct XGuess {
-        double *x = nullptr;
-        double dt = 0;
+        double *x = nullptr, *x2 = nullptr;   // newest, and the earlier step's
+        double dt = 0, t = 0, t2 = 0;
+        bool has2 = false;
         int it = -1;
         int64_t used = 0;
     };
@@ -306,6 +307,7 @@ int sv_factorize_batch(hmec_ctx *const *cs, int n, int precision);
 int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *st);
 int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision);
 int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *st);
+int sv_lincomb(hmec_ctx *c, double *y, double a, const double *u, double b, const double *v);
 void sv_gmres_free(hmec_ctx *c);
 // ---- serving (hmec_serve.cpp) ----
 int batcher_factorize(hmec_batcher *b, hmec_ctx *c, int precision);

@@ -1029,6 +1029,11 @@ __global__ void k_expand(const double *vs, const double *csS, const double *gv, 
     }
     z[k] = s;
 }
+// y = a u + b v
+__global__ void k_lincomb(double *y, double a, const double *u, double b, const double *v, int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) y[t] = a * u[t] + b * v[t];
+}
 // x = z (first) or x += z
 __global__ void k_add(double *x, const double *z, int64_t N, int first) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2072,6 +2077,13 @@ static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *r
 // cycle starts from the true residual; its Arnoldi loop stops once the
 // estimated 2-norm has dropped by the factor the inf-norms still need (with
 // a margin of 2).
+int sv_lincomb(hmec_ctx *c, double *y, double a, const double *u, double b, const double *v) {
+    hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, y, a, u, b, v,
+                       c->N);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
 int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     if (!c->factor_valid) return HMEC_E_ARG;
     const int m = 20;               // restart length (<= 31: GmCoef and the fused dots hold the coefficients)
