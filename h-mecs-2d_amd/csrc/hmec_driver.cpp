@@ -98,13 +98,15 @@ int guess_find(hmec_ctx *c, int it) {
 }
 
 // x0 from entry k: its newest solution, extrapolated to the current step's
-// time through the earlier one when there is one
+// time through the earlier one when there is one (and the current step is
+// no farther from the newest than twice the newest's distance from the
+// earlier one)
 int guess_load(hmec_ctx *c, int k) {
     hmec_ctx::XGuess &g = c->xguess[k];
     g.used = ++c->xguess_clock;
     if (g.has2 && g.t > g.t2 && c->timesum > g.t) {
         const double r = (c->timesum - g.t) / (g.t - g.t2);
-        return sv_lincomb(c, c->d.sol, 1.0 + r, g.x, -r, g.x2);
+        if (r <= 2.0) return sv_lincomb(c, c->d.sol, 1.0 + r, g.x, -r, g.x2);
     }
     HMEC_HIP(hipMemcpyAsync(c->d.sol, g.x, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
     return HMEC_OK;
