@@ -149,7 +149,10 @@ __device__ __forceinline__ float lane_value(float v, int l) {
 // below tau on the first kb pivots) of the NB x NB block D in LDS
 // (identity-padded beyond kb, which the steps leave as it is).  Lane r keeps
 // row r in registers; step k broadcasts the pivot row from lane k, and every
-// lane below it forms its multiplier and updates its row.
+// lane below it forms its multiplier and updates its row.  Rows at or above
+// the pivot take a zero multiplier instead of a select, so each updated entry
+// is one fused multiply-add (the rows above keep their values: the pivot
+// row's entries are finite once the pivot is).
 template <typename T>
 __device__ __forceinline__ void factor_diag_wave(T (*D)[NB + 1], int kb, double tau, int *npert,
                                                  bool count) {
@@ -164,12 +167,9 @@ __device__ __forceinline__ void factor_diag_wave(T (*D)[NB + 1], int kb, double 
             if (count && lane == k) atomicAdd(npert, 1);
         }
         const T d = lane_value(a[k], k);
-        const T l = a[k] / d;
+        const T l = (r > k) ? a[k] / d : (T)0;
 #pragma unroll
-        for (int c = k + 1; c < NB; c++) {
-            const T u = lane_value(a[c], k);
-            if (r > k) a[c] = a[c] - l * u;
-        }
+        for (int c = k + 1; c < NB; c++) a[c] = fma(-l, lane_value(a[c], k), a[c]);
         if (r > k) a[k] = l;
     }
     if (lane < NB) {
@@ -192,7 +192,7 @@ __device__ __forceinline__ void invert_tri_wave(T (*D)[NB + 1], T (*Inv)[NB + 1]
         for (int r = NB - 1; r >= 0; r--) {
             T s = (r == c) ? (T)1 : (T)0;
 #pragma unroll
-            for (int q = r + 1; q < NB; q++) s = s - lane_value(a[q], r) * x[q];
+            for (int q = r + 1; q < NB; q++) s = fma(-lane_value(a[q], r), x[q], s);
             x[r] = (r <= c) ? s / lane_value(a[r], r) : (T)0;
         }
     } else {
@@ -200,7 +200,7 @@ __device__ __forceinline__ void invert_tri_wave(T (*D)[NB + 1], T (*Inv)[NB + 1]
         for (int r = 0; r < NB; r++) {
             T s = (r == c) ? (T)1 : (T)0;
 #pragma unroll
-            for (int q = 0; q < r; q++) s = s - lane_value(a[q], r) * x[q];
+            for (int q = 0; q < r; q++) s = fma(-lane_value(a[q], r), x[q], s);
             x[r] = (r >= c) ? s : (T)0;
         }
     }
