@@ -219,8 +219,12 @@ __device__ __forceinline__ void invert_tri_wave(T (*D)[NB + 1], T (*Inv)[NB + 1]
 //   X = A U_D^-1 for rows [k0+kb, n), column workgroups Y = L_D^-1 A for
 //   columns [k0+kb, n) (products with the inverted triangles: no serial chain
 //   per row).
-template <typename T>
-__global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, const FacPtrs<T> *mem,
+// WG < PR: narrower workgroups for levels whose fronts all have at most WG
+// trailing rows at this panel (one workgroup per front, more fronts resident
+// per CU).  With one wavefront (WG = 64) both inversions run on it in turn,
+// and L_D^-1 overwrites D once D is stored.
+template <typename T, int WG>
+__global__ void __launch_bounds__(WG) k_panel(const FrontTask *tasks, int k0, const FacPtrs<T> *mem,
                                               double tau) {
     T *const arena = mem[blockIdx.z].arena;
     T *const dscr = mem[blockIdx.z].dscr;
@@ -228,7 +232,8 @@ __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, co
     int *const npert = mem[blockIdx.z].npert;
     __shared__ T D[NB][NB + 1];
     __shared__ T Iu[NB][NB + 1];      // U_D^-1 (upper)
-    __shared__ T Il[NB][NB + 1];      // L_D^-1 (unit lower)
+    __shared__ T IlS[WG > 64 ? NB : 1][NB + 1];
+    T (*const Il)[NB + 1] = WG > 64 ? IlS : D;   // L_D^-1 (unit lower)
     const FrontTask tk = tasks[blockIdx.y];
     if (tk.p <= k0) return;
     const int kb = min(NB, tk.p - k0);
@@ -237,7 +242,7 @@ __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, co
     const int nch = (rest + PR - 1) / PR;
     // a front whose trailing rows fit one workgroup is done by workgroup 0
     // alone (one diagonal factorization instead of three)
-    const bool solo = nch <= 1;
+    const bool solo = WG < PR || nch <= 1;
     if ((int)blockIdx.x >= (solo ? 1 : 1 + 2 * nch)) return;
     T *F = arena + tk.off;
     const int tid = threadIdx.x, wave = tid >> 6;
@@ -248,18 +253,24 @@ __global__ void __launch_bounds__(PR) k_panel(const FrontTask *tasks, int k0, co
     __syncthreads();
     if (wave == 0) factor_diag_wave<T>(D, kb, tau, npert, blockIdx.x == 0);
     __syncthreads();
-    if (wave == 0) invert_tri_wave<T, true>(D, Iu);
-    else if (wave == 1) invert_tri_wave<T, false>(D, Il);
+    // factored D for the update launch, and both inverted triangles of this
+    // diagonal block for the solves (block k0/NB of the front: [0] = L^-1
+    // unit lower, [1] = U^-1 upper)
+    T *ds = dscr + (int64_t)blockIdx.y * NB * NB;
+    T *dst = dinv + tk.doff + (int64_t)(k0 / NB) * 2 * NB * NB;
+    if (WG == 64) {
+        for (int e = tid; e < NB * NB; e += blockDim.x) ds[e] = D[e % NB][e / NB];
+        invert_tri_wave<T, true>(D, Iu);
+        invert_tri_wave<T, false>(D, Il);
+    } else {
+        if (wave == 0) invert_tri_wave<T, true>(D, Iu);
+        else if (wave == 1) invert_tri_wave<T, false>(D, Il);
+    }
     __syncthreads();
     if (blockIdx.x == 0) {
-        // factored D for the update launch, and both inverted triangles of
-        // this diagonal block for the solves (block k0/NB of the front:
-        // [0] = L^-1 unit lower, [1] = U^-1 upper)
-        T *ds = dscr + (int64_t)blockIdx.y * NB * NB;
-        T *dst = dinv + tk.doff + (int64_t)(k0 / NB) * 2 * NB * NB;
         for (int e = tid; e < NB * NB; e += blockDim.x) {
             const int r = e % NB, c = e / NB;
-            ds[e] = D[r][c];
+            if (WG > 64) ds[e] = D[r][c];
             dst[e] = Il[r][c];
             dst[NB * NB + e] = Iu[r][c];
         }
@@ -1595,15 +1606,25 @@ static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipS
         // strips, next panel, one update of the rest for both
         const int maxp = dp.maxp[lv];
         auto panel = [&](int k0) -> int {
-            int gx = 1;
+            int gx = 1, rest = 0;
             for (const auto &t : dp.h_tasks[lv]) {
                 if (t.p <= k0) continue;
                 const int kb = std::min(NB, t.p - k0);
                 const int nch = (t.n - k0 - kb + PR - 1) / PR;
                 gx = std::max(gx, nch <= 1 ? 1 : 1 + 2 * nch);
+                rest = std::max(rest, t.n - k0 - kb);
             }
-            hipLaunchKernelGGL(k_panel<T>, dim3(gx, nfr, Bz), dim3(PR), 0, st, dp.d_tasks[lv], k0, mem,
-                               tau);
+            // the narrow shapes take one workgroup per front and need every
+            // front's trailing rows within the workgroup
+            if (rest <= 64)
+                hipLaunchKernelGGL((k_panel<T, 64>), dim3(1, nfr, Bz), dim3(64), 0, st, dp.d_tasks[lv],
+                                   k0, mem, tau);
+            else if (rest <= 128)
+                hipLaunchKernelGGL((k_panel<T, 128>), dim3(1, nfr, Bz), dim3(128), 0, st,
+                                   dp.d_tasks[lv], k0, mem, tau);
+            else
+                hipLaunchKernelGGL((k_panel<T, PR>), dim3(gx, nfr, Bz), dim3(PR), 0, st, dp.d_tasks[lv],
+                                   k0, mem, tau);
             LAUNCH_CHECK();
             return HMEC_OK;
         };
