@@ -1192,6 +1192,14 @@ __global__ void k_vscale(const double *x, double a, double *y, int64_t N) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < N) y[t] = a * x[t];
 }
+// y = x / sqrt(*rr) with the squared norm read on the device (zero when
+// *rr is not positive)
+__global__ void k_vscale_rr(const double *x, const double *rr, double *y, int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double q = *rr;
+    const double a = q > 0 ? 1.0 / sqrt(q) : 0.0;
+    if (t < N) y[t] = a * x[t];
+}
 // x += sum_j c_j Z_j
 __global__ void k_combine(const double *Z, int64_t ldz, int nz, GmCoef c, double *x,
                           int64_t N) {
@@ -2031,8 +2039,9 @@ static int gm_buf(hmec_ctx *c, int m, GmresBuf **out) {
         HMEC_HIP(hipMalloc(&g.rsd, sizeof(double) * c->N));
         const int64_t nbs = std::max<int64_t>(RB, (c->N + SPMV_ROWS - 1) / SPMV_ROWS);
         HMEC_HIP(hipMalloc(&g.part, sizeof(double) * nbs * (m + 2)));
-        HMEC_HIP(hipMalloc(&g.hdev, sizeof(double) * 3 * (m + 2)));
-        HMEC_HIP(hipHostMalloc(&g.hhost, sizeof(double) * 3 * (m + 2), hipHostMallocDefault));
+        // three Hessenberg columns, then the residual norms of a cycle start
+        HMEC_HIP(hipMalloc(&g.hdev, sizeof(double) * (3 * (m + 2) + 8)));
+        HMEC_HIP(hipHostMalloc(&g.hhost, sizeof(double) * (3 * (m + 2) + 8), hipHostMallocDefault));
     }
     *out = &g;
     return HMEC_OK;
@@ -2071,22 +2080,25 @@ static int gm_precond(hmec_ctx *c, const double *v, double *zout) {
 // Relative inf-norm residuals of x: unscaled ||b - A x|| / ||b|| (the
 // reported one) and row-equilibrated ||Dr (b - A x)|| / ||Dr b||; leaves
 // Dr (b - A x) in g.rsd (the GMRES residual)
-static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *rel_u,
-                            double *rel_s, double *rr, int *npert) {
-    SolverDev *s = sdev(c);
-    const int64_t N = c->N;
-    const unsigned gN = (unsigned)((N + 255) / 256);
-    hipStream_t st = c->stream;
+static int gm_residual_launch(hmec_ctx *c, GmresBuf &g, const double *x, double *out) {
     launch_residual(c, x, c->d.work, g.rsd, c->d.rscale);
     LAUNCH_CHECK();
-    HMEC_TRY(launch_norms4(c, c->d.work, c->d.rhs, g.rsd, g.bs, s->norms));
-    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
-    HMEC_HIP(hipStreamSynchronize(st));
-    const double *h = s->h_norms;
+    return launch_norms4(c, c->d.work, c->d.rhs, g.rsd, g.bs, out);
+}
+static void gm_residual_read(const double *h, double *rel_u, double *rel_s, double *rr, int *npert) {
     *rel_u = h[1] > 0 ? h[0] / h[1] : h[0];
     *rel_s = h[3] > 0 ? h[2] / h[3] : h[2];
     *rr = h[4];
     *npert = (int)h[5];
+}
+static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *rel_u,
+                            double *rel_s, double *rr, int *npert) {
+    SolverDev *s = sdev(c);
+    hipStream_t st = c->stream;
+    HMEC_TRY(gm_residual_launch(c, g, x, s->norms));
+    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
+    HMEC_HIP(hipStreamSynchronize(st));
+    gm_residual_read(s->h_norms, rel_u, rel_s, rr, npert);
     return HMEC_OK;
 }
 
@@ -2129,19 +2141,33 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     HMEC_TRY(darcy_e(c, st));
     hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, c->d.rhs, c->d.rscale, g.bs, N);
     LAUNCH_CHECK();
+    // The first cycle does not wait for its starting residual: the first
+    // basis vector is scaled on the device and the residual norms come back
+    // with the first Hessenberg column.  A start that already meets the
+    // tolerance discards that iteration (x is not touched).
+    const int M2 = m + 2;
+    double *const nrm_d = g.hdev + 3 * M2;
+    bool lead = total_it < max_it, done = false;
     for (;;) {
         double rel_s = 0, rr = 0;   // rr = ||Dr (b - A x)||_2^2 from the same pass
-        HMEC_TRY(gm_true_residual(c, g, x, &relinf, &rel_s, &rr, &npert));
-        last_rel_s = rel_s;
-        const double worst = std::max(relinf, rel_s);
-        if (!(worst > tol) || total_it >= max_it) break;
-        const double beta = std::sqrt(rr);
-        if (!(beta > 0)) break;
-        const double target = beta * (tol / worst) * 0.5;
-        hipLaunchKernelGGL(k_vscale, dim3(gN), dim3(256), 0, st, r, 1.0 / beta, g.V, N);
-        LAUNCH_CHECK();
+        double target = 0;
         std::fill(gv.begin(), gv.end(), 0.0);
-        gv[0] = beta;
+        if (lead) {
+            HMEC_TRY(gm_residual_launch(c, g, x, nrm_d));
+            hipLaunchKernelGGL(k_vscale_rr, dim3(gN), dim3(256), 0, st, r, nrm_d + 4, g.V, N);
+            LAUNCH_CHECK();
+        } else {
+            HMEC_TRY(gm_true_residual(c, g, x, &relinf, &rel_s, &rr, &npert));
+            last_rel_s = rel_s;
+            const double worst = std::max(relinf, rel_s);
+            if (!(worst > tol) || total_it >= max_it) break;
+            const double beta = std::sqrt(rr);
+            if (!(beta > 0)) break;
+            target = beta * (tol / worst) * 0.5;
+            hipLaunchKernelGGL(k_vscale, dim3(gN), dim3(256), 0, st, r, 1.0 / beta, g.V, N);
+            LAUNCH_CHECK();
+            gv[0] = beta;
+        }
         int j = 0;
         for (; j < m && total_it < max_it; j++) {
             double *Zj = g.Z + (int64_t)j * N;
@@ -2150,7 +2176,6 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
             // one copy of the Hessenberg column per iteration.  The first
             // pass's dot products come with the operator product, the second
             // pass's with ||w'||^2, and its update writes the next basis vector.
-            const int M2 = m + 2;
             double *h1d = g.hdev, *h2d = g.hdev + M2, *wwd = g.hdev + 2 * M2;
             const unsigned nbs = (unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS);
             launch_spmv_rs_dots(c, Zj, g.w, g.V, j + 1, g.part);
@@ -2171,8 +2196,22 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
             hipLaunchKernelGGL(k_gs_update_norm, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h2d, g.w,
                                j + 1 < m ? g.V + (int64_t)(j + 1) * N : nullptr, wwd, N);
             LAUNCH_CHECK();
-            HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * 3 * M2, hipMemcpyDeviceToHost, st));
+            const bool first = lead && j == 0;
+            HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * (3 * M2 + (first ? 6 : 0)),
+                                    hipMemcpyDeviceToHost, st));
             HMEC_HIP(hipStreamSynchronize(st));
+            if (first) {
+                gm_residual_read(g.hhost + 3 * M2, &relinf, &rel_s, &rr, &npert);
+                last_rel_s = rel_s;
+                const double worst = std::max(relinf, rel_s);
+                const double beta = std::sqrt(rr);
+                if (!(worst > tol) || !(beta > 0)) {
+                    done = true;
+                    break;
+                }
+                target = beta * (tol / worst) * 0.5;
+                gv[0] = beta;
+            }
             for (int i = 0; i <= j; i++) h[i] = g.hhost[i] + g.hhost[M2 + i];
             const double ww = g.hhost[2 * M2];
             const double hn = std::sqrt(ww);
@@ -2197,7 +2236,8 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
                 break;
             }
         }
-        if (j == 0) break;
+        lead = false;
+        if (done || j == 0) break;
         // y = R^-1 g ; x += Z y
         for (int i = j - 1; i >= 0; i--) {
             double t1 = gv[i];
