@@ -94,6 +94,27 @@ def test_fp32_forced_stall_falls_back(hmec, gpu):
         assert np.array_equal(c32.get(f), c64.get(f)), f
 
 
+def test_fp32_start_already_converged(hmec, gpu):
+    """With a loose GMRES tolerance the later Picard solves start from a
+    guess that already meets it: such a solve does no iteration (the first
+    cycle's lead iteration, launched before the starting residual is known,
+    is discarded) and reports the starting residual, within the tolerance."""
+    tol = 1e-3
+    p = hmec.params(1)
+    soa = hmec.generate_markers(p, cfg=1, seed=1)
+    ctx = hmec.Context(p, soa, precision=hmec.PREC_FP32)
+    ctx.set_refinement(max_stale=12, max_fresh=40, tol=tol)
+    zero = 0
+    for s in range(4):
+        i = ctx.step()
+        assert i.fallbacks == 0, s
+        assert i.max_rel_residual <= tol, (s, i.max_rel_residual)
+        zero += max(0, i.solves - i.ir_iterations)
+    assert zero > 0
+    for f in ("vxs", "vys", "pt", "pf"):
+        assert np.isfinite(ctx.get(f)).all(), f
+
+
 @pytest.mark.parametrize("cfg", [1, 2])
 def test_fp32_path_errors(hmec, gpu, cfg):
     """FP32 factor + FP64 GMRES (on the row-equilibrated system) against the
