@@ -4,6 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
 #include <map>
 #include <mutex>
 #include <string>
@@ -331,6 +334,94 @@ extern "C" int hmec_get_field(hmec_ctx *c, int f, double *host, int layout) {
         return HMEC_E_ARG;
     }
     return HMEC_OK;
+}
+
+// ------------------------------------------------------------ EVO streams
+// A value as an ostream with default flags and precision prints it.
+static std::string evo_num(double v) {
+    std::ostringstream s;
+    s << v;
+    return s.str();
+}
+// An Eigen vector in Eigen's default format: every coefficient right-aligned
+// to the widest one, separated by sep (' ' along a row, '\n' down a column).
+static void evo_vec(std::ostream &o, const double *v, int n, char sep) {
+    std::vector<std::string> t(n);
+    size_t w = 0;
+    for (int j = 0; j < n; j++) {
+        t[j] = evo_num(v[j]);
+        w = std::max(w, t[j].size());
+    }
+    for (int j = 0; j < n; j++) {
+        if (j) o << sep;
+        o << std::setw((int)w) << t[j];
+    }
+}
+// row line_fault of field f (first timestep only: plain copy of the field)
+static int evo_field_row(hmec_ctx *c, int f, std::vector<double> &row) {
+    int rows, cols;
+    HMEC_TRY(field_dims(c, f, rows, cols));
+    std::vector<double> tmp((size_t)rows * cols);
+    HMEC_HIP(hipMemcpy(tmp.data(), F(c, f), sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+    row.resize(cols);
+    for (int j = 0; j < cols; j++) row[j] = tmp[c->P.line_fault + (size_t)j * rows];
+    return HMEC_OK;
+}
+
+extern "C" int hmec_write_evo(hmec_ctx *c, const char *dir, int first) {
+    if (!c || !dir) return HMEC_E_ARG;
+    // h_mec_rsf_v9.cpp:1751-1800, in that order
+    static const char *const names[FaultRows::NSERIES] = {
+        "EVO_Vslip.txt", "EVO_viscosity.txt", "EVO_press_flu.txt", "EVO_press_eff.txt", "EVO_SigmaY.txt",
+        "EVO_Sii.txt", "EVO_Theta.txt", "EVO_Visc_comp.txt", "EVO_Elast_comp.txt", "EVO_vxD.txt"};
+    static const int series[FaultRows::NSERIES][2] = {
+        {HMEC_F_VSLIPB, -1}, {HMEC_F_ETA, -1}, {HMEC_F_PF, -1}, {HMEC_F_PT, HMEC_F_PF},
+        {HMEC_F_SIGMAY, -1}, {HMEC_F_SII_FAULT, -1}, {HMEC_F_OM, -1}, {HMEC_F_VIS_COMP, -1},
+        {HMEC_F_EL_DECOM, -1}, {HMEC_F_VXD, -1}};
+    FaultRows a;
+    for (int k = 0; k < FaultRows::NSERIES; k++) {
+        a.f[k] = series[k][0];
+        a.sub[k] = series[k][1];
+        HMEC_TRY(field_dims(c, a.f[k], a.rows[k], a.cols[k]));
+    }
+    const int ld = c->Nx1;
+    std::vector<double> h((size_t)FaultRows::NSERIES * ld);
+    HMEC_TRY(k_fault_rows_gather(c, a, c->d.work));
+    HMEC_HIP(hipMemcpyAsync(h.data(), c->d.work, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    const std::string d = std::string(dir) + "/";
+    if (first) {
+        // :1739-1749
+        std::vector<double> x(c->Nx);
+        linspaced(x.data(), c->Nx, 0, c->P.xsize);
+        std::ofstream o(d + "x_fault.txt");
+        evo_vec(o, x.data(), c->Nx, '\n');
+        o << std::endl;
+        if (!o) return HMEC_E_IO;
+        std::vector<double> ra, rb, rl;
+        HMEC_TRY(evo_field_row(c, HMEC_F_ARSF, ra));
+        HMEC_TRY(evo_field_row(c, HMEC_F_BRSF, rb));
+        HMEC_TRY(evo_field_row(c, HMEC_F_LRSF, rl));
+        std::ofstream r(d + "rsf_fault.txt", std::ios_base::app | std::ios_base::out);
+        evo_vec(r, ra.data(), c->Nx, ' ');
+        r << "\n\n";
+        evo_vec(r, rb.data(), c->Nx, ' ');
+        r << "\n\n";
+        evo_vec(r, rl.data(), c->Nx, ' ');
+        r << std::endl;
+        if (!r) return HMEC_E_IO;
+    }
+    for (int k = 0; k < FaultRows::NSERIES; k++) {
+        std::ofstream o(d + names[k], std::ios_base::app | std::ios_base::out);
+        o << c->timesum << "    " << c->dt << "    ";
+        evo_vec(o, h.data() + (size_t)k * ld, a.cols[k], ' ');
+        o << std::endl;
+        if (!o) return HMEC_E_IO;
+    }
+    std::ofstream o(d + "EVO_data.txt", std::ios_base::app | std::ios_base::out);
+    o << std::setw(20) << c->timesum << std::setw(20) << c->dt << std::setw(20) << c->Vmax << std::setw(20)
+      << c->ynlast + 1 << std::setw(20) << c->iterstep + 1 << std::endl;
+    return o ? HMEC_OK : HMEC_E_IO;
 }
 
 extern "C" int hmec_get_state(hmec_ctx *c, hmec_step_info *s) {

@@ -291,6 +291,14 @@ constexpr int HMEC_FP32_SLOTS = 3;   // FP32 factors kept by default (keyed by d
 int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out);
 int k_iter_commit(hmec_ctx *c, int ynpl, int converged);
 int k_final_stress(hmec_ctx *c, double dt);
+// series of the fault-row output streams: field, subtracted field (-1 none)
+// and grid of each
+struct FaultRows {
+    static constexpr int NSERIES = 10;
+    int f[NSERIES], sub[NSERIES], rows[NSERIES], cols[NSERIES];
+};
+// row line_fault of every series into dev_out[k * Nx1 + j]
+int k_fault_rows_gather(hmec_ctx *c, const FaultRows &a, double *dev_out);
 int k_history(hmec_ctx *c);
 int k_advect(hmec_ctx *c, double dt);
 int k_init_state(hmec_ctx *c);

@@ -16,6 +16,7 @@ extern "C" const char *hmec_strerror(int code) {
     case HMEC_E_NOCONV: return "iterative refinement did not converge";
     case HMEC_E_NODEV: return "no HIP device";
     case HMEC_E_SOLVER: return "solver error";
+    case HMEC_E_IO: return "output file could not be written";
     default: return "unknown error";
     }
 }

@@ -1345,6 +1345,28 @@ int k_final_stress(hmec_ctx *c, double dt) {
     return HMEC_OK;
 }
 
+// ------------------------------------------------------ fault-row streams
+// Row `line` of the fields the EVO_*.txt streams print
+// (h_mec_rsf_v9.cpp:1751-1800), one launch: series k is field f[k] (minus
+// field sub[k] when sub[k] >= 0: pt - pf) on a grid of rows[k] x cols[k],
+// written to out[k * ld + j].
+__global__ void k_fault_rows(const double *f, int64_t ns, int line, FaultRows a, int ld, double *out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (k >= FaultRows::NSERIES || j >= a.cols[k]) return;
+    const int64_t o = line + (int64_t)j * a.rows[k];
+    double v = f[(int64_t)a.f[k] * ns + o];
+    if (a.sub[k] >= 0) v -= f[(int64_t)a.sub[k] * ns + o];
+    out[(int64_t)k * ld + j] = v;
+}
+
+int k_fault_rows_gather(hmec_ctx *c, const FaultRows &a, double *dev_out) {
+    const int ld = c->Nx1;
+    hipLaunchKernelGGL(k_fault_rows, dim3((unsigned)((ld + 255) / 256), FaultRows::NSERIES), dim3(256), 0,
+                       c->stream, c->d.fields, c->ns, c->P.line_fault, a, ld, dev_out);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
 // ------------------------------------------------------ history (a16)
 // h_mec_rsf_v9.cpp:1672-1726
 __global__ void k_history(double *f, int64_t ns, int nstag, int nb) {

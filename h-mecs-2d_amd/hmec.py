@@ -207,6 +207,7 @@ def _declare(L):
     L.hmec_get_marker_cells.argtypes = [vp, P(C.c_int32)]
     L.hmec_set_field.argtypes = [vp, C.c_int, P(C.c_double), C.c_int]
     L.hmec_get_field.argtypes = [vp, C.c_int, P(C.c_double), C.c_int]
+    L.hmec_write_evo.argtypes = [vp, C.c_char_p, C.c_int]
     L.hmec_get_state.argtypes = [vp, P(StepInfo)]
     L.hmec_set_state.argtypes = [vp, P(StepInfo)]
     L.hmec_scatter.argtypes = [vp]
@@ -359,6 +360,11 @@ class Context:
         a = np.zeros((c, r), dtype=np.float64)   # column-major storage
         check(lib().hmec_get_field(self.h, FIELD_ID[name], dptr(a), 0), "get_field " + name)
         return a.T
+
+    def write_evo(self, directory, first=False):
+        """Fault-row EVO_*.txt streams of the last timestep (appended), plus
+        x_fault.txt / rsf_fault.txt when first (h_mec_rsf_v9.cpp:1739-1805)."""
+        check(lib().hmec_write_evo(self.h, os.fsencode(str(directory)), int(bool(first))), "write_evo")
 
     def set(self, name, arr):
         r, c = field_shape(self.p, name)

@@ -41,6 +41,7 @@ extern "C" {
 #define HMEC_E_NOCONV (-5)   /* iterative refinement did not reach tolerance */
 #define HMEC_E_NODEV (-6)    /* no HIP device visible                        */
 #define HMEC_E_SOLVER (-7)   /* external solver error                        */
+#define HMEC_E_IO (-8)       /* an output file could not be written          */
 const char *hmec_strerror(int code);
 
 /* --------------------------------------------------------------- parameters
@@ -222,6 +223,20 @@ int hmec_get_marker_cells(hmec_ctx *ctx, int32_t *cells4xM);
 
 int hmec_set_field(hmec_ctx *ctx, int field_id, const double *host, int layout);
 int hmec_get_field(hmec_ctx *ctx, int field_id, double *host, int layout);
+
+/* Fault-row output streams of the timestep just completed, replacing the
+ * per-timestep output block of h_mec_rsf_v9.cpp:1739-1805 (the stdout summary
+ * aside).  Appends one line to each of EVO_Vslip.txt, EVO_viscosity.txt,
+ * EVO_press_flu.txt, EVO_press_eff.txt (pt - pf), EVO_SigmaY.txt,
+ * EVO_Sii.txt, EVO_Theta.txt, EVO_Visc_comp.txt, EVO_Elast_comp.txt and
+ * EVO_vxD.txt in directory dir: timesum, dt, then row line_fault of the
+ * field, formatted as the reference prints them (default stream precision,
+ * Eigen's column alignment), and to EVO_data.txt (timesum, dt, Vmax, global
+ * and Picard iteration counts in 20-wide columns).  With first != 0 it also
+ * writes x_fault.txt and appends rsf_fault.txt, as the reference does on its
+ * first timestep.  The rows are gathered on the device in one launch.
+ * HMEC_E_IO when a file cannot be written. */
+int hmec_write_evo(hmec_ctx *ctx, const char *dir, int first);
 /* Scalar state (dt, timesum, ...) in/out; timestep is the next step to run. */
 int hmec_get_state(hmec_ctx *ctx, hmec_step_info *info);
 int hmec_set_state(hmec_ctx *ctx, const hmec_step_info *info);
