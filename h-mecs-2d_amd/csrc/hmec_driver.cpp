@@ -305,7 +305,9 @@ extern "C" int hmec_step(hmec_ctx *c, hmec_step_info *info) {
         if (stop) break;
         c->ynlast++;
     }
-    if (c->iterstep >= P.niterglobal) c->iterstep = P.niterglobal - 1;
+    // a loop that runs out leaves iterstep == niterglobal, as in the reference
+    // (:1434-1456): the EVO_data.txt column and the step's iteration count
+    // then read niterglobal + 1
     const double e = now_ms();
     // :1464-1472
     if (c->dt00 > c->dt) c->yndtdecrease = 1;

@@ -1314,7 +1314,8 @@ int orc_step(orc *o, hmec_step_info *info) {
         if (stop) break;
         o->ynlast++;
     }
-    if (o->iterstep >= P->niterglobal) o->iterstep = P->niterglobal - 1;
+    /* a loop that runs out leaves iterstep == niterglobal (:1434-1456), so
+     * EVO_data.txt and the iteration count read niterglobal + 1 */
     double e = now_ms();
     orc_step_end(o);
     double f = now_ms();
@@ -1340,6 +1341,15 @@ int orc_step(orc *o, hmec_step_info *info) {
     o->timestep++;
     return err;
 }
+
+/* the controller of h_mec_rsf_v9.cpp:1310-1436 on given scalars at Picard
+ * iteration iterstep (its pins drive it with constructed scalars); returns 1
+ * when the exit test fires */
+int orc_controller(orc *o, const hmec_iter_scalars *sc, int iterstep) {
+    o->iterstep = iterstep;
+    return controller(o, sc);
+}
+double *orc_dsylsq(orc *o) { return o->DSYLSQ; }
 
 /* scalar state access (dt etc.) */
 void orc_get_state(orc *o, hmec_step_info *info) {

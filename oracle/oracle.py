@@ -73,6 +73,10 @@ def lib():
         L.orc_get_dt.argtypes = [vp]
         L.orc_get_dt.restype = C.c_double
         L.orc_lu_stats.argtypes = [vp, P(C.c_int64), P(C.c_double)]
+        L.orc_controller.argtypes = [vp, vp, C.c_int]
+        L.orc_controller.restype = C.c_int
+        L.orc_dsylsq.argtypes = [vp]
+        L.orc_dsylsq.restype = P(C.c_double)
         _lib = L
     return _lib
 
@@ -220,6 +224,13 @@ class Oracle:
     @dt.setter
     def dt(self, v):
         lib().orc_set_dt(self.h, float(v))
+
+    def controller(self, sc, iterstep):
+        """The Picard / dt controller on constructed scalars; True = exit."""
+        return bool(lib().orc_controller(self.h, C.cast(C.byref(sc), C.c_void_p), int(iterstep)))
+
+    def dsylsq(self, n):
+        return np.ctypeslib.as_array(lib().orc_dsylsq(self.h), shape=(n,)).copy()
 
     def lu_stats(self):
         nnz = C.c_int64()

@@ -103,7 +103,7 @@ def test_scatter_linear_field_interior(hmec, orc):
     rho = o.get("RHO")
     x = np.linspace(0, p.xsize, p.Nx)
     want = 1000.0 + 0.01 * x
-    assert np.allclose(rho[1:-1, 1:-1], want[None, 1:-1], rtol=1e-13)
+    assert np.allclose(rho[1:-1, 1:-1], want[None, 1:-1], rtol=1e-13, atol=0)
 
 
 def test_scatter_edge_rows_never_receive(hmec, orc):
@@ -337,9 +337,9 @@ def test_stress_update_limits(hmec, orc, limit):
     sxy = o.get("SXY")
     eta = o.get("ETA")
     if limit == "viscous":
-        assert np.allclose(sxy[:-1, :], 2 * eta[:-1, :] * (0.5 * g), rtol=1e-12)
+        assert np.allclose(sxy[:-1, :], 2 * eta[:-1, :] * (0.5 * g), rtol=1e-12, atol=0)
     else:
-        assert np.allclose(sxy, 3e6, rtol=1e-12)
+        assert np.allclose(sxy, 3e6, rtol=1e-12, atol=0)
 
 
 def test_aging_law_against_ode(hmec, orc):
@@ -396,7 +396,7 @@ def test_bisection_root(hmec, orc):
     prB = np.maximum(prB, 1e3)
     bracket = tau != 0
     Vt = 2 * p.V0 * np.sinh(np.maximum(tau, 0) / a / prB) * np.exp(-(bb * om + fr) / a)
-    assert np.allclose(Vt[bracket], V[bracket], rtol=1e-9)
+    assert np.allclose(Vt[bracket], V[bracket], rtol=1e-9, atol=0)
 
 
 # ------------------------------------------------------------------ advection
@@ -440,7 +440,7 @@ def test_advection_rotation_invariants(hmec, orc):
     assert np.allclose(nxx + nyy, sxx + syy, rtol=0, atol=1e-6)
     j2a = ((sxx - syy) / 2) ** 2 + sxy ** 2
     j2b = ((nxx - nyy) / 2) ** 2 + nxy ** 2
-    assert np.allclose(j2a, j2b, rtol=1e-12)
+    assert np.allclose(j2a, j2b, rtol=1e-12, atol=0)
     # the rotation angle is the spin times dt
     assert not np.allclose(nxy, sxy)
 
