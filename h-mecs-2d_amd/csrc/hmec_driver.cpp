@@ -196,7 +196,8 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         tm_begin(c, c->t_solve);
         int rc = sv_solve(c, 3, 1e-14, &st);
         tm_end(c, c->t_solve, 0);
-        if (rc != HMEC_OK && rc != HMEC_E_NOCONV) return rc;
+        if (rc == HMEC_E_NOCONV && st.rel_residual <= kAcceptResidual) rc = HMEC_OK;
+        if (rc != HMEC_OK) return rc;
     } else {
         // FP32: GMRES preconditioned by a cached FP32 factor.  The cache keeps
         // a few factors keyed by the dt they were computed at (the
@@ -236,7 +237,9 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         int rc = sv_gmres(c, fresh ? c->ir_max_fresh : c->ir_max_stale, c->ir_tol, &st);
         tm_end(c, c->t_solve, 0);
         acc.ir_iterations += st.ir_iterations;
-        if (rc == HMEC_E_NOCONV && !fresh) {
+        // a stalled refinement, or an FP32 factor with an unrepairable pivot:
+        // a fresh FP32 factor, then an FP64 one
+        if ((rc == HMEC_E_NOCONV || rc == HMEC_E_PIVOT) && !fresh) {
             const int slot = dist <= 0.05 ? k : sv_slot_victim(c, k);
             HMEC_TRY(refactor(slot));
             tm_begin(c, c->t_solve);
@@ -244,12 +247,12 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
             tm_end(c, c->t_solve, 0);
             acc.ir_iterations += st.ir_iterations;
         }
-        if (rc == HMEC_E_NOCONV) {
+        if (rc == HMEC_E_NOCONV || rc == HMEC_E_PIVOT) {
             acc.fallbacks++;
             HMEC_TRY(factor(c, HMEC_PREC_FP64));
             acc.factorizations++;
             rc = sv_solve(c, 3, 1e-14, &st);
-            if (rc == HMEC_E_NOCONV) rc = HMEC_OK;
+            if (rc == HMEC_E_NOCONV && st.rel_residual <= kAcceptResidual) rc = HMEC_OK;
         }
         if (rc != HMEC_OK) return rc;
         HMEC_TRY(guess_keep(c));
@@ -348,5 +351,11 @@ extern "C" int hmec_batch_factorize(hmec_ctx *const *cs, int n, int precision) {
         if (cs[k]->Nx != cs[0]->Nx || cs[k]->Ny != cs[0]->Ny) return HMEC_E_ARG;
         HMEC_TRY(hmec_analyze(cs[k]));
     }
-    return sv_batch_factorize(cs, n, precision);
+    HMEC_TRY(sv_batch_factorize(cs, n, precision));
+    int rc = HMEC_OK;
+    for (int k = 0; k < n; k++) {
+        const int r = factor_check(cs[k]);
+        if (rc == HMEC_OK) rc = r;
+    }
+    return rc;
 }

@@ -26,6 +26,11 @@
         if (rc_ != HMEC_OK) return rc_; \
     } while (0)
 
+// A linear solve whose final ||b - A x||inf / ||b||inf exceeds this bound is
+// reported as HMEC_E_NOCONV (the tolerance the parity and precision targets
+// use); below it a refinement that missed its tighter target is accepted.
+constexpr double kAcceptResidual = 1e-10;
+
 void hmec_set_last_error(const std::string &s);
 double hmec_seeded_normal(uint64_t seed, int64_t idx, unsigned stream);
 
@@ -311,6 +316,8 @@ int hmec_build_pattern(hmec_ctx *c);
 int sv_setup_device(hmec_ctx *c);
 int sv_free_device(hmec_ctx *c);
 int sv_factorize(hmec_ctx *c, int precision);
+// HMEC_E_PIVOT when the last factorization met an unrepairable pivot (syncs)
+int factor_check(hmec_ctx *c);
 int sv_factorize_batch(hmec_ctx *const *cs, int n, int precision);
 int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *st);
 int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision);

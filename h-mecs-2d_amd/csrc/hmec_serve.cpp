@@ -12,8 +12,9 @@
 //
 // Factorization batching: a member that needs a numeric factorization
 // registers with the batcher.  The first requester of a (grid, precision)
-// group waits up to batch_window_ms for the other running same-size members
-// to ask too, then factors the whole group with one launch per kernel
+// group waits up to batch_window_ms (no deadline when it is negative) for the
+// other running same-size members to ask too, then factors the whole group
+// with one launch per kernel
 // (sv_factorize_batch: blockIdx.z selects the member).  Each member's
 // kernels compute exactly what they compute alone, so results do not depend
 // on the mix.
@@ -83,9 +84,17 @@ int batcher_factorize(hmec_batcher *b, hmec_ctx *c, int precision) {
         if (r.group > 1) c->batched_facs++;
         return r.rc;
     }
-    const auto deadline = Clock::now() + std::chrono::microseconds((int64_t)(b->window_ms * 1e3));
     const std::pair<int, int> grid{c->Nx, c->Ny};
-    b->cv.wait_until(lk, deadline, [&] { return (int)b->waiting[key].size() >= b->running[grid]; });
+    auto all_in = [&] { return (int)b->waiting[key].size() >= b->running[grid]; };
+    if (b->window_ms < 0) {
+        // rendezvous without a deadline: every other running same-size member
+        // either asks for a factorization or finishes (both notify), so the
+        // wait ends, and the grouping does not depend on host timing
+        b->cv.wait(lk, all_in);
+    } else {
+        const auto deadline = Clock::now() + std::chrono::microseconds((int64_t)(b->window_ms * 1e3));
+        b->cv.wait_until(lk, deadline, all_in);
+    }
     std::vector<hmec_batcher::Req *> grp;
     grp.swap(b->waiting[key]);
     lk.unlock();
@@ -214,7 +223,7 @@ extern "C" int hmec_server_create(const hmec_server_config *cfg, hmec_server **o
         }
         s->cfg.hbm_budget = (int64_t)(0.9 * (double)fr);
     }
-    s->batcher.window_ms = cfg->batch_window_ms > 0 ? cfg->batch_window_ms : 0.0;
+    s->batcher.window_ms = cfg->batch_window_ms < 0 ? -1.0 : cfg->batch_window_ms;
     *out = s;
     return HMEC_OK;
 }

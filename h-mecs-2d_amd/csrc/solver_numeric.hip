@@ -62,20 +62,28 @@ struct FacPtrs {
 
 // ------------------------------------------------------------ equilibration
 
-__global__ void k_row_scale(const int32_t *rowp, const double *val, int64_t N, double *rs) {
+// max that keeps a NaN (a NaN anywhere must show in the result)
+__device__ __forceinline__ double nan_max(double m, double v) { return (v > m || v != v) ? v : m; }
+
+// (bad: optional counter of rows / columns with no finite non-zero entry,
+// which no pivot perturbation can make solvable)
+__global__ void k_row_scale(const int32_t *rowp, const double *val, int64_t N, double *rs,
+                            int *bad) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= N) return;
     double m = 0;
-    for (int p = rowp[r]; p < rowp[r + 1]; p++) m = fmax(m, fabs(val[p]));
+    for (int p = rowp[r]; p < rowp[r + 1]; p++) m = nan_max(m, fabs(val[p]));
+    if (bad && !(m > 0 && isfinite(m))) atomicAdd(bad, 1);
     rs[r] = pow2_recip(m);
 }
 
 __global__ void k_col_scale(const int32_t *cptr, const int32_t *cidx, const int32_t *crow,
-                            const double *val, const double *rs, int64_t N, double *cs) {
+                            const double *val, const double *rs, int64_t N, double *cs, int *bad) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
     double m = 0;
-    for (int q = cptr[c]; q < cptr[c + 1]; q++) m = fmax(m, fabs(val[cidx[q]] * rs[crow[q]]));
+    for (int q = cptr[c]; q < cptr[c + 1]; q++) m = nan_max(m, fabs(val[cidx[q]] * rs[crow[q]]));
+    if (bad && !(m > 0 && isfinite(m))) atomicAdd(bad, 1);
     cs[c] = pow2_recip(m);
 }
 
@@ -163,8 +171,9 @@ __device__ __forceinline__ void factor_diag_wave(T (*D)[NB + 1], int kb, double 
 #pragma unroll
     for (int k = 0; k < NB; k++) {
         if (r == k && k < kb && !(fabs((double)a[k]) >= tau)) {
+            // a non-finite pivot cannot be repaired: counted apart (npert[1])
+            if (count && lane == k) atomicAdd(isfinite((double)a[k]) ? npert : npert + 1, 1);
             a[k] = (a[k] < (T)0) ? (T)(-tau) : (T)tau;
-            if (count && lane == k) atomicAdd(npert, 1);
         }
         const T d = lane_value(a[k], k);
         const T l = (r > k) ? a[k] / d : (T)0;
@@ -875,7 +884,6 @@ static void launch_spmv_rs(hmec_ctx *c, const double *x, double *y) {
 // out[0] = max|a|, out[1] = max|b| in two stages (block maxima, then one
 // block over them); a NaN anywhere propagates to the result
 constexpr int NRM_BLOCKS = 256;
-__device__ __forceinline__ double nan_max(double m, double v) { return (v > m || v != v) ? v : m; }
 // Inf-norms of NV vectors (x[0..NV)) in two stages: block maxima, then one
 // block over them; a NaN anywhere propagates to the result
 template <int NV>
@@ -912,8 +920,8 @@ __global__ void __launch_bounds__(256) k_norms_part(NormArgs<NV> a, int64_t N, d
         for (int q = 0; q < NO; q++) part[NO * blockIdx.x + q] = r[q][0];
     }
 }
-// (cnt: optional, out[NO] = cnt[0], so the count of perturbed pivots comes back
-// with the norms)
+// (cnt: optional, out[NO] = cnt[0] and out[NO + 1] = cnt[1], so the counts of
+// perturbed and of unrepairable pivots come back with the norms)
 template <int NV, bool SQ = false>
 __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb, double *out,
                                                      const int *cnt) {
@@ -941,7 +949,10 @@ __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb,
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NO; q++) out[q] = r[q][0];
-        if (cnt) out[NO] = (double)cnt[0];
+        if (cnt) {
+            out[NO] = (double)cnt[0];
+            out[NO + 1] = (double)cnt[1];
+        }
     }
 }
 
@@ -950,12 +961,15 @@ __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb,
 // E = -A_DD^-1 A_DR of the current matrix (3 entries per Darcy unknown)
 // (also the Darcy coefficients the preconditioner applications read,
 // dcoef[3d..3d+3) = a_dd, a_nn, a_dn, gathered once per matrix)
+// (bad: optional counter of Darcy diagonals that are zero or non-finite, the
+// pivots of this block elimination, which no perturbation repairs)
 __global__ void k_darcy_e(const double *val, int64_t ND, const int32_t *ddiag,
                           const int32_t *dnbdiag, const int32_t *dnbe, const int32_t *des,
-                          double *e, double *dcoef) {
+                          double *e, double *dcoef, int *bad) {
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ND) return;
     const double add = val[ddiag[d]];
+    if (bad && !(fabs(add) > 0 && isfinite(add))) atomicAdd(bad, 1);
     dcoef[3 * d] = add;
     dcoef[3 * d + 1] = dnbe[d] < 0 ? 0.0 : val[dnbdiag[d]];
     dcoef[3 * d + 2] = dnbe[d] < 0 ? 0.0 : val[dnbe[d]];
@@ -1695,7 +1709,7 @@ static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipS
     return HMEC_OK;
 }
 
-static int darcy_e(hmec_ctx *c, hipStream_t st);
+static int darcy_e(hmec_ctx *c, hipStream_t st, int *bad = nullptr);
 
 // Numeric factorization of n same-size contexts' current matrices with one
 // launch per kernel (n = 1: a single context).  The batch runs on the first
@@ -1729,17 +1743,17 @@ int sv_factorize_batch(hmec_ctx *const *cs, int n, int precision) {
         CondDev &q = c->dc;
         const int64_t NS = cp.NS;
         HMEC_HIP(hipMemsetAsync(c->d.counters, 0, sizeof(int) * 4, st));
-        HMEC_TRY(darcy_e(c, st));
+        HMEC_TRY(darcy_e(c, st, c->d.counters + 1));
         hipLaunchKernelGGL(k_condense_matrix, dim3((unsigned)((NS + 255) / 256)), dim3(256), 0, st,
                            c->d.val, NS, q.srowp, q.ssrc, q.tptr, q.taslot, q.tdk, q.tsslot, q.e,
                            q.sval);
         LAUNCH_CHECK();
         hipLaunchKernelGGL(k_row_scale, dim3((unsigned)((NS + 255) / 256)), dim3(256), 0, st,
-                           c0->dc.srowp, q.sval, NS, q.rs);
+                           c0->dc.srowp, q.sval, NS, q.rs, c->d.counters + 1);
         LAUNCH_CHECK();
         hipLaunchKernelGGL(k_col_scale, dim3((unsigned)((NS + 255) / 256)), dim3(256), 0, st,
                            c0->dp.d_csc_ptr, c0->dp.d_csc_idx, c0->dp.d_csc_row, q.sval, q.rs, NS,
-                           q.cs);
+                           q.cs, c->d.counters + 1);
         LAUNCH_CHECK();
         sv_sweep_invalidate(c, precision);
     }
@@ -1899,11 +1913,11 @@ static int solve_condensed(hmec_ctx *c, double *vs) {
 }
 
 // E = -A_DD^-1 A_DR of the current matrix
-static int darcy_e(hmec_ctx *c, hipStream_t st) {
+static int darcy_e(hmec_ctx *c, hipStream_t st, int *bad) {
     const CondPlan &cp = c->cp;
     CondDev &q = c->dc;
     hipLaunchKernelGGL(k_darcy_e, dim3((unsigned)((cp.ND + 255) / 256)), dim3(256), 0, st, c->d.val,
-                       cp.ND, q.ddiag, q.dnbdiag, q.dnbe, q.des, q.e, q.dcoef);
+                       cp.ND, q.ddiag, q.dnbdiag, q.dnbe, q.des, q.e, q.dcoef, bad);
     LAUNCH_CHECK();
     const int64_t nt = cp.rptr.empty() ? 0 : cp.rptr.back();
     if (nt > 0) {
@@ -1951,7 +1965,7 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
     HMEC_HIP(hipMemcpyAsync(r, c->d.rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
     HMEC_TRY(darcy_e(c, st));
     double res = 0, bnorm = 0, prev = INFINITY;
-    int it = 0, npert = 0;
+    int it = 0, npert = 0, nbad = 0;
     int stalls = 0;
     for (;; it++) {
         HMEC_TRY(precond_apply(c, r, nullptr, z));
@@ -1960,12 +1974,13 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
         launch_residual(c, x, r, nullptr, nullptr);
         LAUNCH_CHECK();
         HMEC_TRY(launch_norms(c, r, c->d.rhs, s->norms));
-        HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+        HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 4, hipMemcpyDeviceToHost, st));
         HMEC_HIP(hipStreamSynchronize(st));
         bnorm = s->h_norms[1];
         res = bnorm > 0 ? s->h_norms[0] / bnorm : s->h_norms[0];
         npert = (int)s->h_norms[2];
-        if (res <= tol || it >= max_ir) break;
+        nbad = (int)s->h_norms[3];
+        if (nbad > 0 || res <= tol || it >= max_ir) break;
         if (res > 0.5 * prev) {
             if (++stalls >= 2) break;
         } else {
@@ -1987,6 +2002,7 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
         out->max_front = c->plan.max_front;
     }
     c->stale_solves++;
+    if (nbad > 0) return HMEC_E_PIVOT;
     return (res <= tol) ? HMEC_OK : HMEC_E_NOCONV;
 }
 
@@ -2085,20 +2101,24 @@ static int gm_residual_launch(hmec_ctx *c, GmresBuf &g, const double *x, double 
     LAUNCH_CHECK();
     return launch_norms4(c, c->d.work, c->d.rhs, g.rsd, g.bs, out);
 }
-static void gm_residual_read(const double *h, double *rel_u, double *rel_s, double *rr, int *npert) {
+// (h[5], h[6]: perturbed and unrepairable pivots of the current factor)
+constexpr int GM_NRM = 7;
+static void gm_residual_read(const double *h, double *rel_u, double *rel_s, double *rr, int *npert,
+                             int *nbad) {
     *rel_u = h[1] > 0 ? h[0] / h[1] : h[0];
     *rel_s = h[3] > 0 ? h[2] / h[3] : h[2];
     *rr = h[4];
     *npert = (int)h[5];
+    *nbad = (int)h[6];
 }
 static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *rel_u,
-                            double *rel_s, double *rr, int *npert) {
+                            double *rel_s, double *rr, int *npert, int *nbad) {
     SolverDev *s = sdev(c);
     hipStream_t st = c->stream;
     HMEC_TRY(gm_residual_launch(c, g, x, s->norms));
-    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 6, hipMemcpyDeviceToHost, st));
+    HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * GM_NRM, hipMemcpyDeviceToHost, st));
     HMEC_HIP(hipStreamSynchronize(st));
-    gm_residual_read(s->h_norms, rel_u, rel_s, rr, npert);
+    gm_residual_read(s->h_norms, rel_u, rel_s, rr, npert, nbad);
     return HMEC_OK;
 }
 
@@ -2129,12 +2149,13 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     hipStream_t st = c->stream;
     double *x = c->d.sol;
     double *r = g.rsd;
-    int total_it = 0, npert = 0;
+    int total_it = 0, npert = 0, nbad = 0;
     double relinf = 0, last_rel_s = 0;
     std::vector<double> H((m + 1) * m), cs(m), sn(m), gv(m + 1), h(m + 2), h2(m + 2), y(m);
     // the operator's row scaling (powers of two) and E of the current matrix
     if (!c->rscale_ready) {   // the assembly forms it with the values
-        hipLaunchKernelGGL(k_row_scale, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.val, N, c->d.rscale);
+        hipLaunchKernelGGL(k_row_scale, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.val, N, c->d.rscale,
+                           nullptr);
         LAUNCH_CHECK();
         c->rscale_ready = true;
     }
@@ -2157,10 +2178,10 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
             hipLaunchKernelGGL(k_vscale_rr, dim3(gN), dim3(256), 0, st, r, nrm_d + 4, g.V, N);
             LAUNCH_CHECK();
         } else {
-            HMEC_TRY(gm_true_residual(c, g, x, &relinf, &rel_s, &rr, &npert));
+            HMEC_TRY(gm_true_residual(c, g, x, &relinf, &rel_s, &rr, &npert, &nbad));
             last_rel_s = rel_s;
             const double worst = std::max(relinf, rel_s);
-            if (!(worst > tol) || total_it >= max_it) break;
+            if (nbad > 0 || !(worst > tol) || total_it >= max_it) break;
             const double beta = std::sqrt(rr);
             if (!(beta > 0)) break;
             target = beta * (tol / worst) * 0.5;
@@ -2197,15 +2218,15 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
                                j + 1 < m ? g.V + (int64_t)(j + 1) * N : nullptr, wwd, N);
             LAUNCH_CHECK();
             const bool first = lead && j == 0;
-            HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * (3 * M2 + (first ? 6 : 0)),
+            HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * (3 * M2 + (first ? GM_NRM : 0)),
                                     hipMemcpyDeviceToHost, st));
             HMEC_HIP(hipStreamSynchronize(st));
             if (first) {
-                gm_residual_read(g.hhost + 3 * M2, &relinf, &rel_s, &rr, &npert);
+                gm_residual_read(g.hhost + 3 * M2, &relinf, &rel_s, &rr, &npert, &nbad);
                 last_rel_s = rel_s;
                 const double worst = std::max(relinf, rel_s);
                 const double beta = std::sqrt(rr);
-                if (!(worst > tol) || !(beta > 0)) {
+                if (nbad > 0 || !(worst > tol) || !(beta > 0)) {
                     done = true;
                     break;
                 }
@@ -2262,5 +2283,6 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
         out->max_front = c->plan.max_front;
     }
     c->stale_solves++;
+    if (nbad > 0) return HMEC_E_PIVOT;
     return (relinf <= tol && last_rel_s <= tol) ? HMEC_OK : HMEC_E_NOCONV;
 }

@@ -142,8 +142,16 @@ class ServerStats(C.Structure):
                 ("factor_batch_members", C.c_int)]
 
 
+# error codes (include/hmec.h)
+E_ARG, E_HIP, E_OOM, E_PIVOT, E_NOCONV, E_NODEV, E_SOLVER, E_IO = -1, -2, -3, -4, -5, -6, -7, -8
+
+
 class HmecError(RuntimeError):
-    pass
+    """A negative HMEC_E_* return code; .code holds it."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 def lib():
@@ -249,7 +257,7 @@ def _declare(L):
 def check(rc, what=""):
     if rc != 0:
         msg = lib().hmec_strerror(rc).decode()
-        raise HmecError(f"{what}: {msg} ({rc})")
+        raise HmecError(f"{what}: {msg} ({rc})", rc)
     return rc
 
 

@@ -267,7 +267,18 @@ int hmec_set_csr_values(hmec_ctx *ctx, const double *values, const double *rhs);
  * analyze = nested-dissection ordering + symbolic multifrontal analysis,
  * done once per grid (the reference redoes it every Picard iteration);
  * factorize = numeric multifrontal LU in the given precision;
- * solve = triangular solves + FP64 iterative refinement of the last CSR. */
+ * solve = triangular solves + FP64 iterative refinement of the last CSR.
+ * Errors (Eigen's info() that the reference never reads, :1050-1052):
+ *  - hmec_factorize / hmec_batch_factorize return HMEC_E_PIVOT when the
+ *    (Darcy-condensed, equilibrated) matrix has a row or column with no finite
+ *    non-zero entry, or elimination meets a non-finite pivot: no static-pivot
+ *    perturbation can repair those.  Small finite pivots are perturbed to
+ *    +-tau and counted (hmec_solve_stats.perturbed_pivots), as PARDISO's
+ *    iparm[9] = 13 does.
+ *  - hmec_solve returns HMEC_E_NOCONV when the final relative residual
+ *    ||b - A x||inf / ||b||inf is above 1e-10 (an FP64 solve that misses its
+ *    1e-14 refinement target but stays within 1e-10 is accepted), and
+ *    HMEC_E_PIVOT for a factor with an unrepairable pivot. */
 typedef struct hmec_solve_stats {
     int ir_iterations;
     int fallback;            /* 1: FP32 factor stalled, FP64 factor used    */
@@ -306,7 +317,11 @@ int hmec_step_end(hmec_ctx *ctx, double dt);
 int hmec_advect(hmec_ctx *ctx, double dt);
 
 /* One complete timestep (h_mec_rsf_v9.cpp:376-1726): the C++ host driver
- * with the reference's Picard / dt controller around the kernels above. */
+ * with the reference's Picard / dt controller around the kernels above.
+ * HMEC_E_NOCONV when a linear solve ends above a relative residual of 1e-10
+ * (under HMEC_PREC_FP32 after the FP64 fallback), HMEC_E_PIVOT when even the
+ * FP64 factor meets an unrepairable pivot; the step is then incomplete and
+ * the context's state is that of the failed Picard iteration. */
 int hmec_step(hmec_ctx *ctx, hmec_step_info *info);
 
 /* ------------------------------------------------- symbolic (host only, no GPU)
@@ -364,7 +379,9 @@ typedef struct {
     int64_t hbm_budget;        /* admission budget in bytes (<= 0: 90% of the
                                   device memory free at creation)             */
     double batch_window_ms;    /* how long a factorization waits for same-size
-                                  partners                                     */
+                                  partners; < 0: until every running same-size
+                                  member asks or finishes (a deterministic
+                                  rendezvous, no deadline)                     */
     int keep_fields;           /* keep the members' final fields for
                                   hmec_server_get_field                         */
 } hmec_server_config;

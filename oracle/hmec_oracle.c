@@ -893,6 +893,16 @@ int orc_solve(orc *o) {
     return err;
 }
 double *orc_solution(orc *o) { return o->S; }
+/* x = A^-1 b with the PARDISO factor of the last orc_solve (phase 33 only,
+ * with its refinement steps): the correction solve of the tests'
+ * extended-precision refinement, which measures the oracle's forward error */
+int orc_resolve(orc *o, const double *b, double *x) {
+    if (!o->use_pardiso || !o->pardiso_ready) return -1;
+    int maxfct = 1, mnum = 1, mtype = 11, n = o->N, nrhs = 1, msg = 0, err = 0, phase = 33;
+    g_pardiso(o->pt, &maxfct, &mnum, &mtype, &phase, &n, o->val, o->rowp, o->coli, NULL, &nrhs,
+              o->iparm, &msg, (void *)b, x, &err);
+    return err;
+}
 void orc_lu_stats(orc *o, int64_t *nnz, double *mflops) {
     if (nnz) *nnz = o->lu_nnz;
     if (mflops) *mflops = o->lu_mflops;

@@ -73,6 +73,8 @@ def lib():
         L.orc_get_dt.argtypes = [vp]
         L.orc_get_dt.restype = C.c_double
         L.orc_lu_stats.argtypes = [vp, P(C.c_int64), P(C.c_double)]
+        L.orc_resolve.argtypes = [vp, P(C.c_double), P(C.c_double)]
+        L.orc_resolve.restype = C.c_int
         L.orc_controller.argtypes = [vp, vp, C.c_int]
         L.orc_controller.restype = C.c_int
         L.orc_dsylsq.argtypes = [vp]
@@ -184,6 +186,14 @@ class Oracle:
     def solution(self):
         n = (self.p.Nx + 1) * (self.p.Ny + 1) * 6
         return np.ctypeslib.as_array(lib().orc_solution(self.h), shape=(n,)).copy()
+
+    def resolve(self, b):
+        """A^-1 b with PARDISO's factor of the last solve (None without it)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        rc = lib().orc_resolve(self.h, b.ctypes.data_as(C.POINTER(C.c_double)),
+                               x.ctypes.data_as(C.POINTER(C.c_double)))
+        return x if rc == 0 else None
 
     def reload(self):
         lib().orc_reload(self.h)

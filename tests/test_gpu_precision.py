@@ -141,7 +141,7 @@ def test_fp32_path_errors(hmec, gpu, cfg):
     errs = group_errors_pair(c32, c64)
     print("fp32 vs fp64 cfg%d:" % cfg, {k: "%.2e" % v for k, v in errs.items()})
     for name, e in errs.items():
-        tol = {"darcy": 1e-4, "stress": 1e-9, "slip_rate": 1e-9}.get(name, 1e-10)
+        tol = {"darcy": 1e-4, "stress": 1e-9, "slip_rate": 1e-9, "fault_stress": 1e-9}.get(name, 1e-10)
         assert e <= tol, (name, e)
 
 
