@@ -193,6 +193,13 @@ extern "C" int hmec_set_refinement(hmec_ctx *c, int max_stale, int max_fresh, do
     return HMEC_OK;
 }
 
+extern "C" int hmec_set_fp64_refinement(hmec_ctx *c, int max_ir, double tol) {
+    if (!c || max_ir < 0 || !(tol > 0)) return HMEC_E_ARG;
+    c->fp64_max_ir = max_ir;
+    c->fp64_tol = tol;
+    return HMEC_OK;
+}
+
 extern "C" int hmec_set_factor_cache(hmec_ctx *c, int slots) {
     if (!c) return HMEC_E_ARG;
     return sv_slots_set(c, slots);
@@ -563,7 +570,7 @@ extern "C" int hmec_solve(hmec_ctx *c, hmec_solve_stats *st) {
     if (!c || !c->factor_valid) return HMEC_E_ARG;
     const bool fp32 = c->factor_prec == HMEC_PREC_FP32;
     tm_begin(c, c->t_solve);
-    int rc = sv_solve(c, fp32 ? 30 : 3, fp32 ? 1e-12 : 1e-14, st);
+    int rc = sv_solve(c, fp32 ? 30 : c->fp64_max_ir, fp32 ? 1e-12 : c->fp64_tol, st);
     tm_end(c, c->t_solve, 0);
     // FP64: a result that missed the 1e-14 target but meets the acceptance
     // bound is kept, like PARDISO's

@@ -194,7 +194,7 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         tm_end(c, c->t_factor, c->plan.flops);
         acc.factorizations++;
         tm_begin(c, c->t_solve);
-        int rc = sv_solve(c, 3, 1e-14, &st);
+        int rc = sv_solve(c, c->fp64_max_ir, c->fp64_tol, &st);
         tm_end(c, c->t_solve, 0);
         if (rc == HMEC_E_NOCONV && st.rel_residual <= kAcceptResidual) rc = HMEC_OK;
         if (rc != HMEC_OK) return rc;
@@ -251,7 +251,7 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
             acc.fallbacks++;
             HMEC_TRY(factor(c, HMEC_PREC_FP64));
             acc.factorizations++;
-            rc = sv_solve(c, 3, 1e-14, &st);
+            rc = sv_solve(c, c->fp64_max_ir, c->fp64_tol, &st);
             if (rc == HMEC_E_NOCONV && st.rel_residual <= kAcceptResidual) rc = HMEC_OK;
         }
         if (rc != HMEC_OK) return rc;

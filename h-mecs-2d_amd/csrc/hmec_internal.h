@@ -271,6 +271,10 @@ struct hmec_ctx {
     int ir_max_stale = 12;           // FP32 policy: GMRES budget with a reused factor
     int ir_max_fresh = 40;           //              ... with a fresh factor
     double ir_tol = 1e-12;           //              target relative inf-norm residual
+    int fp64_max_ir = 3;             // FP64 factor: refinement sweeps after the first solve
+    double fp64_tol = 1e-16;         //              target (unscaled and row-equilibrated):
+                                     //              rounding level, i.e. refine until the
+                                     //              equilibrated residual stops halving
     hmec_batcher *batcher = nullptr; // server member: factorizations go through the batcher
     int batched_facs = 0;            // factorizations done inside a batch of > 1 members
     // statistics of the current step

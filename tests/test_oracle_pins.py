@@ -283,8 +283,9 @@ def test_band_lu_matches_pardiso_cfg1(hmec, orc):
 
 def test_darcy_forward_error_bound(hmec, orc):
     """Measures the oracle's own forward error on the Darcy fluxes against an
-    extended-precision refinement: cancellation-limited at ~1e-8 relative,
-    which is why the GPU parity holds them to 1e-4 instead of 1e-10."""
+    extended-precision refinement: cancellation-limited well above 1e-10
+    (the fluxes are differences of nearly equal pressures), which is why the
+    GPU parity holds them to a multiple of this measured floor instead."""
     p = hmec.params(1)
     o = assembled(hmec, orc, p)
     assert o.solve() == 0
