@@ -1957,7 +1957,11 @@ static int precond_apply(hmec_ctx *c, const double *v, const double *rs, double 
     return HMEC_OK;
 }
 
-int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
+// The refinement's work vectors rsd = Dr (b - A x) and bs = Dr b (N doubles
+// each) are passed in: the context's own (allocated on first use) or the
+// allocation-free entry point's scratch array.
+static int sv_solve_impl(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out, double *rsd,
+                         double *bs) {
     if (!c->factor_valid) return HMEC_E_ARG;
     SolverDev *s = sdev(c);
     const int64_t N = c->N;
@@ -1974,11 +1978,7 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
         LAUNCH_CHECK();
         c->rscale_ready = true;
     }
-    if (!s->rsd) {
-        HMEC_HIP(hipMalloc(&s->rsd, sizeof(double) * N));
-        HMEC_HIP(hipMalloc(&s->bs, sizeof(double) * N));
-    }
-    hipLaunchKernelGGL(k_scale_vec, dim3(g), dim3(256), 0, st, c->d.rhs, c->d.rscale, s->bs, N);
+    hipLaunchKernelGGL(k_scale_vec, dim3(g), dim3(256), 0, st, c->d.rhs, c->d.rscale, bs, N);
     LAUNCH_CHECK();
     // r = b
     HMEC_HIP(hipMemcpyAsync(r, c->d.rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
@@ -1991,9 +1991,9 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
         HMEC_TRY(precond_apply(c, r, nullptr, z));
         hipLaunchKernelGGL(k_add, dim3(g), dim3(256), 0, st, x, z, N, it == 0 ? 1 : 0);
         LAUNCH_CHECK();
-        launch_residual(c, x, r, s->rsd, c->d.rscale);
+        launch_residual(c, x, r, rsd, c->d.rscale);
         LAUNCH_CHECK();
-        HMEC_TRY(launch_norms4(c, r, c->d.rhs, s->rsd, s->bs, s->norms));
+        HMEC_TRY(launch_norms4(c, r, c->d.rhs, rsd, bs, s->norms));
         HMEC_HIP(hipMemcpyAsync(s->h_norms, s->norms, sizeof(double) * 7, hipMemcpyDeviceToHost, st));
         HMEC_HIP(hipStreamSynchronize(st));
         const double *h = s->h_norms;
@@ -2021,6 +2021,71 @@ int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
     c->stale_solves++;
     if (nbad > 0) return HMEC_E_PIVOT;
     return (res <= tol) ? HMEC_OK : HMEC_E_NOCONV;
+}
+
+int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *out) {
+    if (!c->factor_valid) return HMEC_E_ARG;
+    SolverDev *s = sdev(c);
+    if (!s->rsd) {
+        HMEC_HIP(hipMalloc(&s->rsd, sizeof(double) * c->N));
+        HMEC_HIP(hipMalloc(&s->bs, sizeof(double) * c->N));
+    }
+    return sv_solve_impl(c, max_ir, tol, out, s->rsd, s->bs);
+}
+
+// ------------------------------------------------ allocation-free solve
+// hmec_solve_noalloc keeps the refinement's two work vectors in this
+// file-scope device array instead of allocating them on first use: room for
+// kNoallocMaxN unknowns (801x201 has 972,024), 2 x 8 MiB.  Every device the
+// library runs on holds its own copy until the process ends; host code reaches
+// it only through the address hipGetSymbolAddress returns for the current
+// device (the array's name in host code is a stand-in, not that address).
+// Both vectors are fully written before they are read on every call, so
+// nothing needs clearing; calls that overlap on one device share the array and
+// race.
+constexpr int64_t kNoallocMaxN = int64_t(1) << 20;
+static __device__ __align__(256) double scratch[2 * kNoallocMaxN];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// the scratch array's address on the current device (looked up once per device)
+int scratch_address(int device, double **out) {
+    if (device < 0 || device >= kMaxDevices) return HMEC_E_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        HMEC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            hmec_set_last_error("hipGetSymbolAddress(scratch) returned no address");
+            return HMEC_E_HIP;
+        }
+        g_scratch[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch[device];
+    return HMEC_OK;
+}
+}  // namespace
+
+extern "C" int hmec_solve_noalloc(hmec_ctx *c, hmec_solve_stats *st) {
+    if (!c) return HMEC_E_ARG;
+    if (c->N > kNoallocMaxN) {
+        hmec_set_last_error("hmec_solve_noalloc: " + std::to_string(c->N) +
+                            " unknowns, above the " + std::to_string(kNoallocMaxN) +
+                            " its scratch array holds");
+        return HMEC_E_ARG;
+    }
+    if (!c->factor_valid) return HMEC_E_ARG;
+    HMEC_HIP(hipSetDevice(c->device));
+    double *sp = nullptr;
+    HMEC_TRY(scratch_address(c->device, &sp));
+    const bool fp32 = c->factor_prec == HMEC_PREC_FP32;
+    int rc = sv_solve_impl(c, fp32 ? 30 : c->fp64_max_ir, fp32 ? 1e-12 : c->fp64_tol, st, sp,
+                           sp + kNoallocMaxN);
+    if (rc == HMEC_E_NOCONV && !fp32 && c->last_res <= kAcceptResidual) rc = HMEC_OK;
+    return rc;
 }
 
 int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision) {

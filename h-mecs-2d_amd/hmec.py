@@ -209,6 +209,9 @@ def _declare(L):
     L.hmec_set_precision.argtypes = [vp, C.c_int]
     L.hmec_set_refinement.argtypes = [vp, C.c_int, C.c_int, C.c_double]
     L.hmec_set_fp64_refinement.argtypes = [vp, C.c_int, C.c_double]
+    L.hmec_solve_noalloc.argtypes = [vp, P(SolveStats)]
+    L.hmec_last_error.restype = C.c_char_p
+    L.hmec_last_error.argtypes = []
     L.hmec_set_factor_cache.argtypes = [vp, C.c_int]
     L.hmec_init_coseismic.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_uint64]
     L.hmec_set_markers.argtypes = [vp, P(P(C.c_double)), C.c_int64]

@@ -300,6 +300,16 @@ typedef struct hmec_solve_stats {
 int hmec_analyze(hmec_ctx *ctx);
 int hmec_factorize(hmec_ctx *ctx, int precision);
 int hmec_solve(hmec_ctx *ctx, hmec_solve_stats *stats);
+/* hmec_solve without any device allocation, for callers that cannot
+ * allocate (tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same kernels and results as hmec_solve; the
+ * refinement's two work vectors live in a file-scope device array of the
+ * library (2 x 1,048,576 doubles per device, held until the process ends)
+ * instead of being allocated on first use.  The context must have been
+ * created, analyzed and factorized beforehand (those calls allocate).
+ * HMEC_E_ARG above 1,048,576 unknowns (hmec_last_error says so).  Calls that
+ * overlap on one device share that array and race: serialize them. */
+int hmec_solve_noalloc(hmec_ctx *ctx, hmec_solve_stats *stats);
 /* solution vector of the last solve (N doubles, reference unknown order) */
 int hmec_get_solution(hmec_ctx *ctx, double *x);
 
