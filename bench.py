@@ -15,8 +15,15 @@ value = timesteps/s of the mixed-precision path: an FP32 multifrontal factor
         iteration, PardisoLU semantics) is timed beside it.
 roofline = the dominant kernel of the FP32 path, measured with HIP events
         on the context's stream over the timed region.
-cpu_baseline = the CPU oracle (plain-C restatement + MKL PARDISO, sequential)
-        on one timestep of the same workload, on this host.
+sustained = the FP32 path over 200 further steps (its factorization cadence
+        included), with factorizations, p50 and p99 of that window.
+error_vs_oracle = per-field relative error of both GPU paths against the FP64
+        oracle (MKL PARDISO) after the same two timesteps.
+cpu_baseline = the CPU oracle (plain-C restatement + MKL PARDISO) on the same
+        workload on this host: setting A as shipped (MKL sequential, 1 core),
+        setting B with MKL threaded on every core of the process; the second
+        step's cost per Picard iteration scaled to the GPU window's Picard
+        iterations per step.
 
 --serve runs the serving workload instead and prints its own JSON line: a
 Poisson-arrival stream of ensemble members with grids drawn uniformly from
@@ -75,21 +82,99 @@ def run_path(hm, p, soa, prec, warmup, steps):
     return ctx, total, np.array(lat), infos, kt
 
 
-def cpu_baseline(hm, p, soa):
-    """One steady-state timestep of the oracle on the host (MKL sequential)."""
+ERR_FIELDS = ("vxs", "vys", "pt", "pf", "vxD", "vyD", "SXX", "SYY", "SXY", "VSLIPB", "ETA", "OM",
+              "SigmaY", "SII_fault")
+
+
+def host_facts():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+def mkl_version():
+    """Version string of the MKL the oracle loads (same search as the oracle)."""
+    for name in ("libmkl_rt.so", "libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so",
+                 "/opt/conda/lib/libmkl_rt.so.1"):
+        try:
+            L = C.CDLL(name)
+        except OSError:
+            continue
+        buf = C.create_string_buffer(256)
+        L.MKL_Get_Version_String(buf, 256)
+        return buf.value.decode(errors="replace").strip()
+    return None
+
+
+def cpu_sample(args):
+    """Child process: the CPU oracle (plain-C restatement + MKL PARDISO) on
+    the first two timesteps of the workload, with MKL's threading fixed by the
+    parent's environment; prints per-phase timings as JSON and, with
+    --cpu-fields, saves the fields after step 2 for the error report."""
+    hm = load_hmec()
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
     import oracle
     oracle.build()
+    p = hm.params(args.cfg)
+    soa = hm.generate_markers(p, cfg=args.cfg, seed=args.cfg)
     o = oracle.Oracle(p, soa, hm)
-    o.step()                       # untimed: first step (two Picard iterations, MKL init)
-    t = time.perf_counter()
-    info = o.step()
-    dt = time.perf_counter() - t
-    return {"value": 1.0 / dt, "unit": "timesteps/s", "cores": 1, "kind": "port",
-            "sample": "one steady-state timestep (%d Picard iteration) of the 801x201 workload, "
-                      "oracle restatement + MKL PARDISO sequential, after one untimed step"
-                      % info.iterations}
+    steps = []
+    for _ in range(2):
+        t = time.perf_counter()
+        info = o.step()
+        steps.append({"s": time.perf_counter() - t, "iterations": info.iterations,
+                      "scatter_ms": info.ms_scatter, "assemble_ms": info.ms_assemble,
+                      "solve_ms": info.ms_factor, "update_ms": info.ms_update})
+    if args.cpu_fields:
+        np.savez(args.cpu_fields, **{f: o.get(f) for f in ERR_FIELDS})
+    print(json.dumps({"steps": steps, "pardiso": oracle.have_pardiso(), "mkl": mkl_version()}))
+
+
+def cpu_baseline(args, picard_per_step, fields_path):
+    """The oracle timed on this host in two MKL settings (children of this
+    process): A = as shipped, MKL sequential and the single-threaded oracle
+    (1 core); B = MKL threaded on every core of this process's affinity.
+    The second (steady-state) step gives the cost of a scatter and of one
+    Picard iteration (assembly, PARDISO analyze + factorize + solve, update);
+    the value is the timesteps/s at the GPU window's mean Picard iterations
+    per step."""
+    out = {}
+    ncores = len(os.sched_getaffinity(0))
+    for name, env_over, cores in (("A", {"MKL_THREADING_LAYER": "SEQUENTIAL", "MKL_NUM_THREADS": "1"}, 1),
+                                  ("B", {"MKL_THREADING_LAYER": "INTEL",
+                                         "MKL_NUM_THREADS": str(ncores)}, ncores)):
+        env = dict(os.environ)
+        env.update(env_over)
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-sample", "--cfg", str(args.cfg)]
+        if name == "A":
+            cmd += ["--cpu-fields", fields_path]
+        r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            out[name] = {"error": r.stderr[-400:]}
+            continue
+        d = json.loads(r.stdout.strip().splitlines()[-1])
+        st = d["steps"][1]
+        per_it = (st["assemble_ms"] + st["solve_ms"] + st["update_ms"]) / st["iterations"]
+        step_ms = st["scatter_ms"] + per_it * picard_per_step
+        out[name] = {"value": 1e3 / step_ms, "cores": cores, "per_picard_iteration_s": per_it * 1e-3,
+                     "scatter_s": st["scatter_ms"] * 1e-3, "sample_steps": d["steps"],
+                     "pardiso": d["pardiso"], "mkl": d["mkl"]}
+    a = out.get("A", {})
+    res = {"value": a.get("value"), "unit": "timesteps/s", "cores": 1, "kind": "port",
+           "sample": ("the first two timesteps of the 801x201 workload on the oracle (plain-C "
+                      "restatement, MKL PARDISO sequential: setting A, as shipped); the second step's "
+                      "scatter and per-Picard-iteration cost scaled to the GPU window's %.2f Picard "
+                      "iterations per step" % picard_per_step),
+           "solver": ("MKL PARDISO (%s)" % a.get("mkl")) if a.get("pardiso") else "banded LU (no MKL)",
+           "host": host_facts(), "setting_A": a, "setting_B": out.get("B")}
+    return res
 
 
 def serve(hm, args):
@@ -156,6 +241,9 @@ def main():
     ap.add_argument("--serve-interarrival-ms", type=float, default=50.0)
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--serve-fp64", action="store_true")
+    ap.add_argument("--sustained", type=int, default=200, help="steps of the sustained FP32 window")
+    ap.add_argument("--cpu-sample", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-fields", default="", help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -173,6 +261,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each ensemble member "
               "or grid runs on one device); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.cpu_sample:
+        cpu_sample(args)
+        return
     hm = load_hmec()
     if args.serve:
         serve(hm, args)
@@ -234,6 +325,25 @@ def main():
                                                 "traffic": None}))
     roof = max(cands, key=lambda c: c[0])[1] if cands else None
     value = args.steps / t32
+    # the same FP32 context over a longer window: its refactorization cadence
+    # (a far dt gets its own factor; a stalled GMRES refactors) is inside it
+    sustained = None
+    if args.sustained > 0:
+        lat_s, inf_s = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.sustained):
+            a = time.perf_counter()
+            inf_s.append(ctx32.step())
+            lat_s.append(time.perf_counter() - a)
+        ts = time.perf_counter() - t0
+        sustained = {"steps": args.sustained, "timesteps_per_s": args.sustained / ts,
+                     "p50_ms": float(np.percentile(lat_s, 50) * 1e3),
+                     "p99_ms": float(np.percentile(lat_s, 99) * 1e3),
+                     "factorizations": int(sum(i.factorizations for i in inf_s)),
+                     "gmres_iterations": int(sum(i.ir_iterations for i in inf_s)),
+                     "fallbacks": int(sum(i.fallbacks for i in inf_s)),
+                     "picard_iterations": int(sum(i.iterations for i in inf_s)),
+                     "max_rel_residual": float(max(i.max_rel_residual for i in inf_s))}
     out = {
         "metric": METRIC,
         "value": value,
@@ -259,13 +369,36 @@ def main():
                     "picard_iterations": int(sum(i.iterations for i in inf32))},
         "kernels_fp32": {k: {"n": v["n"], "avg_ms": v["avg_ms"]} for k, v in kt32.items()},
         "roofline": roof,
+        "sustained": sustained,
     }
     out.update(res)
     if not args.no_cpu:
+        import shutil
+        import tempfile
+        tmp = tempfile.mkdtemp()
+        fields_path = os.path.join(tmp, "oracle_fields.npz")
         try:
-            out["cpu_baseline"] = cpu_baseline(hm, p, soa)
+            picard = sum(i.iterations for i in inf32) / len(inf32)
+            out["cpu_baseline"] = cpu_baseline(args, picard, fields_path)
+            if os.path.exists(fields_path):
+                ref = np.load(fields_path)
+                ev = {}
+                for name, prec in (("fp64", hm.PREC_FP64), ("fp32", hm.PREC_FP32)):
+                    c = hm.Context(p, soa, precision=prec)
+                    for _ in range(2):
+                        c.step()
+                    ev[name] = {}
+                    for f in ERR_FIELDS:
+                        a, b = c.get(f), ref[f]
+                        nb = np.linalg.norm(b)
+                        ev[name][f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(
+                            np.linalg.norm(a - b))
+                    c.close()
+                out["error_vs_oracle"] = ev
         except Exception as e:     # reported, never fatal for the GPU number
             out["cpu_baseline"] = {"error": str(e)}
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
     print(json.dumps(out))
 
 
