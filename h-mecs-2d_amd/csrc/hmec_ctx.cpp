@@ -88,6 +88,7 @@ static void free_ctx(hmec_ctx *c) {
     sv_sweep_free(c);
     sv_free_device(c);
     DeviceState &d = c->d;
+    hipFree(d.series);
     hipFree(d.fields); hipFree(d.mk); hipFree(d.prop); hipFree(d.cell); hipFree(d.perm);
     hipFree(d.sorted);
     hipFree(d.cell_start); hipFree(d.sort_tmp); hipFree(d.keys_tmp); hipFree(d.coords);
@@ -154,6 +155,8 @@ extern "C" int hmec_create(const hmec_params *p, hmec_ctx **out) {
     const size_t ncount = 16 + 2 * (size_t)c->Nx1 + 64;
     if (hipMalloc(&d.counters, sizeof(int) * ncount) != hipSuccess) return fail(HMEC_E_OOM);
     hipMemset(d.counters, 0, sizeof(int) * ncount);
+    if (hipMalloc(&d.series, sizeof(double) * 6 * kNumTimesteps) != hipSuccess) return fail(HMEC_E_OOM);
+    hipMemset(d.series, 0, sizeof(double) * 6 * kNumTimesteps);
     for (double **v : {&d.rhs, &d.sol, &d.work, &d.work2, &d.rscale, &d.cscale}) {
         if (hipMalloc(v, sizeof(double) * N) != hipSuccess) return fail(HMEC_E_OOM);
         hipMemset(*v, 0, sizeof(double) * N);
@@ -623,6 +626,9 @@ extern "C" int hmec_step_end(hmec_ctx *c, double dt) {
     HMEC_TRY(k_final_stress(c, dt));
     if (c->P.advect_markers) HMEC_TRY(k_advect(c, dt));
     HMEC_TRY(k_history(c));
+    // :1660-1690: timesum and the step's entry of the per-step series
+    c->timesum += dt;
+    HMEC_TRY(k_series_step(c, c->timesum, dt));
     return HMEC_OK;
 }
 

@@ -315,8 +315,7 @@ extern "C" int hmec_step(hmec_ctx *c, hmec_step_info *info) {
     // :1464-1472
     if (c->dt00 > c->dt) c->yndtdecrease = 1;
     c->dt = std::min(std::min(c->dtx, c->dty), std::min(c->dt, c->dtlapusta));
-    HMEC_TRY(hmec_step_end(c, c->dt));
-    c->timesum += c->dt;
+    HMEC_TRY(hmec_step_end(c, c->dt));     // (also timesum += dt, :1661)
     HMEC_HIP(hipStreamSynchronize(c->stream));
     const double f = now_ms();
     acc.ms_update += f - e;

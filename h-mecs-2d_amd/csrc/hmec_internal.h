@@ -31,6 +31,10 @@
 // use); below it a refinement that missed its tighter target is accepted.
 constexpr double kAcceptResidual = 1e-10;
 
+// Length of the per-step series of the snapshot files (constants.hpp:14
+// num_timesteps, the reference's maximum number of timesteps).
+constexpr int64_t kNumTimesteps = 100000;
+
 void hmec_set_last_error(const std::string &s);
 double hmec_seeded_normal(uint64_t seed, int64_t idx, unsigned stream);
 
@@ -218,6 +222,8 @@ struct DeviceState {
     double *work = nullptr;          // N-vector scratch (residual, correction)
     double *work2 = nullptr;
     int *counters = nullptr;         // perturbed pivots etc.
+    double *series = nullptr;        // 6 x kNumTimesteps per-step series (timesumcur, dtcur,
+                                     // maxvxsmod, minvxsmod, maxvysmod, minvysmod)
 };
 
 struct hmec_ctx {
@@ -295,6 +301,7 @@ int k_scale(hmec_ctx *c, double dt, double *ptscale);
 int k_assemble(hmec_ctx *c, double dt);   // ptscale from the device scalars
 int k_reload(hmec_ctx *c, double ptscale);
 int hmec_update_dev(hmec_ctx *c, double dt, hmec_iter_scalars *out);
+int k_series_step(hmec_ctx *c, double timesum, double dt);
 constexpr int SC_PTSCALE_SLOT = 0;   // ptscale's index in the device scalars
 constexpr int HMEC_FP32_SLOTS = 3;   // FP32 factors kept by default (keyed by dt)
 int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out);

@@ -1396,6 +1396,58 @@ int k_history(hmec_ctx *c) {
     return HMEC_OK;
 }
 
+// Per-step series of h_mec_rsf_v9.cpp:1661-1690: timesumcur, dtcur and the
+// extrema of vxs over the nodes i < Ny, j < Nx with RHOX > 2000 and i != 0,
+// of vys over the same range with RHOY > 2000 and j != 0 (start values -1e30
+// and 1e30), stored at entry t of the six series (length nt).  One workgroup.
+constexpr int kSeriesT = 256;
+__global__ void __launch_bounds__(kSeriesT) k_series(const double *f, int64_t ns, int Nx, int Ny,
+                                                     double *series, int64_t nt, int64_t t,
+                                                     double timesum, double dt) {
+    __shared__ double red[4][kSeriesT];
+    const int Ny1 = Ny + 1, tid = threadIdx.x;
+    const double *vxs = f + (int64_t)HMEC_F_VXS * ns, *vys = f + (int64_t)HMEC_F_VYS * ns;
+    const double *rhox = f + (int64_t)HMEC_F_RHOX * ns, *rhoy = f + (int64_t)HMEC_F_RHOY * ns;
+    double mx = -1e30, nx = 1e30, my = -1e30, ny = 1e30;
+    for (int q = tid; q < Nx * Ny; q += kSeriesT) {
+        const int i = q % Ny, j = q / Ny;
+        const int64_t s = i + (int64_t)j * Ny1;
+        if (rhox[s] > 2000 && i != 0) {
+            mx = fmax(mx, vxs[s]);
+            nx = fmin(nx, vxs[s]);
+        }
+        if (rhoy[s] > 2000 && j != 0) {
+            my = fmax(my, vys[s]);
+            ny = fmin(ny, vys[s]);
+        }
+    }
+    red[0][tid] = mx; red[1][tid] = nx; red[2][tid] = my; red[3][tid] = ny;
+    __syncthreads();
+    for (int s = kSeriesT / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[0][tid] = fmax(red[0][tid], red[0][tid + s]);
+            red[1][tid] = fmin(red[1][tid], red[1][tid + s]);
+            red[2][tid] = fmax(red[2][tid], red[2][tid + s]);
+            red[3][tid] = fmin(red[3][tid], red[3][tid + s]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        series[t] = timesum;
+        series[nt + t] = dt;
+        for (int q = 0; q < 4; q++) series[(2 + q) * nt + t] = red[q][0];
+    }
+}
+
+int k_series_step(hmec_ctx *c, double timesum, double dt) {
+    const int64_t t = c->timestep - 1;
+    if (t < 0 || t >= kNumTimesteps) return HMEC_OK;     // beyond the reference's series length
+    hipLaunchKernelGGL(k_series, dim3(1), dim3(kSeriesT), 0, c->stream, c->d.fields, c->ns, c->Nx,
+                       c->Ny, c->d.series, kNumTimesteps, t, timesum, dt);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
 // ---------------------------------------------------- advection (a15)
 // Node -> marker bilinear gather of vx, vy (vx / vy grids) and spin (basic
 // grid) with the reference's RK stages (h_mec_rsf_v9.cpp:1563-1656: stage 3

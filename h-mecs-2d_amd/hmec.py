@@ -111,6 +111,17 @@ class SolveStats(C.Structure):
 _lib = None
 
 
+class Snapshot(C.Structure):
+    """Mirror of hmec_snapshot (host arrays of an HDF5 snapshot)."""
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("markers", C.c_int64),
+                ("basic", C.POINTER(C.c_double) * 24), ("stag", C.POINTER(C.c_double) * 30),
+                ("series", C.POINTER(C.c_double) * 6), ("mk", C.POINTER(C.c_double) * 5),
+                ("values", C.c_double * 9)]
+
+
+SERIES_LEN = 100000
+
+
 class ServerConfig(C.Structure):
     """hmec_server_config (include/hmec.h)."""
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("max_active", C.c_int),
@@ -212,6 +223,15 @@ def _declare(L):
     L.hmec_solve_noalloc.argtypes = [vp, P(SolveStats)]
     L.hmec_last_error.restype = C.c_char_p
     L.hmec_last_error.argtypes = []
+    for f in ("hmec_write_h5", "hmec_read_h5", "hmec_save", "hmec_restart"):
+        getattr(L, f).argtypes = [vp, C.c_char_p]
+    L.hmec_trim_evo.argtypes = [C.c_char_p, C.c_int]
+    L.hmec_get_series.argtypes = [vp, P(C.c_double)]
+    L.hmec_h5_dataset_name.restype = C.c_char_p
+    L.hmec_h5_dataset_name.argtypes = [C.c_int, C.c_int]
+    L.hmec_h5_field_id.argtypes = [C.c_int, C.c_int]
+    L.hmec_h5_write.argtypes = [C.c_char_p, P(Snapshot)]
+    L.hmec_h5_read.argtypes = [C.c_char_p, P(Snapshot)]
     L.hmec_set_factor_cache.argtypes = [vp, C.c_int]
     L.hmec_init_coseismic.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_uint64]
     L.hmec_set_markers.argtypes = [vp, P(P(C.c_double)), C.c_int64]
@@ -339,6 +359,26 @@ class Context:
     def set_refinement(self, max_stale=12, max_fresh=40, tol=1e-12):
         check(lib().hmec_set_refinement(self.h, int(max_stale), int(max_fresh), float(tol)),
               "set_refinement")
+
+    def write_h5(self, path):
+        """Snapshot in the reference's restart layout (hmec_write_h5)."""
+        check(lib().hmec_write_h5(self.h, os.fsencode(path)), "write_h5")
+
+    def read_h5(self, path):
+        check(lib().hmec_read_h5(self.h, os.fsencode(path)), "read_h5")
+
+    def save(self, directory):
+        """file.txt + h_mec_<timestep>.h5 of the step just completed."""
+        check(lib().hmec_save(self.h, os.fsencode(directory)), "save")
+
+    def restart(self, directory):
+        check(lib().hmec_restart(self.h, os.fsencode(directory)), "restart")
+
+    def series(self):
+        """The six per-step series, array (6, SERIES_LEN)."""
+        out = np.zeros((6, SERIES_LEN), dtype=np.float64)
+        check(lib().hmec_get_series(self.h, dptr(out)), "get_series")
+        return out
 
     def set_fp64_refinement(self, max_ir=3, tol=1e-14):
         check(lib().hmec_set_fp64_refinement(self.h, int(max_ir), float(tol)), "set_fp64_refinement")

@@ -246,6 +246,74 @@ int hmec_get_field(hmec_ctx *ctx, int field_id, double *host, int layout);
  * first timestep.  The rows are gathered on the device in one launch.
  * HMEC_E_IO when a file cannot be written. */
 int hmec_write_evo(hmec_ctx *ctx, const char *dir, int first);
+/* ------------------------------------------------------------ HDF5 snapshots
+ * The reference's restart files (write h_mec_rsf_v9.cpp:1807-1876, read
+ * :176-282, helpers cpp/hdf5.cpp:13-150): one HDF5 file with the groups
+ * /Matrix, /Vector and /Value.  /Matrix: the 24 basic-grid matrices (Ny x Nx)
+ * then the 30 staggered ones (Ny1 x Nx1) in the order hmec_h5_dataset_name
+ * gives, NATIVE_DOUBLE, row-major (element [i][j] = M(i, j), hdf5.cpp:46-53);
+ * /Vector: the six per-step series timesumcur, dtcur, maxvxsmod, minvxsmod,
+ * maxvysmod, minvysmod (num_timesteps = 100000 entries, constants.hpp:14),
+ * then the marker vectors xm, ym, sxxm, syym, sxym (marknum entries);
+ * /Value/values: timesum, dt00, dtx, dty, dtlapusta, Vmax, maxvxy, dt,
+ * yndtdecrease.  Datasets are staged in heap buffers (the reference's stack
+ * arrays overflow at 801x201); the file is the same.  The HDF5 C library
+ * (libhdf5.so.103, 1.10) is loaded at first use: without it these calls
+ * return HMEC_E_IO, as do unwritable paths, missing datasets and datasets
+ * whose shape is not the grid's. */
+#define HMEC_H5_NBASIC 24
+#define HMEC_H5_NSTAG 30
+#define HMEC_H5_NSERIES 6
+#define HMEC_H5_NMARKER 5
+#define HMEC_H5_NVALUES 9
+#define HMEC_H5_SERIES_LEN 100000
+/* dataset name k of group 0 (/Matrix basic), 1 (/Matrix staggered),
+ * 2 (/Vector series) or 3 (/Vector markers); NULL out of range */
+const char *hmec_h5_dataset_name(int group, int k);
+/* the hmec_field id of /Matrix dataset k (group 0 or 1), -1 out of range */
+int hmec_h5_field_id(int group, int k);
+/* A snapshot in host memory; matrices column-major (Eigen's MatrixXd, as the
+ * rest of this ABI).  hmec_h5_write reads the arrays, hmec_h5_read fills the
+ * non-NULL ones (Nx, Ny and markers give the shapes it expects). */
+typedef struct hmec_snapshot {
+    int Nx, Ny;                           /* basic grid; staggered (Ny+1)x(Nx+1) */
+    int64_t markers;                      /* marknum                              */
+    double *basic[HMEC_H5_NBASIC];        /* Ny x Nx, hmec_h5_dataset_name(0, k)  */
+    double *stag[HMEC_H5_NSTAG];          /* Ny1 x Nx1, hmec_h5_dataset_name(1, k) */
+    double *series[HMEC_H5_NSERIES];      /* HMEC_H5_SERIES_LEN each              */
+    double *mk[HMEC_H5_NMARKER];          /* xm, ym, sxxm, syym, sxym             */
+    double values[HMEC_H5_NVALUES];
+} hmec_snapshot;
+int hmec_h5_write(const char *filename, const hmec_snapshot *snap);
+int hmec_h5_read(const char *filename, hmec_snapshot *snap);
+/* A context's state to a snapshot file / a snapshot file into a context
+ * (fields, series, the five marker vectors, the nine scalars).  The context
+ * must hold markers before hmec_read_h5: the file keeps only xm, ym, sxxm,
+ * syym, sxym, and the other marker properties are regenerated by the caller,
+ * as the reference's start-up does (:188-209).  Not in the file, as in the
+ * reference: the current stresses SXX, SYY, SXY and the viscosity ETA; a
+ * restarted step therefore sees zero current stresses where the continuous
+ * run saw the last step's, which changes only the SIIB / EIIB diagnostics of
+ * its first Picard iteration (:627-631), not the solution. */
+int hmec_write_h5(hmec_ctx *ctx, const char *filename);
+int hmec_read_h5(hmec_ctx *ctx, const char *filename);
+/* The savestep block (:1807-1876): dir/file.txt holds the timestep just
+ * completed, dir/h_mec_<timestep>.h5 the snapshot (the caller decides when,
+ * timestep % savestep == 0 in the reference). */
+int hmec_save(hmec_ctx *ctx, const char *dir);
+/* The restart start-up (:178-282): timestep T from dir/file.txt; T > 0 reads
+ * dir/h_mec_<T>.h5 into the context, sets the next timestep to T + 1 and trims
+ * the EVO_*.txt streams (hmec_trim_evo); T == 0 leaves the fresh state. */
+int hmec_restart(hmec_ctx *ctx, const char *dir);
+/* :268-282: when dir/EVO_data.txt has at least timestep lines, cut the EVO
+ * streams back to timestep - 1 lines (remove_double_output, :162-174, on the
+ * reference's file list: it names the flux stream "EVO_VxD.txt" while the
+ * writer names it "EVO_vxD.txt", so that stream is left as it is and an
+ * EVO_VxD.txt of empty lines is written, as the reference does). */
+int hmec_trim_evo(const char *dir, int timestep);
+/* the six per-step series, 6 x HMEC_H5_SERIES_LEN doubles (entry t = step t+1) */
+int hmec_get_series(hmec_ctx *ctx, double *series);
+
 /* Scalar state (dt, timesum, ...) in/out; timestep is the next step to run. */
 int hmec_get_state(hmec_ctx *ctx, hmec_step_info *info);
 int hmec_set_state(hmec_ctx *ctx, const hmec_step_info *info);

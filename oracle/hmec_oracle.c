@@ -42,6 +42,8 @@
 #include "../include/hmec.h"
 
 #define SQ(a) ((a) * (a))
+/* constants.hpp:14 num_timesteps: length of the per-step series */
+#define NUM_TIMESTEPS 100000
 static double dmax(double a, double b) { return a > b ? a : b; }
 static double dmin(double a, double b) { return a < b ? a : b; }
 
@@ -73,6 +75,7 @@ typedef struct {
     int pardiso_ready;
     double t_solve, t_assemble, t_scatter, t_update;
     int bisect_capped;
+    double *series;     /* 6 x NUM_TIMESTEPS: timesumcur, dtcur, max/minvxsmod, max/minvysmod */
     int64_t lu_nnz;
     double lu_mflops;
 } orc;
@@ -306,6 +309,7 @@ orc *orc_create(const hmec_params *P, double *const *soa, int64_t M) {
         if (soa && M > 0) memcpy(o->mk[f], soa[f], sizeof(double) * M);
     }
     o->DSYLSQ = calloc((size_t)P->niterglobal + 1, sizeof(double));
+    o->series = calloc((size_t)6 * NUM_TIMESTEPS, sizeof(double));
     o->R = calloc(o->N, sizeof(double));
     o->S = calloc(o->N, sizeof(double));
     o->captrip = (int64_t)o->N * 10;
@@ -371,7 +375,7 @@ void orc_destroy(orc *o) {
     for (int f = 0; f < HMEC_NFIELDS; f++) free(o->F[f]);
     for (int a = 0; a < 31; a++) free(o->acc[a]);
     for (int f = 0; f < HMEC_NMARKER_FIELDS; f++) free(o->mk[f]);
-    free(o->DSYLSQ); free(o->R); free(o->S);
+    free(o->DSYLSQ); free(o->R); free(o->S); free(o->series);
     free(o->ti); free(o->tj); free(o->tv);
     free(o->rowp); free(o->coli); free(o->val); free(o->trip2slot);
     free(o);
@@ -1203,10 +1207,38 @@ static void final_stress(orc *o) {
         }
 }
 
+/* h_mec_rsf_v9.cpp:1661-1690: the step's entry of the per-step series */
+static void series_step(orc *o) {
+    const int64_t t = o->timestep - 1;
+    if (t < 0 || t >= NUM_TIMESTEPS) return;
+    double *sr = o->series;
+    sr[t] = o->timesum;
+    sr[NUM_TIMESTEPS + t] = o->dt;
+    double mx = -1e30, nx = 1e30, my = -1e30, ny = 1e30;
+    for (int i = 0; i < o->Ny; i++)
+        for (int j = 0; j < o->Nx; j++) {
+            if (S_(o, HMEC_F_RHOX, i, j) > 2000 && i != 0) {
+                mx = dmax(mx, S_(o, HMEC_F_VXS, i, j));
+                nx = dmin(nx, S_(o, HMEC_F_VXS, i, j));
+            }
+            if (S_(o, HMEC_F_RHOY, i, j) > 2000 && j != 0) {
+                my = dmax(my, S_(o, HMEC_F_VYS, i, j));
+                ny = dmin(ny, S_(o, HMEC_F_VYS, i, j));
+            }
+        }
+    sr[2 * NUM_TIMESTEPS + t] = mx;
+    sr[3 * NUM_TIMESTEPS + t] = nx;
+    sr[4 * NUM_TIMESTEPS + t] = my;
+    sr[5 * NUM_TIMESTEPS + t] = ny;
+}
+
+double *orc_series(orc *o) { return o->series; }
+
 /* h_mec_rsf_v9.cpp:1658-1726 */
 static void history(orc *o) {
     const int ns = o->Ny1 * o->Nx1;
     o->timesum += o->dt;
+    series_step(o);
     copyf(o, HMEC_F_VX0, HMEC_F_VXS);
     copyf(o, HMEC_F_VY0, HMEC_F_VYS);
     for (int k = 0; k < ns; k++) {

@@ -75,6 +75,8 @@ def lib():
         L.orc_lu_stats.argtypes = [vp, P(C.c_int64), P(C.c_double)]
         L.orc_resolve.argtypes = [vp, P(C.c_double), P(C.c_double)]
         L.orc_resolve.restype = C.c_int
+        L.orc_series.argtypes = [vp]
+        L.orc_series.restype = P(C.c_double)
         L.orc_controller.argtypes = [vp, vp, C.c_int]
         L.orc_controller.restype = C.c_int
         L.orc_dsylsq.argtypes = [vp]
@@ -186,6 +188,12 @@ class Oracle:
     def solution(self):
         n = (self.p.Nx + 1) * (self.p.Ny + 1) * 6
         return np.ctypeslib.as_array(lib().orc_solution(self.h), shape=(n,)).copy()
+
+    def series(self, n):
+        """The six per-step series (timesumcur, dtcur, maxvxsmod, minvxsmod,
+        maxvysmod, minvysmod), first n entries: array (6, n)."""
+        a = np.ctypeslib.as_array(lib().orc_series(self.h), shape=(6 * 100000,))
+        return a.reshape(6, 100000)[:, :n].copy()
 
     def resolve(self, b):
         """A^-1 b with PARDISO's factor of the last solve (None without it)."""

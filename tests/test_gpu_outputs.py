@@ -26,16 +26,31 @@ def eigen_vec(tokens, sep):
     return sep.join(t.rjust(w) for t in tokens)
 
 
-def test_evo_streams(hmec, gpu, tmp_path):
+def test_evo_streams(hmec, orc, gpu, tmp_path):
+    """Every line of the streams equals the context's own fault-row fields in
+    the reference's text format, and the printed values (6 significant
+    digits) equal the oracle's fault rows after the same steps."""
     p = hmec.params(1)
     soa = hmec.generate_markers(p, cfg=1, seed=1)
     ctx = hmec.Context(p, soa, precision=hmec.PREC_FP64)
+    o = orc.Oracle(p, soa, hmec)
     lf = p.line_fault
     expect = {name: [] for name, _, _ in SERIES}
     data = []
     for s in range(3):
         info = ctx.step()
+        o.step()
         ctx.write_evo(tmp_path, first=(s == 0))
+        # the printed values against the oracle's rows: within half a unit
+        # of the 6th significant digit; the Darcy flux agrees with the oracle
+        # only to its cancellation-limited parity (~1e-6 of the row), which
+        # is added to its bound
+        for name, f, sub in SERIES:
+            row_o = o.get(f)[lf] - (o.get(sub)[lf] if sub is not None else 0)
+            got = np.array([float(t) for t in (tmp_path / name).read_text().splitlines()[-1].split()[2:]])
+            assert got.shape == row_o.shape, name
+            extra = 1e-5 * np.abs(row_o).max() if f == "vxD" else 0.0
+            assert np.all(np.abs(got - row_o) <= 5e-6 * np.abs(row_o) + extra + 1e-300), name
         for name, f, sub in SERIES:
             row = ctx.get(f)[lf]
             if sub is not None:
@@ -67,3 +82,83 @@ def test_evo_unwritable_dir(hmec, gpu, tmp_path):
     ctx.step()
     with pytest.raises(hmec.HmecError, match="output file"):
         ctx.write_evo(tmp_path / "missing" / "dir")
+
+
+def test_series_vs_oracle(hmec, orc, gpu):
+    """The per-step series (h_mec_rsf_v9.cpp:1661-1690) on the device against
+    the oracle's over 8 steps: timesum and dt to 1e-12, the masked velocity
+    extrema to 1e-10 of the velocity scale (the vys extrema, ~1e-6 of the vxs
+    ones in this shear flow, are rounding noise on their own); entries of
+    steps not run stay 0."""
+    p = hmec.params(1)
+    soa = hmec.generate_markers(p, cfg=1, seed=1)
+    ctx = hmec.Context(p, soa)
+    o = orc.Oracle(p, soa, hmec)
+    for _ in range(8):
+        ctx.step()
+        o.step()
+    a = ctx.series()
+    b = o.series(8)
+    assert np.all(a[:, 8:] == 0)
+    assert np.all(np.abs(a[:2, :8] - b[:2]) <= 1e-12 * np.abs(b[:2]))
+    vscale = np.abs(b[2:6]).max()
+    for k in range(2, 6):
+        assert np.all(np.abs(a[k, :8] - b[k]) <= 1e-10 * vscale), k
+    assert np.all(b[2] > b[3]) and np.all(b[0] > 0)
+
+
+def test_restart_vs_continuous(hmec, gpu, tmp_path):
+    """Save at step 3 (file.txt + h_mec_3.h5 in the reference's layout), run on
+    to step 5 writing the EVO streams, then start a new context from the same
+    regenerated markers with hmec_restart and run steps 4-6: every field the
+    next steps depend on, the per-step series and the step state equal a
+    continuous 6-step run bit for bit (all fields but SIIB).  Not in the
+    file, as in the reference:
+    the current stresses SXX, SYY, SXY (the restarted step's first Picard
+    iteration reads them only for the SIIB / EIIB diagnostics, :627-631) and
+    ETA (set from ETA00 at the step's start).  The restart trims the EVO
+    streams back to 3 lines, so the restarted run's streams equal the
+    continuous run's; the flux stream EVO_vxD.txt keeps the lines of steps 4
+    and 5 of the first run, because the reference's trim list names it
+    EVO_VxD.txt."""
+    p = hmec.params(1)
+    soa = hmec.generate_markers(p, cfg=1, seed=1)
+    cont = tmp_path / "cont"
+    run = tmp_path / "run"
+    cont.mkdir()
+    run.mkdir()
+    a = hmec.Context(p, soa)
+    for s in range(6):
+        a.step()
+        a.write_evo(cont, first=(s == 0))
+    b = hmec.Context(p, soa)
+    for s in range(5):
+        b.step()
+        b.write_evo(run, first=(s == 0))
+        if s == 2:
+            b.save(run)
+    assert (run / "file.txt").read_text() == "3\n"
+    assert (run / "h_mec_3.h5").exists()
+    c = hmec.Context(p, hmec.generate_markers(p, cfg=1, seed=1))
+    c.restart(run)
+    assert c.state().timestep == 4
+    for name in ("EVO_data.txt", "EVO_Vslip.txt", "EVO_Theta.txt"):
+        assert len((run / name).read_text().splitlines()) == 3, name
+    assert (run / "EVO_VxD.txt").read_text() == "\n" * 3
+    for s in range(3):
+        c.step()
+        c.write_evo(run)
+    # SIIB of a node that is not plastic after the restarted step's first
+    # iteration keeps the value that iteration wrote from the unsaved stresses
+    skip = {"SIIB"}
+    same = [f for f in hmec.FIELDS if f not in skip and np.array_equal(a.get(f), c.get(f))]
+    differ = [f for f in hmec.FIELDS if f not in skip and f not in same]
+    assert not differ, differ
+    assert np.array_equal(a.series(), c.series())
+    sa, sc = a.state(), c.state()
+    for k in ("timestep", "dt", "timesum", "dt00", "dtx", "dty", "dtlapusta", "Vmax", "maxvxy"):
+        assert getattr(sa, k) == getattr(sc, k), k
+    for name in ("EVO_data.txt", "EVO_Vslip.txt", "EVO_viscosity.txt", "EVO_Theta.txt",
+                 "EVO_SigmaY.txt", "x_fault.txt", "rsf_fault.txt"):
+        assert (run / name).read_text() == (cont / name).read_text(), name
+    assert len((run / "EVO_vxD.txt").read_text().splitlines()) == 8
