@@ -88,7 +88,9 @@ static void free_ctx(hmec_ctx *c) {
     sv_sweep_free(c);
     sv_free_device(c);
     DeviceState &d = c->d;
+    if (c->ev_scal) hipEventDestroy(c->ev_scal);
     hipFree(d.series);
+    hipFree(d.dsylsq);
     hipFree(d.fields); hipFree(d.mk); hipFree(d.prop); hipFree(d.cell); hipFree(d.perm);
     hipFree(d.sorted);
     hipFree(d.cell_start); hipFree(d.sort_tmp); hipFree(d.keys_tmp); hipFree(d.coords);
@@ -156,6 +158,8 @@ extern "C" int hmec_create(const hmec_params *p, hmec_ctx **out) {
     if (hipMalloc(&d.counters, sizeof(int) * ncount) != hipSuccess) return fail(HMEC_E_OOM);
     hipMemset(d.counters, 0, sizeof(int) * ncount);
     if (hipMalloc(&d.series, sizeof(double) * 6 * kNumTimesteps) != hipSuccess) return fail(HMEC_E_OOM);
+    if (hipMalloc(&d.dsylsq, sizeof(double) * (c->P.niterglobal + 1)) != hipSuccess) return fail(HMEC_E_OOM);
+    hipMemset(d.dsylsq, 0, sizeof(double) * (c->P.niterglobal + 1));
     hipMemset(d.series, 0, sizeof(double) * 6 * kNumTimesteps);
     for (double **v : {&d.rhs, &d.sol, &d.work, &d.work2, &d.rscale, &d.cscale}) {
         if (hipMalloc(v, sizeof(double) * N) != hipSuccess) return fail(HMEC_E_OOM);
@@ -601,6 +605,15 @@ int hmec_update_dev(hmec_ctx *c, double dt, hmec_iter_scalars *out) {
     HMEC_TRY(k_update(c, dt, out));
     c->Vmax = out->Vmax;
     c->ptscale = c->d.h_scal[SC_PTSCALE_SLOT];
+    return HMEC_OK;
+}
+
+extern "C" int hmec_controller(hmec_ctx *c, const hmec_iter_scalars *sc, int iterstep, int *stop) {
+    if (!c || !sc || !stop || iterstep < 0 || iterstep >= c->P.niterglobal) return HMEC_E_ARG;
+    hipSetDevice(c->device);
+    bool s = false;
+    HMEC_TRY(k_controller_stage(c, *sc, iterstep, &s));
+    *stop = s ? 1 : 0;
     return HMEC_OK;
 }
 

@@ -1,7 +1,7 @@
 // C++ host driver: one timestep of h_mec_rsf_v9.cpp:376-1726 around the HIP
 // stages, with the reference's Picard / timestep controller (:1310-1456)
-// kept on the host (a handful of scalars per iteration), and the serving
-// entry points that advance several contexts concurrently.
+// evaluated on the device (csrc/controller.h), and the serving entry points
+// that advance several contexts concurrently.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -18,56 +18,6 @@ namespace {
 double now_ms() {
     using namespace std::chrono;
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
-}
-
-// Picard / dt controller, h_mec_rsf_v9.cpp:1310-1436.  Returns true when the
-// exit test fires.
-bool controller(hmec_ctx *c, const hmec_iter_scalars &sc) {
-    const hmec_params &P = c->P;
-    const int iterstep = c->iterstep, ynpl = sc.ynpl;
-    const double dx = P.dx, dy = P.dy;
-    bool yn = false;
-    c->DSYLSQ[iterstep] = 0;
-    if (ynpl > 0) c->DSYLSQ[iterstep] = std::sqrt(sc.ddd / ynpl);
-    const double D_iter = c->DSYLSQ[iterstep];
-    double D_iter_quot = 0;
-    if (iterstep != 0) D_iter_quot = D_iter / c->DSYLSQ[iterstep - 1];
-    double dtpl = c->dt;
-    if (ynpl > 0 && iterstep < P.niterglobal && c->ynlast >= P.dtstep &&
-        (c->ynlast > P.ynlastmax || std::log10(D_iter_quot) >= 0 ||
-         std::log10(D_iter_quot) > std::log10(P.errmin / D_iter) / (P.ynlastmax - c->ynlast))) {
-        dtpl = c->dt / P.dtkoef;
-        yn = true;
-    }
-    double maxvxy0 = 0;
-    if (iterstep > 0) maxvxy0 = c->maxvxy;
-    double maxvxs = std::max(sc.vxs_max, std::fabs(sc.vxs_min));
-    double maxvys = std::max(sc.vys_max, std::fabs(sc.vys_min));
-    c->maxvxy = std::sqrt((sc.vxs_max - sc.vxs_min) * (sc.vxs_max - sc.vxs_min) +
-                          (sc.vys_max - sc.vys_min) * (sc.vys_max - sc.vys_min));
-    double stpmaxcur = P.stpmax1;
-    c->dtx = c->dt;
-    if (c->dt > dx * stpmaxcur / maxvxs) { c->dtx = dx / P.dtkoefv * stpmaxcur / maxvxs; yn = true; }
-    c->dty = c->dt;
-    if (c->dt > dy * stpmaxcur / maxvys) { c->dty = dy / P.dtkoefv * stpmaxcur / maxvys; yn = true; }
-    maxvxs = sc.maxvxs_band;
-    maxvys = sc.maxvys_band;
-    stpmaxcur = P.stpmax;
-    if (c->dt > dx * stpmaxcur / maxvxs) { c->dtx = dx / P.dtkoefv * stpmaxcur / maxvxs; yn = true; }
-    if (c->dt > dy * stpmaxcur / maxvys) { c->dty = dy / P.dtkoefv * stpmaxcur / maxvys; yn = true; }
-    double dtslip = sc.dtslip;
-    if (ynpl > 0 && dtslip < c->dt) { yn = true; dtslip = dtslip / P.dtkoefv; }
-    if (yn && c->dt > P.dtmin) {
-        const double dtold = c->dt;
-        c->dt = std::max(std::min(std::min(c->dtx, c->dty), std::min(std::min(dtpl, dtslip), c->dtlapusta)), P.dtmin);
-        if (c->dt < dtold) c->ynlast = 0;
-    } else {
-        yn = false;
-    }
-    double vratio = 0;
-    if (iterstep > 0) vratio = std::log10(c->maxvxy / maxvxy0);
-    return !yn && (ynpl == 0 || (c->DSYLSQ[iterstep] < P.errmin && iterstep > 0 &&
-                                 std::fabs(vratio) < P.vratiomax));
 }
 
 // Numeric factorization, through the server's batcher for server members
@@ -285,28 +235,48 @@ extern "C" int hmec_step(hmec_ctx *c, hmec_step_info *info) {
     const double t1 = now_ms();
     acc.ms_scatter = t1 - t0;
     HMEC_TRY(hmec_step_begin(c));
+    HMEC_TRY(k_controller_begin(c));
+    // Picard iterations (:610-1457).  The controller runs on the device after
+    // the update's reductions, the iteration epilogue (:1440-1448) reads its
+    // decision there, and the next iteration's prologue (VEP pre-update,
+    // scaling, assembly) is enqueued right behind them with dt from the
+    // device and gated on the exit flag: the host reads the decision (one
+    // wait per iteration) while the device already assembles the next system.
+    auto prologue = [&](bool dev) -> int {
+        HMEC_TRY(k_vep_pre(c, c->dt, dev));
+        HMEC_TRY(k_scale(c, c->dt, nullptr, dev));
+        tm_begin(c, c->t_assemble);
+        HMEC_TRY(k_assemble(c, c->dt, dev));
+        tm_end(c, c->t_assemble, 8.0 * (c->nnz + c->N + 26.0 * c->ns) + 4.0 * (c->nnz + c->ntrip));
+        return HMEC_OK;
+    };
+    double a = now_ms();
+    HMEC_TRY(prologue(false));
     int capped = 0;
     for (c->iterstep = 0; c->iterstep < P.niterglobal; c->iterstep++) {
-        const double a = now_ms();
-        HMEC_TRY(hmec_vep_pre(c, c->dt));
-        HMEC_TRY(hmec_assemble(c, c->dt, nullptr));
         sync();
         const double b = now_ms();
         HMEC_TRY(linear_solve(c, acc));
         sync();
         const double cc = now_ms();
+        HMEC_TRY(k_reload(c, 0.));
+        HMEC_TRY(k_update_launch(c, c->dt));
+        HMEC_TRY(k_controller_commit(c, c->iterstep));
+        HMEC_TRY(k_scalars_enqueue(c));
+        if (c->iterstep + 1 < P.niterglobal) HMEC_TRY(prologue(true));
         hmec_iter_scalars sc;
-        HMEC_TRY(hmec_update_dev(c, c->dt, &sc));
+        HMEC_TRY(k_scalars_wait(c, &sc));
+        c->Vmax = sc.Vmax;
+        c->ptscale = c->d.h_scal[SC_PTSCALE_SLOT];
         c->dtlapusta = sc.dtlapusta;
         capped += sc.bisect_capped;
-        const bool stop = controller(c, sc);
-        HMEC_TRY(hmec_iter_commit(c, sc.ynpl, stop ? 1 : 0));
+        const bool stop = k_controller_read(c);   // dt, dtx, dty, maxvxy, ynlast (:1455)
         const double d = now_ms();
         acc.ms_assemble += b - a;
         acc.ms_factor += cc - b;
         acc.ms_update += d - cc;
+        a = d;
         if (stop) break;
-        c->ynlast++;
     }
     // a loop that runs out leaves iterstep == niterglobal, as in the reference
     // (:1434-1456): the EVO_data.txt column and the step's iteration count

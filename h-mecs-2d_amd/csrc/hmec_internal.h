@@ -222,6 +222,7 @@ struct DeviceState {
     double *work = nullptr;          // N-vector scratch (residual, correction)
     double *work2 = nullptr;
     int *counters = nullptr;         // perturbed pivots etc.
+    double *dsylsq = nullptr;        // niterglobal + 1 iteration errors of the device controller
     double *series = nullptr;        // 6 x kNumTimesteps per-step series (timesumcur, dtcur,
                                      // maxvxsmod, minvxsmod, maxvysmod, minvysmod)
 };
@@ -234,6 +235,7 @@ struct hmec_ctx {
     int64_t M = 0;
     int precision = HMEC_PREC_FP64;
     hipStream_t stream = nullptr;
+    hipEvent_t ev_scal = nullptr;    // after the device scalars' copy to the host
     int device = 0;
     DeviceState d;
     // host copies of the pattern
@@ -296,9 +298,18 @@ struct hmec_ctx {
 int k_marker_prep(hmec_ctx *c);          // derived props + anchor cells + sort
 int k_scatter(hmec_ctx *c);
 int k_step_begin_fields(hmec_ctx *c);
-int k_vep_pre(hmec_ctx *c, double dt);
-int k_scale(hmec_ctx *c, double dt, double *ptscale);
-int k_assemble(hmec_ctx *c, double dt);   // ptscale from the device scalars
+// dev: dt from the device controller, launches gated on its exit flag
+int k_vep_pre(hmec_ctx *c, double dt, bool dev = false);
+int k_scale(hmec_ctx *c, double dt, double *ptscale, bool dev = false);
+int k_assemble(hmec_ctx *c, double dt, bool dev = false);   // ptscale from the device scalars
+// device-side Picard / dt controller (csrc/controller.h)
+int k_controller_begin(hmec_ctx *c);
+int k_controller_commit(hmec_ctx *c, int iterstep);
+bool k_controller_read(hmec_ctx *c);
+int k_controller_stage(hmec_ctx *c, const hmec_iter_scalars &sc, int iterstep, bool *stop);
+int k_update_launch(hmec_ctx *c, double dt);
+int k_scalars_enqueue(hmec_ctx *c);
+int k_scalars_wait(hmec_ctx *c, hmec_iter_scalars *out);
 int k_reload(hmec_ctx *c, double ptscale);
 int hmec_update_dev(hmec_ctx *c, double dt, hmec_iter_scalars *out);
 int k_series_step(hmec_ctx *c, double timesum, double dt);

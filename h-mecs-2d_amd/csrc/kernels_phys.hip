@@ -15,6 +15,7 @@
 
 #define HMEC_HD __host__ __device__
 #include "assemble_emit.h"
+#include "controller.h"
 #include "pow2_scale.h"
 #include "xcd_map.h"
 #include "hmec_internal.h"
@@ -24,6 +25,8 @@ enum {
     SC_PTSCALE = 0,
     SC_YNPL, SC_DDD, SC_DTLAP, SC_VMAX, SC_VXMAX, SC_VXMIN, SC_VYMAX, SC_VYMIN,
     SC_BANDX, SC_BANDY, SC_DTSLIP, SC_CAPPED,
+    // device-side Picard / dt controller: its state and the exit flag
+    SC_CDT, SC_CDTX, SC_CDTY, SC_CMAXVXY, SC_CYNLAST, SC_EXIT,
     SC_COUNT
 };
 static_assert(SC_PTSCALE == SC_PTSCALE_SLOT, "ptscale slot");
@@ -493,31 +496,39 @@ int k_copy_fields(hmec_ctx *c, const int *dst, const int *src, int count) {
 
 int k_copy_field(hmec_ctx *c, int dst, int src) { return k_copy_fields(c, &dst, &src, 1); }
 
-// ------------------------------------------------------------ VEP (a7)
+// ------------------------------------------------------------ VEP
 // copy_bounds(pt), copy_bounds(pf) (h_mec_rsf_v9.cpp:616-617, :154-159):
 // columns first, then rows, as in the reference.
-__global__ void k_copy_bounds_cols(double *a, double *b, int Nx, int Ny1) {
+// (ctl: the device controller's scalars when the launch is speculative: it
+// then does nothing once the controller's exit flag is set)
+__global__ void k_copy_bounds_cols(double *a, double *b, int Nx, int Ny1, const double *ctl) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Ny1) return;
+    if (i >= Ny1 || (ctl && ctl[SC_EXIT] != 0)) return;
     a[i] = a[i + Ny1];
     a[i + (int64_t)Nx * Ny1] = a[i + (int64_t)(Nx - 1) * Ny1];
     b[i] = b[i + Ny1];
     b[i + (int64_t)Nx * Ny1] = b[i + (int64_t)(Nx - 1) * Ny1];
 }
-__global__ void k_copy_bounds_rows(double *a, double *b, int Nx1, int Ny, int Ny1) {
+__global__ void k_copy_bounds_rows(double *a, double *b, int Nx1, int Ny, int Ny1, const double *ctl) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= Nx1) return;
+    if (j >= Nx1 || (ctl && ctl[SC_EXIT] != 0)) return;
     a[(int64_t)j * Ny1] = a[1 + (int64_t)j * Ny1];
     a[Ny + (int64_t)j * Ny1] = a[Ny - 1 + (int64_t)j * Ny1];
     b[(int64_t)j * Ny1] = b[1 + (int64_t)j * Ny1];
     b[Ny + (int64_t)j * Ny1] = b[Ny - 1 + (int64_t)j * Ny1];
 }
 
+// etamincur = dt * shearmod * 1e-4 with dt from the controller (ctl) or given
+__device__ __forceinline__ double vep_floor(const double *ctl, double etamincur, double shearmod) {
+    return ctl ? ctl[SC_CDT] * shearmod * 1e-4 : etamincur;
+}
+
 // basic nodes, h_mec_rsf_v9.cpp:620-636
 __global__ void k_vep_basic(double *f, int64_t ns, int Nx, int Ny, double etamincur,
-                            double dy_faultw) {
+                            double dy_faultw, const double *ctl, double shearmod) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= Nx * Ny) return;
+    if (t >= Nx * Ny || (ctl && ctl[SC_EXIT] != 0)) return;
+    etamincur = vep_floor(ctl, etamincur, shearmod);
     const int i = t % Ny, j = t / Ny, Ny1 = Ny + 1;
     double *ETA = f + (int64_t)HMEC_F_ETA * ns;
     const double *ETA0 = f + (int64_t)HMEC_F_ETA0 * ns;
@@ -543,9 +554,10 @@ __global__ void k_vep_basic(double *f, int64_t ns, int Nx, int Ny, double etamin
 
 // pressure nodes, h_mec_rsf_v9.cpp:639-669
 __global__ void k_vep_p(double *f, int64_t ns, int Nx, int Ny, double etamincur,
-                        double dy_faultw) {
+                        double dy_faultw, const double *ctl, double shearmod) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (Nx - 1) * (Ny - 1)) return;
+    if (t >= (Nx - 1) * (Ny - 1) || (ctl && ctl[SC_EXIT] != 0)) return;
+    etamincur = vep_floor(ctl, etamincur, shearmod);
     const int i = 1 + t % (Ny - 1), j = 1 + t / (Ny - 1), Ny1 = Ny + 1;
     const double *IE = f + (int64_t)HMEC_F_IETAPLB * ns;
     const double *YNY0 = f + (int64_t)HMEC_F_YNY0 * ns;
@@ -570,28 +582,32 @@ __global__ void k_vep_p(double *f, int64_t ns, int Nx, int Ny, double etamincur,
     f[(int64_t)HMEC_F_DILP * ns + s] = 0;
 }
 
-int k_vep_pre(hmec_ctx *c, double dt) {
+// dev: dt from the device controller and the launches gated on its exit flag
+// (the next Picard iteration's prologue enqueued before the host has read
+// the controller's decision)
+int k_vep_pre(hmec_ctx *c, double dt, bool dev) {
     const hmec_params &P = c->P;
     const double etamincur = dt * P.shearmod * 1e-4;
     const double dy_faultw = P.dy / P.faultwidth;
+    const double *ctl = dev ? c->d.scal : nullptr;
     hipLaunchKernelGGL(k_copy_bounds_cols, dim3((c->Ny1 + 255) / 256), dim3(256), 0, c->stream,
-                       F(c, HMEC_F_PT), F(c, HMEC_F_PF), c->Nx, c->Ny1);
+                       F(c, HMEC_F_PT), F(c, HMEC_F_PF), c->Nx, c->Ny1, ctl);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_copy_bounds_rows, dim3((c->Nx1 + 255) / 256), dim3(256), 0, c->stream,
-                       F(c, HMEC_F_PT), F(c, HMEC_F_PF), c->Nx1, c->Ny, c->Ny1);
+                       F(c, HMEC_F_PT), F(c, HMEC_F_PF), c->Nx1, c->Ny, c->Ny1, ctl);
     LAUNCH_CHECK();
     const int nb = c->Nx * c->Ny;
     hipLaunchKernelGGL(k_vep_basic, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
-                       c->ns, c->Nx, c->Ny, etamincur, dy_faultw);
+                       c->ns, c->Nx, c->Ny, etamincur, dy_faultw, ctl, P.shearmod);
     LAUNCH_CHECK();
     const int np = (c->Nx - 1) * (c->Ny - 1);
     hipLaunchKernelGGL(k_vep_p, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
-                       c->ns, c->Nx, c->Ny, etamincur, dy_faultw);
+                       c->ns, c->Nx, c->Ny, etamincur, dy_faultw, ctl, P.shearmod);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
 
-// ---------------------------------------------------------- scale (a8)
+// ---------------------------------------------------------- scale
 // pfscale = ptscale = min(ETADX(1:Ny-1, 0:Nx-1)) * dx * (dt > 2e5 ? 1e17/dt^2 : 1)
 // (h_mec_rsf_v9.cpp:671-677); a min is order independent, so exact.
 // two stages: block minima, then one block over them
@@ -615,8 +631,12 @@ __global__ void __launch_bounds__(256) k_scale_part(const double *ETADX, int Nx,
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 __global__ void __launch_bounds__(SCALE_BLOCKS) k_scale_final(const double *part, double dx,
-                                                              double dt, double *scal) {
+                                                              double dt, double *scal, int dev) {
     __shared__ double red[SCALE_BLOCKS];
+    if (dev) {
+        if (scal[SC_EXIT] != 0) return;
+        dt = scal[SC_CDT];
+    }
     red[threadIdx.x] = part[threadIdx.x];
     __syncthreads();
     for (int s = SCALE_BLOCKS / 2; s > 0; s >>= 1) {
@@ -629,14 +649,15 @@ __global__ void __launch_bounds__(SCALE_BLOCKS) k_scale_final(const double *part
     }
 }
 
-// ptscale into the device scalars; *ptscale (when given) waits for it
-int k_scale(hmec_ctx *c, double dt, double *ptscale) {
+// ptscale into the device scalars; *ptscale (when given) waits for it; dev:
+// dt from the device controller, gated on its exit flag
+int k_scale(hmec_ctx *c, double dt, double *ptscale, bool dev) {
     double *part = c->d.work;
     hipLaunchKernelGGL(k_scale_part, dim3(SCALE_BLOCKS), dim3(256), 0, c->stream,
                        F(c, HMEC_F_ETADX), c->Nx, c->Ny, part);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scale_final, dim3(1), dim3(SCALE_BLOCKS), 0, c->stream,
-                       (const double *)part, c->P.dx, dt, c->d.scal);
+                       (const double *)part, c->P.dx, dt, c->d.scal, dev ? 1 : 0);
     LAUNCH_CHECK();
     if (!ptscale) return HMEC_OK;
     HMEC_HIP(hipMemcpyAsync(c->d.h_scal + SC_PTSCALE, c->d.scal + SC_PTSCALE, sizeof(double),
@@ -646,7 +667,7 @@ int k_scale(hmec_ctx *c, double dt, double *ptscale) {
     return HMEC_OK;
 }
 
-// ------------------------------------------------------- assembly (a9)
+// ------------------------------------------------------- assembly
 static AsmIn asm_in(hmec_ctx *c, const double *base, int64_t ns, double dt, double ptscale) {
     const hmec_params &P = c->P;
     AsmIn in;
@@ -793,8 +814,12 @@ __global__ void __launch_bounds__(ASM_T)
 k_assemble(AsmIn in, const double *__restrict__ scal, const int32_t *__restrict__ rowp,
            const int32_t *__restrict__ node_trip, const int32_t *__restrict__ slot,
            double *__restrict__ val, double *__restrict__ R, double *__restrict__ rscale, int Nx1,
-           int Ny1) {
+           int Ny1, int dev) {
     extern __shared__ double lv[];
+    if (dev) {
+        if (scal[SC_EXIT] != 0) return;
+        in.dt = scal[SC_CDT];
+    }
     __shared__ double lr[6 * ASM_T];
     const int tid = threadIdx.x;
     const int n0 = blockIdx.x * ASM_T;
@@ -824,7 +849,7 @@ k_assemble(AsmIn in, const double *__restrict__ scal, const int32_t *__restrict_
     }
 }
 
-int k_assemble(hmec_ctx *c, double dt) {
+int k_assemble(hmec_ctx *c, double dt, bool dev) {
     AsmIn in = asm_in(c, c->d.fields, c->ns, dt, 0.);
     const int n = c->Nx1 * c->Ny1;
     const int nblk = (n + ASM_T - 1) / ASM_T;
@@ -844,13 +869,13 @@ int k_assemble(hmec_ctx *c, double dt) {
     }
     hipLaunchKernelGGL(k_assemble, dim3(nblk), dim3(ASM_T), lds, c->stream, in, c->d.scal,
                        c->d.rowp, c->d.node_trip, c->d.trip_slot, c->d.val, c->d.rhs, c->d.rscale,
-                       c->Nx1, c->Ny1);
+                       c->Nx1, c->Ny1, dev ? 1 : 0);
     LAUNCH_CHECK();
     c->rscale_ready = true;
     return HMEC_OK;
 }
 
-// ------------------------------------------------------- reload (a11)
+// ------------------------------------------------------- reload
 __global__ void k_reload(const double *S, double *f, int64_t ns, int n, const double *scal) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -873,7 +898,7 @@ int k_reload(hmec_ctx *c, double ptscale) {
     return HMEC_OK;
 }
 
-// ------------------------------------------------------ stress (a12)
+// ------------------------------------------------------ stress
 // h_mec_rsf_v9.cpp:1102-1127 (and :1480-1521 with ESP when final != 0)
 __global__ void k_stress(double *f, int64_t ns, int Nx, int Ny, double dx, double dy, double dt,
                          int final_pass) {
@@ -909,7 +934,7 @@ __global__ void k_stress(double *f, int64_t ns, int Nx, int Ny, double dx, doubl
     }
 }
 
-// --------------------------------------------------------- RSF (a13)
+// --------------------------------------------------------- RSF
 // One workgroup.  Vmax (before the solve's RSF, h_mec_rsf_v9.cpp:1077), then
 // the rate-and-state bisection per fault node (:1165-1308), then ynpl / ddd
 // (summed sequentially in node order like the reference) / dtlapusta.
@@ -1189,7 +1214,7 @@ __global__ void __launch_bounds__(RED_BLOCKS) k_reduce_final(const double *part,
     }
 }
 
-int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out) {
+int k_update_launch(hmec_ctx *c, double dt) {
     const hmec_params &P = c->P;
     const int n = c->Nx1 * c->Ny1;
     hipLaunchKernelGGL(k_stress, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
@@ -1224,9 +1249,22 @@ int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out) {
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCKS), 0, c->stream, (const double *)part,
                        c->d.scal);
     LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// the device scalars back to the host: the copy and an event after it are
+// enqueued now (work enqueued later keeps the device busy), k_scalars_wait
+// waits for the event alone and unpacks
+int k_scalars_enqueue(hmec_ctx *c) {
     HMEC_HIP(hipMemcpyAsync(c->d.h_scal, c->d.scal, sizeof(double) * SC_COUNT,
                             hipMemcpyDeviceToHost, c->stream));
-    HMEC_HIP(hipStreamSynchronize(c->stream));
+    if (!c->ev_scal) HMEC_HIP(hipEventCreateWithFlags(&c->ev_scal, hipEventDisableTiming));
+    HMEC_HIP(hipEventRecord(c->ev_scal, c->stream));
+    return HMEC_OK;
+}
+
+int k_scalars_wait(hmec_ctx *c, hmec_iter_scalars *out) {
+    HMEC_HIP(hipEventSynchronize(c->ev_scal));
     const double *h = c->d.h_scal;
     out->ynpl = (int)h[SC_YNPL];
     out->ddd = h[SC_DDD];
@@ -1243,13 +1281,37 @@ int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out) {
     return HMEC_OK;
 }
 
-// ------------------------------------------------ Picard commit (a14)
+int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out) {
+    HMEC_TRY(k_update_launch(c, dt));
+    HMEC_TRY(k_scalars_enqueue(c));
+    return k_scalars_wait(c, out);
+}
+
+// the device controller's outcome after k_update_read: its state into the
+// context, true when the exit test fired
+bool k_controller_read(hmec_ctx *c) {
+    const double *h = c->d.h_scal;
+    c->dt = h[SC_CDT];
+    c->dtx = h[SC_CDTX];
+    c->dty = h[SC_CDTY];
+    c->maxvxy = h[SC_CMAXVXY];
+    c->ynlast = (int)h[SC_CYNLAST];
+    return h[SC_EXIT] != 0;
+}
+
+// ------------------------------------------------ Picard commit
 // ETA = ETA0 when ynpl == 0 (h_mec_rsf_v9.cpp:1315-1317); when not
 // converged: ETA = max(min(ETA5, ETA0), etamin), ETA50 = ETA, YNY0 = YNY,
 // OM = OM5 (:1440-1448).
-__global__ void k_commit(double *f, int64_t ns, int nb, int ynpl, int converged, double etamin) {
+// (ctl: ynpl and the exit decision from the device scalars instead)
+__global__ void k_commit(double *f, int64_t ns, int nb, int ynpl, int converged, double etamin,
+                         const double *ctl) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nb) return;
+    if (ctl) {
+        ynpl = (int)ctl[SC_YNPL];
+        converged = ctl[SC_EXIT] != 0;
+    }
     double *ETA = f + (int64_t)HMEC_F_ETA * ns;
     const double eta0 = f[(int64_t)HMEC_F_ETA0 * ns + t];
     if (ynpl == 0) ETA[t] = eta0;
@@ -1265,8 +1327,102 @@ __global__ void k_commit(double *f, int64_t ns, int nb, int ynpl, int converged,
 int k_iter_commit(hmec_ctx *c, int ynpl, int converged) {
     const int nb = c->Nx * c->Ny;
     hipLaunchKernelGGL(k_commit, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
-                       c->ns, nb, ynpl, converged, c->P.etamin);
+                       c->ns, nb, ynpl, converged, c->P.etamin, (const double *)nullptr);
     LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// ----------------------------------- device-side Picard / dt controller
+// One thread: this iteration's scalars (the update's reductions) and the
+// controller state in the device scalars, DSYLSQ in its own array; writes the
+// new state and the exit flag (ynlast counts on when the loop goes on, :1455).
+__global__ void k_controller(hmec_params P, int iterstep, double *scal, double *dsylsq) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    hmec_iter_scalars sc;
+    sc.ynpl = (int)scal[SC_YNPL];
+    sc.ddd = scal[SC_DDD];
+    sc.dtlapusta = scal[SC_DTLAP];
+    sc.Vmax = scal[SC_VMAX];
+    sc.vxs_max = scal[SC_VXMAX];
+    sc.vxs_min = scal[SC_VXMIN];
+    sc.vys_max = scal[SC_VYMAX];
+    sc.vys_min = scal[SC_VYMIN];
+    sc.maxvxs_band = scal[SC_BANDX];
+    sc.maxvys_band = scal[SC_BANDY];
+    sc.dtslip = scal[SC_DTSLIP];
+    sc.bisect_capped = 0;
+    CtlState s{scal[SC_CDT], scal[SC_CDTX], scal[SC_CDTY], sc.dtlapusta, scal[SC_CMAXVXY],
+               (int)scal[SC_CYNLAST]};
+    const bool stop = controller_eval(P, iterstep, sc, dsylsq, s);
+    if (!stop) s.ynlast++;
+    scal[SC_CDT] = s.dt;
+    scal[SC_CDTX] = s.dtx;
+    scal[SC_CDTY] = s.dty;
+    scal[SC_CMAXVXY] = s.maxvxy;
+    scal[SC_CYNLAST] = s.ynlast;
+    scal[SC_EXIT] = stop ? 1.0 : 0.0;
+}
+
+// controller, then the iteration epilogue with its decision, both on the device
+int k_controller_commit(hmec_ctx *c, int iterstep) {
+    hipLaunchKernelGGL(k_controller, dim3(1), dim3(64), 0, c->stream, c->P, iterstep, c->d.scal,
+                       c->d.dsylsq);
+    LAUNCH_CHECK();
+    const int nb = c->Nx * c->Ny;
+    hipLaunchKernelGGL(k_commit, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
+                       c->ns, nb, 0, 0, c->P.etamin, (const double *)c->d.scal);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// the controller's starting state of a timestep: dt (after the step's growth
+// rule), ynlast = 0, no exit
+int k_controller_begin(hmec_ctx *c) {
+    double *h = c->d.h_scal;
+    h[SC_CDT] = c->dt;
+    h[SC_CDTX] = c->dtx;
+    h[SC_CDTY] = c->dty;
+    h[SC_CMAXVXY] = c->maxvxy;
+    h[SC_CYNLAST] = 0;
+    h[SC_EXIT] = 0;
+    HMEC_HIP(hipMemcpyAsync(c->d.scal + SC_CDT, h + SC_CDT, sizeof(double) * (SC_EXIT - SC_CDT + 1),
+                            hipMemcpyHostToDevice, c->stream));
+    return HMEC_OK;
+}
+
+// One controller evaluation on caller-given scalars (stage-level callers):
+// the scalars and the context's controller state up, k_controller, the
+// outcome back into the context.  ptscale's slot is left alone.
+int k_controller_stage(hmec_ctx *c, const hmec_iter_scalars &sc, int iterstep, bool *stop) {
+    HMEC_HIP(hipStreamSynchronize(c->stream));   // no copy into the host mirror in flight
+    double *h = c->d.h_scal;
+    h[SC_YNPL] = sc.ynpl;
+    h[SC_DDD] = sc.ddd;
+    h[SC_DTLAP] = sc.dtlapusta;
+    h[SC_VMAX] = sc.Vmax;
+    h[SC_VXMAX] = sc.vxs_max;
+    h[SC_VXMIN] = sc.vxs_min;
+    h[SC_VYMAX] = sc.vys_max;
+    h[SC_VYMIN] = sc.vys_min;
+    h[SC_BANDX] = sc.maxvxs_band;
+    h[SC_BANDY] = sc.maxvys_band;
+    h[SC_DTSLIP] = sc.dtslip;
+    h[SC_CAPPED] = sc.bisect_capped;
+    h[SC_CDT] = c->dt;
+    h[SC_CDTX] = c->dtx;
+    h[SC_CDTY] = c->dty;
+    h[SC_CMAXVXY] = c->maxvxy;
+    h[SC_CYNLAST] = c->ynlast;
+    h[SC_EXIT] = 0;
+    const size_t bytes = sizeof(double) * (SC_COUNT - SC_YNPL);
+    HMEC_HIP(hipMemcpyAsync(c->d.scal + SC_YNPL, h + SC_YNPL, bytes, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_controller, dim3(1), dim3(64), 0, c->stream, c->P, iterstep, c->d.scal,
+                       c->d.dsylsq);
+    LAUNCH_CHECK();
+    HMEC_HIP(hipMemcpyAsync(h + SC_YNPL, c->d.scal + SC_YNPL, bytes, hipMemcpyDeviceToHost, c->stream));
+    HMEC_HIP(hipStreamSynchronize(c->stream));
+    c->dtlapusta = sc.dtlapusta;
+    *stop = k_controller_read(c);
     return HMEC_OK;
 }
 
@@ -1367,7 +1523,7 @@ int k_fault_rows_gather(hmec_ctx *c, const FaultRows &a, double *dev_out) {
     return HMEC_OK;
 }
 
-// ------------------------------------------------------ history (a16)
+// ------------------------------------------------------ history
 // h_mec_rsf_v9.cpp:1672-1726
 __global__ void k_history(double *f, int64_t ns, int nstag, int nb) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1448,7 +1604,7 @@ int k_series_step(hmec_ctx *c, double timesum, double dt) {
     return HMEC_OK;
 }
 
-// ---------------------------------------------------- advection (a15)
+// ---------------------------------------------------- advection
 // Node -> marker bilinear gather of vx, vy (vx / vy grids) and spin (basic
 // grid) with the reference's RK stages (h_mec_rsf_v9.cpp:1563-1656: stage 3
 // samples stage 2's position), stress rotation and periodic recycling in x.

@@ -254,6 +254,7 @@ def _declare(L):
     L.hmec_solve.argtypes = [vp, P(SolveStats)]
     L.hmec_get_solution.argtypes = [vp, P(C.c_double)]
     L.hmec_update.argtypes = [vp, C.c_double, C.c_double, P(IterScalars)]
+    L.hmec_controller.argtypes = [vp, P(IterScalars), C.c_int, P(C.c_int)]
     L.hmec_iter_commit.argtypes = [vp, C.c_int, C.c_int]
     L.hmec_step_end.argtypes = [vp, C.c_double]
     L.hmec_advect.argtypes = [vp, C.c_double]
@@ -490,6 +491,13 @@ class Context:
         sc = IterScalars()
         check(lib().hmec_update(self.h, float(dt), float(ptscale), C.byref(sc)), "update")
         return sc
+
+    def controller(self, sc, iterstep):
+        """The Picard / dt controller on the device for this iteration's
+        scalars; updates the context's dt state; True when the exit test fires."""
+        stop = C.c_int()
+        check(lib().hmec_controller(self.h, C.byref(sc), int(iterstep), C.byref(stop)), "controller")
+        return bool(stop.value)
 
     def iter_commit(self, ynpl, converged):
         check(lib().hmec_iter_commit(self.h, int(ynpl), int(bool(converged))), "iter_commit")

@@ -383,7 +383,7 @@ int hmec_get_solution(hmec_ctx *ctx, double *x);
 
 /* Reload (h_mec_rsf_v9.cpp:1060-1075), stress update (:1102-1127), RSF fault
  * solve (:1152-1308) and the reductions feeding the dt controller
- * (:1310-1415).  Returns the scalars the host-side controller needs. */
+ * (:1310-1415).  Returns the scalars the controller (hmec_controller) reads. */
 typedef struct hmec_iter_scalars {
     int ynpl;
     double ddd, dtlapusta, Vmax;       /* Vmax: VSLIPB max before the RSF   */
@@ -393,6 +393,16 @@ typedef struct hmec_iter_scalars {
     int bisect_capped;
 } hmec_iter_scalars;
 int hmec_update(hmec_ctx *ctx, double dt, double ptscale, hmec_iter_scalars *out);
+/* The Picard / dt controller (h_mec_rsf_v9.cpp:1310-1437), evaluated on the
+ * device as hmec_step does once per Picard iteration: the iteration error
+ * DSYLSQ, the plastic, displacement, slip and Lapusta timestep candidates,
+ * the new dt and the exit test, from this iteration's scalars (hmec_update's
+ * output; sc->dtlapusta becomes the context's dtlapusta) and the context's
+ * dt, dtx, dty, maxvxy, ynlast and the errors of this step's earlier
+ * iterations.  Updates that state (hmec_get_state); ynlast counts on when
+ * the loop goes on (:1455).  *stop = 1 when the exit test fires.
+ * HMEC_E_ARG unless 0 <= iterstep < niterglobal. */
+int hmec_controller(hmec_ctx *ctx, const hmec_iter_scalars *sc, int iterstep, int *stop);
 /* Picard-iteration epilogue when not converged: ETA = clamp(ETA5), ETA50,
  * YNY0 = YNY, OM = OM5 (h_mec_rsf_v9.cpp:1440-1448); ETA = ETA0 if ynpl==0
  * (:1315-1317). */

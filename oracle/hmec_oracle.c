@@ -599,7 +599,7 @@ void orc_step_begin(orc *o) {
     memset(o->DSYLSQ, 0, sizeof(double) * (P->niterglobal + 1));
 }
 
-/* ------------------------------------------------------------- VEP (a7) */
+/* ------------------------------------------------------------- VEP */
 /* h_mec_rsf_v9.cpp:612-669 */
 void orc_vep_pre(orc *o) {
     const hmec_params *P = &o->P;
@@ -657,7 +657,7 @@ double orc_scale(orc *o) {
     return pfscale;
 }
 
-/* -------------------------------------------------------- assembly (a9) */
+/* -------------------------------------------------------- assembly */
 static void T(orc *o, int r, int c, double v) {
     if (o->ntrip >= o->captrip) {
         o->captrip *= 2;
@@ -1094,7 +1094,7 @@ void orc_iter_commit(orc *o, int ynpl, int converged) {
     }
 }
 
-/* ------------------------------------------------------ advection (a15) */
+/* ------------------------------------------------------ advection */
 /* h_mec_rsf_v9.cpp:1561-1656 (enabled by params.advect_markers) */
 void orc_advect(orc *o) {
     const hmec_params *P = &o->P;

@@ -55,7 +55,7 @@ def j2_sqrt(sxx, syy, sxy):
     return math.sqrt(0.5 * float(np.sum(s * s)))
 
 
-# ======================================================== VEP pre-update (a7)
+# ======================================================== VEP pre-update
 def test_vep_plastic_strain_rate_and_viscoplastic_viscosities(hmec, orc):
     """ETA is built as the series combination of ETA0 with a plastic viscosity
     eta_pl(i, j) linear in its inverse q = 1/eta_pl; vep_pre must give back
@@ -149,7 +149,7 @@ def test_vep_viscosity_floors(hmec, orc):
     assert o.get("ETAB")[4, 4] == 1e23
 
 
-# ================================================================ RSF (a13)
+# ================================================================ RSF
 def rsf_state(hmec, orc, p, sxy_row, prB=6e7, a=0.011, b=0.017, L=0.02, yny0_row=None,
               eta=None, dt=1e7, om=0.0, seed=3):
     """A constructed fault-row state: uniform pressures (effective pressure prB),
@@ -310,7 +310,7 @@ def test_rsf_lapusta_timestep(hmec, orc, case):
     assert sc.dtlapusta * V.max() / L == pytest.approx(want, rel=1e-12)
 
 
-# ============================================================ controller (a14)
+# ============================================================ controller
 def ctl(hmec, orc, dt=1e7, ynlast=0, maxvxy=0.0, dtlapusta=1e30, iterstep=0, dsylsq_prev=None):
     p = small(hmec)
     o, _ = fresh(hmec, orc, p)
@@ -518,7 +518,7 @@ def test_iter_commit_branches(hmec, orc):
     assert np.array_equal(o.get("ETA"), eta0)
 
 
-# ===================================================== assembly rows (a9)
+# ===================================================== assembly rows
 def prepared(hmec, orc, p, **fields):
     o, _ = fresh(hmec, orc, p)
     o.scatter()
