@@ -157,7 +157,8 @@ extern "C" int hmec_create(const hmec_params *p, hmec_ctx **out) {
     const size_t ncount = 16 + 2 * (size_t)c->Nx1 + 64;
     if (hipMalloc(&d.counters, sizeof(int) * ncount) != hipSuccess) return fail(HMEC_E_OOM);
     hipMemset(d.counters, 0, sizeof(int) * ncount);
-    if (hipMalloc(&d.series, sizeof(double) * 6 * kNumTimesteps) != hipSuccess) return fail(HMEC_E_OOM);
+    if (hipMalloc(&d.series, sizeof(double) * (6 * kNumTimesteps + 4 * kSeriesBlocks)) != hipSuccess)
+        return fail(HMEC_E_OOM);
     if (hipMalloc(&d.dsylsq, sizeof(double) * (c->P.niterglobal + 1)) != hipSuccess) return fail(HMEC_E_OOM);
     hipMemset(d.dsylsq, 0, sizeof(double) * (c->P.niterglobal + 1));
     hipMemset(d.series, 0, sizeof(double) * 6 * kNumTimesteps);

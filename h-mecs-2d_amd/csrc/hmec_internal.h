@@ -34,6 +34,8 @@ constexpr double kAcceptResidual = 1e-10;
 // Length of the per-step series of the snapshot files (constants.hpp:14
 // num_timesteps, the reference's maximum number of timesteps).
 constexpr int64_t kNumTimesteps = 100000;
+// partial extrema per workgroup of the series reduction, kept after the series
+constexpr int kSeriesBlocks = 512;
 
 void hmec_set_last_error(const std::string &s);
 double hmec_seeded_normal(uint64_t seed, int64_t idx, unsigned stream);
@@ -224,7 +226,8 @@ struct DeviceState {
     int *counters = nullptr;         // perturbed pivots etc.
     double *dsylsq = nullptr;        // niterglobal + 1 iteration errors of the device controller
     double *series = nullptr;        // 6 x kNumTimesteps per-step series (timesumcur, dtcur,
-                                     // maxvxsmod, minvxsmod, maxvysmod, minvysmod)
+                                     // maxvxsmod, minvxsmod, maxvysmod, minvysmod), then
+                                     // 4 x kSeriesBlocks partial extrema
 };
 
 struct hmec_ctx {

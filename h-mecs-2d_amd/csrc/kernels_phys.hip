@@ -6,6 +6,7 @@
 //   and the RK4 marker advection.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -1555,17 +1556,33 @@ int k_history(hmec_ctx *c) {
 // Per-step series of h_mec_rsf_v9.cpp:1661-1690: timesumcur, dtcur and the
 // extrema of vxs over the nodes i < Ny, j < Nx with RHOX > 2000 and i != 0,
 // of vys over the same range with RHOY > 2000 and j != 0 (start values -1e30
-// and 1e30), stored at entry t of the six series (length nt).  One workgroup.
+// and 1e30), stored at entry t of the six series (length nt).  Two stages:
+// kSeriesB workgroups stride over the nodes (consecutive lanes on consecutive
+// i, one column after another) and leave 4 partial extrema each in part;
+// one workgroup reduces them.  Extrema are exact in any order.
 constexpr int kSeriesT = 256;
-__global__ void __launch_bounds__(kSeriesT) k_series(const double *f, int64_t ns, int Nx, int Ny,
-                                                     double *series, int64_t nt, int64_t t,
-                                                     double timesum, double dt) {
+__device__ __forceinline__ void series_tree(double (*red)[kSeriesT], int tid) {
+    __syncthreads();
+    for (int s = kSeriesT / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[0][tid] = fmax(red[0][tid], red[0][tid + s]);
+            red[1][tid] = fmin(red[1][tid], red[1][tid + s]);
+            red[2][tid] = fmax(red[2][tid], red[2][tid + s]);
+            red[3][tid] = fmin(red[3][tid], red[3][tid + s]);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kSeriesT) k_series_part(const double *f, int64_t ns, int Nx, int Ny,
+                                                          double *part) {
     __shared__ double red[4][kSeriesT];
     const int Ny1 = Ny + 1, tid = threadIdx.x;
     const double *vxs = f + (int64_t)HMEC_F_VXS * ns, *vys = f + (int64_t)HMEC_F_VYS * ns;
     const double *rhox = f + (int64_t)HMEC_F_RHOX * ns, *rhoy = f + (int64_t)HMEC_F_RHOY * ns;
     double mx = -1e30, nx = 1e30, my = -1e30, ny = 1e30;
-    for (int q = tid; q < Nx * Ny; q += kSeriesT) {
+    const int n = Nx * Ny;
+    for (int q = blockIdx.x * kSeriesT + tid; q < n; q += gridDim.x * kSeriesT) {
         const int i = q % Ny, j = q / Ny;
         const int64_t s = i + (int64_t)j * Ny1;
         if (rhox[s] > 2000 && i != 0) {
@@ -1578,16 +1595,24 @@ __global__ void __launch_bounds__(kSeriesT) k_series(const double *f, int64_t ns
         }
     }
     red[0][tid] = mx; red[1][tid] = nx; red[2][tid] = my; red[3][tid] = ny;
-    __syncthreads();
-    for (int s = kSeriesT / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-            red[0][tid] = fmax(red[0][tid], red[0][tid + s]);
-            red[1][tid] = fmin(red[1][tid], red[1][tid + s]);
-            red[2][tid] = fmax(red[2][tid], red[2][tid + s]);
-            red[3][tid] = fmin(red[3][tid], red[3][tid + s]);
-        }
-        __syncthreads();
+    series_tree(red, tid);
+    if (tid < 4) part[tid * gridDim.x + blockIdx.x] = red[tid][0];
+}
+
+__global__ void __launch_bounds__(kSeriesT) k_series_final(const double *part, int nparts,
+                                                           double *series, int64_t nt, int64_t t,
+                                                           double timesum, double dt) {
+    __shared__ double red[4][kSeriesT];
+    const int tid = threadIdx.x;
+    double v[4] = {-1e30, 1e30, -1e30, 1e30};
+    for (int b = tid; b < nparts; b += kSeriesT) {
+        v[0] = fmax(v[0], part[b]);
+        v[1] = fmin(v[1], part[nparts + b]);
+        v[2] = fmax(v[2], part[2 * nparts + b]);
+        v[3] = fmin(v[3], part[3 * nparts + b]);
     }
+    for (int q = 0; q < 4; q++) red[q][tid] = v[q];
+    series_tree(red, tid);
     if (tid == 0) {
         series[t] = timesum;
         series[nt + t] = dt;
@@ -1598,8 +1623,14 @@ __global__ void __launch_bounds__(kSeriesT) k_series(const double *f, int64_t ns
 int k_series_step(hmec_ctx *c, double timesum, double dt) {
     const int64_t t = c->timestep - 1;
     if (t < 0 || t >= kNumTimesteps) return HMEC_OK;     // beyond the reference's series length
-    hipLaunchKernelGGL(k_series, dim3(1), dim3(kSeriesT), 0, c->stream, c->d.fields, c->ns, c->Nx,
-                       c->Ny, c->d.series, kNumTimesteps, t, timesum, dt);
+    const int n = c->Nx * c->Ny;
+    const int nb = std::max(1, std::min(kSeriesBlocks, (n + kSeriesT - 1) / kSeriesT));
+    double *part = c->d.series + 6 * kNumTimesteps;
+    hipLaunchKernelGGL(k_series_part, dim3(nb), dim3(kSeriesT), 0, c->stream, c->d.fields, c->ns,
+                       c->Nx, c->Ny, part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_series_final, dim3(1), dim3(kSeriesT), 0, c->stream, (const double *)part,
+                       nb, c->d.series, kNumTimesteps, t, timesum, dt);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
