@@ -44,6 +44,12 @@ static hipEvent_t tm_event(Timer &t) {
 
 void tm_begin(hmec_ctx *c, Timer &t) {
     if (!c->timing) return;
+    if (t.sel_gen != c->timing_gen) {
+        t.sel = c->timing_sel.empty() ||
+                std::find(c->timing_sel.begin(), c->timing_sel.end(), t.name) != c->timing_sel.end();
+        t.sel_gen = c->timing_gen;
+    }
+    if (!t.sel) return;
     t.open_ev = tm_event(t);
     hipEventRecord(t.open_ev, c->stream);
     t.open = true;
@@ -119,6 +125,10 @@ extern "C" int hmec_create(const hmec_params *p, hmec_ctx **out) {
         return HMEC_E_NODEV;
     }
     hmec_ctx *c = new hmec_ctx();
+    c->t_scatter.name = "scatter";
+    c->t_assemble.name = "assemble";
+    c->t_factor.name = "factor";
+    c->t_solve.name = "solve";
     c->P = *p;
     if (hmec_params_finalize(&c->P) != HMEC_OK) { delete c; return HMEC_E_ARG; }
     c->Nx = c->P.Nx; c->Ny = c->P.Ny; c->Nx1 = c->Nx + 1; c->Ny1 = c->Ny + 1;
@@ -650,6 +660,25 @@ extern "C" int hmec_timing_enable(hmec_ctx *c, int on) {
     if (!c) return HMEC_E_ARG;
     c->timing = on != 0;
     c->phase_sync = on >= 2;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_timing_select(hmec_ctx *c, const char *names) {
+    if (!c) return HMEC_E_ARG;
+    c->timing_sel.clear();
+    if (names) {
+        std::string cur;
+        for (const char *p = names;; p++) {
+            if (*p == ',' || *p == 0) {
+                if (!cur.empty()) c->timing_sel.push_back(cur);
+                cur.clear();
+                if (*p == 0) break;
+            } else if (*p != ' ') {
+                cur.push_back(*p);
+            }
+        }
+    }
+    c->timing_gen++;
     return HMEC_OK;
 }
 

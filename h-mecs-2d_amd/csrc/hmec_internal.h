@@ -173,6 +173,9 @@ struct Timer {
     double bytes = 0;     // algorithmic work (bytes or flops) of resolved samples
     int n = 0;
     bool open = false;
+    std::string name;     // the name hmec_kernel_time and hmec_timing_select use
+    int sel_gen = -1;     // selection generation sel was computed for
+    bool sel = true;      // recorded under the current selection
 };
 
 struct hmec_ctx_impl;
@@ -295,6 +298,8 @@ struct hmec_ctx {
     bool phase_sync = false;         // wait for the device at step phase boundaries
     Timer t_scatter, t_assemble, t_factor, t_solve;
     std::map<std::string, Timer> kt;    // per kernel class (schur, fwd, bwd, ...)
+    std::vector<std::string> timing_sel;   // timers recorded (empty: all)
+    int timing_gen = 0;
 };
 
 // ---- kernel launch wrappers (kernels_phys.hip) ----
@@ -374,7 +379,11 @@ void tm_end(hmec_ctx *c, Timer &t, double bytes, int launches = 1);
 void tm_resolve(Timer &t);
 void tm_free(Timer &t);
 inline void kt_begin(hmec_ctx *c, const char *name) {
-    if (c->timing) tm_begin(c, c->kt[name]);
+    if (c->timing) {
+        Timer &t = c->kt[name];
+        if (t.name.empty()) t.name = name;
+        tm_begin(c, t);
+    }
 }
 inline void kt_end(hmec_ctx *c, const char *name, double work, int launches = 1) {
     if (c->timing) tm_end(c, c->kt[name], work, launches);

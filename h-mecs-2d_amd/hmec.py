@@ -254,6 +254,7 @@ def _declare(L):
     L.hmec_solve.argtypes = [vp, P(SolveStats)]
     L.hmec_get_solution.argtypes = [vp, P(C.c_double)]
     L.hmec_update.argtypes = [vp, C.c_double, C.c_double, P(IterScalars)]
+    L.hmec_timing_select.argtypes = [vp, C.c_char_p]
     L.hmec_controller.argtypes = [vp, P(IterScalars), C.c_int, P(C.c_int)]
     L.hmec_iter_commit.argtypes = [vp, C.c_int, C.c_int]
     L.hmec_step_end.argtypes = [vp, C.c_double]
@@ -518,6 +519,13 @@ class Context:
         phase boundary of a step (per-phase wall-clock split in StepInfo)."""
         check(lib().hmec_timing_enable(self.h, 2 if (on and phase_sync) else int(bool(on))),
               "timing_enable")
+
+    def timing_select(self, names=None):
+        """Record only the named timers (a list or comma-separated string;
+        None records all)."""
+        if names is not None and not isinstance(names, str):
+            names = ",".join(names)
+        check(lib().hmec_timing_select(self.h, names.encode() if names else None), "timing_select")
 
     def timing_reset(self):
         check(lib().hmec_timing_reset(self.h), "timing_reset")

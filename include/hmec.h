@@ -536,6 +536,10 @@ int hmec_server_get_field(hmec_server *srv, int id, int field, double *buf, int 
 int hmec_kernel_time(hmec_ctx *ctx, const char *name, double *avg_ms, double *bytes);
 int hmec_timing_enable(hmec_ctx *ctx, int on);
 int hmec_timing_reset(hmec_ctx *ctx);
+/* Record only the named timers (comma-separated, e.g. "sweep_fwd,sweep_bwd";
+ * NULL or "" records all): every event pair sits in the stream and costs
+ * device time, so a throughput window keeps only the ones it reports. */
+int hmec_timing_select(hmec_ctx *ctx, const char *names);
 
 #ifdef __cplusplus
 }
