@@ -4,7 +4,7 @@ and the 601x151 coseismic stress test in both precisions."""
 import numpy as np
 import pytest
 
-from parity_util import check_step, field_errors, group_errors, oracle_floor, write_report
+from parity_util import check_step, field_errors, group_errors, oracle_floor, rel, write_report
 
 pytestmark = pytest.mark.gpu
 
@@ -112,8 +112,28 @@ def test_cfg4_coseismic_both_precisions(hmec, gpu):
     assert max(r["vslip_max"] for r in rows) >= 0.1                # seismic slip rates
     errs = group_errors(c32, c64)
     print("cfg4 fp32 vs fp64:", {k: "%.2e" % v[0] for k, v in errs.items()}, flush=True)
-    for name in ("velocity", "pressure", "stress", "slip_rate", "state", "darcy"):
+    for name in ("velocity", "pressure", "stress", "slip_rate", "state", "viscosity", "darcy"):
         assert errs[name][0] <= 1e-6, (name, errs[name][0])
+    # The fault stresses: SigmaY is continuous in the inputs.  SII_fault is
+    # the RSF bisection's last midpoint (h_mec_rsf_v9.cpp:1260) and stays 0
+    # when the bracket test (:1221) fails.  At a node in equilibrium the
+    # first or second trial change (DSIIB1, DSIIB2) is at rounding level, so
+    # a last-bit difference decides that test and the node's value jumps
+    # between 0 and its stress, while its slip rate does not move (the
+    # unbisected trial stress is then already the root).  Where both paths
+    # bisected they agree; where only one did, the node's slip rate agrees.
+    lf = p.line_fault
+    assert rel(c32.get("SigmaY"), c64.get("SigmaY")) <= 1e-6
+    a32, a64 = c32.get("SII_fault")[lf], c64.get("SII_fault")[lf]
+    v32, v64 = c32.get("VSLIPB")[lf], c64.get("VSLIPB")[lf]
+    both = (a32 != 0) & (a64 != 0)
+    assert both.sum() > 0
+    assert np.all(np.abs(a32[both] - a64[both]) <= 1e-6 * np.abs(a64[both]).max())
+    one = (a32 != 0) != (a64 != 0)
+    print("cfg4 SII_fault: bisected on both paths %d nodes, on one %d; slip rate there %.1e" %
+          (both.sum(), one.sum(), float(np.max(np.abs(v32[one] - v64[one]) / np.abs(v64[one]))
+                                       if one.any() else 0.0)), flush=True)
+    assert np.all(np.abs(v32[one] - v64[one]) <= 1e-6 * np.abs(v64[one]))
     rows.append({"fp32_vs_fp64_groups": {k: float(v[0]) for k, v in errs.items()},
                  "fp32_vs_fp64_fields": field_errors(c32, c64)})
     write_report("cfg4_coseismic", rows)
