@@ -192,6 +192,7 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     const int64_t M = hmec_marker_count(&P);
     int64_t b = M * 8 * (HMEC_NMARKER_FIELDS + MP_NPROP + 31) + M * 4 * (4 + 4 + 2) + 16 * ns;
     b += ns * 8 * HMEC_NFIELDS;
+    b += (6 * kNumTimesteps + 4 * kSeriesBlocks) * 8 + (P.niterglobal + 1) * 8;   // series, DSYLSQ
     const int64_t NS = 4 * ns;
     b += z.nnz * (4 + 8) + (N + 1) * 4 + z.ntrip * 4;                     // coupled CSR
     b += z.snnz * (4 + 8 + 8 + 4 + 3 * 4 + 4 + 4 * 4) + (NS + 1) * 16;  // condensed CSR, maps

@@ -17,6 +17,7 @@
 #define HMEC_HD __host__ __device__
 #include "assemble_emit.h"
 #include "controller.h"
+#include "dpp_sum.h"
 #include "pow2_scale.h"
 #include "xcd_map.h"
 #include "hmec_internal.h"
@@ -304,11 +305,12 @@ __device__ __forceinline__ void scatter_store(const double *acc, double *__restr
 // weight dym wx (wx = 1 - dxm in column j, dxm in column j - 1) where those
 // nodes belong to the run.  Inside a cell, lane l takes the cell's markers
 // l, l + L, ... of its cell-ordered run (the L lanes read consecutive
-// addresses), and a butterfly over the L lanes sums each node's partials —
+// addresses), and DPP row moves over the L lanes sum each node's partials —
 // deterministic, no atomics.  The sums then go through the reference's
 // normalization (h_mec_rsf_v9.cpp:531-583).
 constexpr int SCATTER_T = 256;
 constexpr int SCATTER_R = 2;   // nodes per run
+
 template <int G, int L, int R>
 __global__ void __launch_bounds__(SCATTER_T)
 k_scatter_grid(const double *__restrict__ sorted, int64_t M, const int32_t *__restrict__ start,
@@ -370,11 +372,9 @@ k_scatter_grid(const double *__restrict__ sorted, int64_t M, const int32_t *__re
         }
     }
 #pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1)
+    for (int u = 0; u < R; u++)
 #pragma unroll
-        for (int u = 0; u < R; u++)
-#pragma unroll
-            for (int f = 0; f <= NF; f++) acc[u][f] += __shfl_xor(acc[u][f], o, L);
+        for (int f = 0; f <= NF; f++) acc[u][f] = group_sum<L>(acc[u][f]);
     if (l != 0) return;
 #pragma unroll
     for (int u = 0; u < R; u++) {

@@ -22,7 +22,7 @@
 // A sweep is one launch per level and direction; a workgroup owns a block of
 // rows of one front, loads the front's input vector into LDS once, and a
 // group of L lanes (L chosen per level from the row length) forms each row's
-// dot product, reduced by a fixed butterfly.  Forward, every input is
+// dot product, summed by fixed DPP lane moves.  Forward, every input is
 // contiguous and in front order, so a level's loads depend only on its task:
 // the right-hand side at the fronts' pivots arrives in front order (zf,
 // written by the operator that forms it), and a child stores each
@@ -43,6 +43,7 @@
 
 #include "hmec_internal.h"
 #include "mfma_traits.h"
+#include "dpp_sum.h"
 
 
 #define SW_LAUNCH_CHECK()                                                   \
@@ -388,12 +389,10 @@ __global__ void __launch_bounds__(NT) k_sweep(const SwTask *tasks, const T *sa, 
 #pragma unroll
             for (int u = 0; u < RB; u++) acc[u] += V::dot(v0[u], xs + c);
         }
+        // each row's sum in the lane group's last lane
 #pragma unroll
-        for (int u = 0; u < RB; u++) {
-#pragma unroll
-            for (int o = L / 2; o > 0; o >>= 1) acc[u] += __shfl_xor(acc[u], o, L);
-        }
-        if (lane == 0) {
+        for (int u = 0; u < RB; u++) acc[u] = group_total<L>(acc[u]);
+        if (lane == L - 1) {
 #pragma unroll
             for (int u = 0; u < RB; u++) {
                 const int q = q0 + u * NG;
