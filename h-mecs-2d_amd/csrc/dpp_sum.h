@@ -39,3 +39,32 @@ __device__ __forceinline__ double group_total(double x) {
     if (L >= 64) x += dpp_f64<0x143, 0xC>(x);
     return x;
 }
+
+// The same with any commutative, associative operation op (e.g. a maximum):
+// lanes of rows outside a row_bcast's mask combine with 0, so op(x, 0) must
+// leave x unchanged on the values reduced (sums; maxima of values >= 0).
+template <int L, class Op>
+__device__ __forceinline__ double group_total(double x, Op op) {
+    static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "lane group");
+    if (L >= 16) x = op(x, dpp_f64<0x140>(x));
+    if (L >= 8) x = op(x, dpp_f64<0x141>(x));
+    if (L >= 4) x = op(x, dpp_f64<0x4E>(x));
+    if (L >= 2) x = op(x, dpp_f64<0xB1>(x));
+    if (L >= 32) x = op(x, dpp_f64<0x142, 0xA>(x));
+    if (L >= 64) x = op(x, dpp_f64<0x143, 0xC>(x));
+    return x;
+}
+
+// Reduction over a 256-thread workgroup: each wavefront's total (in its lane
+// 63) through 4 LDS slots, combined in the fixed order ((w0, w1), (w2, w3));
+// the result is returned to every thread.  sh: 4 doubles of LDS; every
+// thread of the workgroup must call it.
+template <class Op>
+__device__ __forceinline__ double block_total256(double x, Op op, double *sh) {
+    x = group_total<64>(x, op);
+    if ((threadIdx.x & 63) == 63) sh[threadIdx.x >> 6] = x;
+    __syncthreads();
+    const double r = op(op(sh[0], sh[1]), op(sh[2], sh[3]));
+    __syncthreads();
+    return r;
+}

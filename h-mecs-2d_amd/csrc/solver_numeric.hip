@@ -24,6 +24,7 @@
 
 #include "hmec_internal.h"
 #include "mfma_traits.h"
+#include "dpp_sum.h"
 #include "pow2_scale.h"
 #include "xcd_map.h"
 
@@ -835,10 +836,8 @@ k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
         }
         const int lane = t & 63, wv = t >> 6;
         for (int i = 0; i < nv; i++) {
-            double d = t < nr ? yv * V[(int64_t)i * ldv + r0 + t] : 0.0;
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
-            if (lane == 0) red[wv][i] = d;
+            const double d = group_total<64>(t < nr ? yv * V[(int64_t)i * ldv + r0 + t] : 0.0);
+            if (lane == 63) red[wv][i] = d;
         }
         __syncthreads();
         if (t < nv) {
@@ -894,7 +893,7 @@ struct NormArgs {
 template <int NV, bool SQ = false>
 __global__ void __launch_bounds__(256) k_norms_part(NormArgs<NV> a, int64_t N, double *part) {
     constexpr int NO = NV + (SQ ? 1 : 0);
-    __shared__ double r[NO][256];
+    __shared__ double r[NO][4];
     double m[NO];
 #pragma unroll
     for (int q = 0; q < NO; q++) m[q] = 0;
@@ -904,20 +903,12 @@ __global__ void __launch_bounds__(256) k_norms_part(NormArgs<NV> a, int64_t N, d
         for (int q = 0; q < NV; q++) m[q] = nan_max(m[q], fabs(a.x[q][t]));
         if (SQ) m[NV] += a.x[2][t] * a.x[2][t];
     }
+    const auto mx = [](double u, double v) { return nan_max(u, v); };
+    const auto add = [](double u, double v) { return u + v; };
 #pragma unroll
-    for (int q = 0; q < NO; q++) r[q][threadIdx.x] = m[q];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-#pragma unroll
-            for (int q = 0; q < NV; q++) r[q][threadIdx.x] = nan_max(r[q][threadIdx.x], r[q][threadIdx.x + s]);
-            if (SQ) r[NV][threadIdx.x] += r[NV][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NO; q++) part[NO * blockIdx.x + q] = r[q][0];
+    for (int q = 0; q < NO; q++) {
+        const double v = (SQ && q == NV) ? block_total256(m[q], add, r[q]) : block_total256(m[q], mx, r[q]);
+        if (threadIdx.x == 0) part[NO * blockIdx.x + q] = v;
     }
 }
 // (cnt: optional, out[NO] = cnt[0] and out[NO + 1] = cnt[1], so the counts of
@@ -926,7 +917,7 @@ template <int NV, bool SQ = false>
 __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb, double *out,
                                                      const int *cnt) {
     constexpr int NO = NV + (SQ ? 1 : 0);
-    __shared__ double r[NO][256];
+    __shared__ double r[NO][4];
     double m[NO];
 #pragma unroll
     for (int q = 0; q < NO; q++) m[q] = 0;
@@ -935,20 +926,14 @@ __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb,
         for (int q = 0; q < NV; q++) m[q] = nan_max(m[q], part[NO * k + q]);
         if (SQ) m[NV] += part[NO * k + NV];
     }
+    const auto mx = [](double u, double v) { return nan_max(u, v); };
+    const auto add = [](double u, double v) { return u + v; };
 #pragma unroll
-    for (int q = 0; q < NO; q++) r[q][threadIdx.x] = m[q];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-#pragma unroll
-            for (int q = 0; q < NV; q++) r[q][threadIdx.x] = nan_max(r[q][threadIdx.x], r[q][threadIdx.x + s]);
-            if (SQ) r[NV][threadIdx.x] += r[NV][threadIdx.x + s];
-        }
-        __syncthreads();
+    for (int q = 0; q < NO; q++) {
+        const double v = (SQ && q == NV) ? block_total256(m[q], add, r[q]) : block_total256(m[q], mx, r[q]);
+        if (threadIdx.x == 0) out[q] = v;
     }
     if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NO; q++) out[q] = r[q][0];
         if (cnt) {
             out[NO] = (double)cnt[0];
             out[NO + 1] = (double)cnt[1];
@@ -1101,20 +1086,15 @@ __global__ void __launch_bounds__(256) k_dots_part(const double *V, int64_t ldv,
     }
 }
 // out[i] = sum_b partial[b * nv + i]: one workgroup per vector, fixed
-// two-level order (strided per thread, then a tree)
+// order (strided per thread, DPP wavefront sums, then the four wavefronts)
 __global__ void __launch_bounds__(256) k_dots_final(const double *partial, int nb, int nv,
                                                     double *out) {
-    __shared__ double red[256];
+    __shared__ double red[4];
     const int i = blockIdx.x;
     double s = 0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[(int64_t)b * nv + i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[i] = red[0];
+    s = block_total256(s, [](double u, double v) { return u + v; }, red);
+    if (threadIdx.x == 0) out[i] = s;
 }
 // coefficients passed by value (no upload before the kernel)
 struct GmCoef {
@@ -1154,10 +1134,8 @@ __global__ void __launch_bounds__(256) k_gs_update_dots(const double *V, int64_t
 #pragma unroll
     for (int i = 0; i <= NVMAX; i++) {
         if (i < nv || i == NVMAX) {
-            double d = acc[i];
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
-            if (lane == 0) red[wv][i] = d;
+            const double d = group_total<64>(acc[i]);
+            if (lane == 63) red[wv][i] = d;
         }
     }
     __syncthreads();
