@@ -79,6 +79,12 @@ struct FrontPlan {
     std::vector<std::vector<int>> levels;   // fronts per level
     // maps
     std::vector<int64_t> nz_dst;        // CSR entry -> arena offset
+    // the CSR entries of each front column (rows ascending): column c of
+    // front f has entries [col_ptr[col_base[f] + c], col_ptr[col_base[f] + c + 1])
+    std::vector<int64_t> col_base;      // nfronts + 1
+    std::vector<int32_t> col_ptr;       // sum of front orders + 1
+    std::vector<int32_t> col_row;       // local row of the entry
+    std::vector<int32_t> col_nz;        // CSR entry
     std::vector<int32_t> ea_map;        // child update node -> parent local node
     std::vector<int32_t> dof_local;     // unused placeholder
     std::vector<int32_t> node_front;    // front that eliminates each node
@@ -133,11 +139,27 @@ struct FrontTask {
     int64_t coff;   // offset of the front's pivot-block copy in the inversion scratch (per level)
 };
 
-struct EaTask {       // extend-add of one child CB into its parent
-    int64_t child_off;   // child front offset
-    int64_t parent_off;  // parent front offset
-    int child_n, child_p, parent_n;
-    int64_t map_off;     // into ea_map (child update node -> parent local node)
+// One workgroup's share of a level's front pull: rows [r0, r0 + nr) x
+// columns [c0, c0 + nc) (at most kPullTile^2 entries; whole columns, so a
+// tile is one contiguous piece of the front), with what the pull needs of the
+// front and of its children's contribution blocks, and the tile's original
+// entries ent[e0, e0 + ne).
+struct PullTile {
+    int64_t off;         // arena offset of the front
+    int64_t coff[2];     // arena offsets of the children (slot order)
+    int64_t e0;          // first original entry of the tile
+    int32_t imoff[2];    // per child: parent local node -> child update node (-1: none)
+    int n;               // front order
+    int cn[2], cp[2];    // children's order and pivots
+    int nch;             // children (0..2)
+    int r0, c0, nr, nc, ne;
+};
+constexpr int kPullTile = 64;
+// An original entry of a pull tile: its place in the tile (row | column << 12),
+// the CSR entry of the condensed system, and that entry's global row and
+// column (equilibration scales)
+struct alignas(16) ColEnt {
+    int32_t rc, nz, grow, gcol;
 };
 
 struct DevPlan {
@@ -145,13 +167,14 @@ struct DevPlan {
     // per level task arrays on the device
     std::vector<FrontTask *> d_tasks;     // fronts of the level
     std::vector<std::vector<FrontTask>> h_tasks;
-    std::vector<EaTask *> d_ea[2];        // extend-add of first / second child
-    std::vector<std::vector<EaTask>> h_ea[2];
+    std::vector<PullTile *> d_pull_t;     // per level: the pull's tiles
+    std::vector<int> n_pull_t;
     std::vector<int> maxp, maxn;          // per level maxima
     std::vector<int64_t> lvl_begin, lvl_end;  // arena range touched by the level
-    int64_t *d_nz_dst = nullptr;
     int32_t *d_ea_map = nullptr;
-    int32_t *d_row_of = nullptr;          // CSR row of each entry
+    int32_t *d_imap = nullptr;            // PullTile::imoff maps
+    ColEnt *d_pull_ent = nullptr;         // the pull tiles' original entries
+    int32_t *d_pull_rc = nullptr;         // their places in the tiles (ColEnt::rc)
     int32_t *d_csc_ptr = nullptr;         // column pointers of the transposed pattern
     int32_t *d_csc_idx = nullptr;         // CSR entry index of each CSC entry
     int32_t *d_csc_row = nullptr;         // row of each CSC entry

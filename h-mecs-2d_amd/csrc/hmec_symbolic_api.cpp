@@ -129,6 +129,7 @@ struct PlanSizes {
     int64_t swrows = 0;   // front rows (+1 per front) of the sweep's row source maps
     int64_t icopy = 0;    // pivot-block copies of the largest level (triangle inversion)
     int64_t iscr = 0;     // inversion products of the largest level and doubling stage
+    int64_t pull = 0;     // bytes of the front pull's tiles and child maps
 };
 std::mutex g_ps_mu;
 std::map<std::pair<int, int>, PlanSizes> g_ps;
@@ -166,6 +167,9 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
             z.sweep += n * ((pv + 3) & ~3) + pv * ((n + 3) & ~3);
             z.dinv += ((pv + 31) / 32) * 2 * 32 * 32;
             z.swrows += n + 1;
+            const int64_t nodes = pl.np[f] + pl.nu[f];
+            const int64_t tc = std::max<int64_t>(1, kPullTile * kPullTile / n);
+            z.pull += nodes * 4 * (int64_t)pl.children[f].size() + (n + tc - 1) / tc * (int64_t)sizeof(PullTile);
         }
         for (int lv = 0; lv < pl.nlevels; lv++) {
             int64_t co = 0;
@@ -195,7 +199,8 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     b += (6 * kNumTimesteps + 4 * kSeriesBlocks) * 8 + (P.niterglobal + 1) * 8;   // series, DSYLSQ
     const int64_t NS = 4 * ns;
     b += z.nnz * (4 + 8) + (N + 1) * 4 + z.ntrip * 4;                     // coupled CSR
-    b += z.snnz * (4 + 8 + 8 + 4 + 3 * 4 + 4 + 4 * 4) + (NS + 1) * 16;  // condensed CSR, maps
+    b += z.snnz * (4 + 8 + 16 + 3 * 4 + 4 + 4 * 4) + (NS + 1) * 16;    // condensed CSR, maps
+    b += z.pull + z.snnz * (4 + 8);                                     // front pull plan, values
     b += 6 * N * 8 + 4 * NS * 8 + 3 * N * 4 + z.varena * 8 + z.nfronts * 256;   // vectors, plans
     b += z.swrows * (4 + 4) + 3 * z.varena * 8 + NS * 4;                // sweep row maps, slots, zf
     b += (z.icopy + z.iscr) * 8;                                        // inversion scratch

@@ -59,6 +59,7 @@ struct FacPtrs {
     int *npert;            // perturbed-pivot counter
     const double *val;     // CSR values
     const double *rs, *cs; // equilibration scales
+    T *aval;               // the equilibrated original entries in pull-tile order
 };
 
 // ------------------------------------------------------------ equilibration
@@ -88,50 +89,7 @@ __global__ void k_col_scale(const int32_t *cptr, const int32_t *cidx, const int3
     cs[c] = pow2_recip(m);
 }
 
-template <typename T>
-__global__ void k_fill_zero(const FacPtrs<T> *mem, int64_t n) {
-    T *a = mem[blockIdx.z].arena;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; t < n; t += stride) a[t] = (T)0;
-}
-
-template <typename T>
-__global__ void k_front_assemble(const FacPtrs<T> *mem, const int32_t *row_of, const int32_t *coli,
-                                 const int64_t *dst, int64_t nnz) {
-    const FacPtrs<T> M = mem[blockIdx.z];
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nnz) return;
-    M.arena[dst[p]] = (T)(M.val[p] * M.rs[row_of[p]] * M.cs[coli[p]]);
-}
-
-// ------------------------------------------------------------ extend-add
-// parent(map(a), map(b)) += child CB(a, b); one launch per child slot so no
-// two workgroups of a launch touch the same parent entry.
-template <typename T>
-__global__ void k_extend_add(const EaTask *tasks, const int32_t *ea_map, const FacPtrs<T> *mem,
-                             int dpn) {
-    T *arena = mem[blockIdx.z].arena;
-    const EaTask tk = tasks[blockIdx.y];
-    const int u = tk.child_n - tk.child_p;
-    const int nt = (u + TM - 1) / TM;
-    const int tile = blockIdx.x;
-    if (tile >= nt * nt) return;
-    const int tr = tile % nt, tc = tile / nt;
-    const int32_t *map = ea_map + tk.map_off;
-    const T *ch = arena + tk.child_off;
-    T *pa = arena + tk.parent_off;
-    const int64_t cn = tk.child_n, pn = tk.parent_n;
-    for (int e = threadIdx.x; e < TM * TM; e += blockDim.x) {
-        const int a = tr * TM + (e % TM), b = tc * TM + (e / TM);
-        if (a >= u || b >= u) continue;
-        const int pa_r = map[a / dpn] * dpn + a % dpn, pa_c = map[b / dpn] * dpn + b % dpn;
-        const int ca = tk.child_p + a, cb = tk.child_p + b;
-        pa[(int64_t)pa_c * pn + pa_r] += ch[(int64_t)cb * cn + ca];
-    }
-}
-
-// ------------------------------------------------------------ panel
+// ------------------------------------------------------------ lane broadcasts
 // wave-uniform broadcast of lane k's value
 __device__ __forceinline__ double bcast(double v, int k) {
     const long long b = __double_as_longlong(v);
@@ -154,6 +112,119 @@ __device__ __forceinline__ float lane_value(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+// ------------------------------------------------------------ front pull
+// Forms every front of a level in one launch, replacing a zero fill of the
+// arena, the scatter of the original entries and one extend-add launch per
+// child slot: entry (r, c) of the front is
+//   ((A(r, c) + CB_0(r', c')) + CB_1(r'', c''))
+// where A is the equilibrated original entry (0 if none) and CB_q the
+// contribution block of the child in slot q at the rows / columns the
+// parent's local nodes map to (a term is left out where the child has no
+// such node).  The additions run in that fixed order, so each entry is the
+// value of zeroing, storing A and adding the two children in slot order.
+// One workgroup per tile of at most 64 x 64 entries (a small front is one
+// tile): the tile's original entries are placed in an LDS tile, the
+// children's values are loaded meanwhile, and every column of the tile is
+// stored as 64 consecutive rows.
+// PB entries of a thread: the children's values are all loaded before any
+// entry is stored (the parent never overlaps its children's blocks)
+template <typename T, int PB>
+__device__ __forceinline__ void pull_batch(const PullTile &t, unsigned e0, unsigned E, unsigned nr,
+                                           const int32_t *__restrict__ im0,
+                                           const int32_t *__restrict__ im1,
+                                           const T *__restrict__ ch0, const T *__restrict__ ch1,
+                                           const T *A, T *__restrict__ out) {
+    constexpr int D = 4;                    // unknowns per node (condensed system)
+    int r[PB], c[PB], a0[PB], a1[PB];
+    T v0[PB], v1[PB];
+#pragma unroll
+    for (int j = 0; j < PB; j++) {
+        const unsigned e = e0 + 256 * j;
+        const unsigned cc = e / nr, rr = e - cc * nr;
+        r[j] = t.r0 + (int)rr;
+        c[j] = t.c0 + (int)cc;
+        a0[j] = a1[j] = -1;
+        if (e < E && t.nch > 0) {
+            const int ar = im0[r[j] / D], ac = im0[c[j] / D];
+            if (ar >= 0 && ac >= 0) a0[j] = (t.cp[0] + ac * D + c[j] % D) * t.cn[0] + t.cp[0] + ar * D + r[j] % D;
+        }
+        if (e < E && t.nch > 1) {
+            const int ar = im1[r[j] / D], ac = im1[c[j] / D];
+            if (ar >= 0 && ac >= 0) a1[j] = (t.cp[1] + ac * D + c[j] % D) * t.cn[1] + t.cp[1] + ar * D + r[j] % D;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PB; j++) {
+        v0[j] = a0[j] >= 0 ? ch0[a0[j]] : (T)0;
+        v1[j] = a1[j] >= 0 ? ch1[a1[j]] : (T)0;
+    }
+#pragma unroll
+    for (int j = 0; j < PB; j++) {
+        const unsigned e = e0 + 256 * j;
+        if (e >= E) break;
+        T acc = A[e];
+        if (a0[j] >= 0) acc += v0[j];
+        if (a1[j] >= 0) acc += v1[j];
+        out[e] = acc;
+    }
+}
+
+// The equilibrated original entries of the factored matrix, in pull-tile
+// order (once per factorization, before the levels)
+template <typename T>
+__global__ void k_pull_values(const ColEnt *ent, int64_t nnz, const FacPtrs<T> *mem) {
+    const FacPtrs<T> M = mem[blockIdx.z];
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const ColEnt x = ent[e];
+    M.aval[e] = (T)(M.val[x.nz] * M.rs[x.grow] * M.cs[x.gcol]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_front_pull(const PullTile *tiles, const int32_t *imap,
+                                                    const int32_t *rc, const FacPtrs<T> *mem) {
+    __shared__ T A[kPullTile * kPullTile];  // the tile, column-major with leading dimension nr
+    const FacPtrs<T> M = mem[blockIdx.z];
+    const PullTile t = tiles[blockIdx.x];
+    const int tid = threadIdx.x;
+    const unsigned E = (unsigned)(t.nr * t.nc), nr = (unsigned)t.nr;
+    // the tile's original entries (at most a few per thread) are loaded
+    // before the zero fill's barrier
+    constexpr int EPT = 4;
+    int xr[EPT];
+    T xv[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; j++) {
+        const int e = tid + 256 * j;
+        xr[j] = e < t.ne ? rc[t.e0 + e] : -1;
+        xv[j] = e < t.ne ? M.aval[t.e0 + e] : (T)0;
+    }
+    for (unsigned e = tid; e < E; e += 256) A[e] = (T)0;
+    __syncthreads();
+    for (int base = 0;; base += 256 * EPT) {
+#pragma unroll
+        for (int j = 0; j < EPT; j++)
+            if (xr[j] >= 0) A[(xr[j] >> 12) * nr + (xr[j] & 4095)] = xv[j];
+        if (base + 256 * EPT >= t.ne) break;
+#pragma unroll
+        for (int j = 0; j < EPT; j++) {
+            const int e = base + 256 * EPT + tid + 256 * j;
+            xr[j] = e < t.ne ? rc[t.e0 + e] : -1;
+            xv[j] = e < t.ne ? M.aval[t.e0 + e] : (T)0;
+        }
+    }
+    __syncthreads();
+    // consecutive threads take consecutive rows of a column (a tile is one
+    // contiguous piece of the front)
+    const int32_t *im0 = imap + t.imoff[0], *im1 = imap + t.imoff[1];
+    const T *ch0 = M.arena + t.coff[0], *ch1 = M.arena + t.coff[1];
+    T *out = M.arena + t.off + (int64_t)t.c0 * t.n;
+    constexpr int PB = 8;
+    for (unsigned e0 = tid; e0 < E; e0 += 256 * PB)
+        pull_batch<T, PB>(t, e0, E, nr, im0, im1, ch0, ch1, A, out);
+}
+
+// ------------------------------------------------------------ panel
 // One wavefront: in-place LU (no row exchanges, static-pivot perturbation
 // below tau on the first kb pivots) of the NB x NB block D in LDS
 // (identity-padded beyond kb, which the steps leave as it is).  Lane r keeps
@@ -1218,6 +1289,7 @@ struct SolverDev {
     double *part = nullptr;        // GEMV partial sums of one level
     double *icopy = nullptr;       // pivot-block copies of one level (inversion)
     double *iscr = nullptr;        // inversion products of one level and stage
+    double *aval = nullptr;        // equilibrated original entries (pull-tile order)
     std::vector<std::vector<InvTile *>> inv_tiles;   // per level, per doubling stage
     std::vector<std::vector<int>> inv_ntiles;
     int64_t dinv_size = 0;
@@ -1260,8 +1332,8 @@ int sv_free_device(hmec_ctx *c) {
     }
     DevPlan &dp = c->dp;
     for (auto p : dp.d_tasks) hipFree(p);
-    for (int k = 0; k < 2; k++)
-        for (auto p : dp.d_ea[k]) hipFree(p);
+    for (auto p : dp.d_pull_t) hipFree(p);
+    dp.d_pull_t.clear();
     dp.d_tasks.clear();
     {
         CondDev &q = c->dc;
@@ -1275,9 +1347,10 @@ int sv_free_device(hmec_ctx *c) {
             hipFree(ptr);
         q = CondDev();
     }
-    hipFree(dp.d_nz_dst);
     hipFree(dp.d_ea_map);
-    hipFree(dp.d_row_of);
+    hipFree(dp.d_imap);
+    hipFree(dp.d_pull_ent);
+    hipFree(dp.d_pull_rc);
     hipFree(dp.d_csc_ptr);
     hipFree(dp.d_csc_idx);
     hipFree(dp.d_csc_row);
@@ -1301,6 +1374,7 @@ int sv_free_device(hmec_ctx *c) {
         hipFree(s->part);
         hipFree(s->icopy);
         hipFree(s->iscr);
+        hipFree(s->aval);
         for (auto &v : s->inv_tiles)
             for (InvTile *q : v) hipFree(q);
         s->inv_tiles.clear();
@@ -1411,11 +1485,18 @@ int sv_setup_device(hmec_ctx *c) {
     HMEC_TRY(upload(&s->all_tasks, all));
     std::vector<int> flist;
     dp.h_tasks.assign(pl.nlevels, {});
-    dp.h_ea[0].assign(pl.nlevels, {});
-    dp.h_ea[1].assign(pl.nlevels, {});
     dp.maxp.assign(pl.nlevels, 0);
     dp.maxn.assign(pl.nlevels, 0);
     s->lvl_fronts_off.assign(pl.nlevels + 1, 0);
+    // the pull of each level: its fronts with their children's blocks and the
+    // parent-node -> child-update-node maps, its tiles of at most kPullTile
+    // x kPullTile entries, and each tile's original entries
+    std::vector<int32_t> imap;
+    std::vector<std::vector<PullTile>> hpt(pl.nlevels);
+    std::vector<ColEnt> pent;
+    std::vector<int32_t> row_of(nnz);
+    for (int64_t r = 0; r < N; r++)
+        for (int p = srp[r]; p < srp[r + 1]; p++) row_of[p] = (int32_t)r;
     for (int lv = 0; lv < pl.nlevels; lv++) {
         s->lvl_fronts_off[lv] = (int64_t)flist.size();
         for (int f : pl.levels[lv]) {
@@ -1424,16 +1505,71 @@ int sv_setup_device(hmec_ctx *c) {
             dp.maxp[lv] = std::max(dp.maxp[lv], all[f].p);
             dp.maxn[lv] = std::max(dp.maxn[lv], all[f].n);
             const auto &kids = pl.children[f];
-            for (int q = 0; q < (int)kids.size() && q < 2; q++) {
-                const int k = kids[q];
-                dp.h_ea[q][lv].push_back(EaTask{all[k].off, all[f].off, all[k].n, all[k].p, all[f].n,
-                                                pl.ea_off[k]});
-            }
             if (kids.size() > 2) {
                 hmec_set_last_error("front with more than two children");
                 return HMEC_E_ARG;
             }
+            const int nodes = pl.np[f] + pl.nu[f];
+            const int n = all[f].n;
+            PullTile pt{};
+            pt.off = all[f].off;
+            pt.n = n;
+            pt.nch = (int)kids.size();
+            for (int q = 0; q < (int)kids.size(); q++) {
+                const int k = kids[q];
+                pt.coff[q] = all[k].off;
+                pt.cn[q] = all[k].n;
+                pt.cp[q] = all[k].p;
+                pt.imoff[q] = (int32_t)imap.size();
+                imap.resize(imap.size() + nodes, -1);
+                for (int a = 0; a < pl.nu[k]; a++) {
+                    const int loc = pl.ea_map[pl.ea_off[k] + a];
+                    if (loc < 0 || loc >= nodes) {
+                        hmec_set_last_error("extend-add map outside the parent front");
+                        return HMEC_E_ARG;
+                    }
+                    imap[pt.imoff[q] + loc] = a;
+                }
+            }
+            if (imap.size() > INT32_MAX) {
+                hmec_set_last_error("pull maps exceed 32-bit offsets");
+                return HMEC_E_ARG;
+            }
+            // tiles of whole columns (kPullTile^2 entries at most: a tile is
+            // one contiguous piece of the front); the column lists (rows
+            // ascending) give each tile's entries in order
+            const int tcols = std::max(1, kPullTile * kPullTile / n);
+            if (n > kPullTile * kPullTile) {
+                hmec_set_last_error("front order above the pull tile");
+                return HMEC_E_ARG;
+            }
+            for (int c0 = 0; c0 < n; c0 += tcols) {
+                PullTile t = pt;
+                t.r0 = 0;
+                t.c0 = c0;
+                t.nr = n;
+                t.nc = std::min(tcols, n - c0);
+                t.e0 = (int64_t)pent.size();
+                for (int c = t.c0; c < t.c0 + t.nc; c++) {
+                    const int64_t cb = pl.col_base[f] + c;
+                    for (int e = pl.col_ptr[cb]; e < pl.col_ptr[cb + 1]; e++) {
+                        const int row = pl.col_row[e];
+                        const int32_t p = pl.col_nz[e];
+                        pent.push_back(ColEnt{row | ((c - t.c0) << 12), p, row_of[p], sci[p]});
+                    }
+                }
+                t.ne = (int)((int64_t)pent.size() - t.e0);
+                hpt[lv].push_back(t);
+            }
         }
+    }
+    if ((int64_t)pent.size() != nnz) {
+        hmec_set_last_error("pull tiles do not cover the pattern");
+        return HMEC_E_ARG;
+    }
+    if (dpn != 4) {
+        hmec_set_last_error("the front pull assumes 4 unknowns per node");
+        return HMEC_E_ARG;
     }
     s->lvl_fronts_off[pl.nlevels] = (int64_t)flist.size();
     s->max_level_fronts = 0;
@@ -1460,12 +1596,21 @@ int sv_setup_device(hmec_ctx *c) {
     HMEC_HIP(hipMalloc(&s->dscr, sizeof(double) * NB * NB * std::max(1, s->max_level_fronts)));
     HMEC_TRY(upload(&s->fronts, flist));
     dp.d_tasks.assign(pl.nlevels, nullptr);
-    dp.d_ea[0].assign(pl.nlevels, nullptr);
-    dp.d_ea[1].assign(pl.nlevels, nullptr);
+    dp.d_pull_t.assign(pl.nlevels, nullptr);
+    dp.n_pull_t.assign(pl.nlevels, 0);
     for (int lv = 0; lv < pl.nlevels; lv++) {
         HMEC_TRY(upload(&dp.d_tasks[lv], dp.h_tasks[lv]));
-        for (int q = 0; q < 2; q++) HMEC_TRY(upload(&dp.d_ea[q][lv], dp.h_ea[q][lv]));
+        HMEC_TRY(upload(&dp.d_pull_t[lv], hpt[lv]));
+        dp.n_pull_t[lv] = (int)hpt[lv].size();
     }
+    HMEC_TRY(upload(&dp.d_imap, imap));
+    HMEC_TRY(upload(&dp.d_pull_ent, pent));
+    {
+        std::vector<int32_t> prc(pent.size());
+        for (size_t e = 0; e < pent.size(); e++) prc[e] = pent[e].rc;
+        HMEC_TRY(upload(&dp.d_pull_rc, prc));
+    }
+    HMEC_HIP(hipMalloc(&s->aval, sizeof(double) * std::max<int64_t>(1, nnz)));
     // children lists
     std::vector<int> cptr(pl.nfronts + 1, 0), clist;
     for (int f = 0; f < pl.nfronts; f++) {
@@ -1477,13 +1622,8 @@ int sv_setup_device(hmec_ctx *c) {
     HMEC_TRY(upload(&s->child_list, clist));
     HMEC_TRY(upload(&s->voff, pl.voff));
     HMEC_TRY(upload(&s->ea_off, pl.ea_off));
-    HMEC_TRY(upload(&dp.d_nz_dst, pl.nz_dst));
     HMEC_TRY(upload(&dp.d_ea_map, pl.ea_map));
-    // row of each CSR entry and the transposed pattern
-    std::vector<int32_t> row_of(nnz);
-    for (int64_t r = 0; r < N; r++)
-        for (int p = srp[r]; p < srp[r + 1]; p++) row_of[p] = (int32_t)r;
-    HMEC_TRY(upload(&dp.d_row_of, row_of));
+    // the transposed pattern
     std::vector<int32_t> cptr2(N + 1, 0), cidx(nnz), crow(nnz);
     for (int64_t p = 0; p < nnz; p++) cptr2[sci[p] + 1]++;
     for (int64_t k = 0; k < N; k++) cptr2[k + 1] += cptr2[k];
@@ -1580,29 +1720,18 @@ static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipS
     const FrontPlan &pl = c->plan;
     DevPlan &dp = c->dp;
     SolverDev *s = sdev(c);
-    const int64_t nnz = c->dc.snnz;     // the condensed system
     const unsigned Bz = (unsigned)B;
-    // zero + assemble
-    hipLaunchKernelGGL(k_fill_zero<T>, dim3(4096, 1, Bz), dim3(256), 0, st, mem, pl.arena);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_front_assemble<T>, dim3((unsigned)((nnz + 255) / 256), 1, Bz), dim3(256), 0,
-                       st, mem, dp.d_row_of, c->dc.scoli, dp.d_nz_dst, nnz);
+    const int64_t snnz = c->dc.snnz;
+    hipLaunchKernelGGL(k_pull_values<T>, dim3((unsigned)((snnz + 255) / 256), 1, Bz), dim3(256), 0, st,
+                       dp.d_pull_ent, snnz, mem);
     LAUNCH_CHECK();
     for (int lv = 0; lv < pl.nlevels; lv++) {
         const int nfr = (int)dp.h_tasks[lv].size();
-        // extend-add children
-        for (int q = 0; q < 2; q++) {
-            const auto &ea = dp.h_ea[q][lv];
-            if (ea.empty()) continue;
-            int maxt = 0;
-            for (const auto &t : ea) {
-                const int u = t.child_n - t.child_p;
-                const int nt = (u + TM - 1) / TM;
-                maxt = std::max(maxt, nt * nt);
-            }
-            if (maxt == 0) continue;
-            hipLaunchKernelGGL(k_extend_add<T>, dim3(maxt, (unsigned)ea.size(), Bz), dim3(256), 0, st,
-                               dp.d_ea[q][lv], dp.d_ea_map, mem, pl.dpn);
+        // the level's fronts: original entries plus the children's blocks
+        const int npt = dp.n_pull_t[lv];
+        if (npt > 0) {
+            hipLaunchKernelGGL(k_front_pull<T>, dim3((unsigned)npt, 1, Bz), dim3(256), 0, st,
+                               dp.d_pull_t[lv], dp.d_imap, dp.d_pull_rc, mem);
             LAUNCH_CHECK();
         }
         // pivot panels, in pairs: panel, narrow update of the next panel's
@@ -1748,12 +1877,13 @@ int sv_factorize_batch(hmec_ctx *const *cs, int n, int precision) {
         SolverDev *s = sdev(c);
         if (fp64) {
             FacPtrs<double> f{c->d.arena, s->dinv64, s->dscr, s->icopy, s->iscr, c->d.counters,
-                              c->dc.sval, c->dc.rs, c->dc.cs};
+                              c->dc.sval, c->dc.rs, c->dc.cs, s->aval};
             std::memcpy(host.data() + psz * k, &f, psz);
         } else {
             FacPtrs<float> f{c->d.arena32, s->dinv32, reinterpret_cast<float *>(s->dscr),
                              reinterpret_cast<float *>(s->icopy), reinterpret_cast<float *>(s->iscr),
-                             c->d.counters, c->dc.sval, c->dc.rs, c->dc.cs};
+                             c->d.counters, c->dc.sval, c->dc.rs, c->dc.cs,
+                             reinterpret_cast<float *>(s->aval)};
             std::memcpy(host.data() + psz * k, &f, psz);
         }
     }

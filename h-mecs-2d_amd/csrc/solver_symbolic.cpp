@@ -196,11 +196,21 @@ int hmec_plan_build(int Nx1, int Ny1, const std::vector<int32_t> &rowp,
             row_of[p] = (int32_t)r;
         }
     pl.nz_dst.assign(nnz, -1);
+    // the same entries per front column, rows ascending (the front pull reads
+    // a column's original entries with the column)
+    pl.col_base.assign(nf + 1, 0);
+    for (int f = 0; f < nf; f++) pl.col_base[f + 1] = pl.col_base[f] + (int64_t)dpn * (pl.np[f] + pl.nu[f]);
+    pl.col_ptr.assign(pl.col_base[nf] + 1, 0);
+    pl.col_row.assign(nnz, -1);
+    pl.col_nz.assign(nnz, -1);
     std::vector<int> loc(nn, -1);
+    std::vector<std::pair<int64_t, int64_t>> ent;   // (column * n + row, CSR entry) of one front
+    int64_t ne = 0;
     for (int f = 0; f < nf; f++) {
         const auto &nl = pl.nodes[f];
         for (int l = 0; l < (int)nl.size(); l++) loc[nl[l]] = l;
         const int64_t n = (int64_t)dpn * (int64_t)nl.size();
+        ent.clear();
         for (int64_t q = bucket_cnt[f]; q < bucket_cnt[f + 1]; q++) {
             const int64_t p = bucket[q];
             const int r = row_of[p], cc = coli[p];
@@ -211,8 +221,23 @@ int hmec_plan_build(int Nx1, int Ny1, const std::vector<int32_t> &rowp,
             }
             const int64_t rr = (int64_t)dpn * lr + kVarPos[r % dpn], ccol = (int64_t)dpn * lc + kVarPos[cc % dpn];
             pl.nz_dst[p] = pl.off[f] + ccol * n + rr;
+            ent.emplace_back(ccol * n + rr, p);
+        }
+        std::sort(ent.begin(), ent.end());
+        size_t k = 0;
+        for (int64_t c = 0; c < n; c++) {
+            pl.col_ptr[pl.col_base[f] + c] = (int32_t)ne;
+            for (; k < ent.size() && ent[k].first / n == c; k++, ne++) {
+                pl.col_row[ne] = (int32_t)(ent[k].first % n);
+                pl.col_nz[ne] = (int32_t)ent[k].second;
+            }
         }
         for (int l = 0; l < (int)nl.size(); l++) loc[nl[l]] = -1;
+    }
+    pl.col_ptr[pl.col_base[nf]] = (int32_t)ne;
+    if (ne != nnz) {
+        hmec_set_last_error("front column lists do not cover the pattern");
+        return HMEC_E_ARG;
     }
     // extend-add maps: child update node -> parent local node
     pl.ea_off.assign(nf, -1);
