@@ -226,22 +226,22 @@ __global__ void __launch_bounds__(256) k_front_pull(const PullTile *tiles, const
 
 // ------------------------------------------------------------ panel
 // One wavefront: in-place LU (no row exchanges, static-pivot perturbation
-// below tau on the first kb pivots) of the NB x NB block D in LDS
+// below tau on the first kb pivots) of the KB x KB block D in LDS
 // (identity-padded beyond kb, which the steps leave as it is).  Lane r keeps
 // row r in registers; step k broadcasts the pivot row from lane k, and every
 // lane below it forms its multiplier and updates its row.  Rows at or above
 // the pivot take a zero multiplier instead of a select, so each updated entry
 // is one fused multiply-add (the rows above keep their values: the pivot
 // row's entries are finite once the pivot is).
-template <typename T>
-__device__ __forceinline__ void factor_diag_wave(T (*D)[NB + 1], int kb, double tau, int *npert,
+template <typename T, int KB = NB>
+__device__ __forceinline__ void factor_diag_wave(T (*D)[KB + 1], int kb, double tau, int *npert,
                                                  bool count) {
-    const int lane = threadIdx.x & 63, r = lane & (NB - 1);
-    T a[NB];
+    const int lane = threadIdx.x & 63, r = lane & (KB - 1);
+    T a[KB];
 #pragma unroll
-    for (int c = 0; c < NB; c++) a[c] = D[r][c];
+    for (int c = 0; c < KB; c++) a[c] = D[r][c];
 #pragma unroll
-    for (int k = 0; k < NB; k++) {
+    for (int k = 0; k < KB; k++) {
         if (r == k && k < kb && !(fabs((double)a[k]) >= tau)) {
             // a non-finite pivot cannot be repaired: counted apart (npert[1])
             if (count && lane == k) atomicAdd(isfinite((double)a[k]) ? npert : npert + 1, 1);
@@ -250,44 +250,145 @@ __device__ __forceinline__ void factor_diag_wave(T (*D)[NB + 1], int kb, double 
         const T d = lane_value(a[k], k);
         const T l = (r > k) ? a[k] / d : (T)0;
 #pragma unroll
-        for (int c = k + 1; c < NB; c++) a[c] = fma(-l, lane_value(a[c], k), a[c]);
+        for (int c = k + 1; c < KB; c++) a[c] = fma(-l, lane_value(a[c], k), a[c]);
         if (r > k) a[k] = l;
     }
-    if (lane < NB) {
+    if (lane < KB) {
 #pragma unroll
-        for (int c = 0; c < NB; c++) D[r][c] = a[c];
+        for (int c = 0; c < KB; c++) D[r][c] = a[c];
     }
 }
 
 // One wavefront: inverse of the upper (U, with diagonal) or unit-lower (L)
-// triangle of the factored block D.  Lane c keeps row c of D (broadcast by
-// row) and column c of the inverse in registers.
-template <typename T, bool UPPER>
-__device__ __forceinline__ void invert_tri_wave(T (*D)[NB + 1], T (*Inv)[NB + 1]) {
-    const int lane = threadIdx.x & 63, c = lane & (NB - 1);
-    T a[NB], x[NB];
+// triangle of the factored KB x KB block D (TRANS: its transpose stored).
+// Lane c keeps row c of D (broadcast by row) and column c of the inverse in
+// registers.
+template <typename T, bool UPPER, int KB = NB, bool TRANS = false>
+__device__ __forceinline__ void invert_tri_wave(T (*D)[KB + 1], T (*Inv)[KB + 1]) {
+    const int lane = threadIdx.x & 63, c = lane & (KB - 1);
+    T a[KB], x[KB];
 #pragma unroll
-    for (int q = 0; q < NB; q++) a[q] = D[c][q];
+    for (int q = 0; q < KB; q++) a[q] = D[c][q];
     if (UPPER) {
 #pragma unroll
-        for (int r = NB - 1; r >= 0; r--) {
+        for (int r = KB - 1; r >= 0; r--) {
             T s = (r == c) ? (T)1 : (T)0;
 #pragma unroll
-            for (int q = r + 1; q < NB; q++) s = fma(-lane_value(a[q], r), x[q], s);
+            for (int q = r + 1; q < KB; q++) s = fma(-lane_value(a[q], r), x[q], s);
             x[r] = (r <= c) ? s / lane_value(a[r], r) : (T)0;
         }
     } else {
 #pragma unroll
-        for (int r = 0; r < NB; r++) {
+        for (int r = 0; r < KB; r++) {
             T s = (r == c) ? (T)1 : (T)0;
 #pragma unroll
             for (int q = 0; q < r; q++) s = fma(-lane_value(a[q], r), x[q], s);
             x[r] = (r >= c) ? s : (T)0;
         }
     }
-    if (lane < NB) {
+    if (lane < KB) {
 #pragma unroll
-        for (int r = 0; r < NB; r++) Inv[r][c] = x[r];
+        for (int r = 0; r < KB; r++) {
+            if (TRANS) Inv[c][r] = x[r];   // Inv holds the transpose
+            else Inv[r][c] = x[r];
+        }
+    }
+}
+
+// A front with at most KB pivots (one panel) and at most kSoloMaxN rows,
+// factored completely by one wavefront, for the levels of small fronts (in
+// place of the panel, trailing update, Schur-complement and inversion
+// launches): the KB x KB pivot block is factored and both its triangles
+// inverted as in the panel kernel, and the inverses replace it (the layout
+// the solves read); then
+//   U12 := L11^-1 A12 (lane per column),   L21 := A21 U11^-1 (lane per row),
+//   CB  := A22 - L21 U12 (lane per row; U12 staged in LDS by column chunks
+//                         in the space of the factored block).
+constexpr int kSoloMaxN = 128;
+constexpr int kSoloMaxP = 16;   // wider pivot blocks factor faster in the panel sequence
+template <typename T, int KB>
+__global__ void __launch_bounds__(64) k_front_solo(const FrontTask *tasks, const FacPtrs<T> *mem,
+                                                   double tau) {
+    constexpr int YC = 2 * KB * (KB + 1) / KB >= 64 ? 64 : 32;   // U12 columns per chunk
+    T *const arena = mem[blockIdx.z].arena;
+    int *const npert = mem[blockIdx.z].npert;
+    __shared__ T DL[2][KB][KB + 1];     // factored block, then L11^-1; later U12 chunks
+    __shared__ T IuT[KB][KB + 1];       // U11^-1, transposed
+    T (*const D)[KB + 1] = DL[0];
+    T (*const Il)[KB + 1] = DL[1];
+    T (*const Yt)[KB] = reinterpret_cast<T (*)[KB]>(&DL[0][0][0]);   // Yt[col][q] = U12[q][col]
+    const FrontTask tk = tasks[blockIdx.y];
+    const int kb = tk.p;
+    const int n = tk.n;
+    if (kb <= 0 || kb > KB || n > kSoloMaxN) return;
+    T *const F = arena + tk.off;
+    const int lane = threadIdx.x;
+    for (int e = lane; e < KB * KB; e += 64) {
+        const int r = e % KB, c = e / KB;
+        D[r][c] = (r < kb && c < kb) ? F[(int64_t)c * n + r] : (r == c ? (T)1 : (T)0);
+    }
+    __syncthreads();
+    factor_diag_wave<T, KB>(D, kb, tau, npert, true);
+    __syncthreads();
+    invert_tri_wave<T, true, KB, true>(D, IuT);
+    invert_tri_wave<T, false, KB>(D, Il);
+    __syncthreads();
+    for (int e = lane; e < kb * kb; e += 64) {
+        const int r = e % kb, c = e / kb;
+        F[(int64_t)c * n + r] = r > c ? Il[r][c] : IuT[c][r];
+    }
+    T a[KB];
+    // U12 := L11^-1 A12
+    for (int col = kb + lane; col < n; col += 64) {
+        T *Fc = F + (int64_t)col * n;
+#pragma unroll
+        for (int q = 0; q < KB; q++) a[q] = q < kb ? Fc[q] : (T)0;
+        for (int r = 0; r < kb; r++) {
+            T s = (T)0;
+#pragma unroll
+            for (int q = 0; q < KB; q++) s = fma(Il[r][q], a[q], s);
+            Fc[r] = s;
+        }
+    }
+    // L21 := A21 U11^-1 for this lane's rows (at most two: n <= 128)
+    T x0[KB], x1[KB];
+    const int r0 = kb + lane, r1 = kb + 64 + lane;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int r = h ? r1 : r0;
+        T *x = h ? x1 : x0;
+#pragma unroll
+        for (int q = 0; q < KB; q++) a[q] = (r < n && q < kb) ? F[(int64_t)q * n + r] : (T)0;
+#pragma unroll
+        for (int c = 0; c < KB; c++) {
+            T s = (T)0;
+#pragma unroll
+            for (int q = 0; q < KB; q++) s = fma(a[q], IuT[c][q], s);
+            x[c] = s;
+            if (r < n && c < kb) F[(int64_t)c * n + r] = s;
+        }
+    }
+    // CB := A22 - L21 U12, U12 read back by chunks of YC columns into LDS
+    for (int c0 = kb; c0 < n; c0 += YC) {
+        const int nc = min(YC, n - c0);
+        __syncthreads();
+        for (int e = lane; e < nc * KB; e += 64) {
+            const int q = e % KB, cc = e / KB;
+            Yt[cc][q] = q < kb ? F[(int64_t)(c0 + cc) * n + q] : (T)0;
+        }
+        __syncthreads();
+        for (int cc = 0; cc < nc; cc++) {
+            T *Fc = F + (int64_t)(c0 + cc) * n;
+            T s0 = r0 < n ? Fc[r0] : (T)0, s1 = r1 < n ? Fc[r1] : (T)0;
+#pragma unroll
+            for (int q = 0; q < KB; q++) {
+                const T y = Yt[cc][q];
+                s0 = fma(-x0[q], y, s0);
+                s1 = fma(-x1[q], y, s1);
+            }
+            if (r0 < n) Fc[r0] = s0;
+            if (r1 < n) Fc[r1] = s1;
+        }
     }
 }
 
@@ -1734,9 +1835,17 @@ static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipS
                                dp.d_pull_t[lv], dp.d_imap, dp.d_pull_rc, mem);
             LAUNCH_CHECK();
         }
+        // a level of fronts with at most one panel of pivots: one wavefront
+        // factors each front completely
+        const int maxp = dp.maxp[lv];
+        if (maxp <= kSoloMaxP && dp.maxn[lv] <= kSoloMaxN) {
+            hipLaunchKernelGGL((k_front_solo<T, kSoloMaxP>), dim3(1, nfr, Bz), dim3(64), 0, st,
+                               dp.d_tasks[lv], mem, tau);
+            LAUNCH_CHECK();
+            continue;
+        }
         // pivot panels, in pairs: panel, narrow update of the next panel's
         // strips, next panel, one update of the rest for both
-        const int maxp = dp.maxp[lv];
         auto panel = [&](int k0) -> int {
             int gx = 1, rest = 0;
             for (const auto &t : dp.h_tasks[lv]) {
