@@ -612,8 +612,7 @@ extern "C" int hmec_update(hmec_ctx *c, double dt, double ptscale, hmec_iter_sca
 
 // the update with the ptscale the assembly left in the device scalars
 int hmec_update_dev(hmec_ctx *c, double dt, hmec_iter_scalars *out) {
-    HMEC_TRY(k_reload(c, 0.));
-    HMEC_TRY(k_update(c, dt, out));
+    HMEC_TRY(k_update(c, dt, out));   // reload included
     c->Vmax = out->Vmax;
     c->ptscale = c->d.h_scal[SC_PTSCALE_SLOT];
     return HMEC_OK;

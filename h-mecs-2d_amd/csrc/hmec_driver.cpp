@@ -135,6 +135,88 @@ int guess_keep(hmec_ctx *c) {
 // PardisoLU (h_mec_rsf_v9.cpp:1050-1052).  FP32: the equilibrated FP32 factor
 // is reused while FP64 iterative refinement converges; a stalled refinement
 // refactors in FP32, and a second stall falls back to an FP64 factor.
+//
+// The FP32 solve is split so that hmec_step can enqueue its first GMRES
+// step with the decision on the device (sv_gmres_first) and the rest of the
+// Picard iteration behind it: fp32_begin chooses or refreshes the factor and
+// the initial guess, fp32_finish handles a stalled solve and keeps the
+// solution.
+struct Fp32Solve {
+    double dist = 0;
+    int k = -1;
+    bool fresh = false;
+    int max_it = 0;
+};
+
+int fp32_refactor(hmec_ctx *c, hmec_step_info &acc, Fp32Solve &fs, int slot) {
+    sv_slot_target(c, slot);
+    tm_begin(c, c->t_factor);
+    HMEC_TRY(factor(c, HMEC_PREC_FP32));
+    tm_end(c, c->t_factor, c->plan.flops);
+    acc.factorizations++;
+    c->factor_dt = c->dt;
+    fs.fresh = true;
+    fs.max_it = c->ir_max_fresh;
+    return HMEC_OK;
+}
+
+// FP32: GMRES preconditioned by a cached FP32 factor.  The cache keeps a few
+// factors keyed by the dt they were computed at (the controller alternates
+// between an increased dt on a step's first Picard iteration and the reduced
+// one after it), so each solve starts from the factor of the nearest dt.
+int fp32_begin(hmec_ctx *c, hmec_step_info &acc, Fp32Solve &fs) {
+    fs = Fp32Solve();
+    fs.k = sv_slot_nearest(c, c->dt, &fs.dist);
+    fs.max_it = c->ir_max_stale;
+    // a dt more than 10 % from every kept factor, with a free slot (or one
+    // whose factor no solve has used lately): factor for it right away
+    // rather than spend the GMRES budget on the factor of another dt
+    int idle = -1;
+    if (fs.k >= 0 && fs.dist > 0.1) {
+        const int v = sv_slot_victim(c, fs.k);
+        if (v != fs.k && sv_slot_idle(c, v, 16)) idle = v;
+    }
+    if (fs.k < 0) HMEC_TRY(fp32_refactor(c, acc, fs, sv_slot_victim(c, -1)));
+    else if (idle >= 0) HMEC_TRY(fp32_refactor(c, acc, fs, idle));
+    else HMEC_TRY(sv_slot_activate(c, fs.k));
+    return guess_start(c);
+}
+
+// rc of the first GMRES attempt: if GMRES needs more than its budget the
+// system has moved: refactor in FP32, into the same slot when dt is close to
+// the slot's, else into an empty or the least recently used slot; a fresh
+// factor that still fails falls back to an FP64 factor.
+int fp32_finish(hmec_ctx *c, hmec_step_info &acc, Fp32Solve &fs, int rc, hmec_solve_stats &st) {
+    acc.ir_iterations += st.ir_iterations;
+    // a stalled refinement, or an FP32 factor with an unrepairable pivot:
+    // a fresh FP32 factor, then an FP64 one
+    if ((rc == HMEC_E_NOCONV || rc == HMEC_E_PIVOT) && !fs.fresh) {
+        const int slot = fs.dist <= 0.05 ? fs.k : sv_slot_victim(c, fs.k);
+        HMEC_TRY(fp32_refactor(c, acc, fs, slot));
+        tm_begin(c, c->t_solve);
+        rc = sv_gmres(c, c->ir_max_fresh, c->ir_tol, &st);
+        tm_end(c, c->t_solve, 0);
+        acc.ir_iterations += st.ir_iterations;
+    }
+    if (rc == HMEC_E_NOCONV || rc == HMEC_E_PIVOT) {
+        acc.fallbacks++;
+        HMEC_TRY(factor(c, HMEC_PREC_FP64));
+        acc.factorizations++;
+        rc = sv_solve(c, c->fp64_max_ir, c->fp64_tol, &st);
+        if (rc == HMEC_E_NOCONV && st.rel_residual <= kAcceptResidual) rc = HMEC_OK;
+    }
+    if (rc != HMEC_OK) return rc;
+    HMEC_TRY(guess_keep(c));
+    c->last_solve_dt = c->dt;
+    return HMEC_OK;
+}
+
+void solve_stats(hmec_step_info &acc, const hmec_solve_stats &st) {
+    acc.solves++;
+    acc.perturbed_pivots += st.perturbed_pivots;
+    acc.max_rel_residual = std::max(acc.max_rel_residual, st.rel_residual);
+}
+
 int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
     hmec_solve_stats st;
     std::memset(&st, 0, sizeof(st));
@@ -148,70 +230,16 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         tm_end(c, c->t_solve, 0);
         if (rc == HMEC_E_NOCONV && st.rel_residual <= kAcceptResidual) rc = HMEC_OK;
         if (rc != HMEC_OK) return rc;
-    } else {
-        // FP32: GMRES preconditioned by a cached FP32 factor.  The cache keeps
-        // a few factors keyed by the dt they were computed at (the
-        // controller alternates between an increased dt on a step's first
-        // Picard iteration and the reduced one after it), so each solve
-        // starts from the factor of the nearest dt.  If GMRES needs more than
-        // its budget the system has moved: refactor in FP32, into the same
-        // slot when dt is close to the slot's, else into an empty or the
-        // least recently used slot; a fresh factor that still fails falls
-        // back to an FP64 factor.
-        double dist = 0;
-        int k = sv_slot_nearest(c, c->dt, &dist);
-        bool fresh = false;
-        auto refactor = [&](int slot) -> int {
-            sv_slot_target(c, slot);
-            tm_begin(c, c->t_factor);
-            HMEC_TRY(factor(c, HMEC_PREC_FP32));
-            tm_end(c, c->t_factor, c->plan.flops);
-            acc.factorizations++;
-            c->factor_dt = c->dt;
-            fresh = true;
-            return HMEC_OK;
-        };
-        // a dt more than 10 % from every kept factor, with a free slot (or
-        // one whose factor no solve has used lately): factor for it right
-        // away rather than spend the GMRES budget on the factor of another dt
-        int idle = -1;
-        if (k >= 0 && dist > 0.1) {
-            const int v = sv_slot_victim(c, k);
-            if (v != k && sv_slot_idle(c, v, 16)) idle = v;
-        }
-        if (k < 0) HMEC_TRY(refactor(sv_slot_victim(c, -1)));
-        else if (idle >= 0) HMEC_TRY(refactor(idle));
-        else HMEC_TRY(sv_slot_activate(c, k));
-        HMEC_TRY(guess_start(c));
-        tm_begin(c, c->t_solve);
-        int rc = sv_gmres(c, fresh ? c->ir_max_fresh : c->ir_max_stale, c->ir_tol, &st);
-        tm_end(c, c->t_solve, 0);
         acc.ir_iterations += st.ir_iterations;
-        // a stalled refinement, or an FP32 factor with an unrepairable pivot:
-        // a fresh FP32 factor, then an FP64 one
-        if ((rc == HMEC_E_NOCONV || rc == HMEC_E_PIVOT) && !fresh) {
-            const int slot = dist <= 0.05 ? k : sv_slot_victim(c, k);
-            HMEC_TRY(refactor(slot));
-            tm_begin(c, c->t_solve);
-            rc = sv_gmres(c, c->ir_max_fresh, c->ir_tol, &st);
-            tm_end(c, c->t_solve, 0);
-            acc.ir_iterations += st.ir_iterations;
-        }
-        if (rc == HMEC_E_NOCONV || rc == HMEC_E_PIVOT) {
-            acc.fallbacks++;
-            HMEC_TRY(factor(c, HMEC_PREC_FP64));
-            acc.factorizations++;
-            rc = sv_solve(c, c->fp64_max_ir, c->fp64_tol, &st);
-            if (rc == HMEC_E_NOCONV && st.rel_residual <= kAcceptResidual) rc = HMEC_OK;
-        }
-        if (rc != HMEC_OK) return rc;
-        HMEC_TRY(guess_keep(c));
-        c->last_solve_dt = c->dt;
+    } else {
+        Fp32Solve fs;
+        HMEC_TRY(fp32_begin(c, acc, fs));
+        tm_begin(c, c->t_solve);
+        const int rc = sv_gmres(c, fs.max_it, c->ir_tol, &st);
+        tm_end(c, c->t_solve, 0);
+        HMEC_TRY(fp32_finish(c, acc, fs, rc, st));
     }
-    acc.solves++;
-    if (c->precision == HMEC_PREC_FP64) acc.ir_iterations += st.ir_iterations;
-    acc.perturbed_pivots += st.perturbed_pivots;
-    acc.max_rel_residual = std::max(acc.max_rel_residual, st.rel_residual);
+    solve_stats(acc, st);
     return HMEC_OK;
 }
 
@@ -253,19 +281,73 @@ extern "C" int hmec_step(hmec_ctx *c, hmec_step_info *info) {
     double a = now_ms();
     HMEC_TRY(prologue(false));
     int capped = 0;
+    // the iteration's tail: reload, update and controller, the epilogue with
+    // its decision, the scalars to the host, and the next iteration's
+    // prologue (gated on the exit flag)
+    auto tail = [&](bool gated) -> int {
+        HMEC_TRY(k_update_launch(c, c->dt, c->iterstep, gated));
+        HMEC_TRY(k_controller_commit(c));
+        HMEC_TRY(k_scalars_enqueue(c));
+        if (c->iterstep + 1 < P.niterglobal) HMEC_TRY(prologue(true));
+        return HMEC_OK;
+    };
     for (c->iterstep = 0; c->iterstep < P.niterglobal; c->iterstep++) {
         sync();
         const double b = now_ms();
-        HMEC_TRY(linear_solve(c, acc));
+        // FP32: the solve's first GMRES step is decided on the device and the
+        // tail is enqueued behind it, gated on that decision; the host waits
+        // once, and only a solve that needs more than one step (or a
+        // factor, or a fallback) continues on the host before the tail runs
+        Fp32Solve fs;
+        bool first = false;
+        if (c->precision == HMEC_PREC_FP32 && !sync_timing) {
+            HMEC_TRY(fp32_begin(c, acc, fs));
+            if (fs.max_it >= 1) {
+                tm_begin(c, c->t_solve);
+                HMEC_TRY(sv_gmres_first(c, c->ir_tol));
+                tm_end(c, c->t_solve, 0);
+                first = true;
+            } else {
+                hmec_solve_stats st;
+                std::memset(&st, 0, sizeof(st));
+                const int rc = sv_gmres(c, fs.max_it, c->ir_tol, &st);
+                HMEC_TRY(fp32_finish(c, acc, fs, rc, st));
+                solve_stats(acc, st);
+            }
+        } else {
+            HMEC_TRY(linear_solve(c, acc));
+        }
         sync();
         const double cc = now_ms();
-        HMEC_TRY(k_reload(c, 0.));
-        HMEC_TRY(k_update_launch(c, c->dt));
-        HMEC_TRY(k_controller_commit(c, c->iterstep));
-        HMEC_TRY(k_scalars_enqueue(c));
-        if (c->iterstep + 1 < P.niterglobal) HMEC_TRY(prologue(true));
+        if (first) HMEC_TRY(guess_keep(c));   // kept again if the host finishes the solve
+        HMEC_TRY(tail(first));
         hmec_iter_scalars sc;
-        HMEC_TRY(k_scalars_wait(c, &sc));
+        if (first) {
+            // the first step's outcome arrives before the (gated) tail has run
+            int state = 0;
+            HMEC_TRY(sv_gmres_first_state(c, &state));
+            hmec_solve_stats st;
+            std::memset(&st, 0, sizeof(st));
+            if (state == kGmLeadOk || state == kGmDone) {
+                HMEC_TRY(k_scalars_wait(c, &sc));
+                HMEC_TRY(sv_gmres_first_result(c, state, &st));
+                acc.ir_iterations += st.ir_iterations;
+                c->last_solve_dt = c->dt;
+            } else {
+                // the update and the prologue hold: finish the solve here,
+                // release the flag and run the tail again
+                tm_begin(c, c->t_solve);
+                const int rc = sv_gmres(c, fs.max_it, c->ir_tol, &st, state);
+                tm_end(c, c->t_solve, 0);
+                HMEC_TRY(fp32_finish(c, acc, fs, rc, st));
+                HMEC_TRY(k_exit_release(c));
+                HMEC_TRY(tail(false));
+                HMEC_TRY(k_scalars_wait(c, &sc));
+            }
+            solve_stats(acc, st);
+        } else {
+            HMEC_TRY(k_scalars_wait(c, &sc));
+        }
         c->Vmax = sc.Vmax;
         c->ptscale = c->d.h_scal[SC_PTSCALE_SLOT];
         c->dtlapusta = sc.dtlapusta;

@@ -335,16 +335,24 @@ int k_scale(hmec_ctx *c, double dt, double *ptscale, bool dev = false);
 int k_assemble(hmec_ctx *c, double dt, bool dev = false);   // ptscale from the device scalars
 // device-side Picard / dt controller (csrc/controller.h)
 int k_controller_begin(hmec_ctx *c);
-int k_controller_commit(hmec_ctx *c, int iterstep);
+int k_controller_commit(hmec_ctx *c);
 bool k_controller_read(hmec_ctx *c);
 int k_controller_stage(hmec_ctx *c, const hmec_iter_scalars &sc, int iterstep, bool *stop);
-int k_update_launch(hmec_ctx *c, double dt);
+int k_update_launch(hmec_ctx *c, double dt, int iterstep, bool gated = false);
+int k_exit_release(hmec_ctx *c);
 int k_scalars_enqueue(hmec_ctx *c);
 int k_scalars_wait(hmec_ctx *c, hmec_iter_scalars *out);
-int k_reload(hmec_ctx *c, double ptscale);
 int hmec_update_dev(hmec_ctx *c, double dt, hmec_iter_scalars *out);
 int k_series_step(hmec_ctx *c, double timesum, double dt);
 constexpr int SC_PTSCALE_SLOT = 0;   // ptscale's index in the device scalars
+// the device controller's exit flag: 0 go on, 1 the Picard loop exits, 2 the
+// linear solve left a decision to the host (the update and the next
+// iteration's prologue enqueued behind it do nothing)
+constexpr int SC_EXIT_SLOT = 18;
+constexpr double SC_EXIT_HOLD = 2.0;
+// outcome of a solve's first GMRES step when it was decided on the device
+// (GmState in solver_numeric.hip)
+constexpr int SC_GMSTATE_SLOT = 19;
 constexpr int HMEC_FP32_SLOTS = 3;   // FP32 factors kept by default (keyed by dt)
 int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out);
 int k_iter_commit(hmec_ctx *c, int ynpl, int converged);
@@ -374,7 +382,13 @@ int factor_check(hmec_ctx *c);
 int sv_factorize_batch(hmec_ctx *const *cs, int n, int precision);
 int sv_solve(hmec_ctx *c, int max_ir, double tol, hmec_solve_stats *st);
 int sv_batch_factorize(hmec_ctx *const *cs, int n, int precision);
-int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *st);
+int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *st, int resume = 0);
+// a solve's first GMRES step with its decision on the device (nothing waits);
+// the state lands in the device scalars' SC_GMSTATE_SLOT
+int sv_gmres_first(hmec_ctx *c, double tol);
+int sv_gmres_first_result(hmec_ctx *c, int state, hmec_solve_stats *st);
+int sv_gmres_first_state(hmec_ctx *c, int *state);
+constexpr int kGmLeadOk = 1, kGmContinue = 3, kGmHold = 4, kGmDone = 5, kGmRestart = 6;
 int sv_lincomb(hmec_ctx *c, double *y, double a, const double *u, double b, const double *v);
 void sv_gmres_free(hmec_ctx *c);
 // ---- serving (hmec_serve.cpp) ----

@@ -29,9 +29,12 @@ enum {
     SC_BANDX, SC_BANDY, SC_DTSLIP, SC_CAPPED,
     // device-side Picard / dt controller: its state and the exit flag
     SC_CDT, SC_CDTX, SC_CDTY, SC_CMAXVXY, SC_CYNLAST, SC_EXIT,
+    // outcome of the linear solve's first GMRES step decided on the device
+    SC_GMSTATE,
     SC_COUNT
 };
 static_assert(SC_PTSCALE == SC_PTSCALE_SLOT, "ptscale slot");
+static_assert(SC_EXIT == SC_EXIT_SLOT && SC_GMSTATE == SC_GMSTATE_SLOT, "solver scalar slots");
 static_assert(SC_COUNT <= 32, "scalar slots");
 
 // coordinates: x(Nx), y(Ny), xvx(Nx1), yvx(Ny1), xvy(Nx1), yvy(Ny1), xp(Nx1), yp(Ny1)
@@ -498,37 +501,41 @@ int k_copy_fields(hmec_ctx *c, const int *dst, const int *src, int count) {
 int k_copy_field(hmec_ctx *c, int dst, int src) { return k_copy_fields(c, &dst, &src, 1); }
 
 // ------------------------------------------------------------ VEP
-// copy_bounds(pt), copy_bounds(pf) (h_mec_rsf_v9.cpp:616-617, :154-159):
-// columns first, then rows, as in the reference.
-// (ctl: the device controller's scalars when the launch is speculative: it
-// then does nothing once the controller's exit flag is set)
-__global__ void k_copy_bounds_cols(double *a, double *b, int Nx, int Ny1, const double *ctl) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Ny1 || (ctl && ctl[SC_EXIT] != 0)) return;
-    a[i] = a[i + Ny1];
-    a[i + (int64_t)Nx * Ny1] = a[i + (int64_t)(Nx - 1) * Ny1];
-    b[i] = b[i + Ny1];
-    b[i + (int64_t)Nx * Ny1] = b[i + (int64_t)(Nx - 1) * Ny1];
-}
-__global__ void k_copy_bounds_rows(double *a, double *b, int Nx1, int Ny, int Ny1, const double *ctl) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= Nx1 || (ctl && ctl[SC_EXIT] != 0)) return;
-    a[(int64_t)j * Ny1] = a[1 + (int64_t)j * Ny1];
-    a[Ny + (int64_t)j * Ny1] = a[Ny - 1 + (int64_t)j * Ny1];
-    b[(int64_t)j * Ny1] = b[1 + (int64_t)j * Ny1];
-    b[Ny + (int64_t)j * Ny1] = b[Ny - 1 + (int64_t)j * Ny1];
-}
-
 // etamincur = dt * shearmod * 1e-4 with dt from the controller (ctl) or given
 __device__ __forceinline__ double vep_floor(const double *ctl, double etamincur, double shearmod) {
     return ctl ? ctl[SC_CDT] * shearmod * 1e-4 : etamincur;
 }
 
-// basic nodes, h_mec_rsf_v9.cpp:620-636
+// basic nodes, h_mec_rsf_v9.cpp:620-636; threads past the basic nodes copy
+// the pressure boundaries (copy_bounds of pt and pf, :616-617: the outer
+// ring from the neighbouring interior node, columns then rows, so a corner
+// takes its diagonal interior neighbour; the sources are interior nodes,
+// which nothing here writes, and the VEP update reads no pressure)
 __global__ void k_vep_basic(double *f, int64_t ns, int Nx, int Ny, double etamincur,
                             double dy_faultw, const double *ctl, double shearmod) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= Nx * Ny || (ctl && ctl[SC_EXIT] != 0)) return;
+    if (ctl && ctl[SC_EXIT] != 0) return;
+    if (t >= Nx * Ny) {
+        const int Ny1 = Ny + 1, q = t - Nx * Ny;
+        int i, j;
+        if (q < 2 * Ny1) {
+            i = q % Ny1;
+            j = q < Ny1 ? 0 : Nx;
+        } else if (q < 2 * Ny1 + 2 * (Nx - 1)) {
+            const int q2 = q - 2 * Ny1;
+            j = 1 + q2 % (Nx - 1);
+            i = q2 < Nx - 1 ? 0 : Ny;
+        } else {
+            return;
+        }
+        const int si = i == 0 ? 1 : (i == Ny ? Ny - 1 : i);
+        const int sj = j == 0 ? 1 : (j == Nx ? Nx - 1 : j);
+        const int64_t d = i + (int64_t)j * Ny1, sidx = si + (int64_t)sj * Ny1;
+        double *pt = f + (int64_t)HMEC_F_PT * ns, *pf = f + (int64_t)HMEC_F_PF * ns;
+        pt[d] = pt[sidx];
+        pf[d] = pf[sidx];
+        return;
+    }
     etamincur = vep_floor(ctl, etamincur, shearmod);
     const int i = t % Ny, j = t / Ny, Ny1 = Ny + 1;
     double *ETA = f + (int64_t)HMEC_F_ETA * ns;
@@ -591,13 +598,8 @@ int k_vep_pre(hmec_ctx *c, double dt, bool dev) {
     const double etamincur = dt * P.shearmod * 1e-4;
     const double dy_faultw = P.dy / P.faultwidth;
     const double *ctl = dev ? c->d.scal : nullptr;
-    hipLaunchKernelGGL(k_copy_bounds_cols, dim3((c->Ny1 + 255) / 256), dim3(256), 0, c->stream,
-                       F(c, HMEC_F_PT), F(c, HMEC_F_PF), c->Nx, c->Ny1, ctl);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_copy_bounds_rows, dim3((c->Nx1 + 255) / 256), dim3(256), 0, c->stream,
-                       F(c, HMEC_F_PT), F(c, HMEC_F_PF), c->Nx1, c->Ny, c->Ny1, ctl);
-    LAUNCH_CHECK();
-    const int nb = c->Nx * c->Ny;
+    // basic nodes, then the pressure boundary ring
+    const int nb = c->Nx * c->Ny + 2 * c->Ny1 + 2 * (c->Nx - 1);
     hipLaunchKernelGGL(k_vep_basic, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
                        c->ns, c->Nx, c->Ny, etamincur, dy_faultw, ctl, P.shearmod);
     LAUNCH_CHECK();
@@ -876,43 +878,40 @@ int k_assemble(hmec_ctx *c, double dt, bool dev) {
     return HMEC_OK;
 }
 
-// ------------------------------------------------------- reload
-__global__ void k_reload(const double *S, double *f, int64_t ns, int n, const double *scal) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const double sc = scal[SC_PTSCALE];
-    const int kp = t * 6;
-    f[(int64_t)HMEC_F_PT * ns + t] = S[kp] * sc;
-    f[(int64_t)HMEC_F_VXS * ns + t] = S[kp + 1];
-    f[(int64_t)HMEC_F_VYS * ns + t] = S[kp + 2];
-    f[(int64_t)HMEC_F_PF * ns + t] = S[kp + 3] * sc;
-    f[(int64_t)HMEC_F_VXD * ns + t] = S[kp + 4];
-    f[(int64_t)HMEC_F_VYD * ns + t] = S[kp + 5];
-}
-
-int k_reload(hmec_ctx *c, double ptscale) {
-    (void)ptscale;
-    const int n = c->Nx1 * c->Ny1;
-    hipLaunchKernelGGL(k_reload, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.sol,
-                       c->d.fields, c->ns, n, c->d.scal);
-    LAUNCH_CHECK();
-    return HMEC_OK;
-}
-
-// ------------------------------------------------------ stress
-// h_mec_rsf_v9.cpp:1102-1127 (and :1480-1521 with ESP when final != 0)
+// ------------------------------------------------------- reload + stress
+// h_mec_rsf_v9.cpp:1102-1127 (and :1480-1521 with ESP when final != 0).
+// FROM_SOL (a Picard iteration): the same launch first reloads node t's
+// unknowns from the solution vector S (:1060-1075; pressures scaled by
+// ptscale from the device scalars) and the strain rates read the velocities
+// from S, which holds exactly the values reloaded; otherwise (the final
+// pass) from the fields.
+// (gate: the device scalars when the launch waits on a solve decided on the
+// device; it then does nothing while the exit flag holds)
+template <bool FROM_SOL>
 __global__ void k_stress(double *f, int64_t ns, int Nx, int Ny, double dx, double dy, double dt,
-                         int final_pass) {
+                         int final_pass, const double *S, const double *scal, const double *gate) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int Ny1 = Ny + 1;
-    if (t >= (Nx + 1) * Ny1) return;
+    if (t >= (Nx + 1) * Ny1 || (gate && gate[SC_EXIT] == SC_EXIT_HOLD)) return;
+    if (FROM_SOL) {
+        const double sc = scal[SC_PTSCALE];
+        const int64_t kp = (int64_t)t * 6;
+        f[(int64_t)HMEC_F_PT * ns + t] = S[kp] * sc;
+        f[(int64_t)HMEC_F_VXS * ns + t] = S[kp + 1];
+        f[(int64_t)HMEC_F_VYS * ns + t] = S[kp + 2];
+        f[(int64_t)HMEC_F_PF * ns + t] = S[kp + 3] * sc;
+        f[(int64_t)HMEC_F_VXD * ns + t] = S[kp + 4];
+        f[(int64_t)HMEC_F_VYD * ns + t] = S[kp + 5];
+    }
+    const double *vxf = f + (int64_t)HMEC_F_VXS * ns, *vyf = f + (int64_t)HMEC_F_VYS * ns;
+    auto vxs = [&](int64_t q) { return FROM_SOL ? S[q * 6 + 1] : vxf[q]; };
+    auto vys = [&](int64_t q) { return FROM_SOL ? S[q * 6 + 2] : vyf[q]; };
     const int i = t % Ny1, j = t / Ny1;
-    const double *vxs = f + (int64_t)HMEC_F_VXS * ns, *vys = f + (int64_t)HMEC_F_VYS * ns;
     if (i < Ny && j < Nx) {
         const int64_t b = i + (int64_t)j * Ny;
         const int64_t s = i + (int64_t)j * Ny1;
-        const double dvxdy = (vxs[s + 1] - vxs[s]) / dy;
-        const double dvydx = (vys[s + Ny1] - vys[s]) / dx;
+        const double dvxdy = (vxs(s + 1) - vxs(s)) / dy;
+        const double dvydx = (vys(s + Ny1) - vys(s)) / dx;
         const double exy = .5 * (dvxdy + dvydx);
         if (final_pass) f[(int64_t)HMEC_F_ESP * ns + b] = .5 * (dvydx - dvxdy);
         f[(int64_t)HMEC_F_EXY * ns + b] = exy;
@@ -922,8 +921,8 @@ __global__ void k_stress(double *f, int64_t ns, int Nx, int Ny, double dx, doubl
     }
     if (i >= 1 && i < Ny && j >= 1 && j < Nx) {
         const int64_t s = i + (int64_t)j * Ny1;
-        const double dvx = vxs[s] - vxs[s - Ny1];
-        const double dvy = vys[s] - vys[s - 1];
+        const double dvx = vxs(s) - vxs(s - Ny1);
+        const double dvy = vys(s) - vys(s - 1);
         const double exx = (2 * dvx / dx - dvy / dy) / 3.;
         const double eyy = (2 * dvy / dy - dvx / dx) / 3.;
         f[(int64_t)HMEC_F_EXX * ns + s] = exx;
@@ -935,46 +934,14 @@ __global__ void k_stress(double *f, int64_t ns, int Nx, int Ny, double dx, doubl
     }
 }
 
-// --------------------------------------------------------- RSF
-// One workgroup.  Vmax (before the solve's RSF, h_mec_rsf_v9.cpp:1077), then
-// the rate-and-state bisection per fault node (:1165-1308), then ynpl / ddd
-// (summed sequentially in node order like the reference) / dtlapusta.
-// Nodal rows other than line_fault keep ETA5 = ETA0, DSY = YNY = 0 etc.
-// ETA5 = ETA0, DSY = YNY = SigmaY = SII_fault = 0, OM5 = OM on every basic
-// node (h_mec_rsf_v9.cpp:1152-1159) and block maxima of VSLIPB (the Vmax of
-// the previous iteration, taken before the fault solve overwrites it)
-constexpr int RSF_PREP_T = 256;
-__global__ void __launch_bounds__(RSF_PREP_T) k_rsf_prep(double *f, int64_t ns, int nb,
-                                                         double *vmax_part) {
-    __shared__ double red[RSF_PREP_T];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double mx = -INFINITY;
-    if (t < nb) {
-        mx = f[(int64_t)HMEC_F_VSLIPB * ns + t];
-        f[(int64_t)HMEC_F_ETA5 * ns + t] = f[(int64_t)HMEC_F_ETA0 * ns + t];
-        f[(int64_t)HMEC_F_DSY * ns + t] = 0;
-        f[(int64_t)HMEC_F_YNY * ns + t] = 0;
-        f[(int64_t)HMEC_F_SIGMAY * ns + t] = 0;
-        f[(int64_t)HMEC_F_SII_FAULT * ns + t] = 0;
-        f[(int64_t)HMEC_F_OM5 * ns + t] = f[(int64_t)HMEC_F_OM * ns + t];
-    }
-    red[threadIdx.x] = mx;
-    __syncthreads();
-    for (int s = RSF_PREP_T / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) vmax_part[blockIdx.x] = red[0];
-}
-
-// One thread per fault node: the rate-and-state bisection (:1165-1308) and
-// the node's dtlapusta candidate, bisection-cap flag, DSY^2 and yield flag.
-constexpr int RSF_T = 64;
-__global__ void __launch_bounds__(RSF_T) k_rsf_nodes(double *f, int64_t ns, int Nx, int Ny,
-                                                     hmec_params P, double dt, double *dsy2,
-                                                     int *flag, double *dtln, int *capn) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= Nx) return;
+// --------------------------------------------------------- update reductions
+// The rate-and-state fault solve of one fault node j (h_mec_rsf_v9.cpp:
+// 1165-1308): the bisection for the node's stress, state, slip rate, yield
+// stress and viscosity, and the node's dtlapusta candidate, bisection-cap
+// flag, DSY^2 and yield flag.
+__device__ __forceinline__ void rsf_node(double *f, int64_t ns, int Nx, int Ny, const hmec_params &P,
+                                         double dt, int j, double *dsy2, int *flag, double *dtln,
+                                         int *capn) {
     const int Ny1 = Ny + 1;
     const int i = P.line_fault;
     const double V0 = P.V0, BETASOLID = P.BETASOLID, dx = P.dx;
@@ -1076,179 +1043,219 @@ __global__ void __launch_bounds__(RSF_T) k_rsf_nodes(double *f, int64_t ns, int 
     }
 }
 
-// Vmax (before the solve's RSF, h_mec_rsf_v9.cpp:1077) from the block
-// maxima of k_rsf_prep, min of the nodes' dtlapusta candidates, bisection
-// caps, and ynpl / ddd summed sequentially in node order like the reference
-// (from LDS chunks; an uncounted node has dsy2 = +0, and adding +0 to the
-// non-negative sum is exact).
-constexpr int RSF_FT = 512;
-__global__ void __launch_bounds__(RSF_FT) k_rsf_final(int Nx, int active, double *scal,
-                                                      const double *dsy2, const int *flag,
-                                                      const double *dtln, const int *capn,
-                                                      const double *vmax_part, int nvpart) {
-    __shared__ double red[RSF_FT];
-    __shared__ int cnt[RSF_FT];
-    const int tid = threadIdx.x;
-    double mx = -INFINITY, dtl = 1e7;
-    int cap = 0;
-    for (int t = tid; t < nvpart; t += RSF_FT) mx = fmax(mx, vmax_part[t]);
-    if (active)
-        for (int j = tid; j < Nx; j += RSF_FT) {
-            dtl = fmin(dtl, dtln[j]);
-            cap += capn[j];
-        }
-    red[tid] = mx;
+// One launch after the solve's reload and stresses, thread t:
+// - basic node t (t < Nx Ny): the previous slip rate into the block maximum
+//   (Vmax before this iteration's fault solve, :1077); ETA5 = ETA0, DSY = YNY
+//   = SigmaY = SII_fault = 0, OM5 = OM (:1152-1159); on the fault row, when
+//   the fault is active, the rate-and-state solve of that node; then the
+//   slip-rate timestep candidate dx stpmax / VSLIPB (:1310-1436 inputs);
+// - staggered node t: velocity extrema and fault-band maxima.
+// Block partials (max / min, exact in any order) for k_update_final.
+constexpr int UPD_T = 256;
+constexpr int UPD_NQ = 8;   // Vmax, vx max / min, vy max / min, band x / y, dtslip
+__global__ void __launch_bounds__(UPD_T) k_update_nodes(double *f, int64_t ns, int Nx, int Ny,
+                                                        hmec_params P, double dt, int active,
+                                                        const double *yvx, const double *yvy,
+                                                        double dxstp, double *part, int nblk,
+                                                        double *dsy2, int *flag, double *dtln,
+                                                        int *capn, const double *gate) {
+    __shared__ double red[UPD_NQ][UPD_T];
+    if (gate && gate[SC_EXIT] == SC_EXIT_HOLD) return;
+    const int tid = threadIdx.x, t = blockIdx.x * UPD_T + tid;
+    const int Ny1 = Ny + 1, nb = Nx * Ny;
+    double v[UPD_NQ] = {-INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, 0, 0, 1e30};
+    if (t < nb) {
+        double *VS = f + (int64_t)HMEC_F_VSLIPB * ns;
+        v[0] = VS[t];
+        f[(int64_t)HMEC_F_ETA5 * ns + t] = f[(int64_t)HMEC_F_ETA0 * ns + t];
+        f[(int64_t)HMEC_F_DSY * ns + t] = 0;
+        f[(int64_t)HMEC_F_YNY * ns + t] = 0;
+        f[(int64_t)HMEC_F_SIGMAY * ns + t] = 0;
+        f[(int64_t)HMEC_F_SII_FAULT * ns + t] = 0;
+        f[(int64_t)HMEC_F_OM5 * ns + t] = f[(int64_t)HMEC_F_OM * ns + t];
+        if (active && t % Ny == P.line_fault) rsf_node(f, ns, Nx, Ny, P, dt, t / Ny, dsy2, flag, dtln, capn);
+        const double vs = VS[t];
+        if (vs > 0) v[7] = dxstp / vs;
+    }
+    if (t < (Nx + 1) * Ny1) {
+        const int i = t % Ny1;
+        const double a = f[(int64_t)HMEC_F_VXS * ns + t], b = f[(int64_t)HMEC_F_VYS * ns + t];
+        v[1] = a;
+        v[2] = a;
+        v[3] = b;
+        v[4] = b;
+        if (yvx[i] >= P.upper_block && yvx[i] <= P.lower_block) v[5] = fabs(a);
+        if (yvy[i] >= P.upper_block && yvy[i] <= P.lower_block) v[6] = fabs(b);
+    }
+#pragma unroll
+    for (int q = 0; q < UPD_NQ; q++) red[q][tid] = v[q];
     __syncthreads();
-    for (int s = RSF_FT / 2; s > 0; s >>= 1) {
-        if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
+    for (int s = UPD_T / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[0][tid] = fmax(red[0][tid], red[0][tid + s]);
+            red[1][tid] = fmax(red[1][tid], red[1][tid + s]);
+            red[2][tid] = fmin(red[2][tid], red[2][tid + s]);
+            red[3][tid] = fmax(red[3][tid], red[3][tid + s]);
+            red[4][tid] = fmin(red[4][tid], red[4][tid + s]);
+            red[5][tid] = fmax(red[5][tid], red[5][tid + s]);
+            red[6][tid] = fmax(red[6][tid], red[6][tid + s]);
+            red[7][tid] = fmin(red[7][tid], red[7][tid + s]);
+        }
         __syncthreads();
     }
-    if (tid == 0) scal[SC_VMAX] = red[0];
-    __syncthreads();
-    red[tid] = dtl;
+    if (tid < UPD_NQ) part[tid * nblk + blockIdx.x] = red[tid][0];
+}
+
+// The device-side Picard / dt controller on this iteration's scalars and the
+// controller state in the device scalars, DSYLSQ in its own array; writes the
+// new state and the exit flag (ynlast counts on when the loop goes on, :1455).
+__device__ void controller_device(const hmec_params &P, int iterstep, double *scal, double *dsylsq) {
+    hmec_iter_scalars sc;
+    sc.ynpl = (int)scal[SC_YNPL];
+    sc.ddd = scal[SC_DDD];
+    sc.dtlapusta = scal[SC_DTLAP];
+    sc.Vmax = scal[SC_VMAX];
+    sc.vxs_max = scal[SC_VXMAX];
+    sc.vxs_min = scal[SC_VXMIN];
+    sc.vys_max = scal[SC_VYMAX];
+    sc.vys_min = scal[SC_VYMIN];
+    sc.maxvxs_band = scal[SC_BANDX];
+    sc.maxvys_band = scal[SC_BANDY];
+    sc.dtslip = scal[SC_DTSLIP];
+    sc.bisect_capped = 0;
+    CtlState s{scal[SC_CDT], scal[SC_CDTX], scal[SC_CDTY], sc.dtlapusta, scal[SC_CMAXVXY],
+               (int)scal[SC_CYNLAST]};
+    const bool stop = controller_eval(P, iterstep, sc, dsylsq, s);
+    if (!stop) s.ynlast++;
+    scal[SC_CDT] = s.dt;
+    scal[SC_CDTX] = s.dtx;
+    scal[SC_CDTY] = s.dty;
+    scal[SC_CMAXVXY] = s.maxvxy;
+    scal[SC_CYNLAST] = s.ynlast;
+    scal[SC_EXIT] = stop ? 1.0 : 0.0;
+}
+
+// One workgroup: the update's scalars from k_update_nodes' block partials
+// (Vmax, velocity extrema, band maxima, dtslip), the minimum of the fault
+// nodes' dtlapusta candidates, the bisection caps, and ynpl / ddd summed
+// sequentially in node order like the reference (from LDS chunks; an
+// uncounted node has dsy2 = +0, and adding +0 to the non-negative sum is
+// exact); then, for iterstep >= 0, the controller on them.
+constexpr int UPF_T = 512;
+__global__ void __launch_bounds__(UPF_T) k_update_final(int Nx, int active, double *scal,
+                                                        const double *dsy2, const int *flag,
+                                                        const double *dtln, const int *capn,
+                                                        const double *part, int nblk, int iterstep,
+                                                        hmec_params P, double *dsylsq, int gated) {
+    if (gated && scal[SC_EXIT] == SC_EXIT_HOLD) return;
+    __shared__ double red[UPD_NQ + 1][UPF_T];
+    __shared__ int cnt[UPF_T];
+    const int tid = threadIdx.x;
+    // the partials' eight reductions and the dtlapusta minimum in one tree
+    double v[UPD_NQ + 1] = {-INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, 0, 0, 1e30, 1e7};
+    for (int b = tid; b < nblk; b += UPF_T) {
+        v[0] = fmax(v[0], part[0 * nblk + b]);
+        v[1] = fmax(v[1], part[1 * nblk + b]);
+        v[2] = fmin(v[2], part[2 * nblk + b]);
+        v[3] = fmax(v[3], part[3 * nblk + b]);
+        v[4] = fmin(v[4], part[4 * nblk + b]);
+        v[5] = fmax(v[5], part[5 * nblk + b]);
+        v[6] = fmax(v[6], part[6 * nblk + b]);
+        v[7] = fmin(v[7], part[7 * nblk + b]);
+    }
+    int cap = 0;
+    if (active)
+        for (int j = tid; j < Nx; j += UPF_T) {
+            v[8] = fmin(v[8], dtln[j]);
+            cap += capn[j];
+        }
+#pragma unroll
+    for (int q = 0; q <= UPD_NQ; q++) red[q][tid] = v[q];
     cnt[tid] = cap;
     __syncthreads();
-    for (int s = RSF_FT / 2; s > 0; s >>= 1) {
+    for (int s = UPF_T / 2; s > 0; s >>= 1) {
         if (tid < s) {
-            red[tid] = fmin(red[tid], red[tid + s]);
+            red[0][tid] = fmax(red[0][tid], red[0][tid + s]);
+            red[1][tid] = fmax(red[1][tid], red[1][tid + s]);
+            red[2][tid] = fmin(red[2][tid], red[2][tid + s]);
+            red[3][tid] = fmax(red[3][tid], red[3][tid + s]);
+            red[4][tid] = fmin(red[4][tid], red[4][tid + s]);
+            red[5][tid] = fmax(red[5][tid], red[5][tid + s]);
+            red[6][tid] = fmax(red[6][tid], red[6][tid + s]);
+            red[7][tid] = fmin(red[7][tid], red[7][tid + s]);
+            red[8][tid] = fmin(red[8][tid], red[8][tid + s]);
             cnt[tid] += cnt[tid + s];
         }
         __syncthreads();
     }
-    if (tid == 0) {
-        scal[SC_DTLAP] = red[0];
-        scal[SC_CAPPED] = cnt[0];
-    }
+    double out[UPD_NQ];
+#pragma unroll
+    for (int q = 0; q < UPD_NQ; q++) out[q] = red[q][0];
+    const double dtlap = red[8][0];
+    const int capped = cnt[0];
+    double *seq = red[0];   // reused below for the ordered sums
     int ynpl = 0;
     double ddd = 0;
     if (active) {
-        for (int j0 = 0; j0 < Nx; j0 += RSF_FT) {
+        for (int j0 = 0; j0 < Nx; j0 += UPF_T) {
             __syncthreads();
             if (j0 + tid < Nx) {
-                red[tid] = dsy2[j0 + tid];
+                seq[tid] = dsy2[j0 + tid];
                 cnt[tid] = flag[j0 + tid];
             }
             __syncthreads();
             if (tid == 0) {
-                const int n = min(RSF_FT, Nx - j0);
+                const int n = min(UPF_T, Nx - j0);
                 for (int q = 0; q < n; q++) {
-                    ddd += red[q];
+                    ddd += seq[q];
                     ynpl += cnt[q];
                 }
             }
         }
     }
     if (tid == 0) {
+        scal[SC_VMAX] = out[0];
+        scal[SC_VXMAX] = out[1];
+        scal[SC_VXMIN] = out[2];
+        scal[SC_VYMAX] = out[3];
+        scal[SC_VYMIN] = out[4];
+        scal[SC_BANDX] = out[5];
+        scal[SC_BANDY] = out[6];
+        scal[SC_DTSLIP] = out[7];
+        scal[SC_DTLAP] = dtlap;
+        scal[SC_CAPPED] = capped;
         scal[SC_YNPL] = ynpl;
         scal[SC_DDD] = ddd;
+        if (iterstep >= 0) controller_device(P, iterstep, scal, dsylsq);
     }
 }
 
-// velocity extrema, fault-band maxima and dtslip (h_mec_rsf_v9.cpp:1344-1410)
-// velocity extrema, fault-band maxima and the slip-rate timestep
-// (h_mec_rsf_v9.cpp:1310-1436 inputs) in two stages: block partials, then one
-// block over them (min / max are exact, so the order does not matter)
-constexpr int RED_BLOCKS = 256;
-__global__ void __launch_bounds__(256) k_reduce_part(const double *f, int64_t ns, int Nx, int Ny,
-                                                     const double *yvx, const double *yvy,
-                                                     double upper, double lower, double dxstp,
-                                                     double *part) {
-    __shared__ double red[7][256];
-    const int Ny1 = Ny + 1, tid = threadIdx.x;
-    const double *vxs = f + (int64_t)HMEC_F_VXS * ns, *vys = f + (int64_t)HMEC_F_VYS * ns;
-    const double *VS = f + (int64_t)HMEC_F_VSLIPB * ns;
-    double vxmax = -INFINITY, vxmin = INFINITY, vymax = -INFINITY, vymin = INFINITY, bx = 0, by = 0,
-           dts = 1e30;
-    const int stride = gridDim.x * blockDim.x;
-    for (int t = blockIdx.x * blockDim.x + tid; t < (Nx + 1) * Ny1; t += stride) {
-        const int i = t % Ny1;
-        const double a = vxs[t], b = vys[t];
-        vxmax = fmax(vxmax, a);
-        vxmin = fmin(vxmin, a);
-        vymax = fmax(vymax, b);
-        vymin = fmin(vymin, b);
-        if (yvx[i] >= upper && yvx[i] <= lower) bx = fmax(bx, fabs(a));
-        if (yvy[i] >= upper && yvy[i] <= lower) by = fmax(by, fabs(b));
-    }
-    for (int t = blockIdx.x * blockDim.x + tid; t < Nx * Ny; t += stride)
-        if (VS[t] > 0) dts = fmin(dts, dxstp / VS[t]);
-    red[0][tid] = vxmax; red[1][tid] = vxmin; red[2][tid] = vymax; red[3][tid] = vymin;
-    red[4][tid] = bx; red[5][tid] = by; red[6][tid] = dts;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) {
-            red[0][tid] = fmax(red[0][tid], red[0][tid + s]);
-            red[1][tid] = fmin(red[1][tid], red[1][tid + s]);
-            red[2][tid] = fmax(red[2][tid], red[2][tid + s]);
-            red[3][tid] = fmin(red[3][tid], red[3][tid + s]);
-            red[4][tid] = fmax(red[4][tid], red[4][tid + s]);
-            red[5][tid] = fmax(red[5][tid], red[5][tid + s]);
-            red[6][tid] = fmin(red[6][tid], red[6][tid + s]);
-        }
-        __syncthreads();
-    }
-    if (tid == 0)
-        for (int q = 0; q < 7; q++) part[q * RED_BLOCKS + blockIdx.x] = red[q][0];
-}
-__global__ void __launch_bounds__(RED_BLOCKS) k_reduce_final(const double *part, double *scal) {
-    __shared__ double red[7][RED_BLOCKS];
-    const int tid = threadIdx.x;
-    for (int q = 0; q < 7; q++) red[q][tid] = part[q * RED_BLOCKS + tid];
-    __syncthreads();
-    for (int s = RED_BLOCKS / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-            red[0][tid] = fmax(red[0][tid], red[0][tid + s]);
-            red[1][tid] = fmin(red[1][tid], red[1][tid + s]);
-            red[2][tid] = fmax(red[2][tid], red[2][tid + s]);
-            red[3][tid] = fmin(red[3][tid], red[3][tid + s]);
-            red[4][tid] = fmax(red[4][tid], red[4][tid + s]);
-            red[5][tid] = fmax(red[5][tid], red[5][tid + s]);
-            red[6][tid] = fmin(red[6][tid], red[6][tid + s]);
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        scal[SC_VXMAX] = red[0][0]; scal[SC_VXMIN] = red[1][0];
-        scal[SC_VYMAX] = red[2][0]; scal[SC_VYMIN] = red[3][0];
-        scal[SC_BANDX] = red[4][0]; scal[SC_BANDY] = red[5][0];
-        scal[SC_DTSLIP] = red[6][0];
-    }
-}
-
-int k_update_launch(hmec_ctx *c, double dt) {
+// reload, stresses, fault solve and the update's reductions (three launches);
+// iterstep >= 0: the controller as well; gated: every launch does nothing
+// while the exit flag holds (a solve whose outcome the host decides)
+int k_update_launch(hmec_ctx *c, double dt, int iterstep, bool gated) {
+    const double *gate = gated ? (const double *)c->d.scal : nullptr;
     const hmec_params &P = c->P;
     const int n = c->Nx1 * c->Ny1;
-    hipLaunchKernelGGL(k_stress, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
-                       c->ns, c->Nx, c->Ny, P.dx, P.dy, dt, 0);
+    hipLaunchKernelGGL(k_stress<true>, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
+                       c->ns, c->Nx, c->Ny, P.dx, P.dy, dt, 0, (const double *)c->d.sol,
+                       (const double *)c->d.scal, gate);
     LAUNCH_CHECK();
     const int active = (c->timestep > P.tyield) ? 1 : 0;
     double *dsy2 = c->d.work2;
     int *flag = c->d.counters + 16;
-    // block partials in c->d.work (free between solves)
-    const int nb = c->Nx * c->Ny;
-    const int nvp = (nb + RSF_PREP_T - 1) / RSF_PREP_T;
-    double *part = c->d.work;
-    hipLaunchKernelGGL(k_rsf_prep, dim3(nvp), dim3(RSF_PREP_T), 0, c->stream, c->d.fields, c->ns, nb,
-                       part);
-    LAUNCH_CHECK();
     double *dtln = c->d.work2 + c->Nx;
     int *capn = flag + c->Nx;
-    if (active) {
-        hipLaunchKernelGGL(k_rsf_nodes, dim3((c->Nx + RSF_T - 1) / RSF_T), dim3(RSF_T), 0, c->stream,
-                           c->d.fields, c->ns, c->Nx, c->Ny, P, dt, dsy2, flag, dtln, capn);
-        LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(k_rsf_final, dim3(1), dim3(RSF_FT), 0, c->stream, c->Nx, active, c->d.scal,
-                       (const double *)dsy2, (const int *)flag, (const double *)dtln,
-                       (const int *)capn, (const double *)part, nvp);
-    LAUNCH_CHECK();
+    double *part = c->d.work;          // block partials (free between solves)
+    const int nblk = (n + UPD_T - 1) / UPD_T;
     Coords k = coords(c);
-    hipLaunchKernelGGL(k_reduce_part, dim3(RED_BLOCKS), dim3(256), 0, c->stream, c->d.fields, c->ns,
-                       c->Nx, c->Ny, k.yvx, k.yvy, P.upper_block, P.lower_block, P.dx * P.stpmax,
-                       part);
+    hipLaunchKernelGGL(k_update_nodes, dim3(nblk), dim3(UPD_T), 0, c->stream, c->d.fields, c->ns, c->Nx,
+                       c->Ny, P, dt, active, k.yvx, k.yvy, P.dx * P.stpmax, part, nblk, dsy2, flag,
+                       dtln, capn, gate);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BLOCKS), 0, c->stream, (const double *)part,
-                       c->d.scal);
+    hipLaunchKernelGGL(k_update_final, dim3(1), dim3(UPF_T), 0, c->stream, c->Nx, active, c->d.scal,
+                       (const double *)dsy2, (const int *)flag, (const double *)dtln, (const int *)capn,
+                       (const double *)part, nblk, iterstep, P, c->d.dsylsq, gated ? 1 : 0);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
@@ -1283,7 +1290,7 @@ int k_scalars_wait(hmec_ctx *c, hmec_iter_scalars *out) {
 }
 
 int k_update(hmec_ctx *c, double dt, hmec_iter_scalars *out) {
-    HMEC_TRY(k_update_launch(c, dt));
+    HMEC_TRY(k_update_launch(c, dt, -1, false));
     HMEC_TRY(k_scalars_enqueue(c));
     return k_scalars_wait(c, out);
 }
@@ -1308,7 +1315,7 @@ bool k_controller_read(hmec_ctx *c) {
 __global__ void k_commit(double *f, int64_t ns, int nb, int ynpl, int converged, double etamin,
                          const double *ctl) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nb) return;
+    if (t >= nb || (ctl && ctl[SC_EXIT] == SC_EXIT_HOLD)) return;
     if (ctl) {
         ynpl = (int)ctl[SC_YNPL];
         converged = ctl[SC_EXIT] != 0;
@@ -1334,44 +1341,29 @@ int k_iter_commit(hmec_ctx *c, int ynpl, int converged) {
 }
 
 // ----------------------------------- device-side Picard / dt controller
-// One thread: this iteration's scalars (the update's reductions) and the
-// controller state in the device scalars, DSYLSQ in its own array; writes the
-// new state and the exit flag (ynlast counts on when the loop goes on, :1455).
+// One thread: controller_device on the scalars the caller placed.
 __global__ void k_controller(hmec_params P, int iterstep, double *scal, double *dsylsq) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    hmec_iter_scalars sc;
-    sc.ynpl = (int)scal[SC_YNPL];
-    sc.ddd = scal[SC_DDD];
-    sc.dtlapusta = scal[SC_DTLAP];
-    sc.Vmax = scal[SC_VMAX];
-    sc.vxs_max = scal[SC_VXMAX];
-    sc.vxs_min = scal[SC_VXMIN];
-    sc.vys_max = scal[SC_VYMAX];
-    sc.vys_min = scal[SC_VYMIN];
-    sc.maxvxs_band = scal[SC_BANDX];
-    sc.maxvys_band = scal[SC_BANDY];
-    sc.dtslip = scal[SC_DTSLIP];
-    sc.bisect_capped = 0;
-    CtlState s{scal[SC_CDT], scal[SC_CDTX], scal[SC_CDTY], sc.dtlapusta, scal[SC_CMAXVXY],
-               (int)scal[SC_CYNLAST]};
-    const bool stop = controller_eval(P, iterstep, sc, dsylsq, s);
-    if (!stop) s.ynlast++;
-    scal[SC_CDT] = s.dt;
-    scal[SC_CDTX] = s.dtx;
-    scal[SC_CDTY] = s.dty;
-    scal[SC_CMAXVXY] = s.maxvxy;
-    scal[SC_CYNLAST] = s.ynlast;
-    scal[SC_EXIT] = stop ? 1.0 : 0.0;
+    controller_device(P, iterstep, scal, dsylsq);
 }
 
-// controller, then the iteration epilogue with its decision, both on the device
-int k_controller_commit(hmec_ctx *c, int iterstep) {
-    hipLaunchKernelGGL(k_controller, dim3(1), dim3(64), 0, c->stream, c->P, iterstep, c->d.scal,
-                       c->d.dsylsq);
-    LAUNCH_CHECK();
+// the iteration epilogue with the device controller's decision (the
+// controller ran at the end of k_update_launch)
+int k_controller_commit(hmec_ctx *c) {
     const int nb = c->Nx * c->Ny;
     hipLaunchKernelGGL(k_commit, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
                        c->ns, nb, 0, 0, c->P.etamin, (const double *)c->d.scal);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
+
+// the exit flag back to "go on" after the host decided a held solve (a
+// one-thread kernel: no memset in the stream)
+__global__ void k_exit_release(double *scal) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) scal[SC_EXIT] = 0;
+}
+int k_exit_release(hmec_ctx *c) {
+    hipLaunchKernelGGL(k_exit_release, dim3(1), dim3(64), 0, c->stream, c->d.scal);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
@@ -1490,8 +1482,9 @@ __global__ void k_diagnostics(double *f, int64_t ns, int Nx, int Ny, double dt, 
 
 int k_final_stress(hmec_ctx *c, double dt) {
     const int n = c->Nx1 * c->Ny1;
-    hipLaunchKernelGGL(k_stress, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
-                       c->ns, c->Nx, c->Ny, c->P.dx, c->P.dy, dt, 1);
+    hipLaunchKernelGGL(k_stress<false>, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d.fields,
+                       c->ns, c->Nx, c->Ny, c->P.dx, c->P.dy, dt, 1, (const double *)nullptr,
+                       (const double *)nullptr, (const double *)nullptr);
     LAUNCH_CHECK();
     const int nc = (c->Nx - 1) * (c->Ny - 1);
     if (nc > 0) {

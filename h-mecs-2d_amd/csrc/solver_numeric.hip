@@ -966,6 +966,9 @@ __global__ void k_axpy_scaled(double *x, const double *cs, const double *z, int6
 // and, when y2 is given, y2 = rs * y.
 constexpr int SPMV_ROWS = 256;
 constexpr int SPMV_CHUNK = 2048;
+// gated launches (a solve's first GMRES step decided on the device) run only
+// when the step's state in the device scalars is this one (GM_COMBINE)
+constexpr int GM_GATE_COMBINE = 2;
 // MODE 2: y = rs * (A x) and, per workgroup, the partial dot products of its
 // rows of y with V_0..V_{nv-1} (columns ldv apart) -> part[block * nv + i].
 constexpr int GM_MAXV = 32;
@@ -975,9 +978,11 @@ k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
              const double *__restrict__ val, const double *__restrict__ x,
              const double *__restrict__ rs, const double *__restrict__ b, double *__restrict__ y,
              double *__restrict__ y2, int64_t N, const double *__restrict__ V = nullptr,
-             int64_t ldv = 0, int nv = 0, double *__restrict__ part = nullptr) {
+             int64_t ldv = 0, int nv = 0, double *__restrict__ part = nullptr,
+             const double *__restrict__ gate = nullptr) {
     __shared__ double prod[SPMV_CHUNK];
     __shared__ int32_t rp[SPMV_ROWS + 1];
+    if (gate && gate[SC_GMSTATE_SLOT] != GM_GATE_COMBINE) return;
     // row blocks in contiguous ranges per XCD (the x entries a block reads
     // overlap its neighbours')
     const int64_t nblk = (N + SPMV_ROWS - 1) / SPMV_ROWS;
@@ -1031,11 +1036,13 @@ k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
     }
 }
 // y = b - A x (y2 = rs * y when given)
-static void launch_residual(hmec_ctx *c, const double *x, double *y, double *y2, const double *rs) {
+static void launch_residual(hmec_ctx *c, const double *x, double *y, double *y2, const double *rs,
+                            const double *gate = nullptr) {
     const int64_t N = c->N;
     hipLaunchKernelGGL(k_csr_stream<1>, dim3((unsigned)xcd_grid((N + SPMV_ROWS - 1) / SPMV_ROWS)),
                        dim3(SPMV_ROWS), 0, c->stream, c->d.rowp, c->d.coli, c->d.val, x, rs,
-                       c->d.rhs, y, y2, N);
+                       c->d.rhs, y, y2, N, (const double *)nullptr, (int64_t)0, 0, (double *)nullptr,
+                       gate);
 }
 // y = rs * (A x) with the partial dot products of y with V_0..V_{nv-1}
 static void launch_spmv_rs_dots(hmec_ctx *c, const double *x, double *y, const double *V,
@@ -1063,9 +1070,11 @@ struct NormArgs {
 };
 // (SQ: one more output, out[NV] = x[2] . x[2], summed in a fixed order)
 template <int NV, bool SQ = false>
-__global__ void __launch_bounds__(256) k_norms_part(NormArgs<NV> a, int64_t N, double *part) {
+__global__ void __launch_bounds__(256) k_norms_part(NormArgs<NV> a, int64_t N, double *part,
+                                                    const double *gate = nullptr) {
     constexpr int NO = NV + (SQ ? 1 : 0);
     __shared__ double r[NO][4];
+    if (gate && gate[SC_GMSTATE_SLOT] != GM_GATE_COMBINE) return;
     double m[NO];
 #pragma unroll
     for (int q = 0; q < NO; q++) m[q] = 0;
@@ -1087,9 +1096,10 @@ __global__ void __launch_bounds__(256) k_norms_part(NormArgs<NV> a, int64_t N, d
 // perturbed and of unrepairable pivots come back with the norms)
 template <int NV, bool SQ = false>
 __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb, double *out,
-                                                     const int *cnt) {
+                                                     const int *cnt, const double *gate = nullptr) {
     constexpr int NO = NV + (SQ ? 1 : 0);
     __shared__ double r[NO][4];
+    if (gate && gate[SC_GMSTATE_SLOT] != GM_GATE_COMBINE) return;
     double m[NO];
 #pragma unroll
     for (int q = 0; q < NO; q++) m[q] = 0;
@@ -1120,11 +1130,26 @@ __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb,
 // dcoef[3d..3d+3) = a_dd, a_nn, a_dn, gathered once per matrix)
 // (bad: optional counter of Darcy diagonals that are zero or non-finite, the
 // pivots of this block elimination, which no perturbation repairs)
+// One launch, three independent index ranges: E and the Darcy coefficients
+// per Darcy unknown d < ND; the A_RD coefficients of the condensed
+// right-hand side (rcoef[t] = val[raslot[t]], t < nt); and, when given,
+// out = a * sc over Nv entries (the equilibrated right-hand side of a solve)
 __global__ void k_darcy_e(const double *val, int64_t ND, const int32_t *ddiag,
                           const int32_t *dnbdiag, const int32_t *dnbe, const int32_t *des,
-                          double *e, double *dcoef, int *bad) {
-    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= ND) return;
+                          double *e, double *dcoef, int *bad, const int32_t *raslot, int64_t nt,
+                          double *rcoef, const double *va, const double *vsc, double *vout,
+                          int64_t Nv) {
+    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= ND) {
+        d -= ND;
+        if (d < nt) {
+            rcoef[d] = val[raslot[d]];
+            return;
+        }
+        d -= nt;
+        if (d < Nv) vout[d] = va[d] * vsc[d];
+        return;
+    }
     const double add = val[ddiag[d]];
     if (bad && !(fabs(add) > 0 && isfinite(add))) atomicAdd(bad, 1);
     dcoef[3 * d] = add;
@@ -1172,10 +1197,6 @@ __global__ void k_darcy_gv(const double *v, const double *rs, int64_t ND, const 
     }
 }
 // rcoef[t] = the A_RD entry of right-hand-side term t (once per matrix)
-__global__ void k_cond_coef(const double *val, const int32_t *raslot, int64_t nt, double *rcoef) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nt) rcoef[t] = val[raslot[t]];
-}
 // vs = rsS * (v_R / rs - A_RD gv)   (condensed right-hand side, equilibrated)
 // (zf, when given: the values go to the sweep's front order, zf[fpos[r]],
 // instead; the sweep then writes its solution into vs)
@@ -1928,7 +1949,8 @@ static int factor_levels(hmec_ctx *const *cs, int B, const FacPtrs<T> *mem, hipS
     return HMEC_OK;
 }
 
-static int darcy_e(hmec_ctx *c, hipStream_t st, int *bad = nullptr);
+static int darcy_e(hmec_ctx *c, hipStream_t st, int *bad = nullptr, const double *va = nullptr,
+                   const double *vsc = nullptr, double *vout = nullptr);
 
 // Numeric factorization of n same-size contexts' current matrices with one
 // launch per kernel (n = 1: a single context).  The batch runs on the first
@@ -2111,14 +2133,14 @@ static int launch_norms(hmec_ctx *c, const double *a, const double *b, double *o
 // out[0..4) = inf-norms of a, b, c2, d, out[4] = c2 . c2 in one pass, and
 // out[5] = perturbed pivots of the current factor
 static int launch_norms4(hmec_ctx *c, const double *a, const double *b, const double *c2,
-                         const double *d, double *out) {
+                         const double *d, double *out, const double *gate = nullptr) {
     SolverDev *s = sdev(c);
     NormArgs<4> na{{a, b, c2, d}};
     hipLaunchKernelGGL((k_norms_part<4, true>), dim3(NRM_BLOCKS), dim3(256), 0, c->stream, na, c->N,
-                       s->nrm_part);
+                       s->nrm_part, gate);
     LAUNCH_CHECK();
     hipLaunchKernelGGL((k_norms_final<4, true>), dim3(1), dim3(256), 0, c->stream, s->nrm_part,
-                       NRM_BLOCKS, out, (const int *)c->d.counters);
+                       NRM_BLOCKS, out, (const int *)c->d.counters, gate);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
@@ -2133,18 +2155,20 @@ static int solve_condensed(hmec_ctx *c, double *vs) {
 }
 
 // E = -A_DD^-1 A_DR of the current matrix
-static int darcy_e(hmec_ctx *c, hipStream_t st, int *bad) {
+// E, the Darcy coefficients and the A_RD right-hand-side coefficients of the
+// current matrix; va / vsc given: also vout = va * vsc over N entries
+static int darcy_e(hmec_ctx *c, hipStream_t st, int *bad, const double *va, const double *vsc,
+                   double *vout) {
     const CondPlan &cp = c->cp;
     CondDev &q = c->dc;
-    hipLaunchKernelGGL(k_darcy_e, dim3((unsigned)((cp.ND + 255) / 256)), dim3(256), 0, st, c->d.val,
-                       cp.ND, q.ddiag, q.dnbdiag, q.dnbe, q.des, q.e, q.dcoef, bad);
-    LAUNCH_CHECK();
     const int64_t nt = cp.rptr.empty() ? 0 : cp.rptr.back();
-    if (nt > 0) {
-        hipLaunchKernelGGL(k_cond_coef, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st,
-                           c->d.val, q.raslot, nt, q.rcoef);
-        LAUNCH_CHECK();
-    }
+    const int64_t Nv = va ? c->N : 0;
+    const int64_t tot = cp.ND + nt + Nv;
+    if (tot == 0) return HMEC_OK;
+    hipLaunchKernelGGL(k_darcy_e, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, c->d.val,
+                       cp.ND, q.ddiag, q.dnbdiag, q.dnbe, q.des, q.e, q.dcoef, bad, q.raslot, nt, q.rcoef,
+                       va, vsc, vout, Nv);
+    LAUNCH_CHECK();
     return HMEC_OK;
 }
 
@@ -2320,6 +2344,7 @@ struct GmresBuf {
     double *V = nullptr, *Z = nullptr, *w = nullptr, *part = nullptr, *hdev = nullptr;
     double *bs = nullptr, *rsd = nullptr;   // Dr b, Dr (b - A x)
     double *hhost = nullptr;
+    hipEvent_t ev_first = nullptr;          // the device-decided first step's outcome on the host
     int m = 0;
     int64_t N = 0;
 };
@@ -2334,6 +2359,7 @@ void sv_gmres_free(hmec_ctx *c) {
     hipFree(g.V); hipFree(g.Z); hipFree(g.w); hipFree(g.part); hipFree(g.hdev);
     hipFree(g.bs); hipFree(g.rsd);
     if (g.hhost) hipHostFree(g.hhost);
+    if (g.ev_first) hipEventDestroy(g.ev_first);
     g_gm.erase(it);
 }
 
@@ -2354,9 +2380,10 @@ static int gm_buf(hmec_ctx *c, int m, GmresBuf **out) {
         HMEC_HIP(hipMalloc(&g.rsd, sizeof(double) * c->N));
         const int64_t nbs = std::max<int64_t>(RB, (c->N + SPMV_ROWS - 1) / SPMV_ROWS);
         HMEC_HIP(hipMalloc(&g.part, sizeof(double) * nbs * (m + 2)));
-        // three Hessenberg columns, then the residual norms of a cycle start
-        HMEC_HIP(hipMalloc(&g.hdev, sizeof(double) * (3 * (m + 2) + 8)));
-        HMEC_HIP(hipHostMalloc(&g.hhost, sizeof(double) * (3 * (m + 2) + 8), hipHostMallocDefault));
+        // three Hessenberg columns, the residual norms of a cycle start, the
+        // closing residual norms and the first step's coefficient (device path)
+        HMEC_HIP(hipMalloc(&g.hdev, sizeof(double) * (3 * (m + 2) + 24)));
+        HMEC_HIP(hipHostMalloc(&g.hhost, sizeof(double) * (3 * (m + 2) + 24), hipHostMallocDefault));
     }
     *out = &g;
     return HMEC_OK;
@@ -2395,13 +2422,14 @@ static int gm_precond(hmec_ctx *c, const double *v, double *zout) {
 // Relative inf-norm residuals of x: unscaled ||b - A x|| / ||b|| (the
 // reported one) and row-equilibrated ||Dr (b - A x)|| / ||Dr b||; leaves
 // Dr (b - A x) in g.rsd (the GMRES residual)
-static int gm_residual_launch(hmec_ctx *c, GmresBuf &g, const double *x, double *out) {
-    launch_residual(c, x, c->d.work, g.rsd, c->d.rscale);
+static int gm_residual_launch(hmec_ctx *c, GmresBuf &g, const double *x, double *out,
+                              const double *gate = nullptr) {
+    launch_residual(c, x, c->d.work, g.rsd, c->d.rscale, gate);
     LAUNCH_CHECK();
-    return launch_norms4(c, c->d.work, c->d.rhs, g.rsd, g.bs, out);
+    return launch_norms4(c, c->d.work, c->d.rhs, g.rsd, g.bs, out, gate);
 }
 // (h[5], h[6]: perturbed and unrepairable pivots of the current factor)
-constexpr int GM_NRM = 7;
+constexpr int GM_NRM = 7;   // residual norms of launch_norms4 (with the pivot counts)
 static void gm_residual_read(const double *h, double *rel_u, double *rel_s, double *rr, int *npert,
                              int *nbad) {
     *rel_u = h[1] > 0 ? h[0] / h[1] : h[0];
@@ -2436,10 +2464,223 @@ int sv_lincomb(hmec_ctx *c, double *y, double a, const double *u, double b, cons
     return HMEC_OK;
 }
 
-int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
+// The first Arnoldi step of a solve decided on the device (hmec_step's
+// Picard loop enqueues the rest of the iteration behind it without waiting):
+//   GM_LEAD_OK   the start already meets the tolerance, x is kept;
+//   GM_COMBINE   the step's residual estimate meets its target: x += y0 Z0,
+//                then k_gm_verify checks the true residual;
+//   GM_CONTINUE  more Arnoldi steps are needed (the host continues the cycle);
+//   GM_HOLD      an unrepairable pivot or a zero residual norm: the host
+//                decides, as sv_gmres does;
+//   GM_DONE      verified: both residuals meet the tolerance;
+//   GM_RESTART   the true residual misses it: the host starts the next cycle.
+// GM_CONTINUE, GM_HOLD and GM_RESTART set the exit flag to hold, so the update
+// and the next prologue enqueued behind the solve do nothing until the host
+// has finished the solve and released the flag.
+enum GmState { GM_NONE = 0, GM_LEAD_OK = 1, GM_COMBINE = 2, GM_CONTINUE = 3, GM_HOLD = 4,
+               GM_DONE = 5, GM_RESTART = 6 };
+static_assert(GM_LEAD_OK == kGmLeadOk && GM_CONTINUE == kGmContinue && GM_HOLD == kGmHold &&
+                  GM_DONE == kGmDone && GM_RESTART == kGmRestart && GM_COMBINE == GM_GATE_COMBINE,
+              "GmState values");
+
+// nrm: |r|inf, |b|inf, |Dr r|inf, |Dr b|inf, |Dr r|2^2, perturbed pivots, unrepairable pivots
+__device__ __forceinline__ void gm_rel(const double *nrm, double *rel_u, double *rel_s) {
+    *rel_u = nrm[1] > 0 ? nrm[0] / nrm[1] : nrm[0];
+    *rel_s = nrm[3] > 0 ? nrm[2] / nrm[3] : nrm[2];
+}
+// h: the first Hessenberg column (first and second Gram-Schmidt passes, the
+// squared norm of the new vector) M2 apart, then the start's norms at 3 M2;
+// y0 at 3 M2 + 16
+__global__ void k_gm_step0(double *h, int M2, double tol, double *scal) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double *nrm = h + 3 * M2;
+    double relinf, rel_s;
+    gm_rel(nrm, &relinf, &rel_s);
+    // (the larger as std::max takes it: a NaN first operand stays)
+    const double worst = relinf < rel_s ? rel_s : relinf, beta = sqrt(nrm[4]);
+    int st;
+    if ((int)nrm[6] > 0) {
+        st = GM_HOLD;
+    } else if (!(worst > tol)) {
+        // a NaN norm lands here and is left to the host (not converged)
+        st = (relinf <= tol && rel_s <= tol) ? GM_LEAD_OK : GM_HOLD;
+    } else if (!(beta > 0)) {
+        st = GM_HOLD;
+    } else {
+        const double target = beta * (tol / worst) * 0.5;
+        const double h0 = h[0] + h[M2], hn = sqrt(h[2 * M2]);
+        const double den = hypot(h0, hn);
+        const double cs = den > 0 ? h0 / den : 1.0, sn = den > 0 ? hn / den : 0.0;
+        const double gv1 = -sn * beta, gv0 = cs * beta;
+        if (fabs(gv1) <= target || !(hn > 0)) {
+            h[3 * M2 + 16] = gv0 / den;
+            st = GM_COMBINE;
+        } else {
+            st = GM_CONTINUE;
+        }
+    }
+    scal[SC_GMSTATE_SLOT] = st;
+    if (st == GM_CONTINUE || st == GM_HOLD) scal[SC_EXIT_SLOT] = SC_EXIT_HOLD;
+}
+// x += y0 Z0 when the first step met its target
+__global__ void k_gm_combine0(const double *Z0, const double *y0, const double *scal, double *x,
+                              int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N || scal[SC_GMSTATE_SLOT] != GM_COMBINE) return;
+    double v = x[t];
+    v += y0[0] * Z0[t];
+    x[t] = v;
+}
+// the true residual of the combined x (norms nrm2) against the tolerance
+__global__ void k_gm_verify(const double *nrm2, double tol, double *scal) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || scal[SC_GMSTATE_SLOT] != GM_COMBINE) return;
+    double relinf, rel_s;
+    gm_rel(nrm2, &relinf, &rel_s);
+    if (relinf <= tol && rel_s <= tol) {
+        scal[SC_GMSTATE_SLOT] = GM_DONE;
+    } else {
+        scal[SC_GMSTATE_SLOT] = GM_RESTART;
+        scal[SC_EXIT_SLOT] = SC_EXIT_HOLD;
+    }
+}
+
+// operator scaling, E and Dr b of the current matrix (once per solve)
+static int gm_setup(hmec_ctx *c, GmresBuf &g) {
+    hipStream_t st = c->stream;
+    const int64_t N = c->N;
+    if (!c->rscale_ready) {   // the assembly forms it with the values
+        hipLaunchKernelGGL(k_row_scale, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, c->d.rowp,
+                           c->d.val, N, c->d.rscale, nullptr);
+        LAUNCH_CHECK();
+        c->rscale_ready = true;
+    }
+    return darcy_e(c, st, nullptr, c->d.rhs, c->d.rscale, g.bs);   // and bs = Dr b
+}
+
+// One Arnoldi step j of the current cycle (V_j given): Z_j = M^-1 V_j, the
+// twice-orthogonalized A Z_j with its coefficients left on the device, the
+// next basis vector, and the Hessenberg column copied to the host (with the
+// cycle start's norms when nrm is set); not waited for
+static int gm_arnoldi(hmec_ctx *c, GmresBuf &g, int j, int m, bool nrm) {
+    hipStream_t st = c->stream;
+    const int64_t N = c->N;
+    const unsigned gN = (unsigned)((N + 255) / 256);
+    const int M2 = m + 2;
+    double *Zj = g.Z + (int64_t)j * N;
+    HMEC_TRY(gm_precond(c, g.V + (int64_t)j * N, Zj));
+    // classical Gram-Schmidt, twice, coefficients kept on the device;
+    // one copy of the Hessenberg column per iteration.  The first
+    // pass's dot products come with the operator product, the second
+    // pass's with ||w'||^2, and its update writes the next basis vector.
+    double *h1d = g.hdev, *h2d = g.hdev + M2, *wwd = g.hdev + 2 * M2;
+    const unsigned nbs = (unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS);
+    launch_spmv_rs_dots(c, Zj, g.w, g.V, j + 1, g.part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dots_final, dim3(j + 1), dim3(256), 0, st, g.part, (int)nbs, j + 1, h1d);
+    LAUNCH_CHECK();
+#define GS_GO(NVM)                                                                          \
+    hipLaunchKernelGGL(k_gs_update_dots<NVM>, dim3(RB), dim3(256), 0, st, g.V, N, j + 1, h1d, g.w, \
+                       N, g.part)
+    if (j + 1 <= 4) GS_GO(4);
+    else if (j + 1 <= 8) GS_GO(8);
+    else if (j + 1 <= 16) GS_GO(16);
+    else GS_GO(GM_MAXV);
+#undef GS_GO
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_dots_final, dim3(j + 2), dim3(256), 0, st, g.part, RB, j + 2, h2d);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gs_update_norm, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h2d, g.w,
+                       j + 1 < m ? g.V + (int64_t)(j + 1) * N : nullptr, wwd, N);
+    LAUNCH_CHECK();
+    HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * (3 * M2 + (nrm ? GM_NRM : 0)),
+                            hipMemcpyDeviceToHost, st));
+    return HMEC_OK;
+}
+
+constexpr int GM_M = 20;   // restart length (<= 31: GmCoef and the fused dots hold the coefficients)
+static_assert(GM_M + 1 <= GM_MAXV, "restart length above the fused dot-product width");
+
+// Enqueues a solve's start and its first Arnoldi step with the step's
+// decision on the device (GmState, in the device scalars); nothing waits.
+// The host reads the outcome with the scalars and completes the solve with
+// sv_gmres(..., resume) when the state asks for it.
+int sv_gmres_first(hmec_ctx *c, double tol) {
     if (!c->factor_valid) return HMEC_E_ARG;
-    const int m = 20;               // restart length (<= 31: GmCoef and the fused dots hold the coefficients)
-    static_assert(20 + 1 <= GM_MAXV, "restart length above the fused dot-product width");
+    GmresBuf *gp;
+    HMEC_TRY(gm_buf(c, GM_M, &gp));
+    GmresBuf &g = *gp;
+    const int64_t N = c->N;
+    const unsigned gN = (unsigned)((N + 255) / 256);
+    hipStream_t st = c->stream;
+    const int M2 = GM_M + 2;
+    double *nrm_d = g.hdev + 3 * M2, *nrm2_d = g.hdev + 3 * M2 + 8;
+    HMEC_TRY(gm_setup(c, g));
+    HMEC_TRY(gm_residual_launch(c, g, c->d.sol, nrm_d));
+    hipLaunchKernelGGL(k_vscale_rr, dim3(gN), dim3(256), 0, st, g.rsd, nrm_d + 4, g.V, N);
+    LAUNCH_CHECK();
+    HMEC_TRY(gm_arnoldi(c, g, 0, GM_M, true));
+    hipLaunchKernelGGL(k_gm_step0, dim3(1), dim3(64), 0, st, g.hdev, M2, tol, c->d.scal);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gm_combine0, dim3(gN), dim3(256), 0, st, (const double *)g.Z,
+                       (const double *)(g.hdev + 3 * M2 + 16), (const double *)c->d.scal, c->d.sol, N);
+    LAUNCH_CHECK();
+    HMEC_TRY(gm_residual_launch(c, g, c->d.sol, nrm2_d, c->d.scal));   // only after a combine
+    hipLaunchKernelGGL(k_gm_verify, dim3(1), dim3(64), 0, st, (const double *)nrm2_d, tol, c->d.scal);
+    LAUNCH_CHECK();
+    HMEC_HIP(hipMemcpyAsync(g.hhost + 3 * M2 + 8, nrm2_d, sizeof(double) * GM_NRM,
+                            hipMemcpyDeviceToHost, st));
+    HMEC_HIP(hipMemcpyAsync(g.hhost + 3 * M2 + 20, c->d.scal + SC_GMSTATE_SLOT, sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    if (!g.ev_first) HMEC_HIP(hipEventCreateWithFlags(&g.ev_first, hipEventDisableTiming));
+    HMEC_HIP(hipEventRecord(g.ev_first, st));
+    return HMEC_OK;
+}
+
+// Waits for the outcome of sv_gmres_first (before the work enqueued behind
+// it has run) and returns its GmState
+int sv_gmres_first_state(hmec_ctx *c, int *state) {
+    GmresBuf *gp;
+    HMEC_TRY(gm_buf(c, GM_M, &gp));
+    if (!gp->ev_first) return HMEC_E_ARG;
+    HMEC_HIP(hipEventSynchronize(gp->ev_first));
+    *state = (int)gp->hhost[3 * (GM_M + 2) + 20];
+    return HMEC_OK;
+}
+
+// The outcome of sv_gmres_first for a state that needs no host work
+// (GM_LEAD_OK, GM_DONE), after the host has waited for the scalars
+int sv_gmres_first_result(hmec_ctx *c, int state, hmec_solve_stats *out) {
+    GmresBuf *gp;
+    HMEC_TRY(gm_buf(c, GM_M, &gp));
+    const int M2 = GM_M + 2;
+    const double *h = gp->hhost + 3 * M2 + (state == GM_DONE ? 8 : 0);
+    double relinf, rel_s, rr;
+    int npert, nbad;
+    gm_residual_read(h, &relinf, &rel_s, &rr, &npert, &nbad);
+    const int it = state == GM_DONE ? 1 : 0;
+    c->last_ir = it;
+    c->last_res = relinf;
+    if (out) {
+        out->ir_iterations = it;
+        out->rel_residual = relinf;
+        out->perturbed_pivots = npert;
+        out->factor_flops = c->plan.flops;
+        out->factor_nnz = c->plan.factor_nnz;
+        out->nfronts = c->plan.nfronts;
+        out->nlevels = c->plan.nlevels;
+        out->max_front = c->plan.max_front;
+    }
+    c->stale_solves++;
+    return HMEC_OK;
+}
+
+// resume: 0 a fresh solve; GM_CONTINUE / GM_RESTART / GM_HOLD the solve
+// sv_gmres_first left to the host (its setup done; CONTINUE: the first
+// Hessenberg column and the start's norms already in the host buffer;
+// RESTART: one iteration done, the next cycle starts from the true residual)
+int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out, int resume) {
+    if (!c->factor_valid) return HMEC_E_ARG;
+    const int m = GM_M;
     GmresBuf *gp;
     HMEC_TRY(gm_buf(c, m, &gp));
     GmresBuf &g = *gp;
@@ -2448,34 +2689,28 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
     hipStream_t st = c->stream;
     double *x = c->d.sol;
     double *r = g.rsd;
-    int total_it = 0, npert = 0, nbad = 0;
+    int total_it = resume == GM_RESTART ? 1 : 0, npert = 0, nbad = 0;
     double relinf = 0, last_rel_s = 0;
     std::vector<double> H((m + 1) * m), cs(m), sn(m), gv(m + 1), h(m + 2), h2(m + 2), y(m);
-    // the operator's row scaling (powers of two) and E of the current matrix
-    if (!c->rscale_ready) {   // the assembly forms it with the values
-        hipLaunchKernelGGL(k_row_scale, dim3(gN), dim3(256), 0, st, c->d.rowp, c->d.val, N, c->d.rscale,
-                           nullptr);
-        LAUNCH_CHECK();
-        c->rscale_ready = true;
-    }
-    HMEC_TRY(darcy_e(c, st));
-    hipLaunchKernelGGL(k_scale_vec, dim3(gN), dim3(256), 0, st, c->d.rhs, c->d.rscale, g.bs, N);
-    LAUNCH_CHECK();
+    if (resume == 0) HMEC_TRY(gm_setup(c, g));
     // The first cycle does not wait for its starting residual: the first
     // basis vector is scaled on the device and the residual norms come back
     // with the first Hessenberg column.  A start that already meets the
     // tolerance discards that iteration (x is not touched).
     const int M2 = m + 2;
     double *const nrm_d = g.hdev + 3 * M2;
-    bool lead = total_it < max_it, done = false;
+    bool lead = resume != GM_RESTART && total_it < max_it, done = false;
+    bool have0 = resume == GM_CONTINUE;   // the first column is already on the host
     for (;;) {
         double rel_s = 0, rr = 0;   // rr = ||Dr (b - A x)||_2^2 from the same pass
         double target = 0;
         std::fill(gv.begin(), gv.end(), 0.0);
         if (lead) {
-            HMEC_TRY(gm_residual_launch(c, g, x, nrm_d));
-            hipLaunchKernelGGL(k_vscale_rr, dim3(gN), dim3(256), 0, st, r, nrm_d + 4, g.V, N);
-            LAUNCH_CHECK();
+            if (!have0) {
+                HMEC_TRY(gm_residual_launch(c, g, x, nrm_d));
+                hipLaunchKernelGGL(k_vscale_rr, dim3(gN), dim3(256), 0, st, r, nrm_d + 4, g.V, N);
+                LAUNCH_CHECK();
+            }
         } else {
             HMEC_TRY(gm_true_residual(c, g, x, &relinf, &rel_s, &rr, &npert, &nbad));
             last_rel_s = rel_s;
@@ -2490,36 +2725,11 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
         }
         int j = 0;
         for (; j < m && total_it < max_it; j++) {
-            double *Zj = g.Z + (int64_t)j * N;
-            HMEC_TRY(gm_precond(c, g.V + (int64_t)j * N, Zj));
-            // classical Gram-Schmidt, twice, coefficients kept on the device;
-            // one copy of the Hessenberg column per iteration.  The first
-            // pass's dot products come with the operator product, the second
-            // pass's with ||w'||^2, and its update writes the next basis vector.
-            double *h1d = g.hdev, *h2d = g.hdev + M2, *wwd = g.hdev + 2 * M2;
-            const unsigned nbs = (unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS);
-            launch_spmv_rs_dots(c, Zj, g.w, g.V, j + 1, g.part);
-            LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_dots_final, dim3(j + 1), dim3(256), 0, st, g.part, (int)nbs, j + 1, h1d);
-            LAUNCH_CHECK();
-#define GS_GO(NVM)                                                                          \
-    hipLaunchKernelGGL(k_gs_update_dots<NVM>, dim3(RB), dim3(256), 0, st, g.V, N, j + 1, h1d, g.w, \
-                       N, g.part)
-            if (j + 1 <= 4) GS_GO(4);
-            else if (j + 1 <= 8) GS_GO(8);
-            else if (j + 1 <= 16) GS_GO(16);
-            else GS_GO(GM_MAXV);
-#undef GS_GO
-            LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_dots_final, dim3(j + 2), dim3(256), 0, st, g.part, RB, j + 2, h2d);
-            LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_gs_update_norm, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h2d, g.w,
-                               j + 1 < m ? g.V + (int64_t)(j + 1) * N : nullptr, wwd, N);
-            LAUNCH_CHECK();
+            if (!(have0 && j == 0)) {
+                HMEC_TRY(gm_arnoldi(c, g, j, m, lead && j == 0));
+                HMEC_HIP(hipStreamSynchronize(st));
+            }
             const bool first = lead && j == 0;
-            HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * (3 * M2 + (first ? GM_NRM : 0)),
-                                    hipMemcpyDeviceToHost, st));
-            HMEC_HIP(hipStreamSynchronize(st));
             if (first) {
                 gm_residual_read(g.hhost + 3 * M2, &relinf, &rel_s, &rr, &npert, &nbad);
                 last_rel_s = rel_s;
@@ -2557,6 +2767,7 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out) {
             }
         }
         lead = false;
+        have0 = false;
         if (done || j == 0) break;
         // y = R^-1 g ; x += Z y
         for (int i = j - 1; i >= 0; i--) {
