@@ -1025,6 +1025,32 @@ k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
         }
         return;
     }
+    if (MODE == 1 && part) {
+        // the residual's norms in the same pass: per workgroup |y|inf,
+        // |b|inf, |rs y|inf, |rs b|inf and (rs y).(rs y) -> part[5 blk + q]
+        // (nan_max: a NaN anywhere reaches the result)
+        __shared__ double red[5][4];
+        double m[5] = {0, 0, 0, 0, 0};
+        if (t < nr) {
+            const int64_t r = r0 + t;
+            const double v = b[r] - s, w = rs[r] * v;
+            y[r] = v;
+            y2[r] = w;
+            m[0] = fabs(v);
+            m[1] = fabs(b[r]);
+            m[2] = fabs(w);
+            m[3] = fabs(b[r] * rs[r]);
+            m[4] = w * w;
+        }
+        const auto mx = [](double u, double v) { return nan_max(u, v); };
+        const auto add = [](double u, double v) { return u + v; };
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            const double v = q == 4 ? block_total256(m[q], add, red[q]) : block_total256(m[q], mx, red[q]);
+            if (t == 0) part[5 * blk + q] = v;
+        }
+        return;
+    }
     if (t >= nr) return;
     const int64_t r = r0 + t;
     if (MODE == 0) {
@@ -2422,11 +2448,20 @@ static int gm_precond(hmec_ctx *c, const double *v, double *zout) {
 // Relative inf-norm residuals of x: unscaled ||b - A x|| / ||b|| (the
 // reported one) and row-equilibrated ||Dr (b - A x)|| / ||Dr b||; leaves
 // Dr (b - A x) in g.rsd (the GMRES residual)
+// (the residual kernel forms the norms' block partials with the residual,
+// one more launch reduces them)
 static int gm_residual_launch(hmec_ctx *c, GmresBuf &g, const double *x, double *out,
                               const double *gate = nullptr) {
-    launch_residual(c, x, c->d.work, g.rsd, c->d.rscale, gate);
+    const int64_t N = c->N;
+    const int nblk = (int)((N + SPMV_ROWS - 1) / SPMV_ROWS);
+    hipLaunchKernelGGL(k_csr_stream<1>, dim3((unsigned)xcd_grid(nblk)), dim3(SPMV_ROWS), 0, c->stream,
+                       c->d.rowp, c->d.coli, c->d.val, x, c->d.rscale, c->d.rhs, c->d.work, g.rsd, N,
+                       (const double *)nullptr, (int64_t)0, 0, g.part, gate);
     LAUNCH_CHECK();
-    return launch_norms4(c, c->d.work, c->d.rhs, g.rsd, g.bs, out, gate);
+    hipLaunchKernelGGL((k_norms_final<4, true>), dim3(1), dim3(256), 0, c->stream, (const double *)g.part,
+                       nblk, out, (const int *)c->d.counters, gate);
+    LAUNCH_CHECK();
+    return HMEC_OK;
 }
 // (h[5], h[6]: perturbed and unrepairable pivots of the current factor)
 constexpr int GM_NRM = 7;   // residual norms of launch_norms4 (with the pivot counts)
