@@ -970,7 +970,8 @@ constexpr int SPMV_CHUNK = 2048;
 // when the step's state in the device scalars is this one (GM_COMBINE)
 constexpr int GM_GATE_COMBINE = 2;
 // MODE 2: y = rs * (A x) and, per workgroup, the partial dot products of its
-// rows of y with V_0..V_{nv-1} (columns ldv apart) -> part[block * nv + i].
+// rows of y with V_0..V_{nv-1} (columns ldv apart) -> part[block * nq + i],
+// nq = nv, or nv + 1 with y . y as the last when sq is set.
 constexpr int GM_MAXV = 32;
 template <int MODE>
 __global__ void __launch_bounds__(SPMV_ROWS)
@@ -979,7 +980,7 @@ k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
              const double *__restrict__ rs, const double *__restrict__ b, double *__restrict__ y,
              double *__restrict__ y2, int64_t N, const double *__restrict__ V = nullptr,
              int64_t ldv = 0, int nv = 0, double *__restrict__ part = nullptr,
-             const double *__restrict__ gate = nullptr) {
+             const double *__restrict__ gate = nullptr, int sq = 0) {
     __shared__ double prod[SPMV_CHUNK];
     __shared__ int32_t rp[SPMV_ROWS + 1];
     if (gate && gate[SC_GMSTATE_SLOT] != GM_GATE_COMBINE) return;
@@ -1011,17 +1012,18 @@ k_csr_stream(const int32_t *__restrict__ rowp, const int32_t *__restrict__ coli,
             yv = rs[r0 + t] * s;
             y[r0 + t] = yv;
         }
-        const int lane = t & 63, wv = t >> 6;
-        for (int i = 0; i < nv; i++) {
-            const double d = group_total<64>(t < nr ? yv * V[(int64_t)i * ldv + r0 + t] : 0.0);
+        const int lane = t & 63, wv = t >> 6, nq = nv + (sq ? 1 : 0);
+        for (int i = 0; i < nq; i++) {
+            const double d = group_total<64>(
+                t < nr ? yv * (i < nv ? V[(int64_t)i * ldv + r0 + t] : yv) : 0.0);
             if (lane == 63) red[wv][i] = d;
         }
         __syncthreads();
-        if (t < nv) {
+        if (t < nq) {
             double d = 0.0;
 #pragma unroll
             for (int q = 0; q < SPMV_ROWS / 64; q++) d += red[q][t];
-            part[blk * nv + t] = d;
+            part[blk * nq + t] = d;
         }
         return;
     }
@@ -1072,11 +1074,12 @@ static void launch_residual(hmec_ctx *c, const double *x, double *y, double *y2,
 }
 // y = rs * (A x) with the partial dot products of y with V_0..V_{nv-1}
 static void launch_spmv_rs_dots(hmec_ctx *c, const double *x, double *y, const double *V,
-                                int nv, double *part) {
+                                int nv, double *part, int sq = 0) {
     const int64_t N = c->N;
     hipLaunchKernelGGL(k_csr_stream<2>, dim3((unsigned)xcd_grid((N + SPMV_ROWS - 1) / SPMV_ROWS)),
                        dim3(SPMV_ROWS), 0, c->stream, c->d.rowp, c->d.coli, c->d.val, x,
-                       c->d.rscale, nullptr, y, nullptr, N, V, N, nv, part);
+                       c->d.rscale, nullptr, y, nullptr, N, V, N, nv, part,
+                       (const double *)nullptr, sq);
 }
 // y = rs * (A x)
 static void launch_spmv_rs(hmec_ctx *c, const double *x, double *y) {
@@ -2525,7 +2528,8 @@ __device__ __forceinline__ void gm_rel(const double *nrm, double *rel_u, double 
 }
 // h: the first Hessenberg column (first and second Gram-Schmidt passes, the
 // squared norm of the new vector) M2 apart, then the start's norms at 3 M2;
-// y0 at 3 M2 + 16
+// y0 at 3 M2 + 16; the GmState at 3 M2 + 20 (beside the device scalars' copy,
+// so one copy of h brings the step's whole outcome to the host)
 __global__ void k_gm_step0(double *h, int M2, double tol, double *scal) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double *nrm = h + 3 * M2;
@@ -2542,8 +2546,12 @@ __global__ void k_gm_step0(double *h, int M2, double tol, double *scal) {
     } else if (!(beta > 0)) {
         st = GM_HOLD;
     } else {
+        // from w . V_0 and w . w of the operator product alone (no
+        // Gram-Schmidt pass): |w'|^2 = w . w - (w . V_0)^2.  Its cancellation
+        // only blurs the estimate below ~1e-8 |w|; k_gm_verify checks the
+        // true residual of every combined step.
         const double target = beta * (tol / worst) * 0.5;
-        const double h0 = h[0] + h[M2], hn = sqrt(h[2 * M2]);
+        const double h0 = h[0], hn = sqrt(fmax(h[1] - h0 * h0, 0.0));
         const double den = hypot(h0, hn);
         const double cs = den > 0 ? h0 / den : 1.0, sn = den > 0 ? hn / den : 0.0;
         const double gv1 = -sn * beta, gv0 = cs * beta;
@@ -2555,6 +2563,7 @@ __global__ void k_gm_step0(double *h, int M2, double tol, double *scal) {
         }
     }
     scal[SC_GMSTATE_SLOT] = st;
+    h[3 * M2 + 20] = st;
     if (st == GM_CONTINUE || st == GM_HOLD) scal[SC_EXIT_SLOT] = SC_EXIT_HOLD;
 }
 // x += y0 Z0 when the first step met its target
@@ -2566,17 +2575,16 @@ __global__ void k_gm_combine0(const double *Z0, const double *y0, const double *
     v += y0[0] * Z0[t];
     x[t] = v;
 }
-// the true residual of the combined x (norms nrm2) against the tolerance
-__global__ void k_gm_verify(const double *nrm2, double tol, double *scal) {
+// the true residual of the combined x (norms at h + 3 M2 + 8) against the
+// tolerance
+__global__ void k_gm_verify(double *h, int M2, double tol, double *scal) {
     if (threadIdx.x != 0 || blockIdx.x != 0 || scal[SC_GMSTATE_SLOT] != GM_COMBINE) return;
     double relinf, rel_s;
-    gm_rel(nrm2, &relinf, &rel_s);
-    if (relinf <= tol && rel_s <= tol) {
-        scal[SC_GMSTATE_SLOT] = GM_DONE;
-    } else {
-        scal[SC_GMSTATE_SLOT] = GM_RESTART;
-        scal[SC_EXIT_SLOT] = SC_EXIT_HOLD;
-    }
+    gm_rel(h + 3 * M2 + 8, &relinf, &rel_s);
+    const int st = (relinf <= tol && rel_s <= tol) ? GM_DONE : GM_RESTART;
+    scal[SC_GMSTATE_SLOT] = st;
+    h[3 * M2 + 20] = st;
+    if (st == GM_RESTART) scal[SC_EXIT_SLOT] = SC_EXIT_HOLD;
 }
 
 // operator scaling, E and Dr b of the current matrix (once per solve)
@@ -2595,11 +2603,16 @@ static int gm_setup(hmec_ctx *c, GmresBuf &g) {
 // One Arnoldi step j of the current cycle (V_j given): Z_j = M^-1 V_j, the
 // twice-orthogonalized A Z_j with its coefficients left on the device, the
 // next basis vector, and the Hessenberg column copied to the host (with the
-// cycle start's norms when nrm is set); not waited for
-static int gm_arnoldi(hmec_ctx *c, GmresBuf &g, int j, int m, bool nrm) {
+// cycle start's norms when nrm is set; no copy when copy is unset); not
+// waited for.  Without gs only Z_j, w = A Z_j and its first-pass dot
+// products with V_0..V_j plus w . w (at h[j + 1]) are formed: the first step
+// of a solve decided on the device (k_gm_step0) needs no more, and gm_gs
+// completes the step when the cycle goes on.
+static int gm_gs(hmec_ctx *c, GmresBuf &g, int j, int m, bool nrm, bool copy);
+static int gm_arnoldi(hmec_ctx *c, GmresBuf &g, int j, int m, bool nrm, bool copy = true,
+                      bool gs = true) {
     hipStream_t st = c->stream;
     const int64_t N = c->N;
-    const unsigned gN = (unsigned)((N + 255) / 256);
     const int M2 = m + 2;
     double *Zj = g.Z + (int64_t)j * N;
     HMEC_TRY(gm_precond(c, g.V + (int64_t)j * N, Zj));
@@ -2607,12 +2620,24 @@ static int gm_arnoldi(hmec_ctx *c, GmresBuf &g, int j, int m, bool nrm) {
     // one copy of the Hessenberg column per iteration.  The first
     // pass's dot products come with the operator product, the second
     // pass's with ||w'||^2, and its update writes the next basis vector.
-    double *h1d = g.hdev, *h2d = g.hdev + M2, *wwd = g.hdev + 2 * M2;
+    double *h1d = g.hdev;
     const unsigned nbs = (unsigned)((N + SPMV_ROWS - 1) / SPMV_ROWS);
-    launch_spmv_rs_dots(c, Zj, g.w, g.V, j + 1, g.part);
+    const int nq = j + 1 + (gs ? 0 : 1);
+    launch_spmv_rs_dots(c, Zj, g.w, g.V, j + 1, g.part, gs ? 0 : 1);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_dots_final, dim3(j + 1), dim3(256), 0, st, g.part, (int)nbs, j + 1, h1d);
+    hipLaunchKernelGGL(k_dots_final, dim3(nq), dim3(256), 0, st, g.part, (int)nbs, nq, h1d);
     LAUNCH_CHECK();
+    if (!gs) return HMEC_OK;
+    return gm_gs(c, g, j, m, nrm, copy);
+}
+// The Gram-Schmidt passes of Arnoldi step j after its operator product and
+// first-pass dot products (gm_arnoldi)
+static int gm_gs(hmec_ctx *c, GmresBuf &g, int j, int m, bool nrm, bool copy) {
+    hipStream_t st = c->stream;
+    const int64_t N = c->N;
+    const unsigned gN = (unsigned)((N + 255) / 256);
+    const int M2 = m + 2;
+    double *h1d = g.hdev, *h2d = g.hdev + M2, *wwd = g.hdev + 2 * M2;
 #define GS_GO(NVM)                                                                          \
     hipLaunchKernelGGL(k_gs_update_dots<NVM>, dim3(RB), dim3(256), 0, st, g.V, N, j + 1, h1d, g.w, \
                        N, g.part)
@@ -2627,8 +2652,9 @@ static int gm_arnoldi(hmec_ctx *c, GmresBuf &g, int j, int m, bool nrm) {
     hipLaunchKernelGGL(k_gs_update_norm, dim3(gN), dim3(256), 0, st, g.V, N, j + 1, h2d, g.w,
                        j + 1 < m ? g.V + (int64_t)(j + 1) * N : nullptr, wwd, N);
     LAUNCH_CHECK();
-    HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * (3 * M2 + (nrm ? GM_NRM : 0)),
-                            hipMemcpyDeviceToHost, st));
+    if (copy)
+        HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * (3 * M2 + (nrm ? GM_NRM : 0)),
+                                hipMemcpyDeviceToHost, st));
     return HMEC_OK;
 }
 
@@ -2653,19 +2679,17 @@ int sv_gmres_first(hmec_ctx *c, double tol) {
     HMEC_TRY(gm_residual_launch(c, g, c->d.sol, nrm_d));
     hipLaunchKernelGGL(k_vscale_rr, dim3(gN), dim3(256), 0, st, g.rsd, nrm_d + 4, g.V, N);
     LAUNCH_CHECK();
-    HMEC_TRY(gm_arnoldi(c, g, 0, GM_M, true));
+    HMEC_TRY(gm_arnoldi(c, g, 0, GM_M, true, false, false));
     hipLaunchKernelGGL(k_gm_step0, dim3(1), dim3(64), 0, st, g.hdev, M2, tol, c->d.scal);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_gm_combine0, dim3(gN), dim3(256), 0, st, (const double *)g.Z,
                        (const double *)(g.hdev + 3 * M2 + 16), (const double *)c->d.scal, c->d.sol, N);
     LAUNCH_CHECK();
     HMEC_TRY(gm_residual_launch(c, g, c->d.sol, nrm2_d, c->d.scal));   // only after a combine
-    hipLaunchKernelGGL(k_gm_verify, dim3(1), dim3(64), 0, st, (const double *)nrm2_d, tol, c->d.scal);
+    hipLaunchKernelGGL(k_gm_verify, dim3(1), dim3(64), 0, st, g.hdev, M2, tol, c->d.scal);
     LAUNCH_CHECK();
-    HMEC_HIP(hipMemcpyAsync(g.hhost + 3 * M2 + 8, nrm2_d, sizeof(double) * GM_NRM,
-                            hipMemcpyDeviceToHost, st));
-    HMEC_HIP(hipMemcpyAsync(g.hhost + 3 * M2 + 20, c->d.scal + SC_GMSTATE_SLOT, sizeof(double),
-                            hipMemcpyDeviceToHost, st));
+    // the column, both residuals' norms, y0 and the state in one copy
+    HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * (3 * M2 + 21), hipMemcpyDeviceToHost, st));
     if (!g.ev_first) HMEC_HIP(hipEventCreateWithFlags(&g.ev_first, hipEventDisableTiming));
     HMEC_HIP(hipEventRecord(g.ev_first, st));
     return HMEC_OK;
@@ -2711,7 +2735,7 @@ int sv_gmres_first_result(hmec_ctx *c, int state, hmec_solve_stats *out) {
 
 // resume: 0 a fresh solve; GM_CONTINUE / GM_RESTART / GM_HOLD the solve
 // sv_gmres_first left to the host (its setup done; CONTINUE: the first
-// Hessenberg column and the start's norms already in the host buffer;
+// step's operator product and the start's norms on the device;
 // RESTART: one iteration done, the next cycle starts from the true residual)
 int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out, int resume) {
     if (!c->factor_valid) return HMEC_E_ARG;
@@ -2736,6 +2760,12 @@ int sv_gmres(hmec_ctx *c, int max_it, double tol, hmec_solve_stats *out, int res
     double *const nrm_d = g.hdev + 3 * M2;
     bool lead = resume != GM_RESTART && total_it < max_it, done = false;
     bool have0 = resume == GM_CONTINUE;   // the first column is already on the host
+    if (have0) {
+        // sv_gmres_first formed the first step's operator product only: its
+        // Gram-Schmidt passes and the next basis vector now
+        HMEC_TRY(gm_gs(c, g, 0, m, true, true));
+        HMEC_HIP(hipStreamSynchronize(st));
+    }
     for (;;) {
         double rel_s = 0, rr = 0;   // rr = ||Dr (b - A x)||_2^2 from the same pass
         double target = 0;
