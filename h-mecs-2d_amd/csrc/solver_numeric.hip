@@ -969,6 +969,7 @@ constexpr int SPMV_CHUNK = 2048;
 // gated launches (a solve's first GMRES step decided on the device) run only
 // when the step's state in the device scalars is this one (GM_COMBINE)
 constexpr int GM_GATE_COMBINE = 2;
+constexpr int GM_GATE_DONE = 5, GM_GATE_RESTART = 6;   // the closing check's outcomes
 // MODE 2: y = rs * (A x) and, per workgroup, the partial dot products of its
 // rows of y with V_0..V_{nv-1} (columns ldv apart) -> part[block * nq + i],
 // nq = nv, or nv + 1 with y . y as the last when sq is set.
@@ -1124,8 +1125,13 @@ __global__ void __launch_bounds__(256) k_norms_part(NormArgs<NV> a, int64_t N, d
 // (cnt: optional, out[NO] = cnt[0] and out[NO + 1] = cnt[1], so the counts of
 // perturbed and of unrepairable pivots come back with the norms)
 template <int NV, bool SQ = false>
+// (vscal: the closing check of a solve's first GMRES step decided on the
+// device: both relative residuals of out against vtol give GM_DONE, else
+// GM_RESTART with the exit flag held; the state also at *vstate)
 __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb, double *out,
-                                                     const int *cnt, const double *gate = nullptr) {
+                                                     const int *cnt, const double *gate = nullptr,
+                                                     double vtol = 0, double *vscal = nullptr,
+                                                     double *vstate = nullptr) {
     constexpr int NO = NV + (SQ ? 1 : 0);
     __shared__ double r[NO][4];
     if (gate && gate[SC_GMSTATE_SLOT] != GM_GATE_COMBINE) return;
@@ -1148,6 +1154,14 @@ __global__ void __launch_bounds__(256) k_norms_final(const double *part, int nb,
         if (cnt) {
             out[NO] = (double)cnt[0];
             out[NO + 1] = (double)cnt[1];
+        }
+        if (vscal) {
+            const double rel_u = out[1] > 0 ? out[0] / out[1] : out[0];
+            const double rel_s = out[3] > 0 ? out[2] / out[3] : out[2];
+            const int st = (rel_u <= vtol && rel_s <= vtol) ? GM_GATE_DONE : GM_GATE_RESTART;
+            vscal[SC_GMSTATE_SLOT] = st;
+            *vstate = st;
+            if (st == GM_GATE_RESTART) vscal[SC_EXIT_SLOT] = SC_EXIT_HOLD;
         }
     }
 }
@@ -2454,7 +2468,8 @@ static int gm_precond(hmec_ctx *c, const double *v, double *zout) {
 // (the residual kernel forms the norms' block partials with the residual,
 // one more launch reduces them)
 static int gm_residual_launch(hmec_ctx *c, GmresBuf &g, const double *x, double *out,
-                              const double *gate = nullptr) {
+                              const double *gate = nullptr, double vtol = 0,
+                              double *vstate = nullptr) {
     const int64_t N = c->N;
     const int nblk = (int)((N + SPMV_ROWS - 1) / SPMV_ROWS);
     hipLaunchKernelGGL(k_csr_stream<1>, dim3((unsigned)xcd_grid(nblk)), dim3(SPMV_ROWS), 0, c->stream,
@@ -2462,7 +2477,8 @@ static int gm_residual_launch(hmec_ctx *c, GmresBuf &g, const double *x, double 
                        (const double *)nullptr, (int64_t)0, 0, g.part, gate);
     LAUNCH_CHECK();
     hipLaunchKernelGGL((k_norms_final<4, true>), dim3(1), dim3(256), 0, c->stream, (const double *)g.part,
-                       nblk, out, (const int *)c->d.counters, gate);
+                       nblk, out, (const int *)c->d.counters, gate, vtol,
+                       vstate ? c->d.scal : (double *)nullptr, vstate);
     LAUNCH_CHECK();
     return HMEC_OK;
 }
@@ -2506,7 +2522,8 @@ int sv_lincomb(hmec_ctx *c, double *y, double a, const double *u, double b, cons
 // Picard loop enqueues the rest of the iteration behind it without waiting):
 //   GM_LEAD_OK   the start already meets the tolerance, x is kept;
 //   GM_COMBINE   the step's residual estimate meets its target: x += y0 Z0,
-//                then k_gm_verify checks the true residual;
+//                then the closing residual's norm combine checks the true
+//                residual;
 //   GM_CONTINUE  more Arnoldi steps are needed (the host continues the cycle);
 //   GM_HOLD      an unrepairable pivot or a zero residual norm: the host
 //                decides, as sv_gmres does;
@@ -2518,7 +2535,8 @@ int sv_lincomb(hmec_ctx *c, double *y, double a, const double *u, double b, cons
 enum GmState { GM_NONE = 0, GM_LEAD_OK = 1, GM_COMBINE = 2, GM_CONTINUE = 3, GM_HOLD = 4,
                GM_DONE = 5, GM_RESTART = 6 };
 static_assert(GM_LEAD_OK == kGmLeadOk && GM_CONTINUE == kGmContinue && GM_HOLD == kGmHold &&
-                  GM_DONE == kGmDone && GM_RESTART == kGmRestart && GM_COMBINE == GM_GATE_COMBINE,
+                  GM_DONE == kGmDone && GM_RESTART == kGmRestart && GM_COMBINE == GM_GATE_COMBINE &&
+                  GM_DONE == GM_GATE_DONE && GM_RESTART == GM_GATE_RESTART,
               "GmState values");
 
 // nrm: |r|inf, |b|inf, |Dr r|inf, |Dr b|inf, |Dr r|2^2, perturbed pivots, unrepairable pivots
@@ -2530,63 +2548,53 @@ __device__ __forceinline__ void gm_rel(const double *nrm, double *rel_u, double 
 // squared norm of the new vector) M2 apart, then the start's norms at 3 M2;
 // y0 at 3 M2 + 16; the GmState at 3 M2 + 20 (beside the device scalars' copy,
 // so one copy of h brings the step's whole outcome to the host)
-__global__ void k_gm_step0(double *h, int M2, double tol, double *scal) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The first step's decision from h (the same in every thread that takes it)
+__device__ __forceinline__ int gm_step0_eval(const double *h, int M2, double tol, double *y0) {
     const double *nrm = h + 3 * M2;
     double relinf, rel_s;
     gm_rel(nrm, &relinf, &rel_s);
     // (the larger as std::max takes it: a NaN first operand stays)
     const double worst = relinf < rel_s ? rel_s : relinf, beta = sqrt(nrm[4]);
-    int st;
-    if ((int)nrm[6] > 0) {
-        st = GM_HOLD;
-    } else if (!(worst > tol)) {
-        // a NaN norm lands here and is left to the host (not converged)
-        st = (relinf <= tol && rel_s <= tol) ? GM_LEAD_OK : GM_HOLD;
-    } else if (!(beta > 0)) {
-        st = GM_HOLD;
-    } else {
-        // from w . V_0 and w . w of the operator product alone (no
-        // Gram-Schmidt pass): |w'|^2 = w . w - (w . V_0)^2.  Its cancellation
-        // only blurs the estimate below ~1e-8 |w|; k_gm_verify checks the
-        // true residual of every combined step.
-        const double target = beta * (tol / worst) * 0.5;
-        const double h0 = h[0], hn = sqrt(fmax(h[1] - h0 * h0, 0.0));
-        const double den = hypot(h0, hn);
-        const double cs = den > 0 ? h0 / den : 1.0, sn = den > 0 ? hn / den : 0.0;
-        const double gv1 = -sn * beta, gv0 = cs * beta;
-        if (fabs(gv1) <= target || !(hn > 0)) {
-            h[3 * M2 + 16] = gv0 / den;
-            st = GM_COMBINE;
-        } else {
-            st = GM_CONTINUE;
-        }
+    *y0 = 0.0;
+    if ((int)nrm[6] > 0) return GM_HOLD;
+    // a NaN norm lands here and is left to the host (not converged)
+    if (!(worst > tol)) return (relinf <= tol && rel_s <= tol) ? GM_LEAD_OK : GM_HOLD;
+    if (!(beta > 0)) return GM_HOLD;
+    // from w . V_0 and w . w of the operator product alone (no Gram-Schmidt
+    // pass): |w'|^2 = w . w - (w . V_0)^2.  Its cancellation only blurs the
+    // estimate below ~1e-8 |w|; the true residual of every combined step is
+    // checked after it.
+    const double target = beta * (tol / worst) * 0.5;
+    const double h0 = h[0], hn = sqrt(fmax(h[1] - h0 * h0, 0.0));
+    const double den = hypot(h0, hn);
+    const double cs = den > 0 ? h0 / den : 1.0, sn = den > 0 ? hn / den : 0.0;
+    const double gv1 = -sn * beta, gv0 = cs * beta;
+    if (fabs(gv1) <= target || !(hn > 0)) {
+        *y0 = gv0 / den;
+        return GM_COMBINE;
     }
-    scal[SC_GMSTATE_SLOT] = st;
-    h[3 * M2 + 20] = st;
-    if (st == GM_CONTINUE || st == GM_HOLD) scal[SC_EXIT_SLOT] = SC_EXIT_HOLD;
+    return GM_CONTINUE;
 }
-// x += y0 Z0 when the first step met its target
-__global__ void k_gm_combine0(const double *Z0, const double *y0, const double *scal, double *x,
-                              int64_t N) {
+// The first step's decision and, when it meets its target, x += y0 Z0: every
+// thread evaluates the decision from h (uniform loads), thread 0 of block 0
+// records it (state in the device scalars and at h[3 M2 + 20], y0 at
+// h[3 M2 + 16], the exit flag held for CONTINUE / HOLD).
+__global__ void k_gm_step0_combine(double *h, int M2, double tol, double *scal, const double *Z0,
+                                   double *x, int64_t N) {
+    double y0;
+    const int st = gm_step0_eval(h, M2, tol, &y0);
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N || scal[SC_GMSTATE_SLOT] != GM_COMBINE) return;
+    if (t == 0) {
+        scal[SC_GMSTATE_SLOT] = st;
+        h[3 * M2 + 20] = st;
+        if (st == GM_COMBINE) h[3 * M2 + 16] = y0;
+        if (st == GM_CONTINUE || st == GM_HOLD) scal[SC_EXIT_SLOT] = SC_EXIT_HOLD;
+    }
+    if (t >= N || st != GM_COMBINE) return;
     double v = x[t];
-    v += y0[0] * Z0[t];
+    v += y0 * Z0[t];
     x[t] = v;
 }
-// the true residual of the combined x (norms at h + 3 M2 + 8) against the
-// tolerance
-__global__ void k_gm_verify(double *h, int M2, double tol, double *scal) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || scal[SC_GMSTATE_SLOT] != GM_COMBINE) return;
-    double relinf, rel_s;
-    gm_rel(h + 3 * M2 + 8, &relinf, &rel_s);
-    const int st = (relinf <= tol && rel_s <= tol) ? GM_DONE : GM_RESTART;
-    scal[SC_GMSTATE_SLOT] = st;
-    h[3 * M2 + 20] = st;
-    if (st == GM_RESTART) scal[SC_EXIT_SLOT] = SC_EXIT_HOLD;
-}
-
 // operator scaling, E and Dr b of the current matrix (once per solve)
 static int gm_setup(hmec_ctx *c, GmresBuf &g) {
     hipStream_t st = c->stream;
@@ -2606,7 +2614,7 @@ static int gm_setup(hmec_ctx *c, GmresBuf &g) {
 // cycle start's norms when nrm is set; no copy when copy is unset); not
 // waited for.  Without gs only Z_j, w = A Z_j and its first-pass dot
 // products with V_0..V_j plus w . w (at h[j + 1]) are formed: the first step
-// of a solve decided on the device (k_gm_step0) needs no more, and gm_gs
+// of a solve decided on the device (k_gm_step0_combine) needs no more, and gm_gs
 // completes the step when the cycle goes on.
 static int gm_gs(hmec_ctx *c, GmresBuf &g, int j, int m, bool nrm, bool copy);
 static int gm_arnoldi(hmec_ctx *c, GmresBuf &g, int j, int m, bool nrm, bool copy = true,
@@ -2680,14 +2688,11 @@ int sv_gmres_first(hmec_ctx *c, double tol) {
     hipLaunchKernelGGL(k_vscale_rr, dim3(gN), dim3(256), 0, st, g.rsd, nrm_d + 4, g.V, N);
     LAUNCH_CHECK();
     HMEC_TRY(gm_arnoldi(c, g, 0, GM_M, true, false, false));
-    hipLaunchKernelGGL(k_gm_step0, dim3(1), dim3(64), 0, st, g.hdev, M2, tol, c->d.scal);
+    hipLaunchKernelGGL(k_gm_step0_combine, dim3(gN), dim3(256), 0, st, g.hdev, M2, tol, c->d.scal,
+                       (const double *)g.Z, c->d.sol, N);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gm_combine0, dim3(gN), dim3(256), 0, st, (const double *)g.Z,
-                       (const double *)(g.hdev + 3 * M2 + 16), (const double *)c->d.scal, c->d.sol, N);
-    LAUNCH_CHECK();
-    HMEC_TRY(gm_residual_launch(c, g, c->d.sol, nrm2_d, c->d.scal));   // only after a combine
-    hipLaunchKernelGGL(k_gm_verify, dim3(1), dim3(64), 0, st, g.hdev, M2, tol, c->d.scal);
-    LAUNCH_CHECK();
+    // only after a combine: the true residual, checked against tol
+    HMEC_TRY(gm_residual_launch(c, g, c->d.sol, nrm2_d, c->d.scal, tol, g.hdev + 3 * M2 + 20));
     // the column, both residuals' norms, y0 and the state in one copy
     HMEC_HIP(hipMemcpyAsync(g.hhost, g.hdev, sizeof(double) * (3 * M2 + 21), hipMemcpyDeviceToHost, st));
     if (!g.ev_first) HMEC_HIP(hipEventCreateWithFlags(&g.ev_first, hipEventDisableTiming));
