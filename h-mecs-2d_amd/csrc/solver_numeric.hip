@@ -1223,18 +1223,39 @@ __global__ void k_condense_matrix(const double *val, int64_t NS, const int32_t *
     for (int t = tptr[r]; t < tptr[r + 1]; t++) sval[tsslot[t]] += val[taslot[t]] * e[tdk[t]];
 }
 // gv = A_DD^-1 (v_D / rs)   (rs: optional exact power-of-two row scaling)
+// A GMRES cycle's first basis vector formed where the preconditioner reads
+// it: v = a src with a = 1 / sqrt(rr[0]) (0 when rr[0] <= 0), each thread
+// writing the entry of the unknown it owns into vout (the Darcy unknowns in
+// k_darcy_gv, the others in k_cond_rhs: together every entry once)
+struct VScale {
+    const double *src = nullptr, *rr = nullptr;
+    double *vout = nullptr;
+};
+__device__ __forceinline__ double vscale_a(const VScale &sc) {
+    const double q = *sc.rr;
+    return q > 0 ? 1.0 / sqrt(q) : 0.0;
+}
 __global__ void k_darcy_gv(const double *v, const double *rs, int64_t ND, const int32_t *dglob,
                            const int32_t *dnb, const int32_t *dnbe, const double *dcoef,
-                           double *gv) {
+                           double *gv, VScale sc) {
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ND) return;
     const int64_t k = dglob[d];
-    const double vb = rs ? v[k] / rs[k] : v[k];
+    const double a = sc.src ? vscale_a(sc) : 0.0;
+    double vk;
+    if (sc.src) {
+        vk = a * sc.src[k];
+        sc.vout[k] = vk;
+    } else {
+        vk = v[k];
+    }
+    const double vb = rs ? vk / rs[k] : vk;
     if (dnbe[d] < 0) {
         gv[d] = vb / dcoef[3 * d];
     } else {
         const int64_t n = dnb[d];
-        const double vn = rs ? v[n] / rs[n] : v[n];
+        const double vnr = sc.src ? a * sc.src[n] : v[n];
+        const double vn = rs ? vnr / rs[n] : vnr;
         const double gn = vn / dcoef[3 * d + 1];
         gv[d] = (vb - dcoef[3 * d + 2] * gn) / dcoef[3 * d];
     }
@@ -1246,11 +1267,18 @@ __global__ void k_darcy_gv(const double *v, const double *rs, int64_t ND, const 
 __global__ void k_cond_rhs(const double *v, const double *rs, const double *rcoef, int64_t NS,
                            const int32_t *full_of_s, const int32_t *rptr, const int32_t *rd,
                            const double *gv, const double *rsS, double *vs, double *zf,
-                           const int32_t *fpos) {
+                           const int32_t *fpos, VScale sc) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= NS) return;
     const int64_t k = full_of_s[r];
-    double s = rs ? v[k] / rs[k] : v[k];
+    double vk;
+    if (sc.src) {
+        vk = vscale_a(sc) * sc.src[k];
+        sc.vout[k] = vk;
+    } else {
+        vk = v[k];
+    }
+    double s = rs ? vk / rs[k] : vk;
     for (int t = rptr[r]; t < rptr[r + 1]; t++) s -= rcoef[t] * gv[rd[t]];
     const double o = rsS[r] * s;
     if (zf) zf[fpos[r]] = o;
@@ -2218,12 +2246,14 @@ static int darcy_e(hmec_ctx *c, hipStream_t st, int *bad, const double *va, cons
 // z = A^-1 (v / rs) for the full coupled system by block elimination of the
 // Darcy unknowns around the condensed factor (rs: optional row scaling of
 // the operator, exact powers of two); E must be the current matrix's
-static int precond_apply(hmec_ctx *c, const double *v, const double *rs, double *z) {
+// (sc: v is formed from sc.src on the way in and written to sc.vout)
+static int precond_apply(hmec_ctx *c, const double *v, const double *rs, double *z,
+                         VScale sc = VScale()) {
     const CondPlan &cp = c->cp;
     CondDev &q = c->dc;
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(k_darcy_gv, dim3((unsigned)((cp.ND + 255) / 256)), dim3(256), 0, st, v, rs,
-                       cp.ND, q.dglob, q.dnb, q.dnbe, q.dcoef, q.gv);
+                       cp.ND, q.dglob, q.dnb, q.dnbe, q.dcoef, q.gv, sc);
     LAUNCH_CHECK();
     double *zf = nullptr;
     const int32_t *fpos = nullptr;
@@ -2231,7 +2261,7 @@ static int precond_apply(hmec_ctx *c, const double *v, const double *rs, double 
     const double *srs, *scs;
     sv_active_scales(c, &srs, &scs);
     hipLaunchKernelGGL(k_cond_rhs, dim3((unsigned)((cp.NS + 255) / 256)), dim3(256), 0, st, v, rs,
-                       q.rcoef, cp.NS, q.full_of_s, q.rptr, q.rd, q.gv, srs, q.vs, zf, fpos);
+                       q.rcoef, cp.NS, q.full_of_s, q.rptr, q.rd, q.gv, srs, q.vs, zf, fpos, sc);
     LAUNCH_CHECK();
     if (zf) sv_sweep_rhs_given(c);
     HMEC_TRY(solve_condensed(c, q.vs));
@@ -2458,8 +2488,8 @@ static int gm_dots(hmec_ctx *c, GmresBuf &g, const double *V, int nv, const doub
 }
 
 // z = A^-1 Dr^-1 v: right preconditioner of the row-equilibrated operator Dr A
-static int gm_precond(hmec_ctx *c, const double *v, double *zout) {
-    return precond_apply(c, v, c->d.rscale, zout);
+static int gm_precond(hmec_ctx *c, const double *v, double *zout, VScale sc = VScale()) {
+    return precond_apply(c, v, c->d.rscale, zout, sc);
 }
 
 // Relative inf-norm residuals of x: unscaled ||b - A x|| / ||b|| (the
@@ -2617,13 +2647,21 @@ static int gm_setup(hmec_ctx *c, GmresBuf &g) {
 // of a solve decided on the device (k_gm_step0_combine) needs no more, and gm_gs
 // completes the step when the cycle goes on.
 static int gm_gs(hmec_ctx *c, GmresBuf &g, int j, int m, bool nrm, bool copy);
+// (scaled: V_0 = Dr r / ||Dr r||_2 is formed by the preconditioner's first
+// kernels from g.rsd and the start's squared norm)
 static int gm_arnoldi(hmec_ctx *c, GmresBuf &g, int j, int m, bool nrm, bool copy = true,
-                      bool gs = true) {
+                      bool gs = true, bool scaled = false) {
     hipStream_t st = c->stream;
     const int64_t N = c->N;
     const int M2 = m + 2;
     double *Zj = g.Z + (int64_t)j * N;
-    HMEC_TRY(gm_precond(c, g.V + (int64_t)j * N, Zj));
+    VScale sc;
+    if (scaled) {
+        sc.src = g.rsd;
+        sc.rr = g.hdev + 3 * M2 + 4;
+        sc.vout = g.V + (int64_t)j * N;
+    }
+    HMEC_TRY(gm_precond(c, g.V + (int64_t)j * N, Zj, sc));
     // classical Gram-Schmidt, twice, coefficients kept on the device;
     // one copy of the Hessenberg column per iteration.  The first
     // pass's dot products come with the operator product, the second
@@ -2685,9 +2723,7 @@ int sv_gmres_first(hmec_ctx *c, double tol) {
     double *nrm_d = g.hdev + 3 * M2, *nrm2_d = g.hdev + 3 * M2 + 8;
     HMEC_TRY(gm_setup(c, g));
     HMEC_TRY(gm_residual_launch(c, g, c->d.sol, nrm_d));
-    hipLaunchKernelGGL(k_vscale_rr, dim3(gN), dim3(256), 0, st, g.rsd, nrm_d + 4, g.V, N);
-    LAUNCH_CHECK();
-    HMEC_TRY(gm_arnoldi(c, g, 0, GM_M, true, false, false));
+    HMEC_TRY(gm_arnoldi(c, g, 0, GM_M, true, false, false, true));
     hipLaunchKernelGGL(k_gm_step0_combine, dim3(gN), dim3(256), 0, st, g.hdev, M2, tol, c->d.scal,
                        (const double *)g.Z, c->d.sol, N);
     LAUNCH_CHECK();
