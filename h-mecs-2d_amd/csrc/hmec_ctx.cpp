@@ -103,7 +103,7 @@ static void free_ctx(hmec_ctx *c) {
     hipFree(d.scal);
     if (d.h_scal) hipHostFree(d.h_scal);
     hipFree(d.rowp); hipFree(d.coli); hipFree(d.val); hipFree(d.rhs); hipFree(d.sol);
-    hipFree(d.trip_slot); hipFree(d.node_trip); hipFree(d.rscale); hipFree(d.cscale);
+    hipFree(d.trip_slot); hipFree(d.node_trip); hipFree(d.sec_off); hipFree(d.rscale); hipFree(d.cscale);
     hipFree(d.work); hipFree(d.work2); hipFree(d.counters);
     for (auto &g : c->xguess) {
         hipFree(g.x);

@@ -242,6 +242,7 @@ struct DeviceState {
     double *val = nullptr, *rhs = nullptr, *sol = nullptr;
     int32_t *trip_slot = nullptr;    // CSR slot of each triplet (emission order)
     int32_t *node_trip = nullptr;    // first triplet of each node (N/6 + 1)
+    uint8_t *sec_off = nullptr;      // per node: first triplet of each row section (8 bytes)
     // solver
     double *arena = nullptr;         // FP64 fronts
     float *arena32 = nullptr;        // FP32 fronts

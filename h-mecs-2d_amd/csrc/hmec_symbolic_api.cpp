@@ -10,7 +10,7 @@
 
 int hmec_pattern_host(const hmec_params &P, std::vector<int32_t> &rowp,
                       std::vector<int32_t> &coli, std::vector<int32_t> *slot_out,
-                      std::vector<int32_t> *node_trip_out);
+                      std::vector<int32_t> *node_trip_out, std::vector<uint8_t> *sec_out = nullptr);
 void hmec_footprint_record(int Nx, int Ny, int64_t arena);
 
 extern "C" int hmec_pattern(const hmec_params *p, int64_t *n, int64_t *nnz, int32_t *rowp,
@@ -198,7 +198,7 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     b += ns * 8 * HMEC_NFIELDS;
     b += (6 * kNumTimesteps + 4 * kSeriesBlocks) * 8 + (P.niterglobal + 1) * 8;   // series, DSYLSQ
     const int64_t NS = 4 * ns;
-    b += z.nnz * (4 + 8) + (N + 1) * 4 + z.ntrip * 4;                     // coupled CSR
+    b += z.nnz * (4 + 8) + (N + 1) * 4 + z.ntrip * 4 + ns * 8;            // coupled CSR, sections
     b += z.snnz * (4 + 8 + 16 + 3 * 4 + 4 + 4 * 4) + (NS + 1) * 16;    // condensed CSR, maps
     b += z.pull + z.snnz * (4 + 8);                                     // front pull plan, values
     b += 6 * N * 8 + 4 * NS * 8 + 3 * N * 4 + z.varena * 8 + z.nfronts * 256;   // vectors, plans

@@ -704,9 +704,18 @@ struct PatternEmit {
 // and the triplet -> slot map, built on the host once per grid from the
 // emission order (what Eigen's setFromTriplets + makeCompressed produce,
 // h_mec_rsf_v9.cpp:1046-1049).
+// Triplet emitter that only counts
+struct CountEmit {
+    int n = 0;
+    void add(int, int, double) { n++; }
+    void rhs(int, double) {}
+};
+
+// sec_out: per node, the first triplet of each row section relative to the
+// node's first (8 bytes per node, sections 0..5 used)
 int hmec_pattern_host(const hmec_params &P, std::vector<int32_t> &rowp,
                       std::vector<int32_t> &coli, std::vector<int32_t> *slot_out,
-                      std::vector<int32_t> *node_trip_out) {
+                      std::vector<int32_t> *node_trip_out, std::vector<uint8_t> *sec_out) {
     const int Nx1 = P.Nx + 1, Ny1 = P.Ny + 1;
     const int64_t ns = (int64_t)Nx1 * Ny1, N = ns * 6;
     std::vector<double> base((size_t)ns * HMEC_NFIELDS, 0.0);
@@ -742,6 +751,25 @@ int hmec_pattern_host(const hmec_params &P, std::vector<int32_t> &rowp,
             emit_node(i, j, in, e);
         }
     node_trip[(size_t)ns] = (int32_t)ti.size();
+    if (sec_out) {
+        sec_out->assign((size_t)ns * 8, 0);
+        for (int j = 0; j < Nx1; j++)
+            for (int i = 0; i < Ny1; i++) {
+                const size_t t = (size_t)j * Ny1 + i;
+                int off = 0;
+                for (int q = 0; q < kAsmSections; q++) {
+                    if (off > 255) return HMEC_E_ARG;
+                    (*sec_out)[8 * t + q] = (uint8_t)off;
+                    CountEmit ce;
+                    emit_node(i, j, in, ce, 1 << q);
+                    off += ce.n;
+                }
+                if (off != node_trip[t + 1] - node_trip[t]) {
+                    hmec_set_last_error("assembly: row sections do not cover the node's triplets");
+                    return HMEC_E_ARG;
+                }
+            }
+    }
     const int64_t nt = (int64_t)ti.size();
     std::vector<int32_t> cnt(N + 1, 0);
     for (int64_t t = 0; t < nt; t++) cnt[ti[t] + 1]++;
@@ -777,7 +805,8 @@ int hmec_pattern_host(const hmec_params &P, std::vector<int32_t> &rowp,
 int hmec_build_pattern(hmec_ctx *c) {
     const int64_t N = c->N;
     std::vector<int32_t> slot, node_trip;
-    HMEC_TRY(hmec_pattern_host(c->P, c->h_rowp, c->h_coli, &slot, &node_trip));
+    std::vector<uint8_t> sec;
+    HMEC_TRY(hmec_pattern_host(c->P, c->h_rowp, c->h_coli, &slot, &node_trip, &sec));
     c->nnz = (int64_t)c->h_coli.size();
     const int64_t nt = (int64_t)slot.size();
     c->ntrip = nt;
@@ -786,6 +815,8 @@ int hmec_build_pattern(hmec_ctx *c) {
     HMEC_HIP(hipMalloc(&c->d.val, sizeof(double) * c->nnz));
     HMEC_HIP(hipMalloc(&c->d.trip_slot, sizeof(int32_t) * nt));
     HMEC_HIP(hipMalloc(&c->d.node_trip, sizeof(int32_t) * node_trip.size()));
+    HMEC_HIP(hipMalloc(&c->d.sec_off, sec.size()));
+    HMEC_HIP(hipMemcpy(c->d.sec_off, sec.data(), sec.size(), hipMemcpyHostToDevice));
     HMEC_HIP(hipMemcpy(c->d.rowp, c->h_rowp.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice));
     HMEC_HIP(hipMemcpy(c->d.coli, c->h_coli.data(), sizeof(int32_t) * c->nnz, hipMemcpyHostToDevice));
     HMEC_HIP(hipMemcpy(c->d.trip_slot, slot.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice));
@@ -808,16 +839,20 @@ struct LdsEmit {
     __device__ void rhs(int r, double v) { R[r - rbase] = v; }
 };
 
-// ASM_T nodes per workgroup: the triplets are summed in LDS in emission order
-// (duplicates in the reference's triplet order, as setFromTriplets sums them),
-// then the workgroup's contiguous CSR value range and its right-hand-side rows
-// are stored with coalesced writes.  ptscale comes from the device scalars.
+// ASM_T nodes per workgroup, one wavefront per row section (wave s emits
+// section s of the workgroup's nodes, from the section's first triplet): the
+// triplets are summed in LDS in emission order (duplicates in the
+// reference's triplet order, as setFromTriplets sums them; a section adds
+// only to its own row, so the waves write disjoint entries), then the
+// workgroup's contiguous CSR value range and its right-hand-side rows are
+// stored with coalesced writes.  ptscale comes from the device scalars.
 constexpr int ASM_T = 64;
-__global__ void __launch_bounds__(ASM_T)
+constexpr int ASM_WG = ASM_T * kAsmSections;
+__global__ void __launch_bounds__(ASM_WG)
 k_assemble(AsmIn in, const double *__restrict__ scal, const int32_t *__restrict__ rowp,
-           const int32_t *__restrict__ node_trip, const int32_t *__restrict__ slot,
-           double *__restrict__ val, double *__restrict__ R, double *__restrict__ rscale, int Nx1,
-           int Ny1, int dev) {
+           const int32_t *__restrict__ node_trip, const uint8_t *__restrict__ sec_off,
+           const int32_t *__restrict__ slot, double *__restrict__ val, double *__restrict__ R,
+           double *__restrict__ rscale, int Nx1, int Ny1, int dev) {
     extern __shared__ double lv[];
     if (dev) {
         if (scal[SC_EXIT] != 0) return;
@@ -825,30 +860,31 @@ k_assemble(AsmIn in, const double *__restrict__ scal, const int32_t *__restrict_
     }
     __shared__ double lr[6 * ASM_T];
     const int tid = threadIdx.x;
+    const int sec = __builtin_amdgcn_readfirstlane(tid / ASM_T), q = tid % ASM_T;
     const int n0 = blockIdx.x * ASM_T;
     const int nn = min(ASM_T, Nx1 * Ny1 - n0);
     const int base = rowp[6 * n0], ne = rowp[6 * (n0 + nn)] - base;
-    for (int p = tid; p < ne; p += ASM_T) lv[p] = 0.;
-    for (int r = tid; r < 6 * nn; r += ASM_T) lr[r] = 0.;
+    for (int p = tid; p < ne; p += ASM_WG) lv[p] = 0.;
+    for (int r = tid; r < 6 * nn; r += ASM_WG) lr[r] = 0.;
     in.ptscale = scal[SC_PTSCALE];
     __syncthreads();
-    if (tid < nn) {
-        const int t = n0 + tid;
-        LdsEmit e{lv, slot + node_trip[t], 0, base, lr, 6 * n0};
-        emit_node(t % Ny1, t / Ny1, in, e);
+    if (q < nn) {
+        const int t = n0 + q;
+        LdsEmit e{lv, slot + node_trip[t] + sec_off[8 * t + sec], 0, base, lr, 6 * n0};
+        emit_node(t % Ny1, t / Ny1, in, e, 1 << sec);
     }
     __syncthreads();
-    for (int p = tid; p < ne; p += ASM_T) val[base + p] = lv[p];
-    for (int r = tid; r < 6 * nn; r += ASM_T) R[6 * n0 + r] = lr[r];
-    // power-of-two row scales of the node's six rows (the solver's equilibration)
-    if (tid < nn) {
-        const int64_t k0 = 6 * (int64_t)(n0 + tid);
-#pragma unroll
-        for (int v = 0; v < 6; v++) {
-            double m = 0;
-            for (int p = rowp[k0 + v]; p < rowp[k0 + v + 1]; p++) m = fmax(m, fabs(lv[p - base]));
-            rscale[k0 + v] = pow2_recip(m);
-        }
+    for (int p = tid; p < ne; p += ASM_WG) val[base + p] = lv[p];
+    for (int r = tid; r < 6 * nn; r += ASM_WG) R[6 * n0 + r] = lr[r];
+    // power-of-two row scale of the section's row (the solver's
+    // equilibration): sections vxs vys Pt vxD vyD Pf are rows 1 2 0 4 5 3
+    if (q < nn) {
+        constexpr int kRow[kAsmSections] = {1, 2, 0, 4, 5, 3};
+        const int v = kRow[sec];
+        const int64_t k = 6 * (int64_t)(n0 + q) + v;
+        double m = 0;
+        for (int p = rowp[k]; p < rowp[k + 1]; p++) m = fmax(m, fabs(lv[p - base]));
+        rscale[k] = pow2_recip(m);
     }
 }
 
@@ -870,9 +906,9 @@ int k_assemble(hmec_ctx *c, double dt, bool dev) {
         hmec_set_last_error("assembly: CSR range of a workgroup exceeds the LDS budget");
         return HMEC_E_ARG;
     }
-    hipLaunchKernelGGL(k_assemble, dim3(nblk), dim3(ASM_T), lds, c->stream, in, c->d.scal,
-                       c->d.rowp, c->d.node_trip, c->d.trip_slot, c->d.val, c->d.rhs, c->d.rscale,
-                       c->Nx1, c->Ny1, dev ? 1 : 0);
+    hipLaunchKernelGGL(k_assemble, dim3(nblk), dim3(ASM_WG), lds, c->stream, in, c->d.scal,
+                       c->d.rowp, c->d.node_trip, (const uint8_t *)c->d.sec_off, c->d.trip_slot,
+                       c->d.val, c->d.rhs, c->d.rscale, c->Nx1, c->Ny1, dev ? 1 : 0);
     LAUNCH_CHECK();
     c->rscale_ready = true;
     return HMEC_OK;
