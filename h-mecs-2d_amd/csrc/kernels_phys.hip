@@ -975,13 +975,73 @@ __global__ void k_stress(double *f, int64_t ns, int Nx, int Ny, double dx, doubl
 // 1165-1308): the bisection for the node's stress, state, slip rate, yield
 // stress and viscosity, and the node's dtlapusta candidate, bisection-cap
 // flag, DSY^2 and yield flag.
-__device__ __forceinline__ void rsf_node(double *f, int64_t ns, int Nx, int Ny, const hmec_params &P,
-                                         double dt, int j, double *dsy2, int *flag, double *dtln,
-                                         int *capn) {
+// The bisection of the fault node's rate-and-state solve, emulated exactly
+// by one wavefront: each round, lane q evaluates the midpoint of node q + 1
+// of the complete binary tree of the next six sequential steps (the path
+// bits of q + 1 below its leading one choose SIIB1 / SIIB2 := mid, as the
+// sequential step does, from the round's bracket), then the lanes walk the
+// tree in the sequential order with the sequential tests (cap, the sign
+// rule, |DSIIB| > 1e-3).  Every midpoint and every evaluation is the
+// sequential one, so SIIB4, V and the cap come out bit for bit; six steps
+// cost one evaluation.
+__device__ __forceinline__ void rsf_bisect_wave(double &SIIB1, double &SIIB2, const double DSIIB1,
+                                                const double V0, const double arsf, const double prB,
+                                                const double expf_, const double dx,
+                                                const double eta0, const double siiel,
+                                                const double G, const double dt, const int cap,
+                                                double &SIIB4, double &V, int &capped) {
+    constexpr int D = 6;
+    const int nq = (threadIdx.x & 63) + 1;
+    const int depth = 31 - __clz(nq);
+    int it = 0;
+    bool done = false;
+    while (!done) {
+        double s1 = SIIB1, s2 = SIIB2;
+        for (int k = depth - 1; k >= 0; k--) {
+            const double m = (s1 + s2) / 2.;
+            if ((nq >> k) & 1) s1 = m;
+            else s2 = m;
+        }
+        const double mid = (s1 + s2) / 2.;
+        const double vq = 2 * V0 * sinh(fmax((mid), 0.) / arsf / prB) * expf_;
+        const double eii = vq / dx / 2.;
+        const double etapl = mid / 2. / eii;
+        const double etavp = 1. / (1. / eta0 + 1. / etapl);
+        const double dq = siiel * etavp / (G * dt + etavp) - mid;
+        int node = 1;
+        for (int k = 0; k < D; k++) {
+            if (it >= cap) {
+                capped = 1;
+                done = true;
+                break;
+            }
+            it++;
+            const double m = __shfl(mid, node - 1), d = __shfl(dq, node - 1);
+            SIIB4 = m;
+            V = __shfl(vq, node - 1);
+            const bool right = (d >= 0 && DSIIB1 >= 0) || (d <= 0 && DSIIB1 <= 0);
+            if (right) SIIB1 = m;
+            else SIIB2 = m;
+            if (!(fabs(d) > 1e-3)) {
+                done = true;
+                break;
+            }
+            node = 2 * node + (right ? 1 : 0);
+        }
+    }
+}
+
+// The fault node (line_fault, j) by one wavefront (all lanes compute, lane 0
+// stores); returns the new slip rate
+__device__ __forceinline__ double rsf_node(double *f, int64_t ns, int Nx, int Ny, const hmec_params &P,
+                                           double dt, int j, double *dsy2, int *flag, double *dtln,
+                                           int *capn) {
+    const bool st = (threadIdx.x & 63) == 0;
     const int Ny1 = Ny + 1;
     const int i = P.line_fault;
     const double V0 = P.V0, BETASOLID = P.BETASOLID, dx = P.dx;
     int capped = 0;
+    double vout = 0;
     {
         const int64_t b = i + (int64_t)j * Ny;
         const int64_t s = i + (int64_t)j * Ny1;
@@ -993,7 +1053,7 @@ __device__ __forceinline__ void rsf_node(double *f, int64_t ns, int Nx, int Ny, 
         const double G = BF(HMEC_F_GGG), ETA = BF(HMEC_F_ETA), OMij = BF(HMEC_F_OM);
         const double sxy = BF(HMEC_F_SXY);
         const double SIIBij = sqrt(sxy * sxy + (sxx * sxx + syy * syy + (sxx + syy) * (sxx + syy)) / 32.);
-        BF(HMEC_F_SIIB) = SIIBij;
+        if (st) BF(HMEC_F_SIIB) = SIIBij;
         const double siiel = SIIBij / (ETA / (G * dt + ETA));
         const double prB_temp = (SB(HMEC_F_PT) - SB(HMEC_F_PF)) / 4.;
         double prB = prB_temp;
@@ -1011,30 +1071,18 @@ __device__ __forceinline__ void rsf_node(double *f, int64_t ns, int Nx, int Ny, 
         ETAVP = 1. / (1. / eta0 + 1. / ETAPL);
         const double DSIIB2 = siiel * ETAVP / (G * dt + ETAVP) - SIIB2;
         double SIIB4 = 0.;
-        if ((DSIIB1 >= 0 && DSIIB2 <= 0) || (DSIIB1 <= 0 && DSIIB2 >= 0)) {
-            double DSIIB = 1e9;
-            int it = 0;
-            while (fabs(DSIIB) > 1e-3) {
-                if (it++ >= P.bisect_cap) { capped = 1; break; }
-                SIIB4 = (SIIB1 + SIIB2) / 2.;
-                V = 2 * V0 * sinh(fmax((SIIB4), 0.) / arsf / prB) * expf_;
-                EIISLIP = V / dx / 2.;
-                ETAPL = SIIB4 / 2. / EIISLIP;
-                ETAVP = 1. / (1. / eta0 + 1. / ETAPL);
-                DSIIB = siiel * ETAVP / (G * dt + ETAVP) - SIIB4;
-                if ((DSIIB >= 0 && DSIIB1 >= 0) || (DSIIB <= 0 && DSIIB1 <= 0)) SIIB1 = SIIB4;
-                else SIIB2 = SIIB4;
-            }
-        }
+        if ((DSIIB1 >= 0 && DSIIB2 <= 0) || (DSIIB1 <= 0 && DSIIB2 >= 0))
+            rsf_bisect_wave(SIIB1, SIIB2, DSIIB1, V0, arsf, prB, expf_, dx, eta0, siiel, G, dt,
+                            P.bisect_cap, SIIB4, V, capped);
         double om5;
         if (V * dt / lrsf > 1e-6) om5 = log(V0 / V + (exp(BF(HMEC_F_OM0)) - V0 / V) * exp(-V * dt / lrsf));
         else om5 = log(exp(BF(HMEC_F_OM0)) * (1 - V * dt / lrsf) + V0 * dt / lrsf);
-        BF(HMEC_F_OM5) = om5;
+        if (st) BF(HMEC_F_OM5) = om5;
         const double syield = fmax(P.syieldmin, prB_temp * arsf * asinh(V / 2. / V0 * exp((brsf * om5 + fric) / arsf)));
         double etapl = eta0 * syield / (eta0 * V + syield);
-        BF(HMEC_F_SIGMAY) = syield;
-        BF(HMEC_F_VSLIPB) = V;
-        BF(HMEC_F_SII_FAULT) = SIIB4;
+        if (st) BF(HMEC_F_SIGMAY) = syield;
+        if (st) BF(HMEC_F_VSLIPB) = V;
+        if (st) BF(HMEC_F_SII_FAULT) = SIIB4;
         const double g_temp = 2 * G;
         const double k = g_temp / (M_PI * (1 - ((3. / BETASOLID - g_temp) / (6. / BETASOLID + g_temp))) * dx);
         double dTETAmax;
@@ -1049,43 +1097,50 @@ __device__ __forceinline__ void rsf_node(double *f, int64_t ns, int Nx, int Ny, 
         if (BF(HMEC_F_YNY0) > 0) {
             counted = 1;
             const double dsy = SIIBij - syield;
-            BF(HMEC_F_DSY) = dsy;
+            if (st) BF(HMEC_F_DSY) = dsy;
             d2 = dsy * dsy;
         }
         const double A = syield / siiel;
         if (A < 1) {
             etapl = dt * G * A / (1 - A);
             if (etapl < eta0) {
-                BF(HMEC_F_ETA5) = pow(etapl, 1 - P.etawt) * pow(ETA, P.etawt);
-                BF(HMEC_F_YNY) = 1;
+                if (st) BF(HMEC_F_ETA5) = pow(etapl, 1 - P.etawt) * pow(ETA, P.etawt);
+                if (st) BF(HMEC_F_YNY) = 1;
                 if (!counted) {
                     counted = 1;
                     const double dsy = SIIBij - syield;
-                    BF(HMEC_F_DSY) = dsy;
+                    if (st) BF(HMEC_F_DSY) = dsy;
                     d2 = dsy * dsy;
                 }
             } else {
-                BF(HMEC_F_ETA5) = eta0;
+                if (st) BF(HMEC_F_ETA5) = eta0;
             }
         } else {
-            BF(HMEC_F_ETA5) = eta0;
+            if (st) BF(HMEC_F_ETA5) = eta0;
         }
-        dsy2[j] = d2;
-        flag[j] = counted;
-        dtln[j] = dtl;
-        capn[j] = capped;
+        if (st) {
+            dsy2[j] = d2;
+            flag[j] = counted;
+            dtln[j] = dtl;
+            capn[j] = capped;
+        }
+        vout = V;
 #undef BF
 #undef SB
     }
+    return vout;
 }
 
-// One launch after the solve's reload and stresses, thread t:
+// One launch after the solve's reload and stresses.  Workgroups below nbn,
+// thread t:
 // - basic node t (t < Nx Ny): the previous slip rate into the block maximum
 //   (Vmax before this iteration's fault solve, :1077); ETA5 = ETA0, DSY = YNY
-//   = SigmaY = SII_fault = 0, OM5 = OM (:1152-1159); on the fault row, when
-//   the fault is active, the rate-and-state solve of that node; then the
-//   slip-rate timestep candidate dx stpmax / VSLIPB (:1310-1436 inputs);
+//   = SigmaY = SII_fault = 0, OM5 = OM (:1152-1159); then the slip-rate
+//   timestep candidate dx stpmax / VSLIPB (:1310-1436 inputs);
 // - staggered node t: velocity extrema and fault-band maxima.
+// With the fault active, the fault-row nodes are left to the workgroups from
+// nbn on: one wavefront per fault node does the same for its node around the
+// rate-and-state solve (rsf_node).
 // Block partials (max / min, exact in any order) for k_update_final.
 constexpr int UPD_T = 256;
 constexpr int UPD_NQ = 8;   // Vmax, vx max / min, vy max / min, band x / y, dtslip
@@ -1093,27 +1148,40 @@ __global__ void __launch_bounds__(UPD_T) k_update_nodes(double *f, int64_t ns, i
                                                         hmec_params P, double dt, int active,
                                                         const double *yvx, const double *yvy,
                                                         double dxstp, double *part, int nblk,
-                                                        double *dsy2, int *flag, double *dtln,
-                                                        int *capn, const double *gate) {
+                                                        int nbn, double *dsy2, int *flag,
+                                                        double *dtln, int *capn,
+                                                        const double *gate) {
     __shared__ double red[UPD_NQ][UPD_T];
     if (gate && gate[SC_EXIT] == SC_EXIT_HOLD) return;
     const int tid = threadIdx.x, t = blockIdx.x * UPD_T + tid;
     const int Ny1 = Ny + 1, nb = Nx * Ny;
     double v[UPD_NQ] = {-INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, 0, 0, 1e30};
-    if (t < nb) {
-        double *VS = f + (int64_t)HMEC_F_VSLIPB * ns;
+    double *VS = f + (int64_t)HMEC_F_VSLIPB * ns;
+    auto reset = [&](int64_t q) {
+        f[(int64_t)HMEC_F_ETA5 * ns + q] = f[(int64_t)HMEC_F_ETA0 * ns + q];
+        f[(int64_t)HMEC_F_DSY * ns + q] = 0;
+        f[(int64_t)HMEC_F_YNY * ns + q] = 0;
+        f[(int64_t)HMEC_F_SIGMAY * ns + q] = 0;
+        f[(int64_t)HMEC_F_SII_FAULT * ns + q] = 0;
+        f[(int64_t)HMEC_F_OM5 * ns + q] = f[(int64_t)HMEC_F_OM * ns + q];
+    };
+    if ((int)blockIdx.x >= nbn) {
+        // fault node j: one wavefront
+        const int j = (blockIdx.x - nbn) * (UPD_T / 64) + (tid >> 6);
+        if (j < Nx) {
+            const int64_t q = P.line_fault + (int64_t)j * Ny;
+            v[0] = VS[q];
+            if ((tid & 63) == 0) reset(q);
+            const double vs = rsf_node(f, ns, Nx, Ny, P, dt, j, dsy2, flag, dtln, capn);
+            if (vs > 0) v[7] = dxstp / vs;
+        }
+    } else if (t < nb && !(active && t % Ny == P.line_fault)) {
         v[0] = VS[t];
-        f[(int64_t)HMEC_F_ETA5 * ns + t] = f[(int64_t)HMEC_F_ETA0 * ns + t];
-        f[(int64_t)HMEC_F_DSY * ns + t] = 0;
-        f[(int64_t)HMEC_F_YNY * ns + t] = 0;
-        f[(int64_t)HMEC_F_SIGMAY * ns + t] = 0;
-        f[(int64_t)HMEC_F_SII_FAULT * ns + t] = 0;
-        f[(int64_t)HMEC_F_OM5 * ns + t] = f[(int64_t)HMEC_F_OM * ns + t];
-        if (active && t % Ny == P.line_fault) rsf_node(f, ns, Nx, Ny, P, dt, t / Ny, dsy2, flag, dtln, capn);
+        reset(t);
         const double vs = VS[t];
         if (vs > 0) v[7] = dxstp / vs;
     }
-    if (t < (Nx + 1) * Ny1) {
+    if ((int)blockIdx.x < nbn && t < (Nx + 1) * Ny1) {
         const int i = t % Ny1;
         const double a = f[(int64_t)HMEC_F_VXS * ns + t], b = f[(int64_t)HMEC_F_VYS * ns + t];
         v[1] = a;
@@ -1283,11 +1351,12 @@ int k_update_launch(hmec_ctx *c, double dt, int iterstep, bool gated) {
     double *dtln = c->d.work2 + c->Nx;
     int *capn = flag + c->Nx;
     double *part = c->d.work;          // block partials (free between solves)
-    const int nblk = (n + UPD_T - 1) / UPD_T;
+    const int nbn = (n + UPD_T - 1) / UPD_T;
+    const int nblk = nbn + (active ? (c->Nx + UPD_T / 64 - 1) / (UPD_T / 64) : 0);
     Coords k = coords(c);
     hipLaunchKernelGGL(k_update_nodes, dim3(nblk), dim3(UPD_T), 0, c->stream, c->d.fields, c->ns, c->Nx,
-                       c->Ny, P, dt, active, k.yvx, k.yvy, P.dx * P.stpmax, part, nblk, dsy2, flag,
-                       dtln, capn, gate);
+                       c->Ny, P, dt, active, k.yvx, k.yvy, P.dx * P.stpmax, part, nblk, nbn, dsy2,
+                       flag, dtln, capn, gate);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_update_final, dim3(1), dim3(UPF_T), 0, c->stream, c->Nx, active, c->d.scal,
                        (const double *)dsy2, (const int *)flag, (const double *)dtln, (const int *)capn,
