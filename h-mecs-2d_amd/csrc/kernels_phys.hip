@@ -984,6 +984,12 @@ __global__ void k_stress(double *f, int64_t ns, int Nx, int Ny, double dx, doubl
 // rule, |DSIIB| > 1e-3).  Every midpoint and every evaluation is the
 // sequential one, so SIIB4, V and the cap come out bit for bit; six steps
 // cost one evaluation.
+// lane l's value of v (l wave-uniform: two v_readlane, no LDS crossbar)
+__device__ __forceinline__ double lane_read(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
 __device__ __forceinline__ void rsf_bisect_wave(double &SIIB1, double &SIIB2, const double DSIIB1,
                                                 const double V0, const double arsf, const double prB,
                                                 const double expf_, const double dx,
@@ -1016,9 +1022,9 @@ __device__ __forceinline__ void rsf_bisect_wave(double &SIIB1, double &SIIB2, co
                 break;
             }
             it++;
-            const double m = __shfl(mid, node - 1), d = __shfl(dq, node - 1);
+            const double m = lane_read(mid, node - 1), d = lane_read(dq, node - 1);
             SIIB4 = m;
-            V = __shfl(vq, node - 1);
+            V = lane_read(vq, node - 1);
             const bool right = (d >= 0 && DSIIB1 >= 0) || (d <= 0 && DSIIB1 <= 0);
             if (right) SIIB1 = m;
             else SIIB2 = m;
