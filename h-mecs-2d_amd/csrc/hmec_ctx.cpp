@@ -108,6 +108,7 @@ static void free_ctx(hmec_ctx *c) {
     for (auto &g : c->xguess) {
         hipFree(g.x);
         hipFree(g.x2);
+        hipFree(g.x3);
     }
     for (Timer *t : {&c->t_scatter, &c->t_assemble, &c->t_factor, &c->t_solve}) tm_free(*t);
     for (auto &kv : c->kt) tm_free(kv.second);

@@ -47,15 +47,33 @@ int guess_find(hmec_ctx *c, int it) {
     return -1;
 }
 
+constexpr double kQuadSpacing = 0.05;   // most relative change of the step between kept solutions
+
 // x0 from entry k: its newest solution, extrapolated to the current step's
-// time through the earlier one when there is one (and the current step is
-// no farther from the newest than twice the newest's distance from the
-// earlier one)
+// time through the earlier ones when there are (and the current step is no
+// farther from the newest than twice the newest's distance from the earlier
+// one): linearly through one, or quadratically (Lagrange weights in model
+// time) through two when the three are nearly evenly spaced and the entry
+// has switched to it.  An entry switches order whenever a solve started
+// from it needs more than one GMRES step, so each keeps the order that
+// predicts its solutions (early in a run linear mostly, on the smooth
+// interseismic trajectory later quadratic for the entries linear misses).
 int guess_load(hmec_ctx *c, int k) {
     hmec_ctx::XGuess &g = c->xguess[k];
     g.used = ++c->xguess_clock;
+    c->xguess_used = k;
     if (g.has2 && g.t > g.t2 && c->timesum > g.t) {
-        const double r = (c->timesum - g.t) / (g.t - g.t2);
+        const double T = c->timesum;
+        const double r = (T - g.t) / (g.t - g.t2);
+        // quadratic only over nearly even steps: uneven spacing (dt still
+        // growing) makes the weights amplify the kept solutions' errors
+        const double h1 = g.t - g.t2, h0 = g.t2 - g.t3;
+        if (r <= 2.0 && g.has3 && g.quad && h0 > 0 && std::fabs(h1 / h0 - 1.0) <= kQuadSpacing) {
+            const double w1 = (T - g.t2) * (T - g.t3) / ((g.t - g.t2) * (g.t - g.t3));
+            const double w2 = (T - g.t) * (T - g.t3) / ((g.t2 - g.t) * (g.t2 - g.t3));
+            const double w3 = (T - g.t) * (T - g.t2) / ((g.t3 - g.t) * (g.t3 - g.t2));
+            return sv_lincomb3(c, c->d.sol, w1, g.x, w2, g.x2, w3, g.x3);
+        }
         if (r <= 2.0) return sv_lincomb(c, c->d.sol, 1.0 + r, g.x, -r, g.x2);
     }
     HMEC_HIP(hipMemcpyAsync(c->d.sol, g.x, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
@@ -63,6 +81,7 @@ int guess_load(hmec_ctx *c, int k) {
 }
 
 int guess_start(hmec_ctx *c) {
+    c->xguess_used = -1;
     if (c->iterstep >= 2) {
         const int k = guess_find(c, c->iterstep);
         if (k >= 0) return guess_load(c, k);
@@ -107,10 +126,18 @@ int guess_store(hmec_ctx *c, int it) {
         hmec_ctx::XGuess &g = c->xguess[k];
         if (!g.x) HMEC_HIP(hipMalloc(&g.x, sizeof(double) * c->N));
         g.has2 = false;
+        g.has3 = false;
         g.t = c->timesum;
     }
     hmec_ctx::XGuess &g = c->xguess[k];
     if (g.t != c->timesum) {
+        // x3 <- x2 <- x <- the new solution (buffers rotated)
+        if (g.has2) {
+            if (!g.x3) HMEC_HIP(hipMalloc(&g.x3, sizeof(double) * c->N));
+            std::swap(g.x2, g.x3);
+            g.t3 = g.t2;
+            g.has3 = true;
+        }
         if (!g.x2) HMEC_HIP(hipMalloc(&g.x2, sizeof(double) * c->N));
         std::swap(g.x, g.x2);
         g.t2 = g.t;
@@ -211,6 +238,14 @@ int fp32_finish(hmec_ctx *c, hmec_step_info &acc, Fp32Solve &fs, int rc, hmec_so
     return HMEC_OK;
 }
 
+// after an FP32 solve: a guess that needed more than one GMRES step
+// switches its entry's extrapolation order
+void guess_outcome(hmec_ctx *c, const hmec_solve_stats &st) {
+    if (c->xguess_used >= 0 && st.ir_iterations > 1)
+        c->xguess[c->xguess_used].quad = !c->xguess[c->xguess_used].quad;
+    c->xguess_used = -1;
+}
+
 void solve_stats(hmec_step_info &acc, const hmec_solve_stats &st) {
     acc.solves++;
     acc.perturbed_pivots += st.perturbed_pivots;
@@ -238,6 +273,7 @@ int linear_solve(hmec_ctx *c, hmec_step_info &acc) {
         const int rc = sv_gmres(c, fs.max_it, c->ir_tol, &st);
         tm_end(c, c->t_solve, 0);
         HMEC_TRY(fp32_finish(c, acc, fs, rc, st));
+        guess_outcome(c, st);
     }
     solve_stats(acc, st);
     return HMEC_OK;
@@ -312,6 +348,7 @@ extern "C" int hmec_step(hmec_ctx *c, hmec_step_info *info) {
                 std::memset(&st, 0, sizeof(st));
                 const int rc = sv_gmres(c, fs.max_it, c->ir_tol, &st);
                 HMEC_TRY(fp32_finish(c, acc, fs, rc, st));
+                guess_outcome(c, st);
                 solve_stats(acc, st);
             }
         } else {
@@ -344,6 +381,7 @@ extern "C" int hmec_step(hmec_ctx *c, hmec_step_info *info) {
                 HMEC_TRY(tail(false));
                 HMEC_TRY(k_scalars_wait(c, &sc));
             }
+            guess_outcome(c, st);
             solve_stats(acc, st);
         } else {
             HMEC_TRY(k_scalars_wait(c, &sc));

@@ -296,15 +296,17 @@ struct hmec_ctx {
     // one kept at the key in an earlier step, with the model times of their
     // steps; the dt of the last solve
     struct XGuess {
-        double *x = nullptr, *x2 = nullptr;   // newest, and the earlier step's
-        double dt = 0, t = 0, t2 = 0;
-        bool has2 = false;
+        double *x = nullptr, *x2 = nullptr, *x3 = nullptr;   // newest, and two earlier steps'
+        double dt = 0, t = 0, t2 = 0, t3 = 0;
+        bool has2 = false, has3 = false;
         int it = -1;
+        bool quad = false;           // extrapolate quadratically (else linearly) next time
         int64_t used = 0;
     };
     static constexpr int NGUESS = 6;
     XGuess xguess[NGUESS];
     int64_t xguess_clock = 0;
+    int xguess_used = -1;            // entry the current solve's guess came from
     double last_solve_dt = 0;
     int ir_max_stale = 12;           // FP32 policy: GMRES budget with a reused factor
     int ir_max_fresh = 40;           //              ... with a fresh factor
@@ -391,6 +393,8 @@ int sv_gmres_first_result(hmec_ctx *c, int state, hmec_solve_stats *st);
 int sv_gmres_first_state(hmec_ctx *c, int *state);
 constexpr int kGmLeadOk = 1, kGmContinue = 3, kGmHold = 4, kGmDone = 5, kGmRestart = 6;
 int sv_lincomb(hmec_ctx *c, double *y, double a, const double *u, double b, const double *v);
+int sv_lincomb3(hmec_ctx *c, double *y, double a, const double *u, double b, const double *v,
+                double d, const double *w);
 void sv_gmres_free(hmec_ctx *c);
 // ---- serving (hmec_serve.cpp) ----
 int batcher_factorize(hmec_batcher *b, hmec_ctx *c, int precision);

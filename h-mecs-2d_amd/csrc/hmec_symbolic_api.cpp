@@ -209,7 +209,7 @@ extern "C" int64_t hmec_member_footprint(const hmec_params *p, int precision) {
     } else {
         // the FP32 factor slots (sweep form + scales each)
         b += z.arena * 4 + z.dinv * 4 + HMEC_FP32_SLOTS * (z.sweep * 4 + 2 * NS * 8) + (2 * 20 + 3) * N * 8;
-        b += 2 * hmec_ctx::NGUESS * N * 8;                              // GMRES initial guesses
+        b += 3 * hmec_ctx::NGUESS * N * 8;                              // GMRES initial guesses
     }
     return b;
 }

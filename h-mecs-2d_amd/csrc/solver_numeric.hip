@@ -2541,6 +2541,19 @@ static int gm_true_residual(hmec_ctx *c, GmresBuf &g, const double *x, double *r
 // cycle starts from the true residual; its Arnoldi loop stops once the
 // estimated 2-norm has dropped by the factor the inf-norms still need (with
 // a margin of 2).
+// y = a u + b v + d w
+__global__ void k_lincomb3(double *y, double a, const double *u, double b, const double *v, double d,
+                           const double *w, int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) y[t] = (a * u[t] + b * v[t]) + d * w[t];
+}
+int sv_lincomb3(hmec_ctx *c, double *y, double a, const double *u, double b, const double *v,
+                double d, const double *w) {
+    hipLaunchKernelGGL(k_lincomb3, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, y, a, u,
+                       b, v, d, w, c->N);
+    LAUNCH_CHECK();
+    return HMEC_OK;
+}
 int sv_lincomb(hmec_ctx *c, double *y, double a, const double *u, double b, const double *v) {
     hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, y, a, u, b, v,
                        c->N);
