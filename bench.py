@@ -64,6 +64,7 @@ KT_NAMES = ("scatter", "assemble", "factor", "solve", "schur", "sweep_form", "sw
 # and costs device time, so they record only the sweep (the roofline's
 # kernel); a separate window with every timer gives the breakdown
 ROOF_TIMERS = "sweep_fwd,sweep_bwd"
+ROOF_STRIDE = 4     # the throughput windows time every fourth sweep of each direction
 
 
 def kernel_times(ctx):
@@ -81,6 +82,7 @@ def run_path(hm, p, soa, prec, warmup, steps):
         ctx.step()
     ctx.timing(True)
     ctx.timing_select(ROOF_TIMERS)
+    ctx.timing_sample(ROOF_STRIDE)
     ctx.timing_reset()
     lat = []
     infos = []
@@ -357,6 +359,7 @@ def main():
                      "max_rel_residual": float(max(i.max_rel_residual for i in inf_s))}
     # per-phase breakdown: 10 further steps with every timer recording
     ctx32.timing_select(None)
+    ctx32.timing_sample(1)
     ctx32.timing_reset()
     for _ in range(10):
         ctx32.step()

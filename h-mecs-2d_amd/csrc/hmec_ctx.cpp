@@ -50,6 +50,7 @@ void tm_begin(hmec_ctx *c, Timer &t) {
         t.sel_gen = c->timing_gen;
     }
     if (!t.sel) return;
+    if (c->timing_stride > 1 && (t.calls++ % c->timing_stride) != 0) return;
     t.open_ev = tm_event(t);
     hipEventRecord(t.open_ev, c->stream);
     t.open = true;
@@ -679,6 +680,12 @@ extern "C" int hmec_timing_select(hmec_ctx *c, const char *names) {
         }
     }
     c->timing_gen++;
+    return HMEC_OK;
+}
+
+extern "C" int hmec_timing_sample(hmec_ctx *c, int stride) {
+    if (!c || stride < 1) return HMEC_E_ARG;
+    c->timing_stride = stride;
     return HMEC_OK;
 }
 

@@ -199,6 +199,7 @@ struct Timer {
     std::string name;     // the name hmec_kernel_time and hmec_timing_select use
     int sel_gen = -1;     // selection generation sel was computed for
     bool sel = true;      // recorded under the current selection
+    int64_t calls = 0;    // begin calls (sampling by the context's stride)
 };
 
 struct hmec_ctx_impl;
@@ -326,6 +327,7 @@ struct hmec_ctx {
     std::map<std::string, Timer> kt;    // per kernel class (schur, fwd, bwd, ...)
     std::vector<std::string> timing_sel;   // timers recorded (empty: all)
     int timing_gen = 0;
+    int timing_stride = 1;                 // record every stride-th interval of each timer
 };
 
 // ---- kernel launch wrappers (kernels_phys.hip) ----

@@ -255,6 +255,7 @@ def _declare(L):
     L.hmec_get_solution.argtypes = [vp, P(C.c_double)]
     L.hmec_update.argtypes = [vp, C.c_double, C.c_double, P(IterScalars)]
     L.hmec_timing_select.argtypes = [vp, C.c_char_p]
+    L.hmec_timing_sample.argtypes = [vp, C.c_int]
     L.hmec_controller.argtypes = [vp, P(IterScalars), C.c_int, P(C.c_int)]
     L.hmec_iter_commit.argtypes = [vp, C.c_int, C.c_int]
     L.hmec_step_end.argtypes = [vp, C.c_double]
@@ -519,6 +520,10 @@ class Context:
         phase boundary of a step (per-phase wall-clock split in StepInfo)."""
         check(lib().hmec_timing_enable(self.h, 2 if (on and phase_sync) else int(bool(on))),
               "timing_enable")
+
+    def timing_sample(self, stride=1):
+        """Record every stride-th interval of each timer (1: all)."""
+        check(lib().hmec_timing_sample(self.h, int(stride)), "timing_sample")
 
     def timing_select(self, names=None):
         """Record only the named timers (a list or comma-separated string;

@@ -540,6 +540,10 @@ int hmec_timing_reset(hmec_ctx *ctx);
  * NULL or "" records all): every event pair sits in the stream and costs
  * device time, so a throughput window keeps only the ones it reports. */
 int hmec_timing_select(hmec_ctx *ctx, const char *names);
+/* Record only every stride-th interval of each timer (1: all): a throughput
+ * window samples its kernels' durations at a fraction of the events' cost.
+ * No counterpart in the reference (measurement only). */
+int hmec_timing_sample(hmec_ctx *ctx, int stride);
 
 #ifdef __cplusplus
 }
