@@ -975,8 +975,8 @@ __global__ void k_stress(double *f, int64_t ns, int Nx, int Ny, double dx, doubl
 // 1165-1308): the bisection for the node's stress, state, slip rate, yield
 // stress and viscosity, and the node's dtlapusta candidate, bisection-cap
 // flag, DSY^2 and yield flag.
-// The bisection of the fault node's rate-and-state solve, emulated exactly
-// by one wavefront: each round, lane q evaluates the midpoint of node q + 1
+// The bisection of the fault node's rate-and-state solve
+// (h_mec_rsf_v9.cpp:1221-1243), emulated exactly by one wavefront: each round, lane q evaluates the midpoint of node q + 1
 // of the complete binary tree of the next six sequential steps (the path
 // bits of q + 1 below its leading one choose SIIB1 / SIIB2 := mid, as the
 // sequential step does, from the round's bracket), then the lanes walk the
